@@ -1,0 +1,227 @@
+/*
+ * pc_hydro.h -- C ABI of the MI355X (gfx950) Godunov hydro engine.
+ *
+ * The engine stands behind the reference's hydro path
+ *   PeleC::construct_hydro_source   (Source/Hydro.cpp:5-301)
+ *     -> pc_ctoprim / pc_srctoprim  (Source/Hydro.cpp:106-173)
+ *     -> pc_umdrv                   (Source/Hydro.cpp:304-377, decl Source/Hydro.H:910-935)
+ *        -> pc_umeth_3D             (Source/Godunov.cpp:60-476)
+ *        -> pc_adjust_fluxes        (Source/Hydro.cpp:380-413)
+ *        -> pc_consup               (Source/Hydro.cpp:545-557)
+ * and the step glue around it (construct_Snew, computeTemp, estTimeStep:
+ * Source/Advance.cpp:397-416, Source/PeleC.cpp:829-1034,2222-2307).
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every entry point returns
+ * an int status (PC_OK = 0, negative on error) and never aborts; the message
+ * of the last failure is available from pc_last_error().  Device pointers are
+ * HIP device (or managed) memory on the current device; `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  Work is enqueued
+ * asynchronously on that stream unless the function says otherwise.
+ */
+#ifndef PC_HYDRO_H
+#define PC_HYDRO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Non-owning view of one FAB: the fields of amrex::Array4<double>
+ * (AMReX_Array4.H: p, jstride, kstride, nstride, begin, end, ncomp).
+ * Element (i,j,k,n) is p[(i-begin[0]) + (j-begin[1])*jstride
+ *                       + (k-begin[2])*kstride + n*nstride]. */
+typedef struct pc_array4 {
+  double* p;
+  long long jstride;
+  long long kstride;
+  long long nstride;
+  int begin[3]; /* inclusive lower corner of the index box */
+  int end[3];   /* exclusive upper corner */
+  int ncomp;
+} pc_array4_t;
+
+/* Cell-centred index box, amrex::Box smallEnd/bigEnd (both inclusive). */
+typedef struct pc_box {
+  int lo[3];
+  int hi[3];
+} pc_box_t;
+
+/* Status codes */
+enum {
+  PC_OK = 0,
+  PC_ERR_ARG = -1,         /* null pointer, bad box, wrong ncomp, ...       */
+  PC_ERR_GROW = -2,        /* input FAB has fewer than 4 ghost cells        */
+  PC_ERR_PPM_TYPE = -3,    /* ppm_type not 0 or 1 (Source/Godunov.cpp:203)  */
+  PC_ERR_PLM_IORDER = -4,  /* plm_iorder not 1, 2 or 4                      */
+  PC_ERR_HIP = -5,         /* a HIP runtime call failed                     */
+  PC_ERR_NAN = -6,         /* NaN/Inf found in the state (okToContinue)     */
+  PC_ERR_UNSUPPORTED = -7, /* option outside what this build implements     */
+  PC_ERR_NOMEM = -8,       /* device allocation failed                      */
+  PC_ERR_DT = -9           /* dt fell below dt_cutoff (okToContinue)        */
+};
+
+/* Equation of state / mechanism (compile-time NUM_SPECIES in the reference:
+ * Exec/RegTests/Sod/GNUmakefile selects GammaLaw + Null, Exec/RegTests/PMF
+ * selects Fuego + LiDryer).  The value is NUM_SPECIES. */
+enum { PC_EOS_GAMMALAW = 1, PC_EOS_LIDRYER = 9 };
+
+/* Physical boundary codes, PCPhysBCType (Source/PeleC.H:48-58). */
+enum {
+  PC_BC_INTERIOR = 0,
+  PC_BC_INFLOW = 1,
+  PC_BC_OUTFLOW = 2,
+  PC_BC_SYMMETRY = 3,
+  PC_BC_SLIPWALL = 4,
+  PC_BC_NOSLIPWALL = 5,
+  PC_BC_USERBC = 6
+};
+
+/* Hot-path runtime parameters, named and defaulted as the reference's
+ * pelec.* keys (Source/Params/_cpp_parameters:53,96,103,113,118-130,
+ * 267-289).  pc_params_default() fills the reference defaults. */
+typedef struct pc_params {
+  int ppm_type;        /* 0 = PLM, 1 = PPM                       (default 0) */
+  int plm_iorder;      /* 1, 2 or 4                              (4)         */
+  int use_flattening;  /*                                        (1)         */
+  int use_hybrid_weno; /* must be 0 in this build                (0)         */
+  int weno_scheme;     /*                                        (1)         */
+  double difmag;       /* artificial viscosity coefficient       (0.1)       */
+  double cfl;          /*                                        (0.8)       */
+  double init_shrink;  /*                                        (1.0)       */
+  double change_max;   /*                                        (1.1)       */
+  double max_dt;       /*                                        (1e200)     */
+  double fixed_dt;     /* > 0 overrides the CFL estimate         (-1)        */
+  double dt_cutoff;    /*                                        (0)         */
+  int allow_small_energy;          /* must be 1 in this build    (1)         */
+  int allow_negative_energy;       /* must be 1 in this build    (1)         */
+  int dual_energy_update_E_from_e; /*                            (1)         */
+  double dual_energy_eta2;         /*                            (1e-4)      */
+} pc_params_t;
+
+void pc_params_default(pc_params_t* p);
+
+/* ------------------------------------------------------------ context --- */
+typedef struct pc_ctx pc_ctx_t;
+
+/* A context binds an EOS (PC_EOS_*) to the current HIP device and owns the
+ * scratch workspace of the box drivers (stream-ordered reuse, grown on
+ * demand).  Replaces the reference's The_Async_Arena temporaries
+ * (Source/Hydro.cpp:84-97,333-345; Source/Godunov.cpp:113-388). */
+int pc_ctx_create(int eos_kind, pc_ctx_t** out);
+int pc_ctx_destroy(pc_ctx_t* ctx);
+int pc_ctx_num_species(const pc_ctx_t* ctx); /* NUM_SPECIES */
+int pc_ctx_nvar(const pc_ctx_t* ctx);        /* NVAR = 7 + NUM_SPECIES */
+int pc_ctx_qvar(const pc_ctx_t* ctx);        /* QVAR = 8 + NUM_SPECIES */
+const char* pc_last_error(const pc_ctx_t* ctx);
+
+/* ------------------------------------------------------- box drivers --- */
+
+/* K1: pc_ctoprim on every cell of qbx (Source/Hydro.cpp:106-115 ->
+ * Source/PeleCUtilities.H:242-320).  u: NVAR comps, q: QVAR, qaux: NQAUX=6. */
+int pc_ctoprim_v1(pc_ctx_t* ctx, pc_box_t qbx, pc_array4_t u, pc_array4_t q,
+                  pc_array4_t qaux, void* stream);
+
+/* K2: pc_srctoprim on every cell of qbx (Source/Hydro.cpp:164-173 ->
+ * Source/Hydro.H:14-68). */
+int pc_srctoprim_v1(pc_ctx_t* ctx, pc_box_t qbx, pc_array4_t q,
+                    pc_array4_t qaux, pc_array4_t src, pc_array4_t srcq,
+                    void* stream);
+
+/* pc_umdrv (Source/Hydro.H:910-935, Source/Hydro.cpp:304-377): the CTU
+ * unsplit fluxes (PPM or PLM trace, transverse corrections, Riemann),
+ * artificial viscosity, species-flux normalisation, area scaling and the
+ * flux divergence.  Same arguments in the same order; GpuArrays become
+ * pointers to 3 elements.  Box conventions as the reference: uin, q, qaux,
+ * src_q cover grow(bx,4); flx[d] and a[d] cover surroundingNodes(bx,d);
+ * vol covers bx.  On return (stream-ordered): flx[d] holds the extensive
+ * fluxes, uout += dU/dt on bx.  src_q.p may be NULL for identically zero
+ * sources.  cflLoc is unused, as in the reference (may be NULL).
+ * use_hybrid_weno must be 0. */
+int pc_umdrv_v1(pc_ctx_t* ctx, double time, pc_box_t bx, const int* domlo,
+                const int* domhi, const int* bclo, const int* bchi,
+                pc_array4_t uin, pc_array4_t uout, pc_array4_t q,
+                pc_array4_t qaux, pc_array4_t src_q, const double* dx,
+                double dt, int ppm_type, int plm_iorder, int use_flattening,
+                int use_hybrid_weno, int weno_scheme, double difmag,
+                const pc_array4_t* flx, const pc_array4_t* a, pc_array4_t vol,
+                double* cflLoc, void* stream);
+
+/* One box of construct_hydro_source (Source/Hydro.cpp:73-252): K1 ctoprim
+ * and K2 srctoprim on grow(bx,4) into context scratch, then pc_umdrv.
+ * sarr = state with >= 4 ghost cells; src = sources_for_hydro (NVAR comps,
+ * p may be NULL for zero sources); hyd_src: hydro_source (+= on bx);
+ * flx[d] on surroundingNodes(bx,d) are overwritten; a[d], vol as pc_umdrv. */
+int pc_hydro_source_box_v1(pc_ctx_t* ctx, pc_box_t bx, const int* domlo,
+                           const int* domhi, const int* bclo, const int* bchi,
+                           pc_array4_t sarr, pc_array4_t src,
+                           pc_array4_t hyd_src, const double* dx, double dt,
+                           const pc_params_t* prm, const pc_array4_t* flx,
+                           const pc_array4_t* a, pc_array4_t vol, void* stream);
+
+/* -------------------------------------------------------- level (boxes) --- */
+/* A single AMR level as a table of boxes (a BoxArray on one GPU), each with
+ * its own FAB of NVAR comps and 4 ghost cells in one device arena, uniform
+ * Cartesian geometry (area = dx*dx, vol = dx^3 as amrex::Geometry on a
+ * Cartesian grid, Source/PeleC.cpp:600-617).  The level owns the state and
+ * runs the whole step on the device: ghost fill (intra-level copy, periodic,
+ * FOExtrap/reflect by the BC map of Source/Setup.cpp:42-121), hydro source,
+ * S_new = S_old + dt*hydro_source, reset_internal_energy + computeTemp, and
+ * the next dt (estTimeStep + computeNewDt limits), with dt kept in device
+ * memory so consecutive steps need no host synchronisation. */
+typedef struct pc_level pc_level_t;
+
+typedef struct pc_level_desc {
+  int nboxes;
+  const pc_box_t* boxes; /* host array of nboxes valid boxes            */
+  int domlo[3], domhi[3];
+  int bclo[3], bchi[3];    /* PC_BC_* per direction                       */
+  int periodic_self;       /* 1: every box is its own periodic domain     */
+  double dx[3];
+  double problo[3];
+} pc_level_desc_t;
+
+int pc_level_create_v1(pc_ctx_t* ctx, const pc_level_desc_t* desc,
+                       const pc_params_t* prm, pc_level_t** out);
+int pc_level_destroy_v1(pc_level_t* lev);
+/* Number of doubles in box b's FAB ((n+8)^3 * NVAR) and its offset in the
+ * level arena (the FAB layout is Array4 over grow(box,4), comp slowest). */
+long long pc_level_box_offset_v1(const pc_level_t* lev, int b);
+long long pc_level_box_size_v1(const pc_level_t* lev, int b);
+double* pc_level_state_ptr_v1(pc_level_t* lev); /* device arena base */
+long long pc_level_state_size_v1(const pc_level_t* lev);
+/* Copy box b's whole FAB (grow(box,4) x NVAR doubles, Array4 layout)
+ * between the level arena and `buf` (host or device memory; synchronous). */
+int pc_level_write_box_v1(pc_level_t* lev, int b, const double* buf);
+int pc_level_read_box_v1(pc_level_t* lev, int b, double* buf);
+/* Set / get the device-resident dt (synchronous). */
+int pc_level_set_dt_v1(pc_level_t* lev, double dt);
+int pc_level_get_dt_v1(pc_level_t* lev, double* dt);
+/* dt = init_shrink * estTimeStep(state) (or fixed_dt), on the device. */
+int pc_level_init_dt_v1(pc_level_t* lev, void* stream);
+/* Advance nsteps Godunov steps (asynchronous on `stream`). */
+int pc_level_step_v1(pc_level_t* lev, int nsteps, void* stream);
+/* Fill the 4 ghost cells of every box (asynchronous). */
+int pc_level_fill_ghosts_v1(pc_level_t* lev, void* stream);
+/* estTimeStep over the valid cells of the level: cfl*min(dx/(c+|u|)),
+ * limited by max_dt (Source/PeleC.cpp:829-961).  Synchronous. */
+int pc_level_estdt_v1(pc_level_t* lev, double* dt_out, void* stream);
+/* Device time (ms) of the kernels of the last pc_level_step_v1 call, by
+ * stage, measured with HIP events on `stream`: [ghost, ctoprim, trace,
+ * first flux, transverse, final flux+adjust, update+temp+dt].  Synchronous.
+ * Requires pc_level_enable_timing_v1(lev, 1) before the step. */
+int pc_level_enable_timing_v1(pc_level_t* lev, int on);
+int pc_level_stage_times_v1(pc_level_t* lev, double* ms7);
+/* okToContinue (Source/PeleC.cpp:1310-1340): synchronous; returns
+ * PC_ERR_NAN if any step produced NaN/Inf, PC_ERR_DT if dt < dt_cutoff. */
+int pc_level_status_v1(pc_level_t* lev, int* bad, int* small_dt);
+/* Conservative state of box b's valid cells from primitive fields, the
+ * pc_initdata pattern (Exec/RegTests/Sod/prob.H:25-117 with
+ * Exec/RegTests/Sod/prob.cpp:215-241).  prim covers the valid box with
+ * 5 + NUM_SPECIES comps: mode 0 = (rho, p, u, v, w, Y), T from e = RYP2E
+ * by EY2T; mode 1 = (p, T, u, v, w, Y), rho = p Wbar / (R T). */
+int pc_level_prim_to_cons_v1(pc_level_t* lev, int b, pc_array4_t prim,
+                             int mode, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PC_HYDRO_H */

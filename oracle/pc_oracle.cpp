@@ -1,0 +1,503 @@
+// TEST INFRASTRUCTURE ONLY -- C entry points of the CPU oracle (see
+// pc_oracle.h).  Used by tests/, the smoke check and the bench's cpu_baseline
+// leg; never linked into the product.
+#include "pc_oracle.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle_hydro.hpp"
+
+using namespace orc;
+
+namespace {
+
+A4 to_a4(const pc_array4_t& v) {
+  A4 a;
+  a.p = v.p;
+  a.js = v.jstride;
+  a.ks = v.kstride;
+  a.ns = v.nstride;
+  for (int d = 0; d < 3; ++d) { a.b[d] = v.begin[d]; a.e[d] = v.end[d]; }
+  a.ncomp = v.ncomp;
+  return a;
+}
+Box to_box(const pc_box_t& b) {
+  Box r;
+  for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.hi[d] = b.hi[d]; }
+  return r;
+}
+bool covers(const pc_array4_t& v, const Box& b) {
+  for (int d = 0; d < 3; ++d)
+    if (b.lo[d] < v.begin[d] || b.hi[d] >= v.end[d]) return false;
+  return true;
+}
+
+template <class EOS>
+int hydro_source_box_t(const Box& bx, const int* domlo, const int* domhi,
+                       const int* bclo, const int* bchi, const A4& sarr,
+                       const A4* src, const A4& hyd, const double* dx,
+                       double dt, const pc_params_t& prm, const A4* flx,
+                       const A4* a, const A4& vol) {
+  using H = Hydro<EOS>;
+  const Box qbx = grow(bx, 4);
+  Fab q(qbx, H::QVAR), qaux(qbx, NQAUX), srcq(qbx, H::QVAR);
+  ParallelFor(qbx, [&](int i, int j, int k) { H::ctoprim(i, j, k, sarr, q.a, qaux.a); });
+  Fab zero(qbx, H::NVAR);
+  std::fill(zero.v.begin(), zero.v.end(), 0.0);
+  const A4& s = src ? *src : zero.a;
+  ParallelFor(qbx, [&](int i, int j, int k) { H::srctoprim(i, j, k, q.a, qaux.a, s, srcq.a); });
+  for (int d = 0; d < 3; ++d) {  // flux[dir].setVal(0.0) (Source/Hydro.cpp:88)
+    ParallelFor(surroundingNodes(bx, d), [&](int i, int j, int k) {
+      for (int n = 0; n < H::NVAR; ++n) flx[d](i, j, k, n) = 0.0;
+    });
+  }
+  return H::umdrv(bx, domlo, domhi, bclo, bchi, sarr, hyd, q.a, qaux.a, srcq.a,
+                  dx, dt, prm.ppm_type, prm.plm_iorder, prm.use_flattening != 0,
+                  prm.difmag, flx, a, vol);
+}
+
+// ------------------------------------------------------------- level (host)
+struct HostLevel {
+  int nvar;
+  std::vector<Box> boxes;
+  std::vector<long long> off;
+  Box dom;
+  int bclo[3], bchi[3];
+  bool self;
+  std::vector<int> cellmap;  // domain cell -> box id (domain mode)
+  long long total = 0;
+  A4 fab(double* arena, int b) const {
+    return make_a4(arena + off[b], grow(boxes[b], 4), nvar);
+  }
+};
+
+HostLevel make_level(int nvar, const pc_level_desc_t* d) {
+  HostLevel L;
+  L.nvar = nvar;
+  L.self = d->periodic_self != 0;
+  for (int dd = 0; dd < 3; ++dd) {
+    L.dom.lo[dd] = d->domlo[dd];
+    L.dom.hi[dd] = d->domhi[dd];
+    L.bclo[dd] = d->bclo[dd];
+    L.bchi[dd] = d->bchi[dd];
+  }
+  long long o = 0;
+  for (int b = 0; b < d->nboxes; ++b) {
+    L.boxes.push_back(to_box(d->boxes[b]));
+    L.off.push_back(o);
+    o += grow(L.boxes.back(), 4).npts() * nvar;
+  }
+  L.total = o;
+  return L;
+}
+
+void build_cellmap(HostLevel& L) {
+  if (L.self || !L.cellmap.empty()) return;
+  const Box& D = L.dom;
+  const long nx = D.hi[0] - D.lo[0] + 1, ny = D.hi[1] - D.lo[1] + 1;
+  L.cellmap.assign(D.npts(), -1);
+  for (int b = 0; b < (int)L.boxes.size(); ++b)
+    ParallelFor(L.boxes[b], [&](int i, int j, int k) {
+      L.cellmap[(i - D.lo[0]) + nx * ((j - D.lo[1]) + ny * (long)(k - D.lo[2]))] = b;
+    });
+}
+
+// Map index x in direction d of a ghost cell to its source index; sets odd
+// when the reflecting side flips normal (or, for no-slip, all) momenta.
+int map_index(int x, int lo, int hi, int bcl, int bch, int* refl) {
+  *refl = 0;
+  if (x >= lo && x <= hi) return x;
+  const int L = hi - lo + 1;
+  const int bc = (x < lo) ? bcl : bch;
+  if (bc == BC_INTERIOR) return lo + (((x - lo) % L) + L) % L;
+  if (bc == BC_SYMMETRY || bc == BC_SLIPWALL || bc == BC_NOSLIPWALL) {
+    *refl = (bc == BC_NOSLIPWALL) ? 2 : 1;
+    return (x < lo) ? 2 * lo - 1 - x : 2 * hi + 1 - x;
+  }
+  return (x < lo) ? lo : hi;  // outflow (FOExtrap); inflow/user as FOExtrap
+}
+
+void fill_ghosts(HostLevel& L, double* arena) {
+  build_cellmap(L);
+  const int nb = (int)L.boxes.size();
+#pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < nb; ++b) {
+    const Box& vb = L.boxes[b];
+    const A4 dst = L.fab(arena, b);
+    const Box& D = L.self ? vb : L.dom;
+    int bl[3], bh[3];
+    for (int d = 0; d < 3; ++d) {
+      bl[d] = L.self ? BC_INTERIOR : L.bclo[d];
+      bh[d] = L.self ? BC_INTERIOR : L.bchi[d];
+    }
+    const long nx = D.hi[0] - D.lo[0] + 1, ny = D.hi[1] - D.lo[1] + 1;
+    ParallelFor(grow(vb, 4), [&](int i, int j, int k) {
+      if (i >= vb.lo[0] && i <= vb.hi[0] && j >= vb.lo[1] && j <= vb.hi[1] &&
+          k >= vb.lo[2] && k <= vb.hi[2])
+        return;
+      int r[3];
+      const int s[3] = {map_index(i, D.lo[0], D.hi[0], bl[0], bh[0], &r[0]),
+                        map_index(j, D.lo[1], D.hi[1], bl[1], bh[1], &r[1]),
+                        map_index(k, D.lo[2], D.hi[2], bl[2], bh[2], &r[2])};
+      int sb = b;
+      if (!L.self)
+        sb = L.cellmap[(s[0] - D.lo[0]) + nx * ((s[1] - D.lo[1]) + ny * (long)(s[2] - D.lo[2]))];
+      const A4 src = L.fab(arena, sb);
+      for (int n = 0; n < L.nvar; ++n) {
+        double sg = 1.0;
+        for (int d = 0; d < 3; ++d) {
+          if (r[d] == 0) continue;
+          const bool mom = (n == UMX || n == UMY || n == UMZ);
+          if ((r[d] == 2 && mom) || (r[d] == 1 && n == UMX + d)) sg = -sg;
+        }
+        dst(i, j, k, n) = sg * src(s[0], s[1], s[2], n);
+      }
+    });
+  }
+}
+
+template <class EOS>
+double estdt_level(const HostLevel& L, const pc_params_t& prm,
+                   const double* arena, const double* dx) {
+  // Source/PeleC.cpp:829-961
+  if (prm.fixed_dt > 0.0) return prm.fixed_dt;
+  double estdt = prm.max_dt;
+  double estdt_hydro = prm.max_dt / prm.cfl;
+  double dtc = std::numeric_limits<double>::max();
+  for (int b = 0; b < (int)L.boxes.size(); ++b) {
+    const A4 u = L.fab(const_cast<double*>(arena), b);
+    dtc = mn(dtc, Hydro<EOS>::estdt_hydro(L.boxes[b], u, dx[0], dx[1], dx[2]));
+  }
+  estdt_hydro = mn(estdt_hydro, dtc);
+  estdt_hydro *= prm.cfl;
+  if (estdt_hydro < estdt) estdt = estdt_hydro;
+  return estdt;
+}
+
+template <class EOS>
+int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
+                 double* arena, int nsteps, double* dtp, int nthreads) {
+  using H = Hydro<EOS>;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  HostLevel L = make_level(H::NVAR, desc);
+  const double* dx = desc->dx;
+  const double vol_v = dx[0] * dx[1] * dx[2];
+  const double area_v[3] = {dx[1] * dx[2], dx[2] * dx[0], dx[0] * dx[1]};
+  int dlo[3], dhi[3], blo[3], bhi[3];
+  for (int d = 0; d < 3; ++d) {
+    dlo[d] = desc->domlo[d]; dhi[d] = desc->domhi[d];
+    blo[d] = desc->bclo[d]; bhi[d] = desc->bchi[d];
+  }
+  double dt = *dtp;
+  for (int s = 0; s < nsteps; ++s) {
+    fill_ghosts(L, arena);
+    int err = 0;
+    const int nb = (int)L.boxes.size();
+#pragma omp parallel for schedule(dynamic) reduction(min : err)
+    for (int b = 0; b < nb; ++b) {
+      const Box bx = L.boxes[b];
+      int sdlo[3], sdhi[3], sblo[3], sbhi[3];
+      for (int d = 0; d < 3; ++d) {
+        sdlo[d] = L.self ? bx.lo[d] : dlo[d];
+        sdhi[d] = L.self ? bx.hi[d] : dhi[d];
+        sblo[d] = L.self ? BC_INTERIOR : blo[d];
+        sbhi[d] = L.self ? BC_INTERIOR : bhi[d];
+      }
+      const A4 S = L.fab(arena, b);
+      Fab hsrc(bx, H::NVAR);
+      std::fill(hsrc.v.begin(), hsrc.v.end(), 0.0);  // hydro_source.setVal(0)
+      Fab f0(surroundingNodes(bx, 0), H::NVAR), f1(surroundingNodes(bx, 1), H::NVAR),
+          f2(surroundingNodes(bx, 2), H::NVAR);
+      const A4 flx[3] = {f0.a, f1.a, f2.a};
+      Fab a0(surroundingNodes(bx, 0), 1), a1(surroundingNodes(bx, 1), 1),
+          a2(surroundingNodes(bx, 2), 1), vol(bx, 1);
+      std::fill(a0.v.begin(), a0.v.end(), area_v[0]);
+      std::fill(a1.v.begin(), a1.v.end(), area_v[1]);
+      std::fill(a2.v.begin(), a2.v.end(), area_v[2]);
+      std::fill(vol.v.begin(), vol.v.end(), vol_v);
+      const A4 a[3] = {a0.a, a1.a, a2.a};
+      int rc = hydro_source_box_t<EOS>(bx, sdlo, sdhi, sblo, sbhi, S, nullptr,
+                                       hsrc.a, dx, dt, prm, flx, a, vol.a);
+      if (rc) { err = rc; continue; }
+      // construct_Snew (Source/Advance.cpp:397-416) + computeTemp twice
+      // (Source/Advance.cpp:352,388) + post_timestep computeTemp
+      // (Source/PeleC.cpp:1108-1110).  The second construct_Snew rebuilds
+      // the same S_new from S_old, so the sequence reduces to one update,
+      // reset + e->T from the old T, then reset + e->T from that T.
+      ParallelFor(bx, [&](int i, int j, int k) {
+        for (int n = 0; n < H::NVAR; ++n) S(i, j, k, n) += dt * hsrc.a(i, j, k, n);
+        H::rst_int_e_default(i, j, k, S);
+        H::cmpTemp(i, j, k, S);
+        H::rst_int_e_default(i, j, k, S);
+        H::cmpTemp(i, j, k, S);
+      });
+    }
+    if (err) return err;
+    // computeNewDt (Source/PeleC.cpp:964-1034), single level, no stop_time
+    double dtn = estdt_level<EOS>(L, prm, arena, dx);
+    if (prm.fixed_dt <= 0.0) dtn = mn(dtn, prm.change_max * dt);
+    dt = dtn;
+  }
+  *dtp = dt;
+  return 0;
+}
+
+// pc_initdata restatement (Exec/RegTests/Sod/prob.H:25-117 with
+// Exec/RegTests/Sod/prob.cpp:215-241 for mode 0; the PMF pattern
+// PYT2R + RTY2E for mode 1).  prim comps: mode 0: rho, p, u, v, w, Y;
+// mode 1: p, T, u, v, w, Y.
+template <class EOS>
+int prim_to_cons_t(const Box& bx, const A4& P, int mode, const A4& U) {
+  constexpr int NS = EOS::NS;
+  ParallelFor(bx, [&](int i, int j, int k) {
+    double Y[NS];
+    for (int n = 0; n < NS; ++n) Y[n] = P(i, j, k, 5 + n);
+    const double u = P(i, j, k, 2), v = P(i, j, k, 3), w = P(i, j, k, 4);
+    double rho, e, T;
+    if (mode == 0) {
+      rho = P(i, j, k, 0);
+      const double p = P(i, j, k, 1);
+      EOS::RYP2E(rho, Y, p, e);
+      T = 1.0;
+      EOS::EY2T(e, Y, T);
+    } else {
+      const double p = P(i, j, k, 0);
+      T = P(i, j, k, 1);
+      double wbar;
+      EOS::Y2WBAR(Y, wbar);
+      rho = p * wbar / (RU * T);
+      double ei[NS];
+      EOS::T2Ei(T, ei);
+      e = 0.0;
+      for (int n = 0; n < NS; ++n) e += Y[n] * ei[n];
+    }
+    const double rhoe = rho * e;
+    U(i, j, k, URHO) = rho;
+    U(i, j, k, UMX) = rho * u;
+    U(i, j, k, UMY) = rho * v;
+    U(i, j, k, UMZ) = rho * w;
+    U(i, j, k, UEINT) = rhoe;
+    U(i, j, k, UEDEN) = rhoe + 0.5 * rho * (u * u + v * v + w * w);
+    U(i, j, k, UTEMP) = T;
+    for (int n = 0; n < NS; ++n) U(i, j, k, UFS + n) = rho * Y[n];
+  });
+  return 0;
+}
+
+template <class EOS>
+int eos_point(int which, double a, double b, double c, const double* Y,
+              double* out) {
+  constexpr int NS = EOS::NS;
+  double y[NS];
+  for (int k = 0; k < NS; ++k) y[k] = Y ? Y[k] : (k == 0 ? 1.0 : 0.0);
+  switch (which) {
+    case 0: EOS::RYP2E(a, y, b, out[0]); return 0;
+    case 1: EOS::RPY2Cs(a, b, y, out[0]); return 0;
+    case 2: { double T = c; EOS::REY2T(a, b, y, T); out[0] = T; return 0; }
+    case 3: EOS::RTY2P(a, b, y, out[0]); return 0;
+    case 4: EOS::RTY2Cs(a, b, y, out[0]); return 0;
+    case 5: EOS::RTY2G(a, b, y, out[0]); return 0;
+    case 6: EOS::Y2WBAR(y, out[0]); return 0;
+    case 7: EOS::T2Ei(a, out); return 0;
+    case 8:
+      if constexpr (NS == 9) { LiDryer::CKCVBS(a, y, out[0]); return 0; }
+      else { out[0] = RU * GammaLaw::imw[0] / (GammaLaw::gamma - 1.0); return 0; }
+    case 9:
+      if constexpr (NS == 9) { LiDryer::hRT(a, out); return 0; }
+      else { out[0] = GammaLaw::gamma / (GammaLaw::gamma - 1.0); return 0; }
+    case 10: EOS::RTY2dpde_dpdre(a, b, y, out[0], out[1]); return 0;
+  }
+  return PC_ERR_ARG;
+}
+
+template <class EOS>
+int riemann_point(const double* L5, const double* spl, const double* R5,
+                  const double* spr, double cav, double* o) {
+  double fy[EOS::NS];
+  Hydro<EOS>::riemann(L5[0], L5[1], L5[2], L5[3], L5[4], spl, R5[0], R5[1],
+                      R5[2], R5[3], R5[4], spr, 1, cav, o[0], o[1], fy, o[2],
+                      o[3], o[4], o[5], o[6], o[7], o[8], o[9], o[10], o[11]);
+  return 0;
+}
+
+}  // namespace
+
+#define DISPATCH(eos, call)                                   \
+  do {                                                        \
+    if ((eos) == PC_EOS_GAMMALAW) { using E = GammaLaw; call; } \
+    else if ((eos) == PC_EOS_LIDRYER) { using E = LiDryer; call; } \
+    else return PC_ERR_ARG;                                   \
+  } while (0)
+
+extern "C" {
+
+void pc_ref_params_default(pc_params_t* p) {
+  p->ppm_type = 0;
+  p->plm_iorder = 4;
+  p->use_flattening = 1;
+  p->use_hybrid_weno = 0;
+  p->weno_scheme = 1;
+  p->difmag = 0.1;
+  p->cfl = 0.8;
+  p->init_shrink = 1.0;
+  p->change_max = 1.1;
+  p->max_dt = 1.e200;
+  p->fixed_dt = -1.0;
+  p->dt_cutoff = 0.0;
+  p->allow_small_energy = 1;
+  p->allow_negative_energy = 1;
+  p->dual_energy_update_E_from_e = 1;
+  p->dual_energy_eta2 = 1.0e-4;
+}
+
+int pc_ref_ctoprim(int eos, pc_box_t qbx, pc_array4_t u, pc_array4_t q,
+                   pc_array4_t qaux) {
+  const Box b = to_box(qbx);
+  if (!covers(u, b) || !covers(q, b) || !covers(qaux, b)) return PC_ERR_ARG;
+  const A4 U = to_a4(u), Q = to_a4(q), QA = to_a4(qaux);
+  DISPATCH(eos, ParallelFor(b, [&](int i, int j, int k) { Hydro<E>::ctoprim(i, j, k, U, Q, QA); }));
+  return 0;
+}
+
+int pc_ref_srctoprim(int eos, pc_box_t qbx, pc_array4_t q, pc_array4_t qaux,
+                     pc_array4_t src, pc_array4_t srcq) {
+  const Box b = to_box(qbx);
+  const A4 Q = to_a4(q), QA = to_a4(qaux), S = to_a4(src), SQ = to_a4(srcq);
+  DISPATCH(eos, ParallelFor(b, [&](int i, int j, int k) { Hydro<E>::srctoprim(i, j, k, Q, QA, S, SQ); }));
+  return 0;
+}
+
+int pc_ref_umdrv(int eos, double /*time*/, pc_box_t bx, const int* domlo,
+                 const int* domhi, const int* bclo, const int* bchi,
+                 pc_array4_t uin, pc_array4_t uout, pc_array4_t q,
+                 pc_array4_t qaux, pc_array4_t src_q, const double* dx,
+                 double dt, int ppm_type, int plm_iorder, int use_flattening,
+                 int use_hybrid_weno, int /*weno_scheme*/, double difmag,
+                 const pc_array4_t* flx, const pc_array4_t* a, pc_array4_t vol,
+                 double* /*cflLoc*/) {
+  if (use_hybrid_weno) return PC_ERR_UNSUPPORTED;
+  const Box b = to_box(bx);
+  if (!covers(q, grow(b, 4)) || !covers(uin, grow(b, 4))) return PC_ERR_GROW;
+  const A4 F[3] = {to_a4(flx[0]), to_a4(flx[1]), to_a4(flx[2])};
+  const A4 AA[3] = {to_a4(a[0]), to_a4(a[1]), to_a4(a[2])};
+  int rc = 0;
+  std::vector<double> zero;
+  A4 SQ = to_a4(src_q);
+  if (!SQ.p) {
+    const int qv = q.ncomp;
+    zero.assign((size_t)grow(b, 4).npts() * qv, 0.0);
+    SQ = make_a4(zero.data(), grow(b, 4), qv);
+  }
+  DISPATCH(eos, rc = Hydro<E>::umdrv(b, domlo, domhi, bclo, bchi, to_a4(uin), to_a4(uout),
+                                     to_a4(q), to_a4(qaux), SQ, dx, dt, ppm_type,
+                                     plm_iorder, use_flattening != 0, difmag, F, AA,
+                                     to_a4(vol)));
+  return rc;
+}
+
+int pc_ref_hydro_source_box(int eos, pc_box_t bx, const int* domlo,
+                            const int* domhi, const int* bclo, const int* bchi,
+                            pc_array4_t sarr, pc_array4_t src,
+                            pc_array4_t hyd_src, const double* dx, double dt,
+                            const pc_params_t* prm, const pc_array4_t* flx,
+                            const pc_array4_t* a, pc_array4_t vol) {
+  const Box b = to_box(bx);
+  if (!covers(sarr, grow(b, 4))) return PC_ERR_GROW;
+  const A4 F[3] = {to_a4(flx[0]), to_a4(flx[1]), to_a4(flx[2])};
+  const A4 AA[3] = {to_a4(a[0]), to_a4(a[1]), to_a4(a[2])};
+  const A4 S = to_a4(src);
+  int rc = 0;
+  DISPATCH(eos, rc = hydro_source_box_t<E>(b, domlo, domhi, bclo, bchi, to_a4(sarr),
+                                           S.p ? &S : nullptr, to_a4(hyd_src), dx, dt,
+                                           *prm, F, AA, to_a4(vol)));
+  return rc;
+}
+
+long long pc_ref_level_arena_size(int eos, const pc_level_desc_t* desc,
+                                  long long* offsets) {
+  HostLevel L = make_level(7 + eos, desc);
+  if (offsets)
+    for (size_t b = 0; b < L.off.size(); ++b) offsets[b] = L.off[b];
+  return L.total;
+}
+
+int pc_ref_level_fill_ghosts(int eos, const pc_level_desc_t* desc,
+                             double* arena) {
+  if (eos != 1 && eos != 9) return PC_ERR_ARG;
+  HostLevel L = make_level(7 + eos, desc);
+  fill_ghosts(L, arena);
+  return 0;
+}
+
+int pc_ref_level_estdt(int eos, const pc_level_desc_t* desc,
+                       const pc_params_t* prm, const double* arena,
+                       double* dt_out) {
+  HostLevel L = make_level(7 + eos, desc);
+  DISPATCH(eos, *dt_out = estdt_level<E>(L, *prm, arena, desc->dx));
+  return 0;
+}
+
+int pc_ref_level_step(int eos, const pc_level_desc_t* desc,
+                      const pc_params_t* prm, double* arena, int nsteps,
+                      double* dt, int nthreads) {
+  if (prm->use_hybrid_weno || !prm->allow_small_energy ||
+      !prm->allow_negative_energy)
+    return PC_ERR_UNSUPPORTED;
+  int rc = 0;
+  DISPATCH(eos, rc = level_step_t<E>(desc, *prm, arena, nsteps, dt, nthreads));
+  return rc;
+}
+
+int pc_ref_prim_to_cons(int eos, pc_box_t bx, pc_array4_t prim, int mode,
+                        pc_array4_t u) {
+  if (mode != 0 && mode != 1) return PC_ERR_ARG;
+  int rc = 0;
+  DISPATCH(eos, rc = prim_to_cons_t<E>(to_box(bx), to_a4(prim), mode, to_a4(u)));
+  return rc;
+}
+
+int pc_ref_eos(int eos, int which, double a, double b, double c,
+               const double* Y, double* out) {
+  int rc = 0;
+  DISPATCH(eos, rc = eos_point<E>(which, a, b, c, Y, out));
+  return rc;
+}
+
+int pc_ref_riemann(int eos, const double* left5, const double* spl,
+                   const double* right5, const double* spr, double cav,
+                   double* out12) {
+  int rc = 0;
+  DISPATCH(eos, rc = riemann_point<E>(left5, spl, right5, spr, cav, out12));
+  return rc;
+}
+
+int pc_ref_ppm_reconstruct(const double* s5, double flatn, double* smsp) {
+  Hydro<GammaLaw>::ppm_reconstruct(s5, flatn, smsp[0], smsp[1]);
+  return 0;
+}
+
+int pc_ref_ppm_int_profile(double sm, double sp, double sc, double u, double c,
+                           double dtdx, double* Ip3, double* Im3) {
+  Hydro<GammaLaw>::ppm_int_profile(sm, sp, sc, u, c, dtdx, Ip3, Im3);
+  return 0;
+}
+
+double pc_ref_plm_slope_line(const double* q5, double flat, int order) {
+  // a 5-cell line in x, centre at i = 2
+  double buf[5];
+  for (int i = 0; i < 5; ++i) buf[i] = q5[i];
+  Box b{{0, 0, 0}, {4, 0, 0}};
+  A4 a = make_a4(buf, b, 1);
+  return Hydro<GammaLaw>::plm_slope(2, 0, 0, 0, 0, a, flat, order);
+}
+
+}  // extern "C"

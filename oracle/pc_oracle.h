@@ -1,0 +1,79 @@
+/*
+ * TEST INFRASTRUCTURE ONLY.  The CPU oracle: a restatement of the reference
+ * Godunov path, used by tests/, the smoke check and the bench's cpu_baseline
+ * leg as the checker.  It is never linked into the product library.
+ *
+ * Same entry points as include/pc_hydro.h with a pc_ref_ prefix, operating
+ * on host memory, with the EOS chosen per call (1 = GammaLaw, 9 = LiDryer).
+ */
+#ifndef PC_ORACLE_H
+#define PC_ORACLE_H
+#include "../include/pc_hydro.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void pc_ref_params_default(pc_params_t* p);
+int pc_ref_ctoprim(int eos, pc_box_t qbx, pc_array4_t u, pc_array4_t q,
+                   pc_array4_t qaux);
+int pc_ref_srctoprim(int eos, pc_box_t qbx, pc_array4_t q, pc_array4_t qaux,
+                     pc_array4_t src, pc_array4_t srcq);
+int pc_ref_umdrv(int eos, double time, pc_box_t bx, const int* domlo,
+                 const int* domhi, const int* bclo, const int* bchi,
+                 pc_array4_t uin, pc_array4_t uout, pc_array4_t q,
+                 pc_array4_t qaux, pc_array4_t src_q, const double* dx,
+                 double dt, int ppm_type, int plm_iorder, int use_flattening,
+                 int use_hybrid_weno, int weno_scheme, double difmag,
+                 const pc_array4_t* flx, const pc_array4_t* a, pc_array4_t vol,
+                 double* cflLoc);
+int pc_ref_hydro_source_box(int eos, pc_box_t bx, const int* domlo,
+                            const int* domhi, const int* bclo, const int* bchi,
+                            pc_array4_t sarr, pc_array4_t src,
+                            pc_array4_t hyd_src, const double* dx, double dt,
+                            const pc_params_t* prm, const pc_array4_t* flx,
+                            const pc_array4_t* a, pc_array4_t vol);
+
+/* Level on the host: same arena layout as the device level (box b's FAB over
+ * grow(box,4), NVAR comps, at arena + offset[b]). */
+long long pc_ref_level_arena_size(int eos, const pc_level_desc_t* desc,
+                                  long long* offsets /* nboxes or NULL */);
+int pc_ref_level_fill_ghosts(int eos, const pc_level_desc_t* desc,
+                             double* arena);
+int pc_ref_level_estdt(int eos, const pc_level_desc_t* desc,
+                       const pc_params_t* prm, const double* arena,
+                       double* dt_out);
+/* nsteps Godunov steps; *dt is the dt of the first step on entry and the dt
+ * of the next step on return (computeNewDt).  nthreads > 0 sets the OpenMP
+ * thread count (boxes are the parallel unit). */
+int pc_ref_level_step(int eos, const pc_level_desc_t* desc,
+                      const pc_params_t* prm, double* arena, int nsteps,
+                      double* dt, int nthreads);
+
+/* Conservative state from primitive fields (pc_initdata pattern).
+ * mode 0: prim = (rho, p, u, v, w, Y[NS]); mode 1: prim = (p, T, u, v, w, Y). */
+int pc_ref_prim_to_cons(int eos, pc_box_t bx, pc_array4_t prim, int mode,
+                        pc_array4_t u);
+
+/* Point functions for known-answer tests. */
+/* which: 0 RYP2E(r,Y,p)->e, 1 RPY2Cs(r,p,Y)->c, 2 REY2T(r,e,Y;T0)->T,
+ *        3 RTY2P(r,T,Y)->p, 4 RTY2Cs(r,T,Y)->c, 5 RTY2G(r,T,Y)->G,
+ *        6 Y2WBAR(Y)->W, 7 T2Ei(T)->e_k (NS outputs), 8 cv mix (T,Y),
+ *        9 h_k/(R T) (NS outputs), 10 RTY2dpde_dpdre -> (dpde, dpdr_e) */
+int pc_ref_eos(int eos, int which, double a, double b, double c,
+               const double* Y, double* out);
+/* riemann() of Source/Riemann.H:9-200: out[0..] = ustar, uflx_rho,
+ * uflx_u, uflx_v, uflx_w, uflx_eden, uflx_eint, qint_iu, qint_iv1,
+ * qint_iv2, qint_gdpres, qint_gdgame */
+int pc_ref_riemann(int eos, const double* left5, const double* spl,
+                   const double* right5, const double* spr, double cav,
+                   double* out12);
+int pc_ref_ppm_reconstruct(const double* s5, double flatn, double* smsp);
+int pc_ref_ppm_int_profile(double sm, double sp, double sc, double u, double c,
+                           double dtdx, double* Ip3, double* Im3);
+double pc_ref_plm_slope_line(const double* q5, double flat, int order);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

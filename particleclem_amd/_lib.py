@@ -1,0 +1,104 @@
+"""ctypes binding of the C ABI in include/pc_hydro.h (libpc_hydro.so).
+
+The library is the product: if it is missing this module raises; there is
+no fallback path.
+"""
+import ctypes as C
+import os
+
+from .ctypes_types import Array4, Box, LevelDesc, Params
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libpc_hydro.so")
+
+# every symbol include/pc_hydro.h declares
+EXPORTS = [
+    "pc_params_default", "pc_ctx_create", "pc_ctx_destroy",
+    "pc_ctx_num_species", "pc_ctx_nvar", "pc_ctx_qvar", "pc_last_error",
+    "pc_ctoprim_v1", "pc_srctoprim_v1", "pc_umdrv_v1",
+    "pc_hydro_source_box_v1", "pc_level_create_v1", "pc_level_destroy_v1",
+    "pc_level_box_offset_v1", "pc_level_box_size_v1", "pc_level_state_ptr_v1",
+    "pc_level_state_size_v1", "pc_level_write_box_v1", "pc_level_read_box_v1",
+    "pc_level_set_dt_v1", "pc_level_get_dt_v1", "pc_level_init_dt_v1",
+    "pc_level_step_v1", "pc_level_fill_ghosts_v1", "pc_level_estdt_v1",
+    "pc_level_enable_timing_v1", "pc_level_stage_times_v1",
+    "pc_level_status_v1", "pc_level_prim_to_cons_v1",
+]
+
+PC_OK = 0
+ERRORS = {-1: "PC_ERR_ARG", -2: "PC_ERR_GROW", -3: "PC_ERR_PPM_TYPE",
+          -4: "PC_ERR_PLM_IORDER", -5: "PC_ERR_HIP", -6: "PC_ERR_NAN",
+          -7: "PC_ERR_UNSUPPORTED", -8: "PC_ERR_NOMEM", -9: "PC_ERR_DT"}
+
+_lib = None
+
+
+class HydroError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, "
+            "gfx950).  There is no CPU fallback.")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    i3 = P(C.c_int)
+    d3 = P(C.c_double)
+    L.pc_params_default.argtypes = [P(Params)]
+    L.pc_params_default.restype = None
+    L.pc_ctx_create.argtypes = [C.c_int, P(vp)]
+    L.pc_ctx_destroy.argtypes = [vp]
+    L.pc_ctx_num_species.argtypes = [vp]
+    L.pc_ctx_nvar.argtypes = [vp]
+    L.pc_ctx_qvar.argtypes = [vp]
+    L.pc_last_error.argtypes = [vp]
+    L.pc_last_error.restype = C.c_char_p
+    L.pc_ctoprim_v1.argtypes = [vp, Box, Array4, Array4, Array4, vp]
+    L.pc_srctoprim_v1.argtypes = [vp, Box, Array4, Array4, Array4, Array4, vp]
+    L.pc_umdrv_v1.argtypes = [vp, C.c_double, Box, i3, i3, i3, i3, Array4,
+                              Array4, Array4, Array4, Array4, d3, C.c_double,
+                              C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_double, P(Array4), P(Array4), Array4, d3, vp]
+    L.pc_hydro_source_box_v1.argtypes = [vp, Box, i3, i3, i3, i3, Array4,
+                                         Array4, Array4, d3, C.c_double,
+                                         P(Params), P(Array4), P(Array4),
+                                         Array4, vp]
+    L.pc_level_create_v1.argtypes = [vp, P(LevelDesc), P(Params), P(vp)]
+    L.pc_level_destroy_v1.argtypes = [vp]
+    L.pc_level_box_offset_v1.argtypes = [vp, C.c_int]
+    L.pc_level_box_offset_v1.restype = C.c_longlong
+    L.pc_level_box_size_v1.argtypes = [vp, C.c_int]
+    L.pc_level_box_size_v1.restype = C.c_longlong
+    L.pc_level_state_ptr_v1.argtypes = [vp]
+    L.pc_level_state_ptr_v1.restype = vp
+    L.pc_level_state_size_v1.argtypes = [vp]
+    L.pc_level_state_size_v1.restype = C.c_longlong
+    L.pc_level_write_box_v1.argtypes = [vp, C.c_int, vp]
+    L.pc_level_read_box_v1.argtypes = [vp, C.c_int, vp]
+    L.pc_level_set_dt_v1.argtypes = [vp, C.c_double]
+    L.pc_level_get_dt_v1.argtypes = [vp, d3]
+    L.pc_level_init_dt_v1.argtypes = [vp, vp]
+    L.pc_level_step_v1.argtypes = [vp, C.c_int, vp]
+    L.pc_level_fill_ghosts_v1.argtypes = [vp, vp]
+    L.pc_level_estdt_v1.argtypes = [vp, d3, vp]
+    L.pc_level_enable_timing_v1.argtypes = [vp, C.c_int]
+    L.pc_level_stage_times_v1.argtypes = [vp, d3]
+    L.pc_level_status_v1.argtypes = [vp, P(C.c_int), P(C.c_int)]
+    L.pc_level_prim_to_cons_v1.argtypes = [vp, C.c_int, Array4, C.c_int, vp]
+    _lib = L
+    return L
+
+
+def check(rc, ctx=None):
+    if rc != PC_OK:
+        msg = lib().pc_last_error(ctx).decode() if ctx else ""
+        raise HydroError(rc, msg)
+    return rc

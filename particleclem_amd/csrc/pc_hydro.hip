@@ -1,0 +1,982 @@
+// Host side of the gfx950 Godunov hydro engine: contexts, stage workspaces,
+// launch plans over box tables, the level stepper and the C ABI declared in
+// include/pc_hydro.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pc_hydro.h"
+#include "pc_level_kernels.h"
+
+using namespace pcd;
+
+// ----------------------------------------------------------------- errors --
+struct pc_ctx {
+  int eos = 0;
+  int ns = 0;
+  int device = 0;
+  std::string err;
+  // drop-in scratch (stream-ordered reuse)
+  double* ws = nullptr;
+  size_t ws_bytes = 0;
+  hipStream_t ws_stream = nullptr;
+};
+
+static int fail(pc_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                   \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(ctx, PC_ERR_HIP, "%s failed: %s", #call,                  \
+                  hipGetErrorString(e_));                                   \
+  } while (0)
+
+// ------------------------------------------------------------------ boxes --
+struct IBox {
+  int lo[3], hi[3];
+  long long npts() const {
+    return (long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) *
+           (hi[2] - lo[2] + 1);
+  }
+};
+static IBox mkbox(const int* lo, const int* hi) {
+  IBox b;
+  for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
+  return b;
+}
+static IBox growb(IBox b, int n) {
+  for (int d = 0; d < 3; ++d) { b.lo[d] -= n; b.hi[d] += n; }
+  return b;
+}
+
+// view of a contiguous FAB over box b with nc comps at p
+static V fab_view(double* p, const IBox& b) {
+  V v;
+  v.p = p;
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1,
+            nz = b.hi[2] - b.lo[2] + 1;
+  v.js = nx;
+  v.ks = nx * ny;
+  v.ns = nx * ny * nz;
+  v.b0 = b.lo[0];
+  v.b1 = b.lo[1];
+  v.b2 = b.lo[2];
+  return v;
+}
+static V null_view() {
+  V v;
+  memset(&v, 0, sizeof(v));
+  return v;
+}
+
+// Array4 -> V; fails when an offset would not fit 32 bits or the view does
+// not cover `need` with ncomp >= nc.
+static int a4_view(pc_ctx* c, const pc_array4_t& a, const IBox& need, int nc,
+                   const char* name, V& v) {
+  if (!a.p) return fail(c, PC_ERR_ARG, "%s: null pointer", name);
+  for (int d = 0; d < 3; ++d)
+    if (need.lo[d] < a.begin[d] || need.hi[d] >= a.end[d])
+      return fail(c, (nc > 0 && strcmp(name, "flx") && strcmp(name, "a") &&
+                      strcmp(name, "vol") && strcmp(name, "uout"))
+                         ? PC_ERR_GROW
+                         : PC_ERR_ARG,
+                  "%s does not cover the required box in dir %d", name, d);
+  if (a.ncomp < nc)
+    return fail(c, PC_ERR_ARG, "%s: ncomp %d < %d", name, a.ncomp, nc);
+  const long long ext = (long long)(a.end[0] - a.begin[0]) +
+                        a.jstride * (a.end[1] - a.begin[1]) +
+                        a.kstride * (a.end[2] - a.begin[2]) +
+                        a.nstride * (long long)(a.ncomp > 0 ? a.ncomp : 1);
+  if (ext > 2147483647LL || a.jstride > 2147483647LL ||
+      a.kstride > 2147483647LL || a.nstride > 2147483647LL)
+    return fail(c, PC_ERR_UNSUPPORTED, "%s: FAB too large for 32-bit offsets",
+                name);
+  v.p = a.p;
+  v.js = (int)a.jstride;
+  v.ks = (int)a.kstride;
+  v.ns = (int)a.nstride;
+  v.b0 = a.begin[0];
+  v.b1 = a.begin[1];
+  v.b2 = a.begin[2];
+  return PC_OK;
+}
+
+// ------------------------------------------------------ stage workspace --
+// Sizes (doubles) of one box's intermediates; `full` adds q/qaux/srcq (the
+// box driver computes them) and the final fluxes (level path).
+struct WsLayout {
+  IBox q, tr[3], f1[3], ft[6], fl[3], node;
+  size_t off_q = 0, off_qa = 0, off_srcq = 0, off_tr[3], off_f1[3], off_ft[6],
+         off_fl[3], off_qec[3], off_divu = 0, total = 0;
+};
+
+static WsLayout ws_layout(const IBox& bx, int ns, bool with_q, bool with_src,
+                          bool with_flx) {
+  WsLayout L;
+  const int QVAR = 8 + ns, NVAR = 7 + ns, QT = 6 + ns, FT = 7 + ns;
+  size_t o = 0;
+  auto take = [&](size_t n) {
+    size_t r = o;
+    o += (n + 31) & ~size_t(31);  // 256-byte aligned sub-arrays
+    return r;
+  };
+  L.q = growb(bx, 4);
+  if (with_q) {
+    L.off_q = take(L.q.npts() * QVAR);
+    L.off_qa = take(L.q.npts() * NQAUX);
+    if (with_src) L.off_srcq = take(L.q.npts() * QVAR);
+  }
+  for (int d = 0; d < 3; ++d) {
+    IBox b = growb(bx, 1);
+    b.hi[d] += 1;  // faces [L-1, H+2]
+    L.tr[d] = b;
+    L.off_tr[d] = take(b.npts() * 2 * QT);
+  }
+  for (int d = 0; d < 3; ++d) {
+    IBox b = growb(bx, 1);
+    b.lo[d] = bx.lo[d];
+    b.hi[d] = bx.hi[d] + 1;
+    L.f1[d] = b;
+    L.off_f1[d] = take(b.npts() * FT);
+  }
+  for (int s = 0; s < 6; ++s) {
+    const int D = s / 2;
+    const int T = (s % 2 == 0) ? (D == 0 ? 1 : 0) : (D == 2 ? 1 : 2);
+    const int Tp = 3 - D - T;
+    IBox b = bx;
+    b.hi[D] += 1;
+    b.lo[Tp] -= 1;
+    b.hi[Tp] += 1;
+    L.ft[s] = b;
+    L.off_ft[s] = take(b.npts() * FT);
+  }
+  for (int d = 0; d < 3; ++d) {
+    IBox b = bx;
+    b.hi[d] += 1;
+    L.fl[d] = b;
+    if (with_flx) L.off_fl[d] = take(b.npts() * NVAR);
+    L.off_qec[d] = take(b.npts() * 2);
+  }
+  L.node = bx;
+  for (int d = 0; d < 3; ++d) L.node.hi[d] += 1;
+  L.off_divu = take(L.node.npts());
+  L.total = o;
+  return L;
+}
+
+static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
+                        bool with_src, bool with_flx) {
+  if (with_q) {
+    J.q = fab_view(base + L.off_q, L.q);
+    J.qa = fab_view(base + L.off_qa, L.q);
+    J.srcq = with_src ? fab_view(base + L.off_srcq, L.q) : null_view();
+  }
+  for (int d = 0; d < 3; ++d) {
+    J.tr[d] = fab_view(base + L.off_tr[d], L.tr[d]);
+    J.f1[d] = fab_view(base + L.off_f1[d], L.f1[d]);
+    if (with_flx) J.flx[d] = fab_view(base + L.off_fl[d], L.fl[d]);
+    J.qec[d] = fab_view(base + L.off_qec[d], L.fl[d]);
+  }
+  for (int s = 0; s < 6; ++s) J.ft[s] = fab_view(base + L.off_ft[s], L.ft[s]);
+  J.divu = fab_view(base + L.off_divu, L.node);
+}
+
+static int ctx_scratch(pc_ctx* c, size_t bytes, hipStream_t st) {
+  if (bytes <= c->ws_bytes) {
+    c->ws_stream = st;
+    return PC_OK;
+  }
+  if (c->ws) {
+    // the previous user may still run on its stream
+    if (c->ws_stream) HIPCHK(c, hipStreamSynchronize(c->ws_stream));
+    HIPCHK(c, hipDeviceSynchronize());
+    HIPCHK(c, hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  size_t nb = std::max(bytes, (size_t)64 << 20);
+  if (hipMalloc(&c->ws, nb) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, PC_ERR_NOMEM, "scratch allocation of %zu bytes failed", nb);
+  }
+  c->ws_bytes = nb;
+  c->ws_stream = st;
+  return PC_OK;
+}
+
+// ----------------------------------------------------------- launching --
+static long long region_npts(int st, const Job& J) {
+  int lo[3], n[3];
+  stage_region(st, J.lo, J.hi, lo, n);
+  return (long long)n[0] * n[1] * n[2];
+}
+
+// launch plan of one stage over a job table
+struct StagePlan {
+  int nblocks = 0;
+  int* blkmap = nullptr;    // device
+  int* blkstart = nullptr;  // device
+};
+
+static int grid1(int st, const Job& J) {
+  return (int)((region_npts(st, J) + BLOCK - 1) / BLOCK);
+}
+
+#define LAUNCH(kern, grid, stream, ...)                                      \
+  do {                                                                       \
+    if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0,     \
+                                       stream, __VA_ARGS__);                 \
+  } while (0)
+
+// The CTU stages from primitives to final fluxes for one EOS.  With a job
+// table (sc.jobs) the grids come from `plans`; otherwise from job j0.
+template <class EOS>
+static void run_ctu(const Job& j0, const Sched* scs, const StagePlan* plans,
+                    const KArgs& ka, int ppm_type, hipStream_t st) {
+  auto G = [&](int stage) {
+    return plans ? plans[stage].nblocks : grid1(stage, j0);
+  };
+  auto S = [&](int stage) { return plans ? scs[stage] : Sched{nullptr, nullptr, nullptr}; };
+  LAUNCH(k_divu, G(ST_DIVU), st, j0, S(ST_DIVU), ka);
+  if (ppm_type == 1) {
+    LAUNCH((k_trace_ppm<EOS, 0>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+    LAUNCH((k_trace_ppm<EOS, 1>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+    LAUNCH((k_trace_ppm<EOS, 2>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+  } else {
+    LAUNCH((k_trace_plm<EOS, 0>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+    LAUNCH((k_trace_plm<EOS, 1>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+    LAUNCH((k_trace_plm<EOS, 2>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+  }
+  LAUNCH((k_flux1<EOS, 0>), G(ST_FLUX1_X), st, j0, S(ST_FLUX1_X), ka);
+  LAUNCH((k_flux1<EOS, 1>), G(ST_FLUX1_Y), st, j0, S(ST_FLUX1_Y), ka);
+  LAUNCH((k_flux1<EOS, 2>), G(ST_FLUX1_Z), st, j0, S(ST_FLUX1_Z), ka);
+  LAUNCH((k_trans<EOS, 0, 1>), G(ST_TRANS_XY), st, j0, S(ST_TRANS_XY), ka);
+  LAUNCH((k_trans<EOS, 0, 2>), G(ST_TRANS_XZ), st, j0, S(ST_TRANS_XZ), ka);
+  LAUNCH((k_trans<EOS, 1, 0>), G(ST_TRANS_YX), st, j0, S(ST_TRANS_YX), ka);
+  LAUNCH((k_trans<EOS, 1, 2>), G(ST_TRANS_YZ), st, j0, S(ST_TRANS_YZ), ka);
+  LAUNCH((k_trans<EOS, 2, 0>), G(ST_TRANS_ZX), st, j0, S(ST_TRANS_ZX), ka);
+  LAUNCH((k_trans<EOS, 2, 1>), G(ST_TRANS_ZY), st, j0, S(ST_TRANS_ZY), ka);
+  LAUNCH((k_final<EOS, 0>), G(ST_FINAL_X), st, j0, S(ST_FINAL_X), ka);
+  LAUNCH((k_final<EOS, 1>), G(ST_FINAL_Y), st, j0, S(ST_FINAL_Y), ka);
+  LAUNCH((k_final<EOS, 2>), G(ST_FINAL_Z), st, j0, S(ST_FINAL_Z), ka);
+}
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+void pc_params_default(pc_params_t* p) {
+  p->ppm_type = 0;
+  p->plm_iorder = 4;
+  p->use_flattening = 1;
+  p->use_hybrid_weno = 0;
+  p->weno_scheme = 1;
+  p->difmag = 0.1;
+  p->cfl = 0.8;
+  p->init_shrink = 1.0;
+  p->change_max = 1.1;
+  p->max_dt = 1.e200;
+  p->fixed_dt = -1.0;
+  p->dt_cutoff = 0.0;
+  p->allow_small_energy = 1;
+  p->allow_negative_energy = 1;
+  p->dual_energy_update_E_from_e = 1;
+  p->dual_energy_eta2 = 1.0e-4;
+}
+
+int pc_ctx_create(int eos_kind, pc_ctx_t** out) {
+  if (!out) return PC_ERR_ARG;
+  *out = nullptr;
+  if (eos_kind != PC_EOS_GAMMALAW && eos_kind != PC_EOS_LIDRYER)
+    return PC_ERR_ARG;
+  pc_ctx* c = new pc_ctx();
+  c->eos = eos_kind;
+  c->ns = eos_kind;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return PC_ERR_HIP;
+  }
+  *out = c;
+  return PC_OK;
+}
+
+int pc_ctx_destroy(pc_ctx_t* c) {
+  if (!c) return PC_ERR_ARG;
+  if (c->ws) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(c->ws);
+  }
+  delete c;
+  return PC_OK;
+}
+
+int pc_ctx_num_species(const pc_ctx_t* c) { return c ? c->ns : PC_ERR_ARG; }
+int pc_ctx_nvar(const pc_ctx_t* c) { return c ? 7 + c->ns : PC_ERR_ARG; }
+int pc_ctx_qvar(const pc_ctx_t* c) { return c ? 8 + c->ns : PC_ERR_ARG; }
+const char* pc_last_error(const pc_ctx_t* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+static bool box_ok(const pc_box_t& b) {
+  return b.hi[0] >= b.lo[0] && b.hi[1] >= b.lo[1] && b.hi[2] >= b.lo[2];
+}
+
+int pc_ctoprim_v1(pc_ctx_t* c, pc_box_t qbx, pc_array4_t u, pc_array4_t q,
+                  pc_array4_t qaux, void* stream) {
+  if (!c || !box_ok(qbx)) return fail(c, PC_ERR_ARG, "bad context or box");
+  const IBox b = mkbox(qbx.lo, qbx.hi);
+  Job J;
+  memset(&J, 0, sizeof(J));
+  // the ctoprim region is grow(bx,4): express qbx as bx = shrink(qbx,4)
+  for (int d = 0; d < 3; ++d) { J.lo[d] = b.lo[d] + 4; J.hi[d] = b.hi[d] - 4; }
+  int rc;
+  if ((rc = a4_view(c, u, b, 7 + c->ns, "u", J.U))) return rc;
+  if ((rc = a4_view(c, q, b, 8 + c->ns, "q", J.q))) return rc;
+  if ((rc = a4_view(c, qaux, b, NQAUX, "qaux", J.qa))) return rc;
+  KArgs ka;
+  memset(&ka, 0, sizeof(ka));
+  hipStream_t st = (hipStream_t)stream;
+  const int g = (int)((b.npts() + BLOCK - 1) / BLOCK);
+  if (c->eos == PC_EOS_GAMMALAW)
+    LAUNCH(k_ctoprim<EosGamma>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka);
+  else
+    LAUNCH(k_ctoprim<EosLiDryer>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka);
+  HIPCHK(c, hipGetLastError());
+  return PC_OK;
+}
+
+int pc_srctoprim_v1(pc_ctx_t* c, pc_box_t qbx, pc_array4_t q, pc_array4_t qaux,
+                    pc_array4_t src, pc_array4_t srcq, void* stream) {
+  if (!c || !box_ok(qbx)) return fail(c, PC_ERR_ARG, "bad context or box");
+  const IBox b = mkbox(qbx.lo, qbx.hi);
+  Job J;
+  memset(&J, 0, sizeof(J));
+  for (int d = 0; d < 3; ++d) { J.lo[d] = b.lo[d] + 4; J.hi[d] = b.hi[d] - 4; }
+  int rc;
+  V vs;
+  if ((rc = a4_view(c, q, b, 8 + c->ns, "q", J.q))) return rc;
+  if ((rc = a4_view(c, qaux, b, NQAUX, "qaux", J.qa))) return rc;
+  if ((rc = a4_view(c, src, b, 7 + c->ns, "src", vs))) return rc;
+  if ((rc = a4_view(c, srcq, b, 8 + c->ns, "srcq", J.srcq))) return rc;
+  KArgs ka;
+  memset(&ka, 0, sizeof(ka));
+  hipStream_t st = (hipStream_t)stream;
+  const int g = (int)((b.npts() + BLOCK - 1) / BLOCK);
+  if (c->eos == PC_EOS_GAMMALAW)
+    LAUNCH(k_srctoprim<1>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka, vs);
+  else
+    LAUNCH(k_srctoprim<9>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka, vs);
+  HIPCHK(c, hipGetLastError());
+  return PC_OK;
+}
+
+static int check_opts(pc_ctx* c, int ppm_type, int plm_iorder,
+                      int use_hybrid_weno) {
+  if (ppm_type != 0 && ppm_type != 1)
+    return fail(c, PC_ERR_PPM_TYPE, "PeleC::ppm_type must be 0 (PLM) or 1 (PPM)");
+  if (plm_iorder != 1 && plm_iorder != 2 && plm_iorder != 4)
+    return fail(c, PC_ERR_PLM_IORDER, "plm_iorder must be 1, 2 or 4");
+  if (use_hybrid_weno)
+    return fail(c, PC_ERR_UNSUPPORTED, "use_hybrid_weno is not implemented");
+  return PC_OK;
+}
+
+// shared tail of pc_umdrv_v1 / pc_hydro_source_box_v1: J.q/qa/srcq/U are
+// set; fills the stage workspace views and launches the CTU + consup.
+static int umdrv_core(pc_ctx* c, Job& J, const IBox& bx, const double* dx,
+                      double dt, int ppm_type, int plm_iorder,
+                      int use_flattening, double difmag, const pc_array4_t* flx,
+                      const pc_array4_t* a, const pc_array4_t& vol,
+                      const pc_array4_t& uout, hipStream_t st,
+                      size_t ws_extra_off) {
+  int rc;
+  const WsLayout L = ws_layout(bx, c->ns, false, false, false);
+  ws_fill_job(J, c->ws + ws_extra_off, L, false, false, false);
+  for (int d = 0; d < 3; ++d) {
+    IBox fb = bx;
+    fb.hi[d] += 1;
+    if ((rc = a4_view(c, flx[d], fb, 7 + c->ns, "flx", J.flx[d]))) return rc;
+    if ((rc = a4_view(c, a[d], fb, 1, "a", J.area[d]))) return rc;
+  }
+  if ((rc = a4_view(c, vol, bx, 1, "vol", J.vol))) return rc;
+  if ((rc = a4_view(c, uout, bx, 7 + c->ns, "uout", J.upd))) return rc;
+  KArgs ka;
+  memset(&ka, 0, sizeof(ka));
+  for (int d = 0; d < 3; ++d) ka.dx[d] = dx[d];
+  ka.difmag = difmag;
+  ka.dt_val = dt;
+  ka.dtp = nullptr;
+  ka.plm_iorder = plm_iorder;
+  ka.use_flattening = use_flattening;
+  const int gu = grid1(ST_UPDATE, J);
+  if (c->eos == PC_EOS_GAMMALAW) {
+    run_ctu<EosGamma>(J, nullptr, nullptr, ka, ppm_type, st);
+    LAUNCH(k_consup<1>, gu, st, J, Sched{nullptr, nullptr, nullptr}, ka);
+  } else {
+    run_ctu<EosLiDryer>(J, nullptr, nullptr, ka, ppm_type, st);
+    LAUNCH(k_consup<9>, gu, st, J, Sched{nullptr, nullptr, nullptr}, ka);
+  }
+  HIPCHK(c, hipGetLastError());
+  return PC_OK;
+}
+
+static void set_domain(Job& J, const int* domlo, const int* domhi,
+                       const int* bclo, const int* bchi) {
+  for (int d = 0; d < 3; ++d) {
+    J.dlo[d] = domlo[d];
+    J.dhi[d] = domhi[d];
+    J.bclo[d] = bclo[d];
+    J.bchi[d] = bchi[d];
+  }
+}
+
+int pc_umdrv_v1(pc_ctx_t* c, double /*time*/, pc_box_t bx, const int* domlo,
+                const int* domhi, const int* bclo, const int* bchi,
+                pc_array4_t uin, pc_array4_t uout, pc_array4_t q,
+                pc_array4_t qaux, pc_array4_t src_q, const double* dx,
+                double dt, int ppm_type, int plm_iorder, int use_flattening,
+                int use_hybrid_weno, int /*weno_scheme*/, double difmag,
+                const pc_array4_t* flx, const pc_array4_t* a, pc_array4_t vol,
+                double* /*cflLoc*/, void* stream) {
+  if (!c || !box_ok(bx) || !domlo || !domhi || !bclo || !bchi || !dx || !flx ||
+      !a)
+    return fail(c, PC_ERR_ARG, "null argument or empty box");
+  int rc;
+  if ((rc = check_opts(c, ppm_type, plm_iorder, use_hybrid_weno))) return rc;
+  const IBox b = mkbox(bx.lo, bx.hi);
+  const IBox g4 = growb(b, 4);
+  Job J;
+  memset(&J, 0, sizeof(J));
+  for (int d = 0; d < 3; ++d) { J.lo[d] = b.lo[d]; J.hi[d] = b.hi[d]; }
+  set_domain(J, domlo, domhi, bclo, bchi);
+  if ((rc = a4_view(c, uin, g4, 7 + c->ns, "uin", J.U))) return rc;
+  if ((rc = a4_view(c, q, g4, 8 + c->ns, "q", J.q))) return rc;
+  if ((rc = a4_view(c, qaux, g4, NQAUX, "qaux", J.qa))) return rc;
+  for (int d = 0; d < 3; ++d) J.qe[d] = q.end[d];
+  if (src_q.p) {
+    if ((rc = a4_view(c, src_q, g4, 8 + c->ns, "src_q", J.srcq))) return rc;
+  } else {
+    J.srcq = null_view();
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const WsLayout L = ws_layout(b, c->ns, false, false, false);
+  if ((rc = ctx_scratch(c, L.total * sizeof(double), st))) return rc;
+  return umdrv_core(c, J, b, dx, dt, ppm_type, plm_iorder, use_flattening,
+                    difmag, flx, a, vol, uout, st, 0);
+}
+
+int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
+                           const int* domhi, const int* bclo, const int* bchi,
+                           pc_array4_t sarr, pc_array4_t src,
+                           pc_array4_t hyd_src, const double* dx, double dt,
+                           const pc_params_t* prm, const pc_array4_t* flx,
+                           const pc_array4_t* a, pc_array4_t vol,
+                           void* stream) {
+  if (!c || !prm || !box_ok(bx) || !domlo || !domhi || !bclo || !bchi || !dx ||
+      !flx || !a)
+    return fail(c, PC_ERR_ARG, "null argument or empty box");
+  int rc;
+  if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder, prm->use_hybrid_weno)))
+    return rc;
+  if (!prm->allow_small_energy || !prm->allow_negative_energy)
+    return fail(c, PC_ERR_UNSUPPORTED,
+                "allow_small_energy = allow_negative_energy = 1 required");
+  const IBox b = mkbox(bx.lo, bx.hi);
+  const IBox g4 = growb(b, 4);
+  Job J;
+  memset(&J, 0, sizeof(J));
+  for (int d = 0; d < 3; ++d) { J.lo[d] = b.lo[d]; J.hi[d] = b.hi[d]; }
+  set_domain(J, domlo, domhi, bclo, bchi);
+  if ((rc = a4_view(c, sarr, g4, 7 + c->ns, "sarr", J.U))) return rc;
+  V vsrc = null_view();
+  if (src.p && (rc = a4_view(c, src, g4, 7 + c->ns, "src", vsrc))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  // scratch: [q, qaux, srcq] then the CTU intermediates
+  const WsLayout Lq = ws_layout(b, c->ns, true, src.p != nullptr, false);
+  const WsLayout L = ws_layout(b, c->ns, false, false, false);
+  const size_t qpart = Lq.off_tr[0];
+  if ((rc = ctx_scratch(c, (qpart + L.total) * sizeof(double), st))) return rc;
+  J.q = fab_view(c->ws + Lq.off_q, Lq.q);
+  J.qa = fab_view(c->ws + Lq.off_qa, Lq.q);
+  J.srcq = src.p ? fab_view(c->ws + Lq.off_srcq, Lq.q) : null_view();
+  for (int d = 0; d < 3; ++d) J.qe[d] = Lq.q.hi[d] + 1;
+  KArgs ka;
+  memset(&ka, 0, sizeof(ka));
+  const int gq = grid1(ST_CTOPRIM, J);
+  const Sched s0{nullptr, nullptr, nullptr};
+  if (c->eos == PC_EOS_GAMMALAW) {
+    LAUNCH(k_ctoprim<EosGamma>, gq, st, J, s0, ka);
+    if (src.p) LAUNCH(k_srctoprim<1>, gq, st, J, s0, ka, vsrc);
+  } else {
+    LAUNCH(k_ctoprim<EosLiDryer>, gq, st, J, s0, ka);
+    if (src.p) LAUNCH(k_srctoprim<9>, gq, st, J, s0, ka, vsrc);
+  }
+  HIPCHK(c, hipGetLastError());
+  return umdrv_core(c, J, b, dx, dt, prm->ppm_type, prm->plm_iorder,
+                    prm->use_flattening, prm->difmag, flx, a, vol, hyd_src, st,
+                    qpart);
+}
+
+}  // extern "C"
+
+// ==================================================================== level
+struct pc_level {
+  pc_ctx* ctx = nullptr;
+  pc_params_t prm;
+  std::vector<IBox> boxes;
+  std::vector<long long> off;  // state arena offsets (doubles)
+  long long state_total = 0;
+  double* state = nullptr;     // device arena
+  double* ws = nullptr;        // device stage workspace
+  size_t ws_total = 0;
+  Job* jobs = nullptr;         // device job table
+  int* cellmap = nullptr;      // device domain cell -> box
+  StagePlan plans[ST_COUNT];
+  Sched scs[ST_COUNT];
+  DtState* ds = nullptr;       // device
+  LevelArgs la;
+  KArgs ka;
+  int timing = 0;
+  hipEvent_t ev[8];
+  bool ev_made = false;
+  double times[7];
+  int nsteps_timed = 0;
+};
+
+static void level_free(pc_level* L) {
+  if (L->state) (void)hipFree(L->state);
+  if (L->ws) (void)hipFree(L->ws);
+  if (L->jobs) (void)hipFree(L->jobs);
+  if (L->cellmap) (void)hipFree(L->cellmap);
+  if (L->ds) (void)hipFree(L->ds);
+  for (int s = 0; s < ST_COUNT; ++s) {
+    if (L->plans[s].blkmap) (void)hipFree(L->plans[s].blkmap);
+    if (L->plans[s].blkstart) (void)hipFree(L->plans[s].blkstart);
+  }
+  if (L->ev_made)
+    for (int e = 0; e < 8; ++e) (void)hipEventDestroy(L->ev[e]);
+  delete L;
+}
+
+extern "C" {
+
+int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
+                       const pc_params_t* prm, pc_level_t** out) {
+  if (!c || !d || !prm || !out || d->nboxes <= 0 || !d->boxes)
+    return fail(c, PC_ERR_ARG, "bad level description");
+  int rc;
+  if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder, prm->use_hybrid_weno)))
+    return rc;
+  if (!prm->allow_small_energy || !prm->allow_negative_energy)
+    return fail(c, PC_ERR_UNSUPPORTED,
+                "allow_small_energy = allow_negative_energy = 1 required");
+  *out = nullptr;
+  pc_level* L = new pc_level();
+  L->ctx = c;
+  L->prm = *prm;
+  const int ns = c->ns, NVAR = 7 + ns;
+  long long o = 0;
+  size_t wso = 0;
+  std::vector<size_t> wsoff;
+  for (int b = 0; b < d->nboxes; ++b) {
+    const IBox bx = mkbox(d->boxes[b].lo, d->boxes[b].hi);
+    if (bx.hi[0] < bx.lo[0] || bx.hi[1] < bx.lo[1] || bx.hi[2] < bx.lo[2]) {
+      level_free(L);
+      return fail(c, PC_ERR_ARG, "empty box %d", b);
+    }
+    if (!d->periodic_self)
+      for (int dd = 0; dd < 3; ++dd)
+        if (bx.lo[dd] < d->domlo[dd] || bx.hi[dd] > d->domhi[dd]) {
+          level_free(L);
+          return fail(c, PC_ERR_ARG, "box %d outside the domain", b);
+        }
+    if (growb(bx, 4).npts() * NVAR > 2147483647LL) {
+      level_free(L);
+      return fail(c, PC_ERR_UNSUPPORTED, "box %d too large", b);
+    }
+    L->boxes.push_back(bx);
+    L->off.push_back(o);
+    o += growb(bx, 4).npts() * NVAR;
+    const WsLayout W = ws_layout(bx, ns, true, false, true);
+    wsoff.push_back(wso);
+    wso += W.total;
+  }
+  L->state_total = o;
+  L->ws_total = wso;
+  auto dalloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    return true;
+  };
+  if (!dalloc((void**)&L->state, o * sizeof(double)) ||
+      !dalloc((void**)&L->ws, wso * sizeof(double)) ||
+      !dalloc((void**)&L->ds, sizeof(DtState))) {
+    level_free(L);
+    return fail(c, PC_ERR_NOMEM, "level allocation failed");
+  }
+  // jobs
+  std::vector<Job> jobs(d->nboxes);
+  for (int b = 0; b < d->nboxes; ++b) {
+    Job& J = jobs[b];
+    memset(&J, 0, sizeof(J));
+    const IBox& bx = L->boxes[b];
+    for (int dd = 0; dd < 3; ++dd) {
+      J.lo[dd] = bx.lo[dd];
+      J.hi[dd] = bx.hi[dd];
+      J.dlo[dd] = d->periodic_self ? bx.lo[dd] : d->domlo[dd];
+      J.dhi[dd] = d->periodic_self ? bx.hi[dd] : d->domhi[dd];
+      J.bclo[dd] = d->periodic_self ? PC_BC_INTERIOR : d->bclo[dd];
+      J.bchi[dd] = d->periodic_self ? PC_BC_INTERIOR : d->bchi[dd];
+      J.qe[dd] = bx.hi[dd] + 5;
+    }
+    J.U = fab_view(L->state + L->off[b], growb(bx, 4));
+    const WsLayout W = ws_layout(bx, ns, true, false, true);
+    ws_fill_job(J, L->ws + wsoff[b], W, true, false, true);
+    J.srcq = null_view();
+    for (int dd = 0; dd < 3; ++dd) J.area[dd] = null_view();
+    J.vol = null_view();
+    J.upd = null_view();
+  }
+  if (!dalloc((void**)&L->jobs, sizeof(Job) * jobs.size())) {
+    level_free(L);
+    return fail(c, PC_ERR_NOMEM, "level allocation failed");
+  }
+  if (hipMemcpy(L->jobs, jobs.data(), sizeof(Job) * jobs.size(),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    level_free(L);
+    return fail(c, PC_ERR_HIP, "job upload failed");
+  }
+  // per-stage launch plans
+  for (int s = 0; s < ST_COUNT; ++s) {
+    std::vector<int> start(d->nboxes), map;
+    for (int b = 0; b < d->nboxes; ++b) {
+      start[b] = (int)map.size();
+      const int nb = grid1(s, jobs[b]);
+      for (int k = 0; k < nb; ++k) map.push_back(b);
+    }
+    L->plans[s].nblocks = (int)map.size();
+    if (!dalloc((void**)&L->plans[s].blkmap, sizeof(int) * std::max<size_t>(map.size(), 1)) ||
+        !dalloc((void**)&L->plans[s].blkstart, sizeof(int) * start.size())) {
+      level_free(L);
+      return fail(c, PC_ERR_NOMEM, "plan allocation failed");
+    }
+    (void)hipMemcpy(L->plans[s].blkmap, map.data(), sizeof(int) * map.size(),
+                    hipMemcpyHostToDevice);
+    (void)hipMemcpy(L->plans[s].blkstart, start.data(), sizeof(int) * start.size(),
+                    hipMemcpyHostToDevice);
+    L->scs[s] = Sched{L->jobs, L->plans[s].blkmap, L->plans[s].blkstart};
+  }
+  // ghost-fill cell map over the domain
+  memset(&L->la, 0, sizeof(L->la));
+  L->la.periodic_self = d->periodic_self;
+  for (int dd = 0; dd < 3; ++dd) {
+    L->la.dlo[dd] = d->domlo[dd];
+    L->la.dhi[dd] = d->domhi[dd];
+    L->la.bclo[dd] = d->bclo[dd];
+    L->la.bchi[dd] = d->bchi[dd];
+  }
+  if (!d->periodic_self) {
+    const long long nx = d->domhi[0] - d->domlo[0] + 1,
+                    ny = d->domhi[1] - d->domlo[1] + 1,
+                    nz = d->domhi[2] - d->domlo[2] + 1;
+    std::vector<int> cm((size_t)(nx * ny * nz), -1);
+    for (int b = 0; b < d->nboxes; ++b) {
+      const IBox& bx = L->boxes[b];
+      for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+          for (int i = bx.lo[0]; i <= bx.hi[0]; ++i)
+            cm[(i - d->domlo[0]) + nx * ((j - d->domlo[1]) + ny * (long long)(k - d->domlo[2]))] = b;
+    }
+    for (size_t x = 0; x < cm.size(); ++x)
+      if (cm[x] < 0) {
+        level_free(L);
+        return fail(c, PC_ERR_UNSUPPORTED,
+                    "the boxes must tile the domain (or use periodic_self)");
+      }
+    if (!dalloc((void**)&L->cellmap, sizeof(int) * cm.size())) {
+      level_free(L);
+      return fail(c, PC_ERR_NOMEM, "cellmap allocation failed");
+    }
+    (void)hipMemcpy(L->cellmap, cm.data(), sizeof(int) * cm.size(),
+                    hipMemcpyHostToDevice);
+  }
+  L->la.cellmap = L->cellmap;
+  L->la.cfl = prm->cfl;
+  L->la.max_dt = prm->max_dt;
+  L->la.fixed_dt = prm->fixed_dt;
+  L->la.change_max = prm->change_max;
+  L->la.init_shrink = prm->init_shrink;
+  L->la.dt_cutoff = prm->dt_cutoff;
+  memset(&L->ka, 0, sizeof(L->ka));
+  for (int dd = 0; dd < 3; ++dd) L->ka.dx[dd] = d->dx[dd];
+  // amrex::Geometry on a Cartesian grid: face areas dx1*dx2 etc., volume
+  // dx0*dx1*dx2
+  L->ka.area_u[0] = d->dx[1] * d->dx[2];
+  L->ka.area_u[1] = d->dx[2] * d->dx[0];
+  L->ka.area_u[2] = d->dx[0] * d->dx[1];
+  L->ka.vol_u = d->dx[0] * d->dx[1] * d->dx[2];
+  L->ka.difmag = prm->difmag;
+  L->ka.dtp = &L->ds->dt;
+  L->ka.plm_iorder = prm->plm_iorder;
+  L->ka.use_flattening = prm->use_flattening;
+  DtState z;
+  z.dt = 0.0;
+  z.minbits = 0x7FEFFFFFFFFFFFFFull;
+  z.bad = 0;
+  z.small = 0;
+  (void)hipMemcpy(L->ds, &z, sizeof(z), hipMemcpyHostToDevice);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    level_free(L);
+    return fail(c, PC_ERR_HIP, "level setup failed");
+  }
+  *out = L;
+  return PC_OK;
+}
+
+int pc_level_destroy_v1(pc_level_t* L) {
+  if (!L) return PC_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  level_free(L);
+  return PC_OK;
+}
+
+long long pc_level_box_offset_v1(const pc_level_t* L, int b) {
+  if (!L || b < 0 || b >= (int)L->boxes.size()) return -1;
+  return L->off[b];
+}
+long long pc_level_box_size_v1(const pc_level_t* L, int b) {
+  if (!L || b < 0 || b >= (int)L->boxes.size()) return -1;
+  return growb(L->boxes[b], 4).npts() * (7 + L->ctx->ns);
+}
+double* pc_level_state_ptr_v1(pc_level_t* L) { return L ? L->state : nullptr; }
+long long pc_level_state_size_v1(const pc_level_t* L) {
+  return L ? L->state_total : -1;
+}
+
+int pc_level_write_box_v1(pc_level_t* L, int b, const double* buf) {
+  if (!L || !buf || b < 0 || b >= (int)L->boxes.size()) return PC_ERR_ARG;
+  const size_t n = (size_t)growb(L->boxes[b], 4).npts() * (7 + L->ctx->ns);
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  HIPCHK(L->ctx, hipMemcpy(L->state + L->off[b], buf, n * sizeof(double),
+                           hipMemcpyDefault));
+  return PC_OK;
+}
+
+int pc_level_read_box_v1(pc_level_t* L, int b, double* buf) {
+  if (!L || !buf || b < 0 || b >= (int)L->boxes.size()) return PC_ERR_ARG;
+  const size_t n = (size_t)growb(L->boxes[b], 4).npts() * (7 + L->ctx->ns);
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  HIPCHK(L->ctx, hipMemcpy(buf, L->state + L->off[b], n * sizeof(double),
+                           hipMemcpyDefault));
+  return PC_OK;
+}
+
+int pc_level_set_dt_v1(pc_level_t* L, double dt) {
+  if (!L) return PC_ERR_ARG;
+  HIPCHK(L->ctx, hipMemcpy(&L->ds->dt, &dt, sizeof(double), hipMemcpyHostToDevice));
+  return PC_OK;
+}
+int pc_level_get_dt_v1(pc_level_t* L, double* dt) {
+  if (!L || !dt) return PC_ERR_ARG;
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  HIPCHK(L->ctx, hipMemcpy(dt, &L->ds->dt, sizeof(double), hipMemcpyDeviceToHost));
+  return PC_OK;
+}
+
+static int level_fill(pc_level* L, hipStream_t st) {
+  const int ns = L->ctx->ns;
+  Job dummy;
+  memset(&dummy, 0, sizeof(dummy));
+  if (ns == 1)
+    LAUNCH(k_ghost<1>, L->plans[ST_GHOST].nblocks, st, dummy, L->scs[ST_GHOST], L->ka, L->la);
+  else
+    LAUNCH(k_ghost<9>, L->plans[ST_GHOST].nblocks, st, dummy, L->scs[ST_GHOST], L->ka, L->la);
+  HIPCHK(L->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+int pc_level_fill_ghosts_v1(pc_level_t* L, void* stream) {
+  if (!L) return PC_ERR_ARG;
+  return level_fill(L, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+template <class EOS>
+static void level_estdt_launch(pc_level* L, hipStream_t st) {
+  Job dummy;
+  memset(&dummy, 0, sizeof(dummy));
+  LAUNCH(k_estdt<EOS>, L->plans[ST_UPDATE].nblocks, st, dummy, L->scs[ST_UPDATE], L->ka, L->ds);
+}
+
+extern "C" {
+
+int pc_level_init_dt_v1(pc_level_t* L, void* stream) {
+  if (!L) return PC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (L->ctx->ns == 1) level_estdt_launch<EosGamma>(L, st);
+  else level_estdt_launch<EosLiDryer>(L, st);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 1);
+  HIPCHK(L->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+int pc_level_estdt_v1(pc_level_t* L, double* dt_out, void* stream) {
+  if (!L || !dt_out) return PC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  DtState save;
+  HIPCHK(L->ctx, hipStreamSynchronize(st));
+  HIPCHK(L->ctx, hipMemcpy(&save, L->ds, sizeof(save), hipMemcpyDeviceToHost));
+  if (L->ctx->ns == 1) level_estdt_launch<EosGamma>(L, st);
+  else level_estdt_launch<EosLiDryer>(L, st);
+  LevelArgs la = L->la;
+  la.init_shrink = 1.0;
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, la, 1);
+  HIPCHK(L->ctx, hipStreamSynchronize(st));
+  DtState now;
+  HIPCHK(L->ctx, hipMemcpy(&now, L->ds, sizeof(now), hipMemcpyDeviceToHost));
+  *dt_out = now.dt;
+  HIPCHK(L->ctx, hipMemcpy(L->ds, &save, sizeof(save), hipMemcpyHostToDevice));
+  return PC_OK;
+}
+
+int pc_level_enable_timing_v1(pc_level_t* L, int on) {
+  if (!L) return PC_ERR_ARG;
+  if (on && !L->ev_made) {
+    for (int e = 0; e < 8; ++e) HIPCHK(L->ctx, hipEventCreate(&L->ev[e]));
+    L->ev_made = true;
+  }
+  L->timing = on;
+  for (int s = 0; s < 7; ++s) L->times[s] = 0.0;
+  L->nsteps_timed = 0;
+  return PC_OK;
+}
+
+}  // extern "C"
+
+template <class EOS>
+static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
+  constexpr int NS = EOS::NS;
+  Job d0;
+  memset(&d0, 0, sizeof(d0));
+  const int ppm = L->prm.ppm_type;
+  for (int s = 0; s < nsteps; ++s) {
+    const bool tm = L->timing;
+    if (tm) (void)hipEventRecord(L->ev[0], st);
+    level_fill(L, st);
+    if (tm) (void)hipEventRecord(L->ev[1], st);
+    LAUNCH(k_ctoprim<EOS>, L->plans[ST_CTOPRIM].nblocks, st, d0, L->scs[ST_CTOPRIM], L->ka);
+    if (tm) (void)hipEventRecord(L->ev[2], st);
+    LAUNCH(k_divu, L->plans[ST_DIVU].nblocks, st, d0, L->scs[ST_DIVU], L->ka);
+    const Sched* S = L->scs;
+    const int gT = L->plans[ST_TRACE].nblocks;
+    if (ppm == 1) {
+      LAUNCH((k_trace_ppm<EOS, 0>), gT, st, d0, S[ST_TRACE], L->ka);
+      LAUNCH((k_trace_ppm<EOS, 1>), gT, st, d0, S[ST_TRACE], L->ka);
+      LAUNCH((k_trace_ppm<EOS, 2>), gT, st, d0, S[ST_TRACE], L->ka);
+    } else {
+      LAUNCH((k_trace_plm<EOS, 0>), gT, st, d0, S[ST_TRACE], L->ka);
+      LAUNCH((k_trace_plm<EOS, 1>), gT, st, d0, S[ST_TRACE], L->ka);
+      LAUNCH((k_trace_plm<EOS, 2>), gT, st, d0, S[ST_TRACE], L->ka);
+    }
+    if (tm) (void)hipEventRecord(L->ev[3], st);
+    LAUNCH((k_flux1<EOS, 0>), L->plans[ST_FLUX1_X].nblocks, st, d0, S[ST_FLUX1_X], L->ka);
+    LAUNCH((k_flux1<EOS, 1>), L->plans[ST_FLUX1_Y].nblocks, st, d0, S[ST_FLUX1_Y], L->ka);
+    LAUNCH((k_flux1<EOS, 2>), L->plans[ST_FLUX1_Z].nblocks, st, d0, S[ST_FLUX1_Z], L->ka);
+    if (tm) (void)hipEventRecord(L->ev[4], st);
+    LAUNCH((k_trans<EOS, 0, 1>), L->plans[ST_TRANS_XY].nblocks, st, d0, S[ST_TRANS_XY], L->ka);
+    LAUNCH((k_trans<EOS, 0, 2>), L->plans[ST_TRANS_XZ].nblocks, st, d0, S[ST_TRANS_XZ], L->ka);
+    LAUNCH((k_trans<EOS, 1, 0>), L->plans[ST_TRANS_YX].nblocks, st, d0, S[ST_TRANS_YX], L->ka);
+    LAUNCH((k_trans<EOS, 1, 2>), L->plans[ST_TRANS_YZ].nblocks, st, d0, S[ST_TRANS_YZ], L->ka);
+    LAUNCH((k_trans<EOS, 2, 0>), L->plans[ST_TRANS_ZX].nblocks, st, d0, S[ST_TRANS_ZX], L->ka);
+    LAUNCH((k_trans<EOS, 2, 1>), L->plans[ST_TRANS_ZY].nblocks, st, d0, S[ST_TRANS_ZY], L->ka);
+    if (tm) (void)hipEventRecord(L->ev[5], st);
+    LAUNCH((k_final<EOS, 0>), L->plans[ST_FINAL_X].nblocks, st, d0, S[ST_FINAL_X], L->ka);
+    LAUNCH((k_final<EOS, 1>), L->plans[ST_FINAL_Y].nblocks, st, d0, S[ST_FINAL_Y], L->ka);
+    LAUNCH((k_final<EOS, 2>), L->plans[ST_FINAL_Z].nblocks, st, d0, S[ST_FINAL_Z], L->ka);
+    if (tm) (void)hipEventRecord(L->ev[6], st);
+    LAUNCH(k_update<EOS>, L->plans[ST_UPDATE].nblocks, st, d0, S[ST_UPDATE], L->ka, L->ds);
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 0);
+    if (tm) {
+      (void)hipEventRecord(L->ev[7], st);
+      (void)hipEventSynchronize(L->ev[7]);
+      for (int e = 0; e < 7; ++e) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, L->ev[e], L->ev[e + 1]);
+        L->times[e] += ms;
+      }
+      L->nsteps_timed++;
+    }
+    (void)NS;
+  }
+  HIPCHK(L->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+extern "C" {
+
+int pc_level_step_v1(pc_level_t* L, int nsteps, void* stream) {
+  if (!L || nsteps < 0) return PC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (L->ctx->ns == 1) return level_step_eos<EosGamma>(L, nsteps, st);
+  return level_step_eos<EosLiDryer>(L, nsteps, st);
+}
+
+int pc_level_stage_times_v1(pc_level_t* L, double* ms7) {
+  if (!L || !ms7) return PC_ERR_ARG;
+  for (int s = 0; s < 7; ++s)
+    ms7[s] = L->nsteps_timed ? L->times[s] / L->nsteps_timed : 0.0;
+  return PC_OK;
+}
+
+int pc_level_status_v1(pc_level_t* L, int* bad, int* small_dt) {
+  if (!L) return PC_ERR_ARG;
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  DtState s;
+  HIPCHK(L->ctx, hipMemcpy(&s, L->ds, sizeof(s), hipMemcpyDeviceToHost));
+  if (bad) *bad = s.bad;
+  if (small_dt) *small_dt = s.small;
+  return s.bad ? PC_ERR_NAN : (s.small ? PC_ERR_DT : PC_OK);
+}
+
+int pc_level_prim_to_cons_v1(pc_level_t* L, int b, pc_array4_t prim, int mode,
+                             void* stream) {
+  if (!L || b < 0 || b >= (int)L->boxes.size() || (mode != 0 && mode != 1))
+    return PC_ERR_ARG;
+  const IBox& bx = L->boxes[b];
+  Job J;
+  memset(&J, 0, sizeof(J));
+  for (int d = 0; d < 3; ++d) { J.lo[d] = bx.lo[d]; J.hi[d] = bx.hi[d]; }
+  J.U = fab_view(L->state + L->off[b], growb(bx, 4));
+  V pv;
+  int rc;
+  if ((rc = a4_view(L->ctx, prim, bx, 5 + L->ctx->ns, "prim", pv))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int g = grid1(ST_UPDATE, J);
+  const Sched s0{nullptr, nullptr, nullptr};
+  if (L->ctx->ns == 1)
+    LAUNCH(k_prim_to_cons<EosGamma>, g, st, J, s0, L->ka, pv, mode);
+  else
+    LAUNCH(k_prim_to_cons<EosLiDryer>, g, st, J, s0, L->ka, pv, mode);
+  HIPCHK(L->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+}  // extern "C"

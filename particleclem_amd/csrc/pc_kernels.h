@@ -1,0 +1,870 @@
+// Stage kernels of the Godunov step for gfx950.
+//
+// One launch per stage covers every box of a box table (one box for the
+// drop-in pc_umdrv_v1, a whole level for the level stepper).  Blocks of 256
+// threads (4 wave64) map to boxes through a block->box table; inside a box
+// the thread index runs i-fastest over the stage's region, so every global
+// access is a unit-stride (coalesced) walk along i of an SoA component.
+//
+// Stage regions (bx = [L, H] the valid box) are the dependency cone of the
+// valid outputs, smaller than the reference's grow-2 temporaries:
+//   ctoprim      cells grow(bx,4)
+//   divu         nodes [L, H+1]^3
+//   trace_d      cells grow(bx,1); writes qm on face c+1 and qp on face c
+//   flux1_d      d-faces [L, H+1] x transverse [L-1, H+1]
+//   trans_(d|t)  d-faces [L, H+1] x t in [L, H] x t' in [L-1, H+1]
+//   final_d      d-faces [L, H+1] x transverse [L, H]
+//   update       cells [L, H]
+#pragma once
+#include "pc_phys.h"
+
+namespace pcd {
+
+constexpr int BLOCK = 256;
+
+// SoA view of one FAB (Array4 with 32-bit strides).
+struct V {
+  double* p;
+  int js, ks, ns;
+  int b0, b1, b2;
+  PCD double& operator()(int i, int j, int k, int n) const {
+    return p[(i - b0) + (j - b1) * js + (k - b2) * ks + n * ns];
+  }
+  PCD bool contains(int i, int j, int k, int e0, int e1, int e2) const {
+    return i >= b0 && i < e0 && j >= b1 && j < e1 && k >= b2 && k < e2;
+  }
+};
+
+// Compact layouts of the stage intermediates.
+//   traces: comps [0, QT) = minus state (left of face), [QT, 2QT) = plus
+//           state; per state rho, u, v, w, p, rhoe, Y[NS]  (QT = 6 + NS)
+//   fluxes: rho, mx, my, mz, rhoE, p_gd, un_gd, rhoY[NS]   (FT = 7 + NS)
+struct Job {
+  int lo[3], hi[3];                 // valid box
+  int dlo[3], dhi[3], bclo[3], bchi[3];
+  int qe[3];                        // exclusive end of the q box (contains)
+  V U;                              // state with >= 4 ghosts (uin)
+  V q, qa, srcq;                    // primitives on grow(bx,4); srcq.p may be 0
+  V tr[3];                          // traces per direction
+  V f1[3];                          // first fluxes
+  V ft[6];                          // transverse fluxes, [2*d + slot(d,t)]
+  V flx[3], qec[3];                 // final fluxes (NVAR) and p_gd, un_gd
+  V area[3], vol;                   // p == 0: uniform geometry
+  V upd;                            // uout (dU/dt accumulator) or p == 0
+  V divu;                           // nodes [L, H+1]^3
+};
+
+struct KArgs {
+  double dx[3];
+  double area_u[3];
+  double vol_u;
+  double difmag;
+  double dt_val;
+  const double* dtp;  // device dt (level path) or 0
+  int plm_iorder;
+  int use_flattening;
+  int nblocks;
+};
+
+PCD double get_dt(const KArgs& a) { return a.dtp ? *a.dtp : a.dt_val; }
+
+PCD int tslot(int d, int t) { return (t < d) ? t : t - 1; }
+
+// block -> (job, local thread index)
+struct Sched {
+  const Job* jobs;
+  const int* blkmap;
+  const int* blkstart;
+};
+
+PCD const Job& job_of(const Job& j0, const Sched& s, int& local) {
+  if (!s.jobs) {
+    local = blockIdx.x * BLOCK + threadIdx.x;
+    return j0;
+  }
+  const int b = s.blkmap[blockIdx.x];
+  local = (blockIdx.x - s.blkstart[b]) * BLOCK + threadIdx.x;
+  return s.jobs[b];
+}
+
+PCD bool region_ijk(const int lo[3], const int n[3], int local, int& i,
+                    int& j, int& k) {
+  const int nxy = n[0] * n[1];
+  if (local >= nxy * n[2]) return false;
+  k = local / nxy;
+  const int r = local - k * nxy;
+  j = r / n[0];
+  i = r - j * n[0];
+  i += lo[0];
+  j += lo[1];
+  k += lo[2];
+  return true;
+}
+
+// Stage regions (see file comment).
+enum Stage {
+  ST_CTOPRIM = 0,
+  ST_DIVU,
+  ST_TRACE,
+  ST_FLUX1_X, ST_FLUX1_Y, ST_FLUX1_Z,
+  ST_TRANS_XY, ST_TRANS_XZ, ST_TRANS_YX, ST_TRANS_YZ, ST_TRANS_ZX, ST_TRANS_ZY,
+  ST_FINAL_X, ST_FINAL_Y, ST_FINAL_Z,
+  ST_UPDATE,
+  ST_GHOST,
+  ST_COUNT
+};
+
+__host__ __device__ inline void stage_region(int st, const int L[3],
+                                             const int H[3], int lo[3],
+                                             int n[3]) {
+  int a[3], b[3];
+  for (int d = 0; d < 3; ++d) { a[d] = L[d]; b[d] = H[d]; }
+  auto set = [&](int d, int x, int y) { a[d] = x; b[d] = y; };
+  switch (st) {
+    case ST_CTOPRIM:
+    case ST_GHOST:
+      for (int d = 0; d < 3; ++d) set(d, L[d] - 4, H[d] + 4);
+      break;
+    case ST_DIVU:
+      for (int d = 0; d < 3; ++d) set(d, L[d], H[d] + 1);
+      break;
+    case ST_TRACE:
+      for (int d = 0; d < 3; ++d) set(d, L[d] - 1, H[d] + 1);
+      break;
+    case ST_FLUX1_X: case ST_FLUX1_Y: case ST_FLUX1_Z: {
+      const int D = st - ST_FLUX1_X;
+      for (int d = 0; d < 3; ++d) set(d, L[d] - 1, H[d] + 1);
+      set(D, L[D], H[D] + 1);
+      break;
+    }
+    case ST_TRANS_XY: case ST_TRANS_XZ: case ST_TRANS_YX:
+    case ST_TRANS_YZ: case ST_TRANS_ZX: case ST_TRANS_ZY: {
+      const int idx = st - ST_TRANS_XY;
+      const int D = idx / 2;
+      const int T = (idx % 2 == 0) ? (D == 0 ? 1 : 0) : (D == 2 ? 1 : 2);
+      const int Tp = 3 - D - T;
+      set(D, L[D], H[D] + 1);
+      set(T, L[T], H[T]);
+      set(Tp, L[Tp] - 1, H[Tp] + 1);
+      break;
+    }
+    case ST_FINAL_X: case ST_FINAL_Y: case ST_FINAL_Z: {
+      const int D = st - ST_FINAL_X;
+      set(D, L[D], H[D] + 1);
+      break;
+    }
+    case ST_UPDATE:
+    default:
+      break;
+  }
+  for (int d = 0; d < 3; ++d) { lo[d] = a[d]; n[d] = b[d] - a[d] + 1; }
+}
+
+// ----------------------------------------------------------------- loads --
+template <int NS>
+PCD void load_trace(const V& tr, int i, int j, int k, int side,
+                    FaceState<NS>& s) {
+  constexpr int QT = 6 + NS;
+  const int o = side * QT;
+  s.rho = tr(i, j, k, o + 0);
+  s.u = tr(i, j, k, o + 1);
+  s.v = tr(i, j, k, o + 2);
+  s.w = tr(i, j, k, o + 3);
+  s.p = tr(i, j, k, o + 4);
+  s.rhoe = tr(i, j, k, o + 5);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) s.Y[n] = tr(i, j, k, o + 6 + n);
+}
+template <int NS>
+PCD void store_trace(const V& tr, int i, int j, int k, int side,
+                     const FaceState<NS>& s) {
+  constexpr int QT = 6 + NS;
+  const int o = side * QT;
+  tr(i, j, k, o + 0) = s.rho;
+  tr(i, j, k, o + 1) = s.u;
+  tr(i, j, k, o + 2) = s.v;
+  tr(i, j, k, o + 3) = s.w;
+  tr(i, j, k, o + 4) = s.p;
+  tr(i, j, k, o + 5) = s.rhoe;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) tr(i, j, k, o + 6 + n) = s.Y[n];
+}
+template <int NS>
+PCD void store_flux(const V& f, int i, int j, int k, const Flux<NS>& F) {
+  f(i, j, k, 0) = F.f[0];
+  f(i, j, k, 1) = F.f[1];
+  f(i, j, k, 2) = F.f[2];
+  f(i, j, k, 3) = F.f[3];
+  f(i, j, k, 4) = F.f[4];
+  f(i, j, k, 5) = F.pgd;
+  f(i, j, k, 6) = F.ugd;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) f(i, j, k, 7 + n) = F.f[6 + n];
+}
+
+template <int D>
+PCD void step(int& i, int& j, int& k, int s) {
+  if (D == 0) i += s;
+  else if (D == 1) j += s;
+  else k += s;
+}
+
+// ------------------------------------------------------------ K1 ctoprim --
+// Source/PeleCUtilities.H:242-320
+template <class EOS>
+__global__ void __launch_bounds__(BLOCK)
+    k_ctoprim(Job j0, Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const V& u = J.U;
+  const V& q = J.q;
+  const V& qa = J.qa;
+  const double rho = u(i, j, k, URHO);
+  const double rhoinv = 1.0 / rho;
+  const double vx = u(i, j, k, UMX) * rhoinv;
+  const double vy = u(i, j, k, UMY) * rhoinv;
+  const double vz = u(i, j, k, UMZ) * rhoinv;
+  const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz) -
+                        0.5 * rho * 0.0 * 0.0 * 0.0 * 0.0;
+  double Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double y = u(i, j, k, UFS + s) / rho;
+    if ((-1e-4 * DEPS < y) && (y < 1e-4 * DEPS)) y = 0.0;
+    Y[s] = y;
+  }
+  const double e = (u(i, j, k, UEDEN) - kineng) * rhoinv;
+  const double T = EOS::EY2T(e, Y, u(i, j, k, UTEMP));
+  const double w = EOS::wbar(Y);
+  const double p = EOS::RTY2P(rho, T, Y);
+  const double cs = EOS::RTY2Cs(rho, T, Y);
+  const double gam1 = EOS::RTY2G(rho, T, Y);
+  double dpde, dpdr_e;
+  EOS::RTY2dpde_dpdre(rho, T, Y, dpde, dpdr_e);
+  q(i, j, k, QRHO) = rho;
+  q(i, j, k, QU) = vx;
+  q(i, j, k, QV) = vy;
+  q(i, j, k, QW) = vz;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) q(i, j, k, QFS + s) = Y[s];
+  q(i, j, k, QTEMP) = T;
+  q(i, j, k, QREINT) = e * rho;
+  q(i, j, k, QPRES) = p;
+  q(i, j, k, QGAME) = p / (e * rho) + 1.0;
+  qa(i, j, k, QDPDR) = dpdr_e;
+  qa(i, j, k, QDPDE) = dpde;
+  qa(i, j, k, QGAMC) = gam1;
+  qa(i, j, k, QC) = cs;
+  qa(i, j, k, QCSML) = mx(small_num, small_num * cs);
+  qa(i, j, k, QRSPEC) = RU / w;
+}
+
+// ---------------------------------------------------------- K2 srctoprim --
+// Source/Hydro.H:14-68 (src: NVAR comps over the q box)
+template <int NS>
+__global__ void __launch_bounds__(BLOCK)
+    k_srctoprim(Job j0, Sched sc, KArgs ka, V src) {
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const V& q = J.q;
+  const V& qa = J.qa;
+  const V& srcq = J.srcq;
+  const double srcrho = src(i, j, k, URHO);
+  const double rhoinv = 1.0 / q(i, j, k, QRHO);
+  const double vx = q(i, j, k, QU);
+  const double vy = q(i, j, k, QV);
+  const double vz = q(i, j, k, QW);
+  const double umx = src(i, j, k, UMX);
+  const double umy = src(i, j, k, UMY);
+  const double umz = src(i, j, k, UMZ);
+  srcq(i, j, k, QRHO) = srcrho;
+  srcq(i, j, k, QU) = (umx - vx * srcrho) * rhoinv;
+  srcq(i, j, k, QV) = (umy - vy * srcrho) * rhoinv;
+  srcq(i, j, k, QW) = (umz - vz * srcrho) * rhoinv;
+  const double sre = src(i, j, k, UEDEN) - vx * umx - vy * umy - vz * umz +
+                     srcrho * (vx * vx + vy * vy + vz * vz);
+  srcq(i, j, k, QREINT) = sre;
+  srcq(i, j, k, QPRES) =
+      qa(i, j, k, QDPDE) * (sre - q(i, j, k, QREINT) * srcrho * rhoinv) *
+          rhoinv +
+      qa(i, j, k, QDPDR) * srcrho;
+#pragma unroll
+  for (int s = 0; s < NS; s++)
+    srcq(i, j, k, QFS + s) =
+        (src(i, j, k, UFS + s) - q(i, j, k, QFS + s) * srcrho) * rhoinv;
+}
+
+// --------------------------------------------------------------- K24 divu --
+// Source/Hydro.H:476-517 on the nodes of the valid box
+__global__ void __launch_bounds__(BLOCK) k_divu(Job j0, Sched sc, KArgs ka) {
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_DIVU, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const V& q = J.q;
+  const double ux =
+      0.25 *
+      (q(i, j, k, QU) - q(i - 1, j, k, QU) + q(i, j, k - 1, QU) -
+       q(i - 1, j, k - 1, QU) + q(i, j - 1, k, QU) - q(i - 1, j - 1, k, QU) +
+       q(i, j - 1, k - 1, QU) - q(i - 1, j - 1, k - 1, QU)) /
+      ka.dx[0];
+  const double vy =
+      0.25 *
+      (q(i, j, k, QV) - q(i, j - 1, k, QV) + q(i, j, k - 1, QV) -
+       q(i, j - 1, k - 1, QV) + q(i - 1, j, k, QV) - q(i - 1, j - 1, k, QV) +
+       q(i - 1, j, k - 1, QV) - q(i - 1, j - 1, k - 1, QV)) /
+      ka.dx[1];
+  const double wz =
+      0.25 *
+      (q(i, j, k, QW) - q(i, j, k - 1, QW) + q(i - 1, j, k, QW) -
+       q(i - 1, j, k - 1, QW) + q(i, j - 1, k, QW) - q(i, j - 1, k - 1, QW) +
+       q(i - 1, j - 1, k, QW) - q(i - 1, j - 1, k - 1, QW)) /
+      ka.dx[2];
+  J.divu(i, j, k, 0) = ux + vy + wz;
+}
+
+// flattening coefficient of cell (i,j,k): min over the three directions
+PCD double flat_cell(const Job& J, int i, int j, int k) {
+  const V& q = J.q;
+  double flat = 1.0;
+#pragma unroll
+  for (int df = 0; df < 3; ++df) {
+    int oi = (df == 0), oj = (df == 1), ok = (df == 2);
+    const bool inside =
+        q.contains(i + 3 * oi, j + 3 * oj, k + 3 * ok, J.qe[0], J.qe[1],
+                   J.qe[2]) &&
+        q.contains(i - 3 * oi, j - 3 * oj, k - 3 * ok, J.qe[0], J.qe[1],
+                   J.qe[2]);
+    auto pq = [&](int m) { return q(i + m * oi, j + m * oj, k + m * ok, QPRES); };
+    auto uq = [&](int m) {
+      return q(i + m * oi, j + m * oj, k + m * ok, QU + df);
+    };
+    flat = mn(flat, flatten_line(inside, pq, uq));
+  }
+  return flat;
+}
+
+// ------------------------------------------------------ K3-5 trace (PPM) --
+// Source/PPM.cpp:82-423 for one cell and one direction (no WENO).
+template <class EOS, int D>
+__global__ void __launch_bounds__(BLOCK)
+    k_trace_ppm(Job j0, Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_TRACE, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const V& q = J.q;
+  const double dtdx = get_dt(ka) / ka.dx[D];
+  constexpr int QUN = QU + D;
+  constexpr int QUT = (D == 0) ? QV : ((D == 1) ? QW : QU);
+  constexpr int QUTT = (D == 0) ? QW : ((D == 1) ? QU : QV);
+  auto at = [&](int m, int comp) {
+    int ii = i, jj = j, kk = k;
+    step<D>(ii, jj, kk, m);
+    return q(ii, jj, kk, comp);
+  };
+  double Yc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Yc[s] = q(i, j, k, QFS + s);
+  const double cc = EOS::RPY2Cs(q(i, j, k, QRHO), q(i, j, k, QPRES), Yc);
+  const double un = q(i, j, k, QUN);
+  const double flat = ka.use_flattening ? flat_cell(J, i, j, k) : 1.0;
+  const double spd0 = un - cc, spd1 = un, spd2 = un + cc;
+  double sm, sp, s6;
+  auto recon = [&](int comp) {
+    double s[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) s[m] = at(m - 2, comp);
+    ppm_reconstruct(s, flat, sm, sp);
+    s6 = ppm_s6(sm, sp, s[2]);
+  };
+  // integrals needed by the characteristic projection
+  recon(QRHO);
+  const double Im_r0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const double Im_r1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const double Ip_r2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  const double Ip_r1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  recon(QUN);
+  const double Im_u0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const double Im_u2 = ppm_im(sm, sp, s6, spd2, dtdx);
+  const double Ip_u2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  const double Ip_u0 = ppm_ip(sm, sp, s6, spd0, dtdx);
+  recon(QPRES);
+  const double Im_p0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const double Im_p1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const double Im_p2 = ppm_im(sm, sp, s6, spd2, dtdx);
+  const double Ip_p0 = ppm_ip(sm, sp, s6, spd0, dtdx);
+  const double Ip_p1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  const double Ip_p2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  recon(QREINT);
+  const double Im_e0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const double Im_e1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const double Ip_e2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  const double Ip_e1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  recon(QUT);
+  const double Im_t1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const double Ip_t1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  recon(QUTT);
+  const double Im_tt1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const double Ip_tt1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  FaceState<NS> P, M;
+  double YrefP[NS], YrefM[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    recon(QFS + s);
+    YrefP[s] = ppm_im(sm, sp, s6, spd0, dtdx);
+    P.Y[s] = ppm_im(sm, sp, s6, spd1, dtdx);
+    YrefM[s] = ppm_ip(sm, sp, s6, spd2, dtdx);
+    M.Y[s] = ppm_ip(sm, sp, s6, spd1, dtdx);
+  }
+  double vel[3];
+  // ---- plus state on face i (reference state: fastest wave to the left)
+  {
+    double rho_ref = Im_r0;
+    const double un_ref = Im_u0;
+    double p_ref = Im_p0;
+    const double rhoe_g_ref = Im_e0;
+    rho_ref = mx(rho_ref, DMIN);
+    const double rho_ref_inv = 1.0 / rho_ref;
+    p_ref = mx(p_ref, DMIN);
+    const double cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefP);
+    const double csq_ref = cc_ref * cc_ref;
+    const double cc_ref_inv = 1.0 / cc_ref;
+    (void)rhoe_g_ref;
+    const double dum = un_ref - Im_u0;
+    const double dptotm = p_ref - Im_p0;
+    const double drho = rho_ref - Im_r1;
+    const double dptot = p_ref - Im_p1;
+    const double dup = un_ref - Im_u2;
+    const double dptotp = p_ref - Im_p2;
+    double alpham =
+        0.5 * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+    double alphap =
+        0.5 * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+    double alpha0r = drho - dptot / csq_ref;
+    alpham = un - cc > 0.0 ? 0.0 : -alpham;
+    alphap = un + cc > 0.0 ? 0.0 : -alphap;
+    alpha0r = un > 0.0 ? 0.0 : -alpha0r;
+    P.rho = mx(DMIN, rho_ref + alphap + alpham + alpha0r);
+    vel[D] = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
+    P.p = mx(DMIN, p_ref + (alphap + alpham) * csq_ref);
+    vel[QUT - QU] = Im_t1;
+    vel[QUTT - QU] = Im_tt1;
+    P.u = vel[0];
+    P.v = vel[1];
+    P.w = vel[2];
+    const double eint = EOS::RYP2E(P.rho, P.Y, P.p);
+    P.rhoe = P.rho * eint;
+  }
+  // ---- minus state on face i+1 (fastest wave to the right)
+  {
+    double rho_ref = Ip_r2;
+    const double un_ref = Ip_u2;
+    double p_ref = Ip_p2;
+    rho_ref = mx(rho_ref, DMIN);
+    const double rho_ref_inv = 1.0 / rho_ref;
+    p_ref = mx(p_ref, DMIN);
+    const double cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefM);
+    const double csq_ref = cc_ref * cc_ref;
+    const double cc_ref_inv = 1.0 / cc_ref;
+    const double dum = un_ref - Ip_u0;
+    const double dptotm = p_ref - Ip_p0;
+    const double drho = rho_ref - Ip_r1;
+    const double dptot = p_ref - Ip_p1;
+    const double dup = un_ref - Ip_u2;
+    const double dptotp = p_ref - Ip_p2;
+    double alpham =
+        0.5 * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+    double alphap =
+        0.5 * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+    double alpha0r = drho - dptot / csq_ref;
+    alpham = un - cc > 0.0 ? -alpham : 0.0;
+    alphap = un + cc > 0.0 ? -alphap : 0.0;
+    alpha0r = un > 0.0 ? -alpha0r : 0.0;
+    M.rho = mx(DMIN, rho_ref + alphap + alpham + alpha0r);
+    vel[D] = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
+    M.p = mx(DMIN, p_ref + (alphap + alpham) * csq_ref);
+    vel[QUT - QU] = Ip_t1;
+    vel[QUTT - QU] = Ip_tt1;
+    M.u = vel[0];
+    M.v = vel[1];
+    M.w = vel[2];
+    const double eint = EOS::RYP2E(M.rho, M.Y, M.p);
+    M.rhoe = M.rho * eint;
+  }
+  (void)Ip_e2;
+  (void)Ip_e1;
+  (void)Im_e1;
+  const V& tr = J.tr[D];
+  store_trace<NS>(tr, i, j, k, 1, P);
+  int i1 = i, j1 = j, k1 = k;
+  step<D>(i1, j1, k1, 1);
+  store_trace<NS>(tr, i1, j1, k1, 0, M);
+}
+
+// --------------------------------------------------- K3' trace (PLM) ----
+// Source/Godunov.cpp:139-180 (one direction per launch)
+template <class EOS, int D>
+__global__ void __launch_bounds__(BLOCK)
+    k_trace_plm(Job j0, Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_TRACE, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const V& q = J.q;
+  const double dtdx = get_dt(ka) / ka.dx[D];
+  const double flat = ka.use_flattening ? flat_cell(J, i, j, k) : 1.0;
+  auto slope = [&](int comp) {
+    double s[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      int ii = i, jj = j, kk = k;
+      step<D>(ii, jj, kk, m - 2);
+      s[m] = q(ii, jj, kk, comp);
+    }
+    return plm_slope(s, flat, ka.plm_iorder);
+  };
+  FaceState<NS> c, sl, P, M;
+  c.rho = q(i, j, k, QRHO);
+  c.u = q(i, j, k, QU);
+  c.v = q(i, j, k, QV);
+  c.w = q(i, j, k, QW);
+  c.p = q(i, j, k, QPRES);
+  c.rhoe = q(i, j, k, QREINT);
+  sl.rho = slope(QRHO);
+  sl.u = slope(QU);
+  sl.v = slope(QV);
+  sl.w = slope(QW);
+  sl.p = slope(QPRES);
+  sl.rhoe = slope(QREINT);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    c.Y[s] = q(i, j, k, QFS + s);
+    sl.Y[s] = slope(QFS + s);
+  }
+  plm_trace<NS>(D, c, sl, J.qa(i, j, k, QC), dtdx, P, M);
+  const V& tr = J.tr[D];
+  store_trace<NS>(tr, i, j, k, 1, P);
+  int i1 = i, j1 = j, k1 = k;
+  step<D>(i1, j1, k1, 1);
+  store_trace<NS>(tr, i1, j1, k1, 0, M);
+}
+
+// -------------------------------------------------- K6-8 first fluxes ----
+template <class EOS, int D>
+__global__ void __launch_bounds__(BLOCK)
+    k_flux1(Job j0, Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_FLUX1_X + D, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  FaceState<NS> L, R;
+  load_trace<NS>(J.tr[D], i, j, k, 0, L);
+  load_trace<NS>(J.tr[D], i, j, k, 1, R);
+  int im = i, jm = j, km = k;
+  step<D>(im, jm, km, -1);
+  const double cav = 0.5 * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  Flux<NS> F;
+  cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
+  store_flux<NS>(J.f1[D], i, j, k, F);
+}
+
+// flux differences across cell c in direction T from a compact flux array
+template <int NS, int T>
+PCD void dflux(const V& f, int i, int j, int k, double c, DFlux<NS>& D) {
+  int ip = i, jp = j, kp = k;
+  step<T>(ip, jp, kp, 1);
+#pragma unroll
+  for (int m = 0; m < 5; ++m) D.d[m] = c * (f(ip, jp, kp, m) - f(i, j, k, m));
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    D.ds[s] = c * (f(ip, jp, kp, 7 + s) - f(i, j, k, 7 + s));
+  D.pgp = f(ip, jp, kp, 5);
+  D.pgm = f(i, j, k, 5);
+  D.ugp = f(ip, jp, kp, 6);
+  D.ugm = f(i, j, k, 6);
+}
+
+// ------------------------------------- K9-16 transverse (d|t) fluxes ----
+// pc_transdo on the two cells of a d-face, then pc_cmpflx
+// (Source/Godunov.cpp:246-384).
+template <class EOS, int D, int T>
+__global__ void __launch_bounds__(BLOCK)
+    k_trans(Job j0, Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_TRANS_XY + 2 * D + tslot(D, T), J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const double cdtdx = 1.0 / 3.0 * get_dt(ka) / ka.dx[T];
+  int im = i, jm = j, km = k;
+  step<D>(im, jm, km, -1);
+  FaceState<NS> s, L, R;
+  DFlux<NS> df;
+  // minus side: cell f - e_D, state on its high face
+  load_trace<NS>(J.tr[D], i, j, k, 0, s);
+  dflux<NS, T>(J.f1[T], im, jm, km, cdtdx, df);
+  transdo_side<NS>(s, df, J.qa(im, jm, km, QGAMC), cdtdx, L);
+  // plus side: cell f, state on its low face
+  load_trace<NS>(J.tr[D], i, j, k, 1, s);
+  dflux<NS, T>(J.f1[T], i, j, k, cdtdx, df);
+  transdo_side<NS>(s, df, J.qa(i, j, k, QGAMC), cdtdx, R);
+  const double cav = 0.5 * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  Flux<NS> F;
+  cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
+  store_flux<NS>(J.ft[2 * D + tslot(D, T)], i, j, k, F);
+}
+
+// transdd flux differences across cell (i,j,k) for final direction D
+template <int NS, int D>
+PCD void final_corr(const Job& J, int i, int j, int k, double hdt,
+                    double c0, double c1, const FaceState<NS>& s,
+                    FaceState<NS>& o) {
+  constexpr int T0 = (D == 0) ? 1 : 0;
+  constexpr int T1 = (D == 2) ? 1 : 2;
+  const V& fx = J.ft[2 * T0 + tslot(T0, T1)];  // F_{T0|T1} on T0-faces
+  const V& fy = J.ft[2 * T1 + tslot(T1, T0)];  // F_{T1|T0} on T1-faces
+  int i0 = i, j0 = j, k0 = k;
+  step<T0>(i0, j0, k0, 1);
+  int i1 = i, j1 = j, k1 = k;
+  step<T1>(i1, j1, k1, 1);
+  double dd[5], dds[NS];
+#pragma unroll
+  for (int m = 0; m < 5; ++m)
+    dd[m] = c0 * (fx(i0, j0, k0, m) - fx(i, j, k, m)) +
+            c1 * (fy(i1, j1, k1, m) - fy(i, j, k, m));
+#pragma unroll
+  for (int s2 = 0; s2 < NS; ++s2)
+    dds[s2] = c0 * (fx(i0, j0, k0, 7 + s2) - fx(i, j, k, 7 + s2)) +
+              c1 * (fy(i1, j1, k1, 7 + s2) - fy(i, j, k, 7 + s2));
+  const double gamc = J.qa(i, j, k, QGAMC);
+  const double pggpx = fx(i0, j0, k0, 5), pggmx = fx(i, j, k, 5);
+  const double ugpx = fx(i0, j0, k0, 6), ugmx = fx(i, j, k, 6);
+  const double dAupx = pggpx * ugpx - pggmx * ugmx;
+  const double pavx = 0.5 * (pggpx + pggmx);
+  const double dAux = ugpx - ugmx;
+  const double pggpy = fy(i1, j1, k1, 5), pggmy = fy(i, j, k, 5);
+  const double ugpy = fy(i1, j1, k1, 6), ugmy = fy(i, j, k, 6);
+  const double dAupy = pggpy * ugpy - pggmy * ugmy;
+  const double pavy = 0.5 * (pggpy + pggmy);
+  const double dAuy = ugpy - ugmy;
+  const double pxnew = c0 * (dAupx + pavx * dAux * (gamc - 1.0));
+  const double pynew = c1 * (dAupy + pavy * dAuy * (gamc - 1.0));
+  double src[6 + NS];
+  if (J.srcq.p) {
+    src[0] = J.srcq(i, j, k, QRHO);
+    src[1] = J.srcq(i, j, k, QU);
+    src[2] = J.srcq(i, j, k, QV);
+    src[3] = J.srcq(i, j, k, QW);
+    src[4] = J.srcq(i, j, k, QREINT);
+    src[5] = J.srcq(i, j, k, QPRES);
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) src[6 + s2] = J.srcq(i, j, k, QFS + s2);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 6 + NS; ++m) src[m] = 0.0;
+  }
+  transdd_side<NS>(s, dd, dds, pxnew, pynew, src, hdt, o);
+}
+
+// PLM state of cell (i,j,k) in direction D (pc_low_order_boundary)
+template <class EOS, int D>
+PCD void plm_cell(const Job& J, const KArgs& ka, double dtdx, int i, int j,
+                  int k, FaceState<EOS::NS>& P, FaceState<EOS::NS>& M) {
+  constexpr int NS = EOS::NS;
+  const V& q = J.q;
+  const double flat = ka.use_flattening ? flat_cell(J, i, j, k) : 1.0;
+  auto slope = [&](int comp) {
+    double s[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      int ii = i, jj = j, kk = k;
+      step<D>(ii, jj, kk, m - 2);
+      s[m] = q(ii, jj, kk, comp);
+    }
+    return plm_slope(s, flat, ka.plm_iorder);
+  };
+  FaceState<NS> c, sl;
+  c.rho = q(i, j, k, QRHO);
+  c.u = q(i, j, k, QU);
+  c.v = q(i, j, k, QV);
+  c.w = q(i, j, k, QW);
+  c.p = q(i, j, k, QPRES);
+  c.rhoe = q(i, j, k, QREINT);
+  sl.rho = slope(QRHO);
+  sl.u = slope(QU);
+  sl.v = slope(QV);
+  sl.w = slope(QW);
+  sl.p = slope(QPRES);
+  sl.rhoe = slope(QREINT);
+  for (int s = 0; s < NS; ++s) {
+    c.Y[s] = q(i, j, k, QFS + s);
+    sl.Y[s] = slope(QFS + s);
+  }
+  plm_trace<NS>(D, c, sl, J.qa(i, j, k, QC), dtdx, P, M);
+}
+
+// ----------------------------- K17-22 + K25-27 final fluxes + adjust ----
+// pc_transdd + pc_cmpflx (Source/Godunov.cpp:392-438), the low-order
+// boundary override (:440-468), then pc_artif_visc, pc_norm_spec_flx and
+// pc_ext_flx (Source/Hydro.cpp:380-413).
+template <class EOS, int D>
+__global__ void __launch_bounds__(BLOCK)
+    k_final(Job j0, Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  constexpr int NVAR = 7 + NS;
+  constexpr int T0 = (D == 0) ? 1 : 0;
+  constexpr int T1 = (D == 2) ? 1 : 2;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const double dt = get_dt(ka);
+  const double hdt = 0.5 * dt;
+  const double c0 = 0.5 * dt / ka.dx[T0];
+  const double c1 = 0.5 * dt / ka.dx[T1];
+  int im = i, jm = j, km = k;
+  step<D>(im, jm, km, -1);
+  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  const bool lowo =
+      (idx == J.dlo[D] && (J.bclo[D] == BC_INFLOW || J.bclo[D] == BC_USER)) ||
+      (idx == J.dhi[D] + 1 && (J.bchi[D] == BC_INFLOW || J.bchi[D] == BC_USER));
+  FaceState<NS> L, R;
+  if (!lowo) {
+    FaceState<NS> s;
+    load_trace<NS>(J.tr[D], i, j, k, 0, s);
+    final_corr<NS, D>(J, im, jm, km, hdt, c0, c1, s, L);
+    load_trace<NS>(J.tr[D], i, j, k, 1, s);
+    final_corr<NS, D>(J, i, j, k, hdt, c0, c1, s, R);
+  } else {
+    const double dtdx = dt / ka.dx[D];
+    FaceState<NS> unused;
+    plm_cell<EOS, D>(J, ka, dtdx, im, jm, km, unused, L);
+    plm_cell<EOS, D>(J, ka, dtdx, i, j, k, R, unused);
+  }
+  const double cav = 0.5 * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  Flux<NS> F;
+  cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
+  // lab-frame NVAR layout (UTEMP flux starts at zero)
+  double fl[NVAR];
+  fl[URHO] = F.f[0];
+  fl[UMX] = F.f[1];
+  fl[UMY] = F.f[2];
+  fl[UMZ] = F.f[3];
+  fl[UEDEN] = F.f[4];
+  fl[UEINT] = F.f[5];
+  fl[UTEMP] = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) fl[UFS + s] = F.f[6 + s];
+  // pc_artif_visc (Source/Godunov.H:1067-1138)
+  {
+    const V& dv = J.divu;
+    int a0 = i, b0 = j, e0 = k;
+    step<T0>(a0, b0, e0, 1);
+    int a1 = i, b1 = j, e1 = k;
+    step<T1>(a1, b1, e1, 1);
+    int a2 = a0, b2 = b0, e2 = e0;
+    step<T1>(a2, b2, e2, 1);
+    const double div =
+        ka.difmag * mn(0.0, 0.25 * (dv(i, j, k, 0) + dv(a0, b0, e0, 0) +
+                                    dv(a1, b1, e1, 0) + dv(a2, b2, e2, 0)));
+    const bool at_bndry = (idx == J.dlo[D] && J.bclo[D] == BC_NOSLIPWALL) ||
+                          (idx == J.dhi[D] + 1 && J.bchi[D] == BC_NOSLIPWALL);
+    const double dxd = ka.dx[D];
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m) {
+      if (m == UTEMP) continue;
+      const bool tang = (m == UMX + T0) || (m == UMX + T1);
+      if (!(at_bndry && tang))
+        fl[m] += dxd * div * (J.U(i, j, k, m) - J.U(im, jm, km, m));
+    }
+    fl[UTEMP] = 0.0;
+  }
+  // pc_norm_spec_flx (Source/Hydro.H:835-849)
+  {
+    double summ = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) summ += fl[UFS + s];
+    const double fac = (summ != 0.0) ? fl[URHO] / summ : 1.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) fl[UFS + s] *= fac;
+  }
+  // pc_ext_flx (Source/Hydro.H:854-865)
+  const double area = J.area[D].p ? J.area[D](i, j, k, 0) : ka.area_u[D];
+  const V& fo = J.flx[D];
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m) fo(i, j, k, m) = fl[m] * area;
+  J.qec[D](i, j, k, 0) = F.pgd;
+  J.qec[D](i, j, k, 1) = F.ugd;
+}
+
+// pdivu (Source/Godunov.H:1033-1060) and the flux divergence of one cell
+// (Source/Hydro.H:888-907); h receives dU/dt.
+template <int NS>
+PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
+                         double* h) {
+  constexpr int NVAR = 7 + NS;
+  auto A = [&](int d, int a, int b, int c) {
+    return J.area[d].p ? J.area[d](a, b, c, 0) : ka.area_u[d];
+  };
+  const double v = J.vol.p ? J.vol(i, j, k, 0) : ka.vol_u;
+  const V& q1 = J.qec[0];
+  const V& q2 = J.qec[1];
+  const V& q3 = J.qec[2];
+  const double pdivu =
+      0.5 *
+      ((q1(i + 1, j, k, 0) + q1(i, j, k, 0)) *
+           (q1(i + 1, j, k, 1) * A(0, i + 1, j, k) - q1(i, j, k, 1) * A(0, i, j, k)) +
+       (q2(i, j + 1, k, 0) + q2(i, j, k, 0)) *
+           (q2(i, j + 1, k, 1) * A(1, i, j + 1, k) - q2(i, j, k, 1) * A(1, i, j, k)) +
+       (q3(i, j, k + 1, 0) + q3(i, j, k, 0)) *
+           (q3(i, j, k + 1, 1) * A(2, i, j, k + 1) - q3(i, j, k, 1) * A(2, i, j, k))) /
+      v;
+  const V& f0 = J.flx[0];
+  const V& f1 = J.flx[1];
+  const V& f2 = J.flx[2];
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m)
+    h[m] += (f0(i, j, k, m) - f0(i + 1, j, k, m) + f1(i, j, k, m) -
+             f1(i, j + 1, k, m) + f2(i, j, k, m) - f2(i, j, k + 1, m)) /
+            v;
+  h[UEINT] -= pdivu;
+}
+
+// ----------------------------------------------------------- K28 consup --
+template <int NS>
+__global__ void __launch_bounds__(BLOCK)
+    k_consup(Job j0, Sched sc, KArgs ka) {
+  constexpr int NVAR = 7 + NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  double h[NVAR];
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m) h[m] = J.upd(i, j, k, m);
+  cell_divergence<NS>(J, ka, i, j, k, h);
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m) J.upd(i, j, k, m) = h[m];
+}
+
+}  // namespace pcd

@@ -1,0 +1,267 @@
+// Level-path kernels: ghost fill over a box table, the fused cell update
+// (flux divergence + construct_Snew + reset_internal_energy + computeTemp +
+// estTimeStep partial minimum), the device-side dt finalisation and the
+// primitive -> conservative initialiser.
+#pragma once
+#include "pc_kernels.h"
+
+namespace pcd {
+
+// Device-resident time-step state.
+struct DtState {
+  double dt;                 // dt of the next step
+  unsigned long long minbits;  // running min of dx/(c+|u|) (positive double bits)
+  int bad;                   // NaN/Inf seen in the new state
+  int small;                 // dt fell below dt_cutoff
+};
+
+struct LevelArgs {
+  int periodic_self;
+  int dlo[3], dhi[3], bclo[3], bchi[3];
+  const int* cellmap;  // domain cell -> box (domain mode)
+  double cfl, max_dt, fixed_dt, change_max, init_shrink, dt_cutoff;
+};
+
+// Source index of ghost coordinate x (Source/Setup.cpp:42-121 BC map:
+// Interior -> periodic, Outflow/Inflow/UserBC -> first-order extrapolation,
+// Symmetry/SlipWall -> reflect (normal momentum odd), NoSlipWall -> reflect
+// (all momenta odd)).  refl: 0 none, 1 normal-odd, 2 all-momenta-odd.
+PCD int ghost_src(int x, int lo, int hi, int bcl, int bch, int& refl) {
+  refl = 0;
+  if (x >= lo && x <= hi) return x;
+  const int L = hi - lo + 1;
+  const int bc = (x < lo) ? bcl : bch;
+  if (bc == BC_INTERIOR) return lo + (((x - lo) % L) + L) % L;
+  if (bc == BC_SYMMETRY || bc == BC_SLIPWALL || bc == BC_NOSLIPWALL) {
+    refl = (bc == BC_NOSLIPWALL) ? 2 : 1;
+    return (x < lo) ? 2 * lo - 1 - x : 2 * hi + 1 - x;
+  }
+  return (x < lo) ? lo : hi;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(BLOCK)
+    k_ghost(Job j0, Sched sc, KArgs ka, LevelArgs la) {
+  constexpr int NVAR = 7 + NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_GHOST, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (i >= J.lo[0] && i <= J.hi[0] && j >= J.lo[1] && j <= J.hi[1] &&
+      k >= J.lo[2] && k <= J.hi[2])
+    return;
+  int D0[3], D1[3], bl[3], bh[3];
+  for (int d = 0; d < 3; ++d) {
+    D0[d] = la.periodic_self ? J.lo[d] : la.dlo[d];
+    D1[d] = la.periodic_self ? J.hi[d] : la.dhi[d];
+    bl[d] = la.periodic_self ? BC_INTERIOR : la.bclo[d];
+    bh[d] = la.periodic_self ? BC_INTERIOR : la.bchi[d];
+  }
+  int r0, r1, r2;
+  const int s0 = ghost_src(i, D0[0], D1[0], bl[0], bh[0], r0);
+  const int s1 = ghost_src(j, D0[1], D1[1], bl[1], bh[1], r1);
+  const int s2 = ghost_src(k, D0[2], D1[2], bl[2], bh[2], r2);
+  const V* src = &J.U;
+  if (!la.periodic_self) {
+    const long nx = D1[0] - D0[0] + 1, ny = D1[1] - D0[1] + 1;
+    const int sb = la.cellmap[(s0 - D0[0]) + nx * ((s1 - D0[1]) + ny * (long)(s2 - D0[2]))];
+    src = &sc.jobs[sb].U;
+  }
+  const int r[3] = {r0, r1, r2};
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m) {
+    double sg = 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const bool mom = (m == UMX || m == UMY || m == UMZ);
+      if ((r[d] == 2 && mom) || (r[d] == 1 && m == UMX + d)) sg = -sg;
+    }
+    J.U(i, j, k, m) = sg * (*src)(s0, s1, s2, m);
+  }
+}
+
+PCD double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = mn(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// block-wide minimum of positive doubles -> atomic minimum on the bits
+PCD void block_min_to(double v, unsigned long long* dst) {
+  __shared__ double red[BLOCK / 64];
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < BLOCK / 64; ++w) m = mn(m, red[w]);
+    if (m < 1.0e300) atomicMin(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// pc_estdt_hydro for one cell (Source/Timestep.H:62-84)
+template <class EOS>
+PCD double cell_estdt(const V& u, int i, int j, int k, const double* dx) {
+  constexpr int NS = EOS::NS;
+  const double rho = u(i, j, k, URHO);
+  const double rhoInv = 1.0 / rho;
+  const double T = u(i, j, k, UTEMP);
+  double Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Y[s] = u(i, j, k, UFS + s) * rhoInv;
+  const double c = EOS::RTY2Cs(rho, T, Y);
+  double dt = 1.0e308 * 1.79;  // > any finite candidate; min() replaces it
+  dt = 1.7976931348623157e308;
+  const double ux = u(i, j, k, UMX) * rhoInv;
+  dt = mn(dt, dx[0] / (c + fabs(ux)));
+  const double uy = u(i, j, k, UMY) * rhoInv;
+  dt = mn(dt, dx[1] / (c + fabs(uy)));
+  const double uz = u(i, j, k, UMZ) * rhoInv;
+  dt = mn(dt, dx[2] / (c + fabs(uz)));
+  return dt;
+}
+
+// reset_internal_energy with allow_small_energy = allow_negative_energy = 1
+// (Source/PeleCUtilities.H:227-236) and pc_cmpTemp (:70-84).
+template <class EOS>
+PCD void reset_and_temp(double* S) {
+  constexpr int NS = EOS::NS;
+  {
+    const double rho = S[URHO];
+    const double rhoInv = 1.0 / rho;
+    const double Up = S[UMX] * rhoInv;
+    const double Vp = S[UMY] * rhoInv;
+    const double Wp = S[UMZ] * rhoInv;
+    const double ke =
+        0.5 * (Up * Up + Vp * Vp + Wp * Wp) - 0.5 * 0.0 * 0.0 * 0.0 * 0.0;
+    S[UEINT] = S[UEDEN] - rho * ke;
+  }
+  const double rhoInv = 1.0 / S[URHO];
+  const double e = S[UEINT] * rhoInv;
+  double Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Y[s] = S[UFS + s] * rhoInv;
+  S[UTEMP] = EOS::EY2T(e, Y, S[UTEMP]);
+}
+
+// Fused per-cell step end: dU/dt from the fluxes, S_new = S_old + dt dU/dt
+// (construct_Snew, Source/Advance.cpp:397-416), reset + e->T twice
+// (computeTemp in do_sdc_iteration and in post_timestep), NaN check and the
+// CFL partial minimum on the new state.
+template <class EOS>
+__global__ void __launch_bounds__(BLOCK)
+    k_update(Job j0, Sched sc, KArgs ka, DtState* ds) {
+  constexpr int NS = EOS::NS;
+  constexpr int NVAR = 7 + NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
+  double mydt = 1.7976931348623157e308;
+  if (region_ijk(lo, n, local, i, j, k)) {
+    const double dt = get_dt(ka);
+    double h[NVAR];
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m) h[m] = 0.0;
+    cell_divergence<NS>(J, ka, i, j, k, h);
+    double S[NVAR];
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m) S[m] = J.U(i, j, k, m) + dt * h[m];
+    reset_and_temp<EOS>(S);
+    reset_and_temp<EOS>(S);
+    bool bad = false;
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m) {
+      J.U(i, j, k, m) = S[m];
+      bad |= !isfinite(S[m]);
+    }
+    if (bad) ds->bad = 1;
+    mydt = cell_estdt<EOS>(J.U, i, j, k, ka.dx);
+  }
+  block_min_to(mydt, &ds->minbits);
+}
+
+// estTimeStep partial minimum over the valid cells (no update)
+template <class EOS>
+__global__ void __launch_bounds__(BLOCK)
+    k_estdt(Job j0, Sched sc, KArgs ka, DtState* ds) {
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
+  double mydt = 1.7976931348623157e308;
+  if (region_ijk(lo, n, local, i, j, k)) mydt = cell_estdt<EOS>(J.U, i, j, k, ka.dx);
+  block_min_to(mydt, &ds->minbits);
+}
+
+// estTimeStep (Source/PeleC.cpp:829-961) from the partial minimum, then
+// computeNewDt (:964-1034, single level, no stop_time) or, with init = 1,
+// initialTimeStep (:811-826).
+__global__ void k_dt_finalize(DtState* ds, LevelArgs la, int init) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double est;
+  if (la.fixed_dt > 0.0) {
+    est = la.fixed_dt;
+  } else {
+    est = la.max_dt;
+    double eh = la.max_dt / la.cfl;
+    const double dtc = __longlong_as_double((long long)ds->minbits);
+    eh = mn(eh, dtc);
+    eh *= la.cfl;
+    if (eh < est) est = eh;
+  }
+  double dtn;
+  if (init) {
+    dtn = la.init_shrink * est;
+  } else {
+    dtn = est;
+    if (la.fixed_dt <= 0.0) dtn = mn(dtn, la.change_max * ds->dt);
+  }
+  ds->dt = dtn;
+  if (dtn < la.dt_cutoff) ds->small = 1;
+  ds->minbits = 0x7FEFFFFFFFFFFFFFull;
+}
+
+// primitive -> conservative (pc_initdata pattern; see workloads.py)
+template <class EOS>
+__global__ void __launch_bounds__(BLOCK)
+    k_prim_to_cons(Job j0, Sched sc, KArgs ka, V prim, int mode) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const Job& J = job_of(j0, sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  double Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Y[s] = prim(i, j, k, 5 + s);
+  const double u = prim(i, j, k, 2), v = prim(i, j, k, 3), w = prim(i, j, k, 4);
+  double rho, e, T;
+  if (mode == 0) {
+    rho = prim(i, j, k, 0);
+    const double p = prim(i, j, k, 1);
+    e = EOS::RYP2E(rho, Y, p);
+    T = EOS::EY2T(e, Y, 1.0);
+  } else {
+    const double p = prim(i, j, k, 0);
+    T = prim(i, j, k, 1);
+    const double wb = EOS::wbar(Y);
+    rho = p * wb / (RU * T);
+    e = EOS::TY2E(T, Y);
+  }
+  const double rhoe = rho * e;
+  const V& U = J.U;
+  U(i, j, k, URHO) = rho;
+  U(i, j, k, UMX) = rho * u;
+  U(i, j, k, UMY) = rho * v;
+  U(i, j, k, UMZ) = rho * w;
+  U(i, j, k, UEINT) = rhoe;
+  U(i, j, k, UEDEN) = rhoe + 0.5 * rho * (u * u + v * v + w * w);
+  U(i, j, k, UTEMP) = T;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) U(i, j, k, UFS + s) = rho * Y[s];
+}
+
+}  // namespace pcd
