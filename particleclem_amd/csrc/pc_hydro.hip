@@ -307,9 +307,11 @@ int pc_ctx_create(int eos_kind, pc_ctx_t** out) {
   pc_ctx* c = new pc_ctx();
   c->eos = eos_kind;
   c->ns = eos_kind;
+  // no device is needed to create a context or validate arguments; the
+  // first launch reports a missing device as PC_ERR_HIP
   if (hipGetDevice(&c->device) != hipSuccess) {
-    delete c;
-    return PC_ERR_HIP;
+    (void)hipGetLastError();
+    c->device = -1;
   }
   *out = c;
   return PC_OK;

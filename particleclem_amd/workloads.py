@@ -88,7 +88,7 @@ def _box_centres(lo, hi, dx, problo=(0.0, 0.0, 0.0)):
 
 
 def sod(n=64, seed=DEFAULT_SEED, nsteps=10, cfl=0.3, max_grid=None,
-        ny=None, nz=None, idir=0, xbc="outflow"):
+        ny=None, nz=None, idir=0, xbc="outflow", pert=1e-3, mirror=False):
     """3D Sod shock tube, gamma-law gas, 1 species.  Left (x <= 0.5):
     rho=1, p=1; right: rho=0.125, p=0.1; u=v=w=0; rho carries a seeded
     uniform +-1e-3 relative perturbation.  PPM, CFL 0.3, periodic in the
@@ -110,15 +110,16 @@ def sod(n=64, seed=DEFAULT_SEED, nsteps=10, cfl=0.3, max_grid=None,
                                min(j0 + mg, ncell[1]) - 1,
                                min(k0 + mg, ncell[2]) - 1]))
     rng = np.random.Generator(np.random.Philox(seed))
+    pert_amp = pert
     pert = rng.uniform(-1.0, 1.0, size=(ncell[2], ncell[1], ncell[0]))
     prims = []
     for lo, hi in boxes:
         xs = _box_centres(lo, hi, dx)
         X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")[2 - idir]
-        left = X <= 0.5
+        left = (X >= 0.5) if mirror else (X <= 0.5)
         sl = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1),
               slice(lo[0], hi[0] + 1))
-        rho = np.where(left, 1.0, 0.125) * (1.0 + 1e-3 * pert[sl])
+        rho = np.where(left, 1.0, 0.125) * (1.0 + pert_amp * pert[sl])
         p = np.where(left, 1.0, 0.1)
         P = np.zeros((6,) + rho.shape)
         P[0] = rho
