@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Godunov hydro step throughput on one MI355X.
+
+Default workload: premixed H2/air, 9 species (LiDryer ideal-gas mixture),
+hydro only, 128^3 cells as 64 boxes of 32^3 with 4 ghost cells, PPM, the
+reference's approximate Riemann solver, CFL 0.3 (the BASELINE's FP64
+premixed configuration).  One "step" = ghost fill + the full hydro step
+(ctoprim -> CTU fluxes -> flux divergence -> S_new -> internal-energy reset
+and e->T) + the next dt, for every box of the level, on the GPU.
+
+Prints ONE JSON line.  `value` = valid cells x timed steps / seconds / 1e6
+(Mcell-updates/s) with the state resident in HBM.  `roofline` is the
+dominant kernel against HBM bandwidth, from its stage-compulsory bytes and
+its HIP-event duration measured here; `hbm_frac_step` is the whole step
+against HBM on the reference's algorithmic bytes (2 * NVAR * 8 per
+cell-update).  `cpu_baseline` is the CPU oracle (this repository's C++
+restatement of the reference path, OpenMP over boxes) on one step of the
+same level, timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
+METRIC = ("Mcell-updates/s per hydro step + % HBM roofline; "
+          "p99 ensemble-solve latency")
+
+
+def stage_bytes(nspec, boxes):
+    """Compulsory bytes of one launch of each kernel over the level: every
+    input element read once and every output element written once on the
+    kernel's region (stencil re-reads are not counted)."""
+    Q, N, QT, FT = 8 + nspec, 7 + nspec, 6 + nspec, 7 + nspec
+    tot = {}
+
+    def add(k, units, per):
+        tot[k] = tot.get(k, 0) + units * per * 8
+
+    for lo, hi in boxes:
+        n = [hi[d] - lo[d] + 1 for d in range(3)]
+        g4 = (n[0] + 8) * (n[1] + 8) * (n[2] + 8)
+        v = n[0] * n[1] * n[2]
+        g1 = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+        add("ghost", g4 - v, 2 * N)
+        add("ctoprim", g4, N + Q + 6)
+        add("divu", (n[0] + 1) * (n[1] + 1) * (n[2] + 1), 4)
+        for d, a in enumerate("xyz"):
+            t = [x + 2 for x in n]
+            add("trace_" + a, g1, QT + 2 * QT)
+            f1 = list(t)
+            f1[d] = n[d] + 1
+            add("flux1_" + a, f1[0] * f1[1] * f1[2], 2 * QT + 1 + FT)
+            fin = list(n)
+            fin[d] = n[d] + 1
+            add("final_" + a, fin[0] * fin[1] * fin[2], 2 * QT + 2 * FT + 2 * N + 5)
+        for D, T, name in [(0, 1, "xy"), (0, 2, "xz"), (1, 0, "yx"),
+                           (1, 2, "yz"), (2, 0, "zx"), (2, 1, "zy")]:
+            r = list(n)
+            r[D] = n[D] + 1
+            r[3 - D - T] = n[3 - D - T] + 2
+            add("trans_" + name, r[0] * r[1] * r[2], 2 * QT + 2 * FT + 2)
+        add("update", v, 6 * N + 6)
+        add("dt", 0, 0)
+    return tot
+
+
+def cpu_baseline(w, gpu_fabs, dt, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    H = ob.HostLevel(w.nspec, w.boxes, w.domlo, w.domhi, w.bclo, w.bchi, w.dx,
+                     periodic_self=w.periodic_self)
+    for b, f in enumerate(gpu_fabs):
+        H.fab(b)[...] = f
+    prm = ob.params(**w.params)
+    t0 = time.perf_counter()
+    H.step(prm, steps, dt, threads)
+    t1 = time.perf_counter()
+    return w.valid_cells() * steps / (t1 - t0) / 1e6, t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="premixed", choices=["premixed", "sod"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--box", type=int, default=32)
+    ap.add_argument("--kernel-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the hydro path on a "
+              "single MI355X (multi-GPU sharding is design-only); use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from particleclem_amd import workloads
+    from particleclem_amd.hydro import DeviceLevel, HydroContext
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+
+    if args.workload == "premixed":
+        n = args.n or 128
+        w = workloads.premixed(n=n, box=args.box, nsteps=args.steps)
+        desc = (f"premixed H2/air, 9 species (LiDryer ideal-gas mixture), hydro "
+                f"only, {n}^3 as {len(w.boxes)} boxes of {args.box}^3 + 4 ghosts, "
+                f"PPM, approximate Riemann solver, CFL 0.3, x/y periodic, "
+                f"z outflow")
+    else:
+        n = args.n or 64
+        w = workloads.sod(n=n, nsteps=args.steps)
+        desc = (f"3D Sod, gamma-law gas, {n}^3 single box + 4 ghosts, PPM, "
+                f"approximate Riemann solver, CFL 0.3")
+
+    ctx = HydroContext(w.nspec)
+    lev = DeviceLevel.from_workload(ctx, w)
+    lev.init_dt()
+    torch.cuda.synchronize()
+    cells = w.valid_cells()
+
+    # warm-up (JIT-free: kernels are precompiled; this settles clocks/caches)
+    lev.step(args.warmup)
+    torch.cuda.synchronize()
+    fabs0 = None
+    dt_cpu = None
+    if not args.no_cpu_baseline:
+        fabs0 = [lev.read_box(b) for b in range(len(w.boxes))]
+        dt_cpu = lev.get_dt()
+
+    # timed region: K full steps, state resident in HBM
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lev.step(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rc, bad, small = lev.status()
+    if rc != 0:
+        print(f"state check failed: status {rc} (nan={bad}, small_dt={small})",
+              file=sys.stderr)
+        sys.exit(1)
+    sec = t1 - t0
+    ms_step = sec / args.steps * 1e3
+    value = cells * args.steps / sec / 1e6
+
+    # per-kernel device times (events around every launch, separate steps)
+    lev.enable_timing(True)
+    lev.step(args.kernel_steps)
+    torch.cuda.synchronize()
+    kt = lev.kernel_times()
+    lev.enable_timing(False)
+    kbytes = stage_bytes(w.nspec, w.boxes)
+    dom = max(kt, key=lambda k: kt[k])
+    dom_s = kt[dom] * 1e-3
+    achieved = kbytes[dom] / dom_s
+    roof = {"bound": "hbm", "kernel": f"k_{dom.split('_')[0]} ({dom})",
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "bytes_per_launch": kbytes[dom], "ms_per_launch": kt[dom]}
+    nvar = w.nvar
+    step_bytes = cells * 2 * nvar * 8
+    hbm_frac_step = step_bytes / (ms_step * 1e-3) / HBM_PEAK
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+        v, s = cpu_baseline(w, fabs0, dt_cpu, args.cpu_steps, threads)
+        cpu = {"value": v, "unit": "Mcell-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} full step(s) of the same level "
+                         f"({cells} cells) from the GPU run's state, "
+                         f"{s:.1f} s, OpenMP over boxes"}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc, "valid_cells": cells,
+                   "boxes": len(w.boxes), "nspec": w.nspec,
+                   "parallelism": "single GPU, all boxes per launch"},
+        "roofline": roof,
+        "hbm_frac_step": hbm_frac_step,
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    lev.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

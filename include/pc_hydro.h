@@ -204,12 +204,21 @@ int pc_level_fill_ghosts_v1(pc_level_t* lev, void* stream);
 /* estTimeStep over the valid cells of the level: cfl*min(dx/(c+|u|)),
  * limited by max_dt (Source/PeleC.cpp:829-961).  Synchronous. */
 int pc_level_estdt_v1(pc_level_t* lev, double* dt_out, void* stream);
-/* Device time (ms) of the kernels of the last pc_level_step_v1 call, by
- * stage, measured with HIP events on `stream`: [ghost, ctoprim, trace,
- * first flux, transverse, final flux+adjust, update+temp+dt].  Synchronous.
- * Requires pc_level_enable_timing_v1(lev, 1) before the step. */
+/* Per-kernel device time: with timing on, every launch of a step is
+ * bracketed by HIP events on the step's stream and the step waits for its
+ * last event (so timed steps do not overlap the host).  Kernel ids, in
+ * launch order: */
+enum {
+  PC_K_GHOST = 0, PC_K_CTOPRIM, PC_K_DIVU, PC_K_TRACE_X, PC_K_TRACE_Y,
+  PC_K_TRACE_Z, PC_K_FLUX1_X, PC_K_FLUX1_Y, PC_K_FLUX1_Z, PC_K_TRANS_XY,
+  PC_K_TRANS_XZ, PC_K_TRANS_YX, PC_K_TRANS_YZ, PC_K_TRANS_ZX, PC_K_TRANS_ZY,
+  PC_K_FINAL_X, PC_K_FINAL_Y, PC_K_FINAL_Z, PC_K_UPDATE, PC_K_DT,
+  PC_NKERN
+};
+/* on = 1 also resets the accumulated times. */
 int pc_level_enable_timing_v1(pc_level_t* lev, int on);
-int pc_level_stage_times_v1(pc_level_t* lev, double* ms7);
+/* Average ms per timed step of each kernel id (n <= PC_NKERN entries). */
+int pc_level_kernel_times_v1(pc_level_t* lev, double* ms, int n);
 /* okToContinue (Source/PeleC.cpp:1310-1340): synchronous; returns
  * PC_ERR_NAN if any step produced NaN/Inf, PC_ERR_DT if dt < dt_cutoff. */
 int pc_level_status_v1(pc_level_t* lev, int* bad, int* small_dt);

@@ -21,11 +21,15 @@ EXPORTS = [
     "pc_level_state_size_v1", "pc_level_write_box_v1", "pc_level_read_box_v1",
     "pc_level_set_dt_v1", "pc_level_get_dt_v1", "pc_level_init_dt_v1",
     "pc_level_step_v1", "pc_level_fill_ghosts_v1", "pc_level_estdt_v1",
-    "pc_level_enable_timing_v1", "pc_level_stage_times_v1",
+    "pc_level_enable_timing_v1", "pc_level_kernel_times_v1",
     "pc_level_status_v1", "pc_level_prim_to_cons_v1",
 ]
 
 PC_OK = 0
+KERNELS = ["ghost", "ctoprim", "divu", "trace_x", "trace_y", "trace_z",
+           "flux1_x", "flux1_y", "flux1_z", "trans_xy", "trans_xz",
+           "trans_yx", "trans_yz", "trans_zx", "trans_zy", "final_x",
+           "final_y", "final_z", "update", "dt"]
 ERRORS = {-1: "PC_ERR_ARG", -2: "PC_ERR_GROW", -3: "PC_ERR_PPM_TYPE",
           -4: "PC_ERR_PLM_IORDER", -5: "PC_ERR_HIP", -6: "PC_ERR_NAN",
           -7: "PC_ERR_UNSUPPORTED", -8: "PC_ERR_NOMEM", -9: "PC_ERR_DT"}
@@ -90,7 +94,7 @@ def lib():
     L.pc_level_fill_ghosts_v1.argtypes = [vp, vp]
     L.pc_level_estdt_v1.argtypes = [vp, d3, vp]
     L.pc_level_enable_timing_v1.argtypes = [vp, C.c_int]
-    L.pc_level_stage_times_v1.argtypes = [vp, d3]
+    L.pc_level_kernel_times_v1.argtypes = [vp, d3, C.c_int]
     L.pc_level_status_v1.argtypes = [vp, P(C.c_int), P(C.c_int)]
     L.pc_level_prim_to_cons_v1.argtypes = [vp, C.c_int, Array4, C.c_int, vp]
     _lib = L

@@ -554,9 +554,9 @@ struct pc_level {
   LevelArgs la;
   KArgs ka;
   int timing = 0;
-  hipEvent_t ev[8];
+  hipEvent_t ev[PC_NKERN + 1];
   bool ev_made = false;
-  double times[7];
+  double ktime[PC_NKERN];
   int nsteps_timed = 0;
 };
 
@@ -571,7 +571,7 @@ static void level_free(pc_level* L) {
     if (L->plans[s].blkstart) (void)hipFree(L->plans[s].blkstart);
   }
   if (L->ev_made)
-    for (int e = 0; e < 8; ++e) (void)hipEventDestroy(L->ev[e]);
+    for (int e = 0; e <= PC_NKERN; ++e) (void)hipEventDestroy(L->ev[e]);
   delete L;
 }
 
@@ -862,71 +862,94 @@ int pc_level_estdt_v1(pc_level_t* L, double* dt_out, void* stream) {
 int pc_level_enable_timing_v1(pc_level_t* L, int on) {
   if (!L) return PC_ERR_ARG;
   if (on && !L->ev_made) {
-    for (int e = 0; e < 8; ++e) HIPCHK(L->ctx, hipEventCreate(&L->ev[e]));
+    for (int e = 0; e <= PC_NKERN; ++e) HIPCHK(L->ctx, hipEventCreate(&L->ev[e]));
     L->ev_made = true;
   }
   L->timing = on;
-  for (int s = 0; s < 7; ++s) L->times[s] = 0.0;
+  for (int s = 0; s < PC_NKERN; ++s) L->ktime[s] = 0.0;
   L->nsteps_timed = 0;
   return PC_OK;
 }
 
 }  // extern "C"
 
+// One Godunov step of the whole level: 21 launches, each covering every box.
+// With timing on, an event follows every launch and the step ends with a
+// host wait, so per-kernel device times accumulate in ktime[].
 template <class EOS>
 static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
-  constexpr int NS = EOS::NS;
   Job d0;
   memset(&d0, 0, sizeof(d0));
   const int ppm = L->prm.ppm_type;
+  const Sched* S = L->scs;
+  const StagePlan* P = L->plans;
+  const KArgs& ka = L->ka;
   for (int s = 0; s < nsteps; ++s) {
     const bool tm = L->timing;
-    if (tm) (void)hipEventRecord(L->ev[0], st);
+    int ne = 0;
+    auto mark = [&]() {
+      if (tm) (void)hipEventRecord(L->ev[ne++], st);
+    };
+    mark();
     level_fill(L, st);
-    if (tm) (void)hipEventRecord(L->ev[1], st);
-    LAUNCH(k_ctoprim<EOS>, L->plans[ST_CTOPRIM].nblocks, st, d0, L->scs[ST_CTOPRIM], L->ka);
-    if (tm) (void)hipEventRecord(L->ev[2], st);
-    LAUNCH(k_divu, L->plans[ST_DIVU].nblocks, st, d0, L->scs[ST_DIVU], L->ka);
-    const Sched* S = L->scs;
-    const int gT = L->plans[ST_TRACE].nblocks;
+    mark();  // PC_K_GHOST
+    LAUNCH(k_ctoprim<EOS>, P[ST_CTOPRIM].nblocks, st, d0, S[ST_CTOPRIM], ka);
+    mark();
+    LAUNCH(k_divu, P[ST_DIVU].nblocks, st, d0, S[ST_DIVU], ka);
+    mark();
+    const int gT = P[ST_TRACE].nblocks;
     if (ppm == 1) {
-      LAUNCH((k_trace_ppm<EOS, 0>), gT, st, d0, S[ST_TRACE], L->ka);
-      LAUNCH((k_trace_ppm<EOS, 1>), gT, st, d0, S[ST_TRACE], L->ka);
-      LAUNCH((k_trace_ppm<EOS, 2>), gT, st, d0, S[ST_TRACE], L->ka);
+      LAUNCH((k_trace_ppm<EOS, 0>), gT, st, d0, S[ST_TRACE], ka);
+      mark();
+      LAUNCH((k_trace_ppm<EOS, 1>), gT, st, d0, S[ST_TRACE], ka);
+      mark();
+      LAUNCH((k_trace_ppm<EOS, 2>), gT, st, d0, S[ST_TRACE], ka);
+      mark();
     } else {
-      LAUNCH((k_trace_plm<EOS, 0>), gT, st, d0, S[ST_TRACE], L->ka);
-      LAUNCH((k_trace_plm<EOS, 1>), gT, st, d0, S[ST_TRACE], L->ka);
-      LAUNCH((k_trace_plm<EOS, 2>), gT, st, d0, S[ST_TRACE], L->ka);
+      LAUNCH((k_trace_plm<EOS, 0>), gT, st, d0, S[ST_TRACE], ka);
+      mark();
+      LAUNCH((k_trace_plm<EOS, 1>), gT, st, d0, S[ST_TRACE], ka);
+      mark();
+      LAUNCH((k_trace_plm<EOS, 2>), gT, st, d0, S[ST_TRACE], ka);
+      mark();
     }
-    if (tm) (void)hipEventRecord(L->ev[3], st);
-    LAUNCH((k_flux1<EOS, 0>), L->plans[ST_FLUX1_X].nblocks, st, d0, S[ST_FLUX1_X], L->ka);
-    LAUNCH((k_flux1<EOS, 1>), L->plans[ST_FLUX1_Y].nblocks, st, d0, S[ST_FLUX1_Y], L->ka);
-    LAUNCH((k_flux1<EOS, 2>), L->plans[ST_FLUX1_Z].nblocks, st, d0, S[ST_FLUX1_Z], L->ka);
-    if (tm) (void)hipEventRecord(L->ev[4], st);
-    LAUNCH((k_trans<EOS, 0, 1>), L->plans[ST_TRANS_XY].nblocks, st, d0, S[ST_TRANS_XY], L->ka);
-    LAUNCH((k_trans<EOS, 0, 2>), L->plans[ST_TRANS_XZ].nblocks, st, d0, S[ST_TRANS_XZ], L->ka);
-    LAUNCH((k_trans<EOS, 1, 0>), L->plans[ST_TRANS_YX].nblocks, st, d0, S[ST_TRANS_YX], L->ka);
-    LAUNCH((k_trans<EOS, 1, 2>), L->plans[ST_TRANS_YZ].nblocks, st, d0, S[ST_TRANS_YZ], L->ka);
-    LAUNCH((k_trans<EOS, 2, 0>), L->plans[ST_TRANS_ZX].nblocks, st, d0, S[ST_TRANS_ZX], L->ka);
-    LAUNCH((k_trans<EOS, 2, 1>), L->plans[ST_TRANS_ZY].nblocks, st, d0, S[ST_TRANS_ZY], L->ka);
-    if (tm) (void)hipEventRecord(L->ev[5], st);
-    LAUNCH((k_final<EOS, 0>), L->plans[ST_FINAL_X].nblocks, st, d0, S[ST_FINAL_X], L->ka);
-    LAUNCH((k_final<EOS, 1>), L->plans[ST_FINAL_Y].nblocks, st, d0, S[ST_FINAL_Y], L->ka);
-    LAUNCH((k_final<EOS, 2>), L->plans[ST_FINAL_Z].nblocks, st, d0, S[ST_FINAL_Z], L->ka);
-    if (tm) (void)hipEventRecord(L->ev[6], st);
-    LAUNCH(k_update<EOS>, L->plans[ST_UPDATE].nblocks, st, d0, S[ST_UPDATE], L->ka, L->ds);
+    LAUNCH((k_flux1<EOS, 0>), P[ST_FLUX1_X].nblocks, st, d0, S[ST_FLUX1_X], ka);
+    mark();
+    LAUNCH((k_flux1<EOS, 1>), P[ST_FLUX1_Y].nblocks, st, d0, S[ST_FLUX1_Y], ka);
+    mark();
+    LAUNCH((k_flux1<EOS, 2>), P[ST_FLUX1_Z].nblocks, st, d0, S[ST_FLUX1_Z], ka);
+    mark();
+    LAUNCH((k_trans<EOS, 0, 1>), P[ST_TRANS_XY].nblocks, st, d0, S[ST_TRANS_XY], ka);
+    mark();
+    LAUNCH((k_trans<EOS, 0, 2>), P[ST_TRANS_XZ].nblocks, st, d0, S[ST_TRANS_XZ], ka);
+    mark();
+    LAUNCH((k_trans<EOS, 1, 0>), P[ST_TRANS_YX].nblocks, st, d0, S[ST_TRANS_YX], ka);
+    mark();
+    LAUNCH((k_trans<EOS, 1, 2>), P[ST_TRANS_YZ].nblocks, st, d0, S[ST_TRANS_YZ], ka);
+    mark();
+    LAUNCH((k_trans<EOS, 2, 0>), P[ST_TRANS_ZX].nblocks, st, d0, S[ST_TRANS_ZX], ka);
+    mark();
+    LAUNCH((k_trans<EOS, 2, 1>), P[ST_TRANS_ZY].nblocks, st, d0, S[ST_TRANS_ZY], ka);
+    mark();
+    LAUNCH((k_final<EOS, 0>), P[ST_FINAL_X].nblocks, st, d0, S[ST_FINAL_X], ka);
+    mark();
+    LAUNCH((k_final<EOS, 1>), P[ST_FINAL_Y].nblocks, st, d0, S[ST_FINAL_Y], ka);
+    mark();
+    LAUNCH((k_final<EOS, 2>), P[ST_FINAL_Z].nblocks, st, d0, S[ST_FINAL_Z], ka);
+    mark();
+    LAUNCH(k_update<EOS>, P[ST_UPDATE].nblocks, st, d0, S[ST_UPDATE], ka, L->ds);
+    mark();
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 0);
+    mark();
     if (tm) {
-      (void)hipEventRecord(L->ev[7], st);
-      (void)hipEventSynchronize(L->ev[7]);
-      for (int e = 0; e < 7; ++e) {
+      (void)hipEventSynchronize(L->ev[ne - 1]);
+      for (int e = 0; e + 1 < ne && e < PC_NKERN; ++e) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, L->ev[e], L->ev[e + 1]);
-        L->times[e] += ms;
+        L->ktime[e] += ms;
       }
       L->nsteps_timed++;
     }
-    (void)NS;
   }
   HIPCHK(L->ctx, hipGetLastError());
   return PC_OK;
@@ -941,10 +964,10 @@ int pc_level_step_v1(pc_level_t* L, int nsteps, void* stream) {
   return level_step_eos<EosLiDryer>(L, nsteps, st);
 }
 
-int pc_level_stage_times_v1(pc_level_t* L, double* ms7) {
-  if (!L || !ms7) return PC_ERR_ARG;
-  for (int s = 0; s < 7; ++s)
-    ms7[s] = L->nsteps_timed ? L->times[s] / L->nsteps_timed : 0.0;
+int pc_level_kernel_times_v1(pc_level_t* L, double* ms, int n) {
+  if (!L || !ms || n <= 0) return PC_ERR_ARG;
+  for (int s = 0; s < n && s < PC_NKERN; ++s)
+    ms[s] = L->nsteps_timed ? L->ktime[s] / L->nsteps_timed : 0.0;
   return PC_OK;
 }
 

@@ -215,7 +215,9 @@ class DeviceLevel:
     def enable_timing(self, on=True):
         self._chk(self.L.pc_level_enable_timing_v1(self.h, 1 if on else 0))
 
-    def stage_times(self):
-        a = (C.c_double * 7)()
-        self._chk(self.L.pc_level_stage_times_v1(self.h, a))
-        return list(a)
+    def kernel_times(self):
+        """{kernel name: average ms per timed step}"""
+        n = len(_lib.KERNELS)
+        a = (C.c_double * n)()
+        self._chk(self.L.pc_level_kernel_times_v1(self.h, a, n))
+        return dict(zip(_lib.KERNELS, list(a)))
