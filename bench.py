@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (the "
+                         "reference kernel for scaling HBM counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -122,6 +125,15 @@ def main():
         w = workloads.sod(n=n, nsteps=args.steps)
         desc = (f"3D Sod, gamma-law gas, {n}^3 single box + 4 ghosts, PPM, "
                 f"approximate Riemann solver, CFL 0.3")
+
+    if args.copy_calibration_mib:
+        nel = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ctx = HydroContext(w.nspec)
     lev = DeviceLevel.from_workload(ctx, w)

@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <limits>
+#include <utility>
 
 #define PCD __device__ __forceinline__
 
@@ -142,77 +143,88 @@ struct LiDryerTab {
        -9.227977e+02}};
 };
 
+// Coefficient combinations of species K as compile-time literals (constexpr
+// locals are evaluated by the compiler in IEEE double, the same roundings as
+// the run-time expressions they replace).
+template <int K, bool LO>
+struct Nasa {
+  static constexpr const double* a = LO ? LiDryerTab::lo[K] : LiDryerTab::hi[K];
+  static constexpr double c0 = a[0] - 1.0;
+  static constexpr double e1 = a[1] / 2.0, e2 = a[2] / 3.0, e3 = a[3] / 4.0,
+                          e4 = a[4] / 5.0, e5 = a[5];
+  static constexpr double c1 = a[1], c2 = a[2], c3 = a[3], c4 = a[4];
+  static constexpr double imw = 1.0 / LiDryerTab::mw[K];
+  // e_K / (R T) and cv_K / R
+  PCD static double eRT(double T, double T2, double T3, double T4,
+                        double invT) {
+    return c0 + e1 * T + e2 * T2 + e3 * T3 + e4 * T4 + e5 * invT;
+  }
+  PCD static double cvR(double T, double T2, double T3, double T4) {
+    return c0 + c1 * T + c2 * T2 + c3 * T3 + c4 * T4;
+  }
+};
+
 struct EosLiDryer {
   static constexpr int NS = 9;
   using D = LiDryerTab;
-  static constexpr double imw(int k) { return 1.0 / D::mw[k]; }
+  using Seq = std::make_integer_sequence<int, 9>;
 
-  // sum_k Y_k e_k(T) (erg/g), CKUBMS
-  template <bool LO>
-  PCD static double ubms_tab(double T, const double* Y) {
+  // sum_k Y_k e_k(T) (erg/g), CKUBMS; the sum runs k = 0..8 in order
+  template <bool LO, int... K>
+  PCD static double ubms_seq(double T, const double* Y,
+                             std::integer_sequence<int, K...>) {
     const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, invT = 1.0 / T;
     const double RT = RU * T;
     double E = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      const double* a = LO ? D::lo[k] : D::hi[k];
-      const double e = (a[0] - 1.0) + (a[1] / 2.0) * T + (a[2] / 3.0) * T2 +
-                       (a[3] / 4.0) * T3 + (a[4] / 5.0) * T4 + a[5] * invT;
-      E += Y[k] * (e * RT * imw(k));
-    }
+    ((E += Y[K] * (Nasa<K, LO>::eRT(T, T2, T3, T4, invT) * RT *
+                   Nasa<K, LO>::imw)),
+     ...);
     return E;
   }
-  PCD static double ubms(double T, const double* Y) {
-    return (T < 1000.0) ? ubms_tab<true>(T, Y) : ubms_tab<false>(T, Y);
-  }
   // sum_k Y_k cv_k(T), CKCVBS
-  template <bool LO>
-  PCD static double cvbs_tab(double T, const double* Y) {
+  template <bool LO, int... K>
+  PCD static double cvbs_seq(double T, const double* Y,
+                             std::integer_sequence<int, K...>) {
     const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
     double cv = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      const double* a = LO ? D::lo[k] : D::hi[k];
-      const double c =
-          (a[0] - 1.0) + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-      cv += Y[k] * (c * RU * imw(k));
-    }
+    ((cv += Y[K] * (Nasa<K, LO>::cvR(T, T2, T3, T4) * RU * Nasa<K, LO>::imw)),
+     ...);
     return cv;
   }
-  PCD static double cvbs(double T, const double* Y) {
-    return (T < 1000.0) ? cvbs_tab<true>(T, Y) : cvbs_tab<false>(T, Y);
-  }
-  // both at one temperature
-  template <bool LO>
-  PCD static void ucv_tab(double T, const double* Y, double& E, double& cv) {
+  template <bool LO, int... K>
+  PCD static void ucv_seq(double T, const double* Y, double& E, double& cv,
+                          std::integer_sequence<int, K...>) {
     const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, invT = 1.0 / T;
     const double RT = RU * T;
     E = 0.0;
     cv = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      const double* a = LO ? D::lo[k] : D::hi[k];
-      const double e = (a[0] - 1.0) + (a[1] / 2.0) * T + (a[2] / 3.0) * T2 +
-                       (a[3] / 4.0) * T3 + (a[4] / 5.0) * T4 + a[5] * invT;
-      E += Y[k] * (e * RT * imw(k));
-      const double c =
-          (a[0] - 1.0) + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-      cv += Y[k] * (c * RU * imw(k));
-    }
+    ((E += Y[K] * (Nasa<K, LO>::eRT(T, T2, T3, T4, invT) * RT *
+                   Nasa<K, LO>::imw),
+      cv += Y[K] * (Nasa<K, LO>::cvR(T, T2, T3, T4) * RU * Nasa<K, LO>::imw)),
+     ...);
+  }
+  PCD static double ubms(double T, const double* Y) {
+    return (T < 1000.0) ? ubms_seq<true>(T, Y, Seq{})
+                        : ubms_seq<false>(T, Y, Seq{});
+  }
+  PCD static double cvbs(double T, const double* Y) {
+    return (T < 1000.0) ? cvbs_seq<true>(T, Y, Seq{})
+                        : cvbs_seq<false>(T, Y, Seq{});
   }
   PCD static void ucv(double T, const double* Y, double& E, double& cv) {
     if (T < 1000.0)
-      ucv_tab<true>(T, Y, E, cv);
+      ucv_seq<true>(T, Y, E, cv, Seq{});
     else
-      ucv_tab<false>(T, Y, E, cv);
+      ucv_seq<false>(T, Y, E, cv, Seq{});
   }
 
-  PCD static double wbar(const double* Y) {
+  template <int... K>
+  PCD static double sum_y_imw(const double* Y, std::integer_sequence<int, K...>) {
     double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) s += Y[k] * imw(k);
-    return 1.0 / s;
+    ((s += Y[K] * Nasa<K, true>::imw), ...);
+    return s;
   }
+  PCD static double wbar(const double* Y) { return 1.0 / sum_y_imw(Y, Seq{}); }
   PCD static double RYP2E(double R, const double* Y, double P) {
     const double w = wbar(Y);
     const double T = P * w / (R * RU);
@@ -235,10 +247,7 @@ struct EosLiDryer {
     Cs = sqrt(G * P / R);
   }
   PCD static double RTY2P(double R, double T, const double* Y) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) s += Y[k] * imw(k);
-    return R * RU * T * s;
+    return R * RU * T * sum_y_imw(Y, Seq{});
   }
   PCD static double RTY2Cs(double R, double T, const double* Y) {
     const double P = RTY2P(R, T, Y);
@@ -264,14 +273,14 @@ struct EosLiDryer {
   // |dT| clipped to 100 K, bracket [90, 4000] K, 200 iterations)
   PCD static double EY2T(double E, const double* Y, double Tguess) {
     const double tmin = 90.0, tmax = 4000.0;
-    const double emin = ubms(tmin, Y);
-    const double emax = ubms(tmax, Y);
+    const double emin = ubms_seq<true>(tmin, Y, Seq{});
+    const double emax = ubms_seq<false>(tmax, Y, Seq{});
     if (E < emin) {
-      const double cv = cvbs(tmin, Y);
+      const double cv = cvbs_seq<true>(tmin, Y, Seq{});
       return tmin - (emin - E) / cv;
     }
     if (E > emax) {
-      const double cv = cvbs(tmax, Y);
+      const double cv = cvbs_seq<false>(tmax, Y, Seq{});
       return tmax - (emax - E) / cv;
     }
     double t1 = Tguess;
