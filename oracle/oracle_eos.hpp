@@ -20,7 +20,12 @@
 // CHEMKIN thermodynamic data the Li-Dryer H2/O2 mechanism ships with, and a
 // Newton e->T inversion with the Fuego-generated stopping rule (T bracket
 // [90, 4000] K, |dT| < 1e-12 or T+dT == T, |dT| clipped to 100 K, 200
-// iterations).  Pinned by ideal-gas identities, NASA continuity at T_mid, the
+// iterations).  Evaluation scheme (this restatement's own; the reference's
+// EOS source is absent): polynomials by Horner's rule in std::fma with
+// coefficients scaled by RU/W_k; species sums as fma chains in species
+// order; functions of a known composition (RTY2G, RTY2Cs, RTY2dpde_dpdre,
+// EY2T/REY2T) on mass-weighted mixture coefficients formed once per call.
+// The same operations on the GPU give the same bits.  Pinned by ideal-gas identities, NASA continuity at T_mid, the
 // PMF flame file's molecular weights and the adiabatic enthalpy balances of
 // Exec/RegTests/PMF and Exec/RegTests/TGReact (tests/test_oracle_eos.py).
 // The exact coefficient digits and Newton details of the reference build are
@@ -81,6 +86,12 @@ struct GammaLaw {
   static void T2Ei(double T, double* Ei) {
     for (int k = 0; k < NS; ++k) Ei[k] = RU * T * imw[k] / (gamma - 1.0);
   }
+  static void TY2E(double T, const double* Y, double& E) {
+    double ei[NS];
+    T2Ei(T, ei);
+    E = 0.0;
+    for (int k = 0; k < NS; ++k) E += Y[k] * ei[k];
+  }
 };
 
 // NASA-7 data, species order H2 O2 H2O H O OH HO2 H2O2 N2.
@@ -130,60 +141,95 @@ struct LiDryerData {
 struct LiDryer {
   static constexpr int NS = 9;
   using D = LiDryerData;
+  static constexpr double TMID = 1000.0;
 
   static double imw(int k) { return 1.0 / D::mw[k]; }
-
-  // e_k/(R T) for every species (polynomial in T, chosen by T < 1000 K).
-  static void eRT(double T, double* e) {
-    const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, invT = 1.0 / T;
-    for (int k = 0; k < NS; ++k) {
-      const double* a = (T < 1000.0) ? D::lo[k] : D::hi[k];
-      e[k] = (a[0] - 1.0) + (a[1] / 2.0) * T + (a[2] / 3.0) * T2 +
-             (a[3] / 4.0) * T3 + (a[4] / 5.0) * T4 + a[5] * invT;
-    }
+  static const double* coef(int k, double T) {
+    return (T < TMID) ? D::lo[k] : D::hi[k];
   }
-  // cv_k / R
-  static void cvR(double T, double* c) {
-    const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-    for (int k = 0; k < NS; ++k) {
-      const double* a = (T < 1000.0) ? D::lo[k] : D::hi[k];
-      c[k] = (a[0] - 1.0) + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-    }
+  // Species k's polynomials with the coefficients scaled by RU / W_k:
+  //   cv_k(T) = v0 + v1 T + ... + v4 T^4,  v = RU/W (a0 - 1, a1, a2, a3, a4)
+  //   e_k(T)  = u0 + u1 T + ... + u5 T^5,  u = RU/W (a5, a0 - 1, a1/2, a2/3,
+  //             a3/4, a4/5)
+  // i.e. e_k = RT (a0 - 1 + a1 T/2 + ... + a4 T^4/5 + a5/T) multiplied
+  // through by T; both by Horner's rule in fused multiply-adds.
+  static void ucoef(const double* a, int k, double* u) {
+    const double s = RU / D::mw[k];
+    u[0] = a[5] * s;
+    u[1] = (a[0] - 1.0) * s;
+    u[2] = a[1] / 2.0 * s;
+    u[3] = a[2] / 3.0 * s;
+    u[4] = a[3] / 4.0 * s;
+    u[5] = a[4] / 5.0 * s;
+  }
+  static double e_k(int k, double T) {
+    double u[6];
+    ucoef(coef(k, T), k, u);
+    return std::fma(T, std::fma(T, std::fma(T, std::fma(T, std::fma(T, u[5], u[4]), u[3]), u[2]), u[1]), u[0]);
+  }
+  static double cv_k(int k, double T) {
+    const double* a = coef(k, T);
+    const double s = RU / D::mw[k];
+    const double v0 = (a[0] - 1.0) * s, v1 = a[1] * s, v2 = a[2] * s,
+                 v3 = a[3] * s, v4 = a[4] * s;
+    return std::fma(T, std::fma(T, std::fma(T, std::fma(T, v4, v3), v2), v1), v0);
   }
   // h_k / (R T)  (for tests: enthalpy balances)
   static void hRT(double T, double* h) {
     const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, invT = 1.0 / T;
     for (int k = 0; k < NS; ++k) {
-      const double* a = (T < 1000.0) ? D::lo[k] : D::hi[k];
+      const double* a = coef(k, T);
       h[k] = a[0] + (a[1] / 2.0) * T + (a[2] / 3.0) * T2 + (a[3] / 4.0) * T3 +
              (a[4] / 5.0) * T4 + a[5] * invT;
     }
   }
   // species internal energies, erg/g
   static void CKUMS(double T, double* ei) {
-    double e[NS];
-    eRT(T, e);
-    const double RT = RU * T;
-    for (int k = 0; k < NS; ++k) ei[k] = e[k] * RT * imw(k);
+    for (int k = 0; k < NS; ++k) ei[k] = e_k(k, T);
   }
   static void CKUBMS(double T, const double* Y, double& E) {
-    double ei[NS];
-    CKUMS(T, ei);
     E = 0.0;
-    for (int k = 0; k < NS; ++k) E += Y[k] * ei[k];
+    for (int k = 0; k < NS; ++k) E = std::fma(Y[k], e_k(k, T), E);
   }
   static void CKCVBS(double T, const double* Y, double& cv) {
-    double c[NS];
-    cvR(T, c);
     cv = 0.0;
-    for (int k = 0; k < NS; ++k) cv += Y[k] * (c[k] * RU * imw(k));
+    for (int k = 0; k < NS; ++k) cv = std::fma(Y[k], cv_k(k, T), cv);
+  }
+  static double sum_y_imw(const double* Y) {
+    double s = 0.0;
+    for (int k = 0; k < NS; ++k) s = std::fma(Y[k], imw(k), s);
+    return s;
   }
 
-  static void Y2WBAR(const double* Y, double& wbar) {
-    double s = 0.0;
-    for (int k = 0; k < NS; ++k) s += Y[k] * imw(k);
-    wbar = 1.0 / s;
-  }
+  // Mass-weighted mixture polynomial e(T) = sum_m U[m] T^m of one
+  // composition in each NASA range (cv = de/dT); the state functions of a
+  // known composition (RTY2*, EY2T) evaluate these.
+  struct Mix {
+    double lo[6], hi[6], sy;
+    explicit Mix(const double* Y) {
+      for (int m = 0; m < 6; ++m) lo[m] = hi[m] = 0.0;
+      for (int k = 0; k < NS; ++k) {
+        double ul[6], uh[6];
+        ucoef(D::lo[k], k, ul);
+        ucoef(D::hi[k], k, uh);
+        for (int m = 0; m < 6; ++m) {
+          lo[m] = std::fma(Y[k], ul[m], lo[m]);
+          hi[m] = std::fma(Y[k], uh[m], hi[m]);
+        }
+      }
+      sy = sum_y_imw(Y);
+    }
+    static double pe(const double* U, double T) {
+      return std::fma(T, std::fma(T, std::fma(T, std::fma(T, std::fma(T, U[5], U[4]), U[3]), U[2]), U[1]), U[0]);
+    }
+    static double pcv(const double* U, double T) {
+      return std::fma(T, std::fma(T, std::fma(T, std::fma(T, 5.0 * U[5], 4.0 * U[4]), 3.0 * U[3]), 2.0 * U[2]), U[1]);
+    }
+    double e(double T) const { return (T < TMID) ? pe(lo, T) : pe(hi, T); }
+    double cv(double T) const { return (T < TMID) ? pcv(lo, T) : pcv(hi, T); }
+  };
+
+  static void Y2WBAR(const double* Y, double& wbar) { wbar = 1.0 / sum_y_imw(Y); }
   static void RYP2E(double R, const double* Y, double P, double& E) {
     double wbar;
     Y2WBAR(Y, wbar);
@@ -200,64 +246,49 @@ struct LiDryer {
     Cs = std::sqrt(G * P / R);
   }
   static void RTY2P(double R, double T, const double* Y, double& P) {
-    double s = 0.0;
-    for (int k = 0; k < NS; ++k) s += Y[k] * imw(k);
-    P = R * RU * T * s;
-  }
-  static void RTY2Cs(double R, double T, const double* Y, double& Cs) {
-    double P;
-    RTY2P(R, T, Y, P);
-    double wbar;
-    Y2WBAR(Y, wbar);
-    double cv;
-    CKCVBS(T, Y, cv);
-    const double G = (wbar * cv + RU) / (wbar * cv);
-    Cs = std::sqrt(G * P / R);
+    P = R * RU * T * sum_y_imw(Y);
   }
   static void RTY2G(double /*R*/, double T, const double* Y, double& G) {
-    double wbar;
-    Y2WBAR(Y, wbar);
-    double cv;
-    CKCVBS(T, Y, cv);
+    const Mix m(Y);
+    const double wbar = 1.0 / m.sy;
+    const double cv = m.cv(T);
     G = (wbar * cv + RU) / (wbar * cv);
+  }
+  static void RTY2Cs(double R, double T, const double* Y, double& Cs) {
+    double P, G;
+    RTY2P(R, T, Y, P);
+    RTY2G(R, T, Y, G);
+    Cs = std::sqrt(G * P / R);
   }
   static void RTY2dpde_dpdre(double R, double T, const double* Y, double& dpde,
                              double& dpdr_e) {
-    double wbar;
-    Y2WBAR(Y, wbar);
-    double cv;
-    CKCVBS(T, Y, cv);
-    dpde = R * RU / (wbar * cv);
+    const Mix m(Y);
+    const double wbar = 1.0 / m.sy;
+    dpde = R * RU / (wbar * m.cv(T));
     dpdr_e = RU * T / wbar;
   }
   static void T2Ei(double T, double* Ei) { CKUMS(T, Ei); }
+  static void TY2E(double T, const double* Y, double& E) { CKUBMS(T, Y, E); }
   static void EY2T(double E, const double* Y, double& T) {
     const int maxiter = 200;
     const double tol = 1.0e-12;
     const double tmin = 90.0, tmax = 4000.0;
-    double emin, emax;
-    CKUBMS(tmin, Y, emin);
-    CKUBMS(tmax, Y, emax);
+    const Mix m(Y);
+    const double emin = Mix::pe(m.lo, tmin);
+    const double emax = Mix::pe(m.hi, tmax);
     if (E < emin) {
-      double cv;
-      CKCVBS(tmin, Y, cv);
-      T = tmin - (emin - E) / cv;
+      T = tmin - (emin - E) / Mix::pcv(m.lo, tmin);
       return;
     }
     if (E > emax) {
-      double cv;
-      CKCVBS(tmax, Y, cv);
-      T = tmax - (emax - E) / cv;
+      T = tmax - (emax - E) / Mix::pcv(m.hi, tmax);
       return;
     }
     double t1 = T;
     if (t1 < tmin || t1 > tmax)
       t1 = tmin + (tmax - tmin) / (emax - emin) * (E - emin);
     for (int it = 0; it < maxiter; ++it) {
-      double e1, cv;
-      CKUBMS(t1, Y, e1);
-      CKCVBS(t1, Y, cv);
-      double dt = (E - e1) / cv;
+      double dt = (E - m.e(t1)) / m.cv(t1);
       if (dt > 100.0) {
         dt = 100.0;
       } else if (dt < -100.0) {

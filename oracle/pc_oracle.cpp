@@ -275,10 +275,7 @@ int prim_to_cons_t(const Box& bx, const A4& P, int mode, const A4& U) {
       double wbar;
       EOS::Y2WBAR(Y, wbar);
       rho = p * wbar / (RU * T);
-      double ei[NS];
-      EOS::T2Ei(T, ei);
-      e = 0.0;
-      for (int n = 0; n < NS; ++n) e += Y[n] * ei[n];
+      EOS::TY2E(T, Y, e);
     }
     const double rhoe = rho * e;
     U(i, j, k, URHO) = rho;
@@ -315,6 +312,9 @@ int eos_point(int which, double a, double b, double c, const double* Y,
       if constexpr (NS == 9) { LiDryer::hRT(a, out); return 0; }
       else { out[0] = GammaLaw::gamma / (GammaLaw::gamma - 1.0); return 0; }
     case 10: EOS::RTY2dpde_dpdre(a, b, y, out[0], out[1]); return 0;
+    case 11:  // cv of the composition from mixture coefficients
+      if constexpr (NS == 9) { out[0] = LiDryer::Mix(y).cv(a); return 0; }
+      else { out[0] = RU * GammaLaw::imw[0] / (GammaLaw::gamma - 1.0); return 0; }
   }
   return PC_ERR_ARG;
 }

@@ -58,8 +58,9 @@ int pc_ref_prim_to_cons(int eos, pc_box_t bx, pc_array4_t prim, int mode,
 /* Point functions for known-answer tests. */
 /* which: 0 RYP2E(r,Y,p)->e, 1 RPY2Cs(r,p,Y)->c, 2 REY2T(r,e,Y;T0)->T,
  *        3 RTY2P(r,T,Y)->p, 4 RTY2Cs(r,T,Y)->c, 5 RTY2G(r,T,Y)->G,
- *        6 Y2WBAR(Y)->W, 7 T2Ei(T)->e_k (NS outputs), 8 cv mix (T,Y),
- *        9 h_k/(R T) (NS outputs), 10 RTY2dpde_dpdre -> (dpde, dpdr_e) */
+ *        6 Y2WBAR(Y)->W, 7 T2Ei(T)->e_k (NS outputs), 8 CKCVBS(T,Y)
+ *        (species sum), 9 h_k/(R T) (NS outputs), 10 RTY2dpde_dpdre ->
+ *        (dpde, dpdr_e), 11 cv(T,Y) from mixture coefficients */
 int pc_ref_eos(int eos, int which, double a, double b, double c,
                const double* Y, double* out);
 /* riemann() of Source/Riemann.H:9-200: out[0..] = ustar, uflx_rho,

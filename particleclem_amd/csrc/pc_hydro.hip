@@ -25,6 +25,14 @@ struct pc_ctx {
   double* ws = nullptr;
   size_t ws_bytes = 0;
   hipStream_t ws_stream = nullptr;
+  // job descriptors of the box drivers: a ring of device slots filled from
+  // pinned host slots; a slot is reused once the event of its last use fired
+  static constexpr int RING = 64;
+  Job* djobs = nullptr;
+  Job* hjobs = nullptr;
+  hipEvent_t jev[RING];
+  bool jev_used[RING] = {};
+  int jnext = 0;
 };
 
 static int fail(pc_ctx* c, int code, const char* fmt, ...) {
@@ -197,6 +205,35 @@ static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
   J.divu = fab_view(base + L.off_divu, L.node);
 }
 
+// Stage job J for the kernels of one box-driver call on stream st.
+static int ctx_job(pc_ctx* c, const Job& J, hipStream_t st, Sched* sc,
+                   int* slot) {
+  if (!c->djobs) {
+    if (hipMalloc(&c->djobs, sizeof(Job) * pc_ctx::RING) != hipSuccess ||
+        hipHostMalloc(&c->hjobs, sizeof(Job) * pc_ctx::RING) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, PC_ERR_NOMEM, "job ring allocation failed");
+    }
+    for (int e = 0; e < pc_ctx::RING; ++e)
+      HIPCHK(c, hipEventCreateWithFlags(&c->jev[e], hipEventDisableTiming));
+  }
+  const int s = c->jnext;
+  c->jnext = (c->jnext + 1) % pc_ctx::RING;
+  if (c->jev_used[s]) HIPCHK(c, hipEventSynchronize(c->jev[s]));
+  c->hjobs[s] = J;
+  HIPCHK(c, hipMemcpyAsync(c->djobs + s, c->hjobs + s, sizeof(Job),
+                           hipMemcpyHostToDevice, st));
+  *sc = Sched{c->djobs + s, nullptr, nullptr};
+  *slot = s;
+  return PC_OK;
+}
+
+static int ctx_job_done(pc_ctx* c, int slot, hipStream_t st) {
+  HIPCHK(c, hipEventRecord(c->jev[slot], st));
+  c->jev_used[slot] = true;
+  return PC_OK;
+}
+
 static int ctx_scratch(pc_ctx* c, size_t bytes, hipStream_t st) {
   if (bytes <= c->ws_bytes) {
     c->ws_stream = st;
@@ -247,37 +284,87 @@ static int grid1(int st, const Job& J) {
 // The CTU stages from primitives to final fluxes for one EOS.  With a job
 // table (sc.jobs) the grids come from `plans`; otherwise from job j0.
 template <class EOS>
-static void run_ctu(const Job& j0, const Sched* scs, const StagePlan* plans,
-                    const KArgs& ka, int ppm_type, hipStream_t st) {
-  auto G = [&](int stage) {
-    return plans ? plans[stage].nblocks : grid1(stage, j0);
-  };
-  auto S = [&](int stage) { return plans ? scs[stage] : Sched{nullptr, nullptr, nullptr}; };
-  LAUNCH(k_divu, G(ST_DIVU), st, j0, S(ST_DIVU), ka);
+static void run_ctu(const Job& j0, const Sched& s1, const KArgs& ka,
+                    int ppm_type, hipStream_t st) {
+  auto G = [&](int stage) { return grid1(stage, j0); };
+  auto S = [&](int) { return s1; };
+  LAUNCH(k_divu, G(ST_DIVU), st, S(ST_DIVU), ka);
   if (ppm_type == 1) {
-    LAUNCH((k_trace_ppm<EOS, 0>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
-    LAUNCH((k_trace_ppm<EOS, 1>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
-    LAUNCH((k_trace_ppm<EOS, 2>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+    LAUNCH((k_trace_ppm<EOS, 0>), G(ST_TRACE), st, S(ST_TRACE), ka);
+    LAUNCH((k_trace_ppm<EOS, 1>), G(ST_TRACE), st, S(ST_TRACE), ka);
+    LAUNCH((k_trace_ppm<EOS, 2>), G(ST_TRACE), st, S(ST_TRACE), ka);
   } else {
-    LAUNCH((k_trace_plm<EOS, 0>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
-    LAUNCH((k_trace_plm<EOS, 1>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
-    LAUNCH((k_trace_plm<EOS, 2>), G(ST_TRACE), st, j0, S(ST_TRACE), ka);
+    LAUNCH((k_trace_plm<EOS, 0>), G(ST_TRACE), st, S(ST_TRACE), ka);
+    LAUNCH((k_trace_plm<EOS, 1>), G(ST_TRACE), st, S(ST_TRACE), ka);
+    LAUNCH((k_trace_plm<EOS, 2>), G(ST_TRACE), st, S(ST_TRACE), ka);
   }
-  LAUNCH((k_flux1<EOS, 0>), G(ST_FLUX1_X), st, j0, S(ST_FLUX1_X), ka);
-  LAUNCH((k_flux1<EOS, 1>), G(ST_FLUX1_Y), st, j0, S(ST_FLUX1_Y), ka);
-  LAUNCH((k_flux1<EOS, 2>), G(ST_FLUX1_Z), st, j0, S(ST_FLUX1_Z), ka);
-  LAUNCH((k_trans<EOS, 0, 1>), G(ST_TRANS_XY), st, j0, S(ST_TRANS_XY), ka);
-  LAUNCH((k_trans<EOS, 0, 2>), G(ST_TRANS_XZ), st, j0, S(ST_TRANS_XZ), ka);
-  LAUNCH((k_trans<EOS, 1, 0>), G(ST_TRANS_YX), st, j0, S(ST_TRANS_YX), ka);
-  LAUNCH((k_trans<EOS, 1, 2>), G(ST_TRANS_YZ), st, j0, S(ST_TRANS_YZ), ka);
-  LAUNCH((k_trans<EOS, 2, 0>), G(ST_TRANS_ZX), st, j0, S(ST_TRANS_ZX), ka);
-  LAUNCH((k_trans<EOS, 2, 1>), G(ST_TRANS_ZY), st, j0, S(ST_TRANS_ZY), ka);
-  LAUNCH((k_final<EOS, 0>), G(ST_FINAL_X), st, j0, S(ST_FINAL_X), ka);
-  LAUNCH((k_final<EOS, 1>), G(ST_FINAL_Y), st, j0, S(ST_FINAL_Y), ka);
-  LAUNCH((k_final<EOS, 2>), G(ST_FINAL_Z), st, j0, S(ST_FINAL_Z), ka);
+  LAUNCH((k_flux1<EOS, 0>), G(ST_FLUX1_X), st, S(ST_FLUX1_X), ka);
+  LAUNCH((k_flux1<EOS, 1>), G(ST_FLUX1_Y), st, S(ST_FLUX1_Y), ka);
+  LAUNCH((k_flux1<EOS, 2>), G(ST_FLUX1_Z), st, S(ST_FLUX1_Z), ka);
+  LAUNCH((k_trans<EOS, 0, 1>), G(ST_TRANS_XY), st, S(ST_TRANS_XY), ka);
+  LAUNCH((k_trans<EOS, 0, 2>), G(ST_TRANS_XZ), st, S(ST_TRANS_XZ), ka);
+  LAUNCH((k_trans<EOS, 1, 0>), G(ST_TRANS_YX), st, S(ST_TRANS_YX), ka);
+  LAUNCH((k_trans<EOS, 1, 2>), G(ST_TRANS_YZ), st, S(ST_TRANS_YZ), ka);
+  LAUNCH((k_trans<EOS, 2, 0>), G(ST_TRANS_ZX), st, S(ST_TRANS_ZX), ka);
+  LAUNCH((k_trans<EOS, 2, 1>), G(ST_TRANS_ZY), st, S(ST_TRANS_ZY), ka);
+  if (j0.srcq.p) {
+    LAUNCH((k_final<EOS, 0, true>), G(ST_FINAL_X), st, S(ST_FINAL_X), ka);
+    LAUNCH((k_final<EOS, 1, true>), G(ST_FINAL_Y), st, S(ST_FINAL_Y), ka);
+    LAUNCH((k_final<EOS, 2, true>), G(ST_FINAL_Z), st, S(ST_FINAL_Z), ka);
+  } else {
+    LAUNCH((k_final<EOS, 0, false>), G(ST_FINAL_X), st, S(ST_FINAL_X), ka);
+    LAUNCH((k_final<EOS, 1, false>), G(ST_FINAL_Y), st, S(ST_FINAL_Y), ka);
+    LAUNCH((k_final<EOS, 2, false>), G(ST_FINAL_Z), st, S(ST_FINAL_Z), ka);
+  }
+  auto lowo = [&](int d) {
+    auto is = [](int b) { return b == PC_BC_INFLOW || b == PC_BC_USERBC; };
+    return is(j0.bclo[d]) || is(j0.bchi[d]);
+  };
+  if (lowo(0)) LAUNCH((k_final_loworder<EOS, 0>), G(ST_FINAL_X), st, S(ST_FINAL_X), ka);
+  if (lowo(1)) LAUNCH((k_final_loworder<EOS, 1>), G(ST_FINAL_Y), st, S(ST_FINAL_Y), ka);
+  if (lowo(2)) LAUNCH((k_final_loworder<EOS, 2>), G(ST_FINAL_Z), st, S(ST_FINAL_Z), ka);
 }
 
 // ------------------------------------------------------------------ C ABI --
+// Workspace views plus the caller's flx/area/vol/uout views of a box-driver
+// job; ws points at the CTU part of the context scratch.
+static int finish_umdrv_job(pc_ctx* c, Job& J, const IBox& bx, double* ws,
+                            const pc_array4_t* flx, const pc_array4_t* a,
+                            const pc_array4_t& vol, const pc_array4_t& uout) {
+  int rc;
+  const WsLayout L = ws_layout(bx, c->ns, false, false, false);
+  ws_fill_job(J, ws, L, false, false, false);
+  for (int d = 0; d < 3; ++d) {
+    IBox fb = bx;
+    fb.hi[d] += 1;
+    if ((rc = a4_view(c, flx[d], fb, 7 + c->ns, "flx", J.flx[d]))) return rc;
+    if ((rc = a4_view(c, a[d], fb, 1, "a", J.area[d]))) return rc;
+  }
+  if ((rc = a4_view(c, vol, bx, 1, "vol", J.vol))) return rc;
+  if ((rc = a4_view(c, uout, bx, 7 + c->ns, "uout", J.upd))) return rc;
+  return PC_OK;
+}
+
+static KArgs umdrv_args(const double* dx, double dt, int plm_iorder,
+                        int use_flattening, double difmag) {
+  KArgs ka;
+  memset(&ka, 0, sizeof(ka));
+  for (int d = 0; d < 3; ++d) ka.dx[d] = dx[d];
+  ka.difmag = difmag;
+  ka.dt_val = dt;
+  ka.dtp = nullptr;
+  ka.plm_iorder = plm_iorder;
+  ka.use_flattening = use_flattening;
+  return ka;
+}
+
+template <class EOS>
+static void launch_ctu_consup(const Job& J, const Sched& sc, const KArgs& ka,
+                              int ppm_type, hipStream_t st) {
+  run_ctu<EOS>(J, sc, ka, ppm_type, st);
+  LAUNCH(k_consup<EOS::NS>, grid1(ST_UPDATE, J), st, sc, ka);
+}
+
 extern "C" {
 
 void pc_params_default(pc_params_t* p) {
@@ -319,9 +406,12 @@ int pc_ctx_create(int eos_kind, pc_ctx_t** out) {
 
 int pc_ctx_destroy(pc_ctx_t* c) {
   if (!c) return PC_ERR_ARG;
-  if (c->ws) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(c->ws);
+  if (c->ws || c->djobs) (void)hipDeviceSynchronize();
+  if (c->ws) (void)hipFree(c->ws);
+  if (c->djobs) {
+    (void)hipFree(c->djobs);
+    (void)hipHostFree(c->hjobs);
+    for (int e = 0; e < pc_ctx::RING; ++e) (void)hipEventDestroy(c->jev[e]);
   }
   delete c;
   return PC_OK;
@@ -352,14 +442,18 @@ int pc_ctoprim_v1(pc_ctx_t* c, pc_box_t qbx, pc_array4_t u, pc_array4_t q,
   if ((rc = a4_view(c, qaux, b, NQAUX, "qaux", J.qa))) return rc;
   KArgs ka;
   memset(&ka, 0, sizeof(ka));
+  ka.full_prims = 1;
   hipStream_t st = (hipStream_t)stream;
   const int g = (int)((b.npts() + BLOCK - 1) / BLOCK);
+  Sched sc;
+  int slot;
+  if ((rc = ctx_job(c, J, st, &sc, &slot))) return rc;
   if (c->eos == PC_EOS_GAMMALAW)
-    LAUNCH(k_ctoprim<EosGamma>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka);
+    LAUNCH(k_ctoprim<EosGamma>, g, st, sc, ka);
   else
-    LAUNCH(k_ctoprim<EosLiDryer>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka);
+    LAUNCH(k_ctoprim<EosLiDryer>, g, st, sc, ka);
   HIPCHK(c, hipGetLastError());
-  return PC_OK;
+  return ctx_job_done(c, slot, st);
 }
 
 int pc_srctoprim_v1(pc_ctx_t* c, pc_box_t qbx, pc_array4_t q, pc_array4_t qaux,
@@ -379,12 +473,15 @@ int pc_srctoprim_v1(pc_ctx_t* c, pc_box_t qbx, pc_array4_t q, pc_array4_t qaux,
   memset(&ka, 0, sizeof(ka));
   hipStream_t st = (hipStream_t)stream;
   const int g = (int)((b.npts() + BLOCK - 1) / BLOCK);
+  Sched sc;
+  int slot;
+  if ((rc = ctx_job(c, J, st, &sc, &slot))) return rc;
   if (c->eos == PC_EOS_GAMMALAW)
-    LAUNCH(k_srctoprim<1>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka, vs);
+    LAUNCH(k_srctoprim<1>, g, st, sc, ka, vs);
   else
-    LAUNCH(k_srctoprim<9>, g, st, J, Sched{nullptr, nullptr, nullptr}, ka, vs);
+    LAUNCH(k_srctoprim<9>, g, st, sc, ka, vs);
   HIPCHK(c, hipGetLastError());
-  return PC_OK;
+  return ctx_job_done(c, slot, st);
 }
 
 static int check_opts(pc_ctx* c, int ppm_type, int plm_iorder,
@@ -400,43 +497,6 @@ static int check_opts(pc_ctx* c, int ppm_type, int plm_iorder,
 
 // shared tail of pc_umdrv_v1 / pc_hydro_source_box_v1: J.q/qa/srcq/U are
 // set; fills the stage workspace views and launches the CTU + consup.
-static int umdrv_core(pc_ctx* c, Job& J, const IBox& bx, const double* dx,
-                      double dt, int ppm_type, int plm_iorder,
-                      int use_flattening, double difmag, const pc_array4_t* flx,
-                      const pc_array4_t* a, const pc_array4_t& vol,
-                      const pc_array4_t& uout, hipStream_t st,
-                      size_t ws_extra_off) {
-  int rc;
-  const WsLayout L = ws_layout(bx, c->ns, false, false, false);
-  ws_fill_job(J, c->ws + ws_extra_off, L, false, false, false);
-  for (int d = 0; d < 3; ++d) {
-    IBox fb = bx;
-    fb.hi[d] += 1;
-    if ((rc = a4_view(c, flx[d], fb, 7 + c->ns, "flx", J.flx[d]))) return rc;
-    if ((rc = a4_view(c, a[d], fb, 1, "a", J.area[d]))) return rc;
-  }
-  if ((rc = a4_view(c, vol, bx, 1, "vol", J.vol))) return rc;
-  if ((rc = a4_view(c, uout, bx, 7 + c->ns, "uout", J.upd))) return rc;
-  KArgs ka;
-  memset(&ka, 0, sizeof(ka));
-  for (int d = 0; d < 3; ++d) ka.dx[d] = dx[d];
-  ka.difmag = difmag;
-  ka.dt_val = dt;
-  ka.dtp = nullptr;
-  ka.plm_iorder = plm_iorder;
-  ka.use_flattening = use_flattening;
-  const int gu = grid1(ST_UPDATE, J);
-  if (c->eos == PC_EOS_GAMMALAW) {
-    run_ctu<EosGamma>(J, nullptr, nullptr, ka, ppm_type, st);
-    LAUNCH(k_consup<1>, gu, st, J, Sched{nullptr, nullptr, nullptr}, ka);
-  } else {
-    run_ctu<EosLiDryer>(J, nullptr, nullptr, ka, ppm_type, st);
-    LAUNCH(k_consup<9>, gu, st, J, Sched{nullptr, nullptr, nullptr}, ka);
-  }
-  HIPCHK(c, hipGetLastError());
-  return PC_OK;
-}
-
 static void set_domain(Job& J, const int* domlo, const int* domhi,
                        const int* bclo, const int* bchi) {
   for (int d = 0; d < 3; ++d) {
@@ -478,8 +538,17 @@ int pc_umdrv_v1(pc_ctx_t* c, double /*time*/, pc_box_t bx, const int* domlo,
   hipStream_t st = (hipStream_t)stream;
   const WsLayout L = ws_layout(b, c->ns, false, false, false);
   if ((rc = ctx_scratch(c, L.total * sizeof(double), st))) return rc;
-  return umdrv_core(c, J, b, dx, dt, ppm_type, plm_iorder, use_flattening,
-                    difmag, flx, a, vol, uout, st, 0);
+  if ((rc = finish_umdrv_job(c, J, b, c->ws, flx, a, vol, uout))) return rc;
+  const KArgs ka = umdrv_args(dx, dt, plm_iorder, use_flattening, difmag);
+  Sched sc;
+  int slot;
+  if ((rc = ctx_job(c, J, st, &sc, &slot))) return rc;
+  if (c->eos == PC_EOS_GAMMALAW)
+    launch_ctu_consup<EosGamma>(J, sc, ka, ppm_type, st);
+  else
+    launch_ctu_consup<EosLiDryer>(J, sc, ka, ppm_type, st);
+  HIPCHK(c, hipGetLastError());
+  return ctx_job_done(c, slot, st);
 }
 
 int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
@@ -517,21 +586,28 @@ int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
   J.qa = fab_view(c->ws + Lq.off_qa, Lq.q);
   J.srcq = src.p ? fab_view(c->ws + Lq.off_srcq, Lq.q) : null_view();
   for (int d = 0; d < 3; ++d) J.qe[d] = Lq.q.hi[d] + 1;
-  KArgs ka;
-  memset(&ka, 0, sizeof(ka));
+  if ((rc = finish_umdrv_job(c, J, b, c->ws + qpart, flx, a, vol, hyd_src)))
+    return rc;
+  KArgs ka = umdrv_args(dx, dt, prm->plm_iorder, prm->use_flattening,
+                        prm->difmag);
+  // srctoprim reads qaux's dp/de and dp/drho, which ctoprim only writes
+  // when asked for the full primitive set
+  if (src.p) ka.full_prims = 1;
   const int gq = grid1(ST_CTOPRIM, J);
-  const Sched s0{nullptr, nullptr, nullptr};
+  Sched sc;
+  int slot;
+  if ((rc = ctx_job(c, J, st, &sc, &slot))) return rc;
   if (c->eos == PC_EOS_GAMMALAW) {
-    LAUNCH(k_ctoprim<EosGamma>, gq, st, J, s0, ka);
-    if (src.p) LAUNCH(k_srctoprim<1>, gq, st, J, s0, ka, vsrc);
+    LAUNCH(k_ctoprim<EosGamma>, gq, st, sc, ka);
+    if (src.p) LAUNCH(k_srctoprim<1>, gq, st, sc, ka, vsrc);
+    launch_ctu_consup<EosGamma>(J, sc, ka, prm->ppm_type, st);
   } else {
-    LAUNCH(k_ctoprim<EosLiDryer>, gq, st, J, s0, ka);
-    if (src.p) LAUNCH(k_srctoprim<9>, gq, st, J, s0, ka, vsrc);
+    LAUNCH(k_ctoprim<EosLiDryer>, gq, st, sc, ka);
+    if (src.p) LAUNCH(k_srctoprim<9>, gq, st, sc, ka, vsrc);
+    launch_ctu_consup<EosLiDryer>(J, sc, ka, prm->ppm_type, st);
   }
   HIPCHK(c, hipGetLastError());
-  return umdrv_core(c, J, b, dx, dt, prm->ppm_type, prm->plm_iorder,
-                    prm->use_flattening, prm->difmag, flx, a, vol, hyd_src, st,
-                    qpart);
+  return ctx_job_done(c, slot, st);
 }
 
 }  // extern "C"
@@ -553,6 +629,7 @@ struct pc_level {
   DtState* ds = nullptr;       // device
   LevelArgs la;
   KArgs ka;
+  int lowo[3] = {0, 0, 0};  // a Hard/UserBC face exists in direction d
   int timing = 0;
   hipEvent_t ev[PC_NKERN + 1];
   bool ev_made = false;
@@ -720,6 +797,10 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
                     hipMemcpyHostToDevice);
   }
   L->la.cellmap = L->cellmap;
+  for (int dd = 0; dd < 3; ++dd) {
+    auto is = [](int b) { return b == PC_BC_INFLOW || b == PC_BC_USERBC; };
+    L->lowo[dd] = !d->periodic_self && (is(d->bclo[dd]) || is(d->bchi[dd]));
+  }
   L->la.cfl = prm->cfl;
   L->la.max_dt = prm->max_dt;
   L->la.fixed_dt = prm->fixed_dt;
@@ -807,9 +888,9 @@ static int level_fill(pc_level* L, hipStream_t st) {
   Job dummy;
   memset(&dummy, 0, sizeof(dummy));
   if (ns == 1)
-    LAUNCH(k_ghost<1>, L->plans[ST_GHOST].nblocks, st, dummy, L->scs[ST_GHOST], L->ka, L->la);
+    LAUNCH(k_ghost<1>, L->plans[ST_GHOST].nblocks, st, L->scs[ST_GHOST], L->ka, L->la);
   else
-    LAUNCH(k_ghost<9>, L->plans[ST_GHOST].nblocks, st, dummy, L->scs[ST_GHOST], L->ka, L->la);
+    LAUNCH(k_ghost<9>, L->plans[ST_GHOST].nblocks, st, L->scs[ST_GHOST], L->ka, L->la);
   HIPCHK(L->ctx, hipGetLastError());
   return PC_OK;
 }
@@ -825,7 +906,7 @@ template <class EOS>
 static void level_estdt_launch(pc_level* L, hipStream_t st) {
   Job dummy;
   memset(&dummy, 0, sizeof(dummy));
-  LAUNCH(k_estdt<EOS>, L->plans[ST_UPDATE].nblocks, st, dummy, L->scs[ST_UPDATE], L->ka, L->ds);
+  LAUNCH(k_estdt<EOS>, L->plans[ST_UPDATE].nblocks, st, L->scs[ST_UPDATE], L->ka, L->ds);
 }
 
 extern "C" {
@@ -893,51 +974,54 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
     mark();
     level_fill(L, st);
     mark();  // PC_K_GHOST
-    LAUNCH(k_ctoprim<EOS>, P[ST_CTOPRIM].nblocks, st, d0, S[ST_CTOPRIM], ka);
+    LAUNCH(k_ctoprim<EOS>, P[ST_CTOPRIM].nblocks, st, S[ST_CTOPRIM], ka);
     mark();
-    LAUNCH(k_divu, P[ST_DIVU].nblocks, st, d0, S[ST_DIVU], ka);
+    LAUNCH(k_divu, P[ST_DIVU].nblocks, st, S[ST_DIVU], ka);
     mark();
     const int gT = P[ST_TRACE].nblocks;
     if (ppm == 1) {
-      LAUNCH((k_trace_ppm<EOS, 0>), gT, st, d0, S[ST_TRACE], ka);
+      LAUNCH((k_trace_ppm<EOS, 0>), gT, st, S[ST_TRACE], ka);
       mark();
-      LAUNCH((k_trace_ppm<EOS, 1>), gT, st, d0, S[ST_TRACE], ka);
+      LAUNCH((k_trace_ppm<EOS, 1>), gT, st, S[ST_TRACE], ka);
       mark();
-      LAUNCH((k_trace_ppm<EOS, 2>), gT, st, d0, S[ST_TRACE], ka);
+      LAUNCH((k_trace_ppm<EOS, 2>), gT, st, S[ST_TRACE], ka);
       mark();
     } else {
-      LAUNCH((k_trace_plm<EOS, 0>), gT, st, d0, S[ST_TRACE], ka);
+      LAUNCH((k_trace_plm<EOS, 0>), gT, st, S[ST_TRACE], ka);
       mark();
-      LAUNCH((k_trace_plm<EOS, 1>), gT, st, d0, S[ST_TRACE], ka);
+      LAUNCH((k_trace_plm<EOS, 1>), gT, st, S[ST_TRACE], ka);
       mark();
-      LAUNCH((k_trace_plm<EOS, 2>), gT, st, d0, S[ST_TRACE], ka);
+      LAUNCH((k_trace_plm<EOS, 2>), gT, st, S[ST_TRACE], ka);
       mark();
     }
-    LAUNCH((k_flux1<EOS, 0>), P[ST_FLUX1_X].nblocks, st, d0, S[ST_FLUX1_X], ka);
+    LAUNCH((k_flux1<EOS, 0>), P[ST_FLUX1_X].nblocks, st, S[ST_FLUX1_X], ka);
     mark();
-    LAUNCH((k_flux1<EOS, 1>), P[ST_FLUX1_Y].nblocks, st, d0, S[ST_FLUX1_Y], ka);
+    LAUNCH((k_flux1<EOS, 1>), P[ST_FLUX1_Y].nblocks, st, S[ST_FLUX1_Y], ka);
     mark();
-    LAUNCH((k_flux1<EOS, 2>), P[ST_FLUX1_Z].nblocks, st, d0, S[ST_FLUX1_Z], ka);
+    LAUNCH((k_flux1<EOS, 2>), P[ST_FLUX1_Z].nblocks, st, S[ST_FLUX1_Z], ka);
     mark();
-    LAUNCH((k_trans<EOS, 0, 1>), P[ST_TRANS_XY].nblocks, st, d0, S[ST_TRANS_XY], ka);
+    LAUNCH((k_trans<EOS, 0, 1>), P[ST_TRANS_XY].nblocks, st, S[ST_TRANS_XY], ka);
     mark();
-    LAUNCH((k_trans<EOS, 0, 2>), P[ST_TRANS_XZ].nblocks, st, d0, S[ST_TRANS_XZ], ka);
+    LAUNCH((k_trans<EOS, 0, 2>), P[ST_TRANS_XZ].nblocks, st, S[ST_TRANS_XZ], ka);
     mark();
-    LAUNCH((k_trans<EOS, 1, 0>), P[ST_TRANS_YX].nblocks, st, d0, S[ST_TRANS_YX], ka);
+    LAUNCH((k_trans<EOS, 1, 0>), P[ST_TRANS_YX].nblocks, st, S[ST_TRANS_YX], ka);
     mark();
-    LAUNCH((k_trans<EOS, 1, 2>), P[ST_TRANS_YZ].nblocks, st, d0, S[ST_TRANS_YZ], ka);
+    LAUNCH((k_trans<EOS, 1, 2>), P[ST_TRANS_YZ].nblocks, st, S[ST_TRANS_YZ], ka);
     mark();
-    LAUNCH((k_trans<EOS, 2, 0>), P[ST_TRANS_ZX].nblocks, st, d0, S[ST_TRANS_ZX], ka);
+    LAUNCH((k_trans<EOS, 2, 0>), P[ST_TRANS_ZX].nblocks, st, S[ST_TRANS_ZX], ka);
     mark();
-    LAUNCH((k_trans<EOS, 2, 1>), P[ST_TRANS_ZY].nblocks, st, d0, S[ST_TRANS_ZY], ka);
+    LAUNCH((k_trans<EOS, 2, 1>), P[ST_TRANS_ZY].nblocks, st, S[ST_TRANS_ZY], ka);
     mark();
-    LAUNCH((k_final<EOS, 0>), P[ST_FINAL_X].nblocks, st, d0, S[ST_FINAL_X], ka);
+    LAUNCH((k_final<EOS, 0, false>), P[ST_FINAL_X].nblocks, st, S[ST_FINAL_X], ka);
     mark();
-    LAUNCH((k_final<EOS, 1>), P[ST_FINAL_Y].nblocks, st, d0, S[ST_FINAL_Y], ka);
+    LAUNCH((k_final<EOS, 1, false>), P[ST_FINAL_Y].nblocks, st, S[ST_FINAL_Y], ka);
     mark();
-    LAUNCH((k_final<EOS, 2>), P[ST_FINAL_Z].nblocks, st, d0, S[ST_FINAL_Z], ka);
+    LAUNCH((k_final<EOS, 2, false>), P[ST_FINAL_Z].nblocks, st, S[ST_FINAL_Z], ka);
+    if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0>), P[ST_FINAL_X].nblocks, st, S[ST_FINAL_X], ka);
+    if (L->lowo[1]) LAUNCH((k_final_loworder<EOS, 1>), P[ST_FINAL_Y].nblocks, st, S[ST_FINAL_Y], ka);
+    if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2>), P[ST_FINAL_Z].nblocks, st, S[ST_FINAL_Z], ka);
     mark();
-    LAUNCH(k_update<EOS>, P[ST_UPDATE].nblocks, st, d0, S[ST_UPDATE], ka, L->ds);
+    LAUNCH(k_update<EOS>, P[ST_UPDATE].nblocks, st, S[ST_UPDATE], ka, L->ds);
     mark();
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 0);
     mark();
@@ -995,11 +1079,11 @@ int pc_level_prim_to_cons_v1(pc_level_t* L, int b, pc_array4_t prim, int mode,
   if ((rc = a4_view(L->ctx, prim, bx, 5 + L->ctx->ns, "prim", pv))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const int g = grid1(ST_UPDATE, J);
-  const Sched s0{nullptr, nullptr, nullptr};
+  const Sched s0{L->jobs + b, nullptr, nullptr};
   if (L->ctx->ns == 1)
-    LAUNCH(k_prim_to_cons<EosGamma>, g, st, J, s0, L->ka, pv, mode);
+    LAUNCH(k_prim_to_cons<EosGamma>, g, st, s0, L->ka, pv, mode);
   else
-    LAUNCH(k_prim_to_cons<EosLiDryer>, g, st, J, s0, L->ka, pv, mode);
+    LAUNCH(k_prim_to_cons<EosLiDryer>, g, st, s0, L->ka, pv, mode);
   HIPCHK(L->ctx, hipGetLastError());
   return PC_OK;
 }

@@ -22,13 +22,17 @@ namespace pcd {
 
 constexpr int BLOCK = 256;
 
+// Global-memory double (address space 1): loads and stores through it are
+// global_load/global_store rather than flat accesses.
+typedef __attribute__((address_space(1))) double gdouble;
+
 // SoA view of one FAB (Array4 with 32-bit strides).
 struct V {
   double* p;
   int js, ks, ns;
   int b0, b1, b2;
-  PCD double& operator()(int i, int j, int k, int n) const {
-    return p[(i - b0) + (j - b1) * js + (k - b2) * ks + n * ns];
+  PCD gdouble& operator()(int i, int j, int k, int n) const {
+    return ((gdouble*)p)[(i - b0) + (j - b1) * js + (k - b2) * ks + n * ns];
   }
   PCD bool contains(int i, int j, int k, int e0, int e1, int e2) const {
     return i >= b0 && i < e0 && j >= b1 && j < e1 && k >= b2 && k < e2;
@@ -64,6 +68,7 @@ struct KArgs {
   int plm_iorder;
   int use_flattening;
   int nblocks;
+  int full_prims;  // 1: write every q/qaux comp; 0: only what the step reads
 };
 
 PCD double get_dt(const KArgs& a) { return a.dtp ? *a.dtp : a.dt_val; }
@@ -77,14 +82,35 @@ struct Sched {
   const int* blkstart;
 };
 
-PCD const Job& job_of(const Job& j0, const Sched& s, int& local) {
-  if (!s.jobs) {
-    local = blockIdx.x * BLOCK + threadIdx.x;
-    return j0;
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i % 8).  Renumber them so that each XCD walks one contiguous range of the
+// logical block order: stencil neighbours (the next rows and planes of a
+// box) then sit in the L2 of the XCD that just read them.
+PCD int xcd_swizzle(int i, int n) {
+  const int x = i & 7, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + (i >> 3);
+}
+
+// The job table is read-only during a launch: read it through the constant
+// address space so the (block-uniform) descriptor fields are scalar loads.
+typedef const __attribute__((address_space(4))) Job CJob;
+typedef const __attribute__((address_space(4))) int CInt;
+
+PCD Job load_job(const Sched& s, int& local) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  int b = 0, start = 0;
+  if (s.blkmap) {
+    b = ((CInt*)s.blkmap)[blk];
+    start = ((CInt*)s.blkstart)[b];
   }
-  const int b = s.blkmap[blockIdx.x];
-  local = (blockIdx.x - s.blkstart[b]) * BLOCK + threadIdx.x;
-  return s.jobs[b];
+  local = (blk - start) * BLOCK + threadIdx.x;
+  return ((CJob*)s.jobs)[b];
+#else
+  // the host compilation pass parses device code but never runs it
+  local = 0;
+  return *s.jobs;
+#endif
 }
 
 PCD bool region_ijk(const int lo[3], const int n[3], int local, int& i,
@@ -213,10 +239,10 @@ PCD void step(int& i, int& j, int& k, int s) {
 // Source/PeleCUtilities.H:242-320
 template <class EOS>
 __global__ void __launch_bounds__(BLOCK)
-    k_ctoprim(Job j0, Sched sc, KArgs ka) {
+    k_ctoprim(Sched sc, KArgs ka) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -238,38 +264,41 @@ __global__ void __launch_bounds__(BLOCK)
     Y[s] = y;
   }
   const double e = (u(i, j, k, UEDEN) - kineng) * rhoinv;
-  const double T = EOS::EY2T(e, Y, u(i, j, k, UTEMP));
-  const double w = EOS::wbar(Y);
-  const double p = EOS::RTY2P(rho, T, Y);
-  const double cs = EOS::RTY2Cs(rho, T, Y);
-  const double gam1 = EOS::RTY2G(rho, T, Y);
-  double dpde, dpdr_e;
-  EOS::RTY2dpde_dpdre(rho, T, Y, dpde, dpdr_e);
+  const typename EOS::Thermo th(Y);
+  const double T = th.EY2T(e, u(i, j, k, UTEMP));
+  const double p = th.RTY2P(rho, T);
+  const double cs = th.RTY2Cs(rho, T);
+  const double gam1 = th.RTY2G(rho, T);
   q(i, j, k, QRHO) = rho;
   q(i, j, k, QU) = vx;
   q(i, j, k, QV) = vy;
   q(i, j, k, QW) = vz;
 #pragma unroll
   for (int s = 0; s < NS; ++s) q(i, j, k, QFS + s) = Y[s];
-  q(i, j, k, QTEMP) = T;
   q(i, j, k, QREINT) = e * rho;
   q(i, j, k, QPRES) = p;
-  q(i, j, k, QGAME) = p / (e * rho) + 1.0;
-  qa(i, j, k, QDPDR) = dpdr_e;
-  qa(i, j, k, QDPDE) = dpde;
   qa(i, j, k, QGAMC) = gam1;
   qa(i, j, k, QC) = cs;
-  qa(i, j, k, QCSML) = mx(small_num, small_num * cs);
-  qa(i, j, k, QRSPEC) = RU / w;
+  if (ka.full_prims) {
+    const double w = th.wbar();
+    double dpde, dpdr_e;
+    th.RTY2dpde_dpdre(rho, T, dpde, dpdr_e);
+    q(i, j, k, QTEMP) = T;
+    q(i, j, k, QGAME) = p / (e * rho) + 1.0;
+    qa(i, j, k, QDPDR) = dpdr_e;
+    qa(i, j, k, QDPDE) = dpde;
+    qa(i, j, k, QCSML) = mx(small_num, small_num * cs);
+    qa(i, j, k, QRSPEC) = RU / w;
+  }
 }
 
 // ---------------------------------------------------------- K2 srctoprim --
 // Source/Hydro.H:14-68 (src: NVAR comps over the q box)
 template <int NS>
 __global__ void __launch_bounds__(BLOCK)
-    k_srctoprim(Job j0, Sched sc, KArgs ka, V src) {
+    k_srctoprim(Sched sc, KArgs ka, V src) {
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -303,9 +332,9 @@ __global__ void __launch_bounds__(BLOCK)
 
 // --------------------------------------------------------------- K24 divu --
 // Source/Hydro.H:476-517 on the nodes of the valid box
-__global__ void __launch_bounds__(BLOCK) k_divu(Job j0, Sched sc, KArgs ka) {
+__global__ void __launch_bounds__(BLOCK) k_divu(Sched sc, KArgs ka) {
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_DIVU, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -356,10 +385,10 @@ PCD double flat_cell(const Job& J, int i, int j, int k) {
 // Source/PPM.cpp:82-423 for one cell and one direction (no WENO).
 template <class EOS, int D>
 __global__ void __launch_bounds__(BLOCK)
-    k_trace_ppm(Job j0, Sched sc, KArgs ka) {
+    k_trace_ppm(Sched sc, KArgs ka) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -516,10 +545,10 @@ __global__ void __launch_bounds__(BLOCK)
 // Source/Godunov.cpp:139-180 (one direction per launch)
 template <class EOS, int D>
 __global__ void __launch_bounds__(BLOCK)
-    k_trace_plm(Job j0, Sched sc, KArgs ka) {
+    k_trace_plm(Sched sc, KArgs ka) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -565,10 +594,10 @@ __global__ void __launch_bounds__(BLOCK)
 // -------------------------------------------------- K6-8 first fluxes ----
 template <class EOS, int D>
 __global__ void __launch_bounds__(BLOCK)
-    k_flux1(Job j0, Sched sc, KArgs ka) {
+    k_flux1(Sched sc, KArgs ka) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_FLUX1_X + D, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -605,10 +634,10 @@ PCD void dflux(const V& f, int i, int j, int k, double c, DFlux<NS>& D) {
 // (Source/Godunov.cpp:246-384).
 template <class EOS, int D, int T>
 __global__ void __launch_bounds__(BLOCK)
-    k_trans(Job j0, Sched sc, KArgs ka) {
+    k_trans(Sched sc, KArgs ka) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRANS_XY + 2 * D + tslot(D, T), J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -633,7 +662,7 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // transdd flux differences across cell (i,j,k) for final direction D
-template <int NS, int D>
+template <int NS, int D, bool SRC>
 PCD void final_corr(const Job& J, int i, int j, int k, double hdt,
                     double c0, double c1, const FaceState<NS>& s,
                     FaceState<NS>& o) {
@@ -668,7 +697,7 @@ PCD void final_corr(const Job& J, int i, int j, int k, double hdt,
   const double pxnew = c0 * (dAupx + pavx * dAux * (gamc - 1.0));
   const double pynew = c1 * (dAupy + pavy * dAuy * (gamc - 1.0));
   double src[6 + NS];
-  if (J.srcq.p) {
+  if (SRC) {
     src[0] = J.srcq(i, j, k, QRHO);
     src[1] = J.srcq(i, j, k, QU);
     src[2] = J.srcq(i, j, k, QV);
@@ -722,44 +751,25 @@ PCD void plm_cell(const Job& J, const KArgs& ka, double dtdx, int i, int j,
 }
 
 // ----------------------------- K17-22 + K25-27 final fluxes + adjust ----
-// pc_transdd + pc_cmpflx (Source/Godunov.cpp:392-438), the low-order
-// boundary override (:440-468), then pc_artif_visc, pc_norm_spec_flx and
-// pc_ext_flx (Source/Hydro.cpp:380-413).
+// pc_transdd + pc_cmpflx (Source/Godunov.cpp:392-438), then pc_artif_visc,
+// pc_norm_spec_flx and pc_ext_flx (Source/Hydro.cpp:380-413) on the face.
+// Faces of Hard/UserBC domain boundaries are left to k_final_loworder.
+PCD bool loworder_face(const Job& J, int D, int idx) {
+  return (idx == J.dlo[D] && (J.bclo[D] == BC_INFLOW || J.bclo[D] == BC_USER)) ||
+         (idx == J.dhi[D] + 1 &&
+          (J.bchi[D] == BC_INFLOW || J.bchi[D] == BC_USER));
+}
+
 template <class EOS, int D>
-__global__ void __launch_bounds__(BLOCK)
-    k_final(Job j0, Sched sc, KArgs ka) {
+PCD void final_tail(const Job& J, const KArgs& ka, int i, int j, int k,
+                    const FaceState<EOS::NS>& L, const FaceState<EOS::NS>& R) {
   constexpr int NS = EOS::NS;
   constexpr int NVAR = 7 + NS;
   constexpr int T0 = (D == 0) ? 1 : 0;
   constexpr int T1 = (D == 2) ? 1 : 2;
-  int local;
-  const Job& J = job_of(j0, sc, local);
-  int lo[3], n[3], i, j, k;
-  stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
-  const double dt = get_dt(ka);
-  const double hdt = 0.5 * dt;
-  const double c0 = 0.5 * dt / ka.dx[T0];
-  const double c1 = 0.5 * dt / ka.dx[T1];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-  const bool lowo =
-      (idx == J.dlo[D] && (J.bclo[D] == BC_INFLOW || J.bclo[D] == BC_USER)) ||
-      (idx == J.dhi[D] + 1 && (J.bchi[D] == BC_INFLOW || J.bchi[D] == BC_USER));
-  FaceState<NS> L, R;
-  if (!lowo) {
-    FaceState<NS> s;
-    load_trace<NS>(J.tr[D], i, j, k, 0, s);
-    final_corr<NS, D>(J, im, jm, km, hdt, c0, c1, s, L);
-    load_trace<NS>(J.tr[D], i, j, k, 1, s);
-    final_corr<NS, D>(J, i, j, k, hdt, c0, c1, s, R);
-  } else {
-    const double dtdx = dt / ka.dx[D];
-    FaceState<NS> unused;
-    plm_cell<EOS, D>(J, ka, dtdx, im, jm, km, unused, L);
-    plm_cell<EOS, D>(J, ka, dtdx, i, j, k, R, unused);
-  }
   const double cav = 0.5 * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
   Flux<NS> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
@@ -816,6 +826,56 @@ __global__ void __launch_bounds__(BLOCK)
   J.qec[D](i, j, k, 1) = F.ugd;
 }
 
+template <class EOS, int D, bool SRC>
+__global__ void __launch_bounds__(BLOCK)
+    k_final(Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  constexpr int T0 = (D == 0) ? 1 : 0;
+  constexpr int T1 = (D == 2) ? 1 : 2;
+  int local;
+  const Job J = load_job(sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  if (loworder_face(J, D, idx)) return;
+  const double dt = get_dt(ka);
+  const double hdt = 0.5 * dt;
+  const double c0 = 0.5 * dt / ka.dx[T0];
+  const double c1 = 0.5 * dt / ka.dx[T1];
+  int im = i, jm = j, km = k;
+  step<D>(im, jm, km, -1);
+  FaceState<NS> L, R, s;
+  load_trace<NS>(J.tr[D], i, j, k, 0, s);
+  final_corr<NS, D, SRC>(J, im, jm, km, hdt, c0, c1, s, L);
+  load_trace<NS>(J.tr[D], i, j, k, 1, s);
+  final_corr<NS, D, SRC>(J, i, j, k, hdt, c0, c1, s, R);
+  final_tail<EOS, D>(J, ka, i, j, k, L, R);
+}
+
+// pc_low_order_boundary (Source/Godunov.cpp:7-55,440-468): at Hard/UserBC
+// domain faces the flux comes from first-order-in-time PLM states without
+// transverse terms.  Launched only when such a boundary exists.
+template <class EOS, int D>
+__global__ void __launch_bounds__(BLOCK)
+    k_final_loworder(Sched sc, KArgs ka) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const Job J = load_job(sc, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
+  if (!region_ijk(lo, n, local, i, j, k)) return;
+  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  if (!loworder_face(J, D, idx)) return;
+  const double dtdx = get_dt(ka) / ka.dx[D];
+  int im = i, jm = j, km = k;
+  step<D>(im, jm, km, -1);
+  FaceState<NS> L, R, unused;
+  plm_cell<EOS, D>(J, ka, dtdx, im, jm, km, unused, L);
+  plm_cell<EOS, D>(J, ka, dtdx, i, j, k, R, unused);
+  final_tail<EOS, D>(J, ka, i, j, k, L, R);
+}
+
 // pdivu (Source/Godunov.H:1033-1060) and the flux divergence of one cell
 // (Source/Hydro.H:888-907); h receives dU/dt.
 template <int NS>
@@ -852,10 +912,10 @@ PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
 // ----------------------------------------------------------- K28 consup --
 template <int NS>
 __global__ void __launch_bounds__(BLOCK)
-    k_consup(Job j0, Sched sc, KArgs ka) {
+    k_consup(Sched sc, KArgs ka) {
   constexpr int NVAR = 7 + NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;

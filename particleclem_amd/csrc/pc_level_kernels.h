@@ -41,10 +41,10 @@ PCD int ghost_src(int x, int lo, int hi, int bcl, int bch, int& refl) {
 
 template <int NS>
 __global__ void __launch_bounds__(BLOCK)
-    k_ghost(Job j0, Sched sc, KArgs ka, LevelArgs la) {
+    k_ghost(Sched sc, KArgs ka, LevelArgs la) {
   constexpr int NVAR = 7 + NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_GHOST, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -62,11 +62,11 @@ __global__ void __launch_bounds__(BLOCK)
   const int s0 = ghost_src(i, D0[0], D1[0], bl[0], bh[0], r0);
   const int s1 = ghost_src(j, D0[1], D1[1], bl[1], bh[1], r1);
   const int s2 = ghost_src(k, D0[2], D1[2], bl[2], bh[2], r2);
-  const V* src = &J.U;
+  V src = J.U;
   if (!la.periodic_self) {
     const long nx = D1[0] - D0[0] + 1, ny = D1[1] - D0[1] + 1;
     const int sb = la.cellmap[(s0 - D0[0]) + nx * ((s1 - D0[1]) + ny * (long)(s2 - D0[2]))];
-    src = &sc.jobs[sb].U;
+    src = sc.jobs[sb].U;
   }
   const int r[3] = {r0, r1, r2};
 #pragma unroll
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(BLOCK)
       const bool mom = (m == UMX || m == UMY || m == UMZ);
       if ((r[d] == 2 && mom) || (r[d] == 1 && m == UMX + d)) sg = -sg;
     }
-    J.U(i, j, k, m) = sg * (*src)(s0, s1, s2, m);
+    J.U(i, j, k, m) = sg * src(s0, s1, s2, m);
   }
 }
 
@@ -101,33 +101,38 @@ PCD void block_min_to(double v, unsigned long long* dst) {
   }
 }
 
-// pc_estdt_hydro for one cell (Source/Timestep.H:62-84)
-template <class EOS>
-PCD double cell_estdt(const V& u, int i, int j, int k, const double* dx) {
-  constexpr int NS = EOS::NS;
-  const double rho = u(i, j, k, URHO);
-  const double rhoInv = 1.0 / rho;
-  const double T = u(i, j, k, UTEMP);
-  double Y[NS];
+// Mass fractions of a conserved state
+template <int NS>
+PCD void state_Y(const double* S, double* Y) {
+  const double rhoInv = 1.0 / S[URHO];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) Y[s] = u(i, j, k, UFS + s) * rhoInv;
-  const double c = EOS::RTY2Cs(rho, T, Y);
-  double dt = 1.0e308 * 1.79;  // > any finite candidate; min() replaces it
-  dt = 1.7976931348623157e308;
-  const double ux = u(i, j, k, UMX) * rhoInv;
+  for (int s = 0; s < NS; ++s) Y[s] = S[UFS + s] * rhoInv;
+}
+
+// pc_estdt_hydro for one cell (Source/Timestep.H:62-84) from its conserved
+// state and the thermodynamics of its composition
+template <class EOS>
+PCD double cell_estdt(const double* S, const typename EOS::Thermo& th,
+                      const double* dx) {
+  const double rho = S[URHO];
+  const double rhoInv = 1.0 / rho;
+  const double T = S[UTEMP];
+  const double c = th.RTY2Cs(rho, T);
+  double dt = 1.7976931348623157e308;  // numeric_limits<double>::max()
+  const double ux = S[UMX] * rhoInv;
   dt = mn(dt, dx[0] / (c + fabs(ux)));
-  const double uy = u(i, j, k, UMY) * rhoInv;
+  const double uy = S[UMY] * rhoInv;
   dt = mn(dt, dx[1] / (c + fabs(uy)));
-  const double uz = u(i, j, k, UMZ) * rhoInv;
+  const double uz = S[UMZ] * rhoInv;
   dt = mn(dt, dx[2] / (c + fabs(uz)));
   return dt;
 }
 
 // reset_internal_energy with allow_small_energy = allow_negative_energy = 1
-// (Source/PeleCUtilities.H:227-236) and pc_cmpTemp (:70-84).
+// (Source/PeleCUtilities.H:227-236) and pc_cmpTemp (:70-84).  Neither
+// changes rho or rho Y, so th (the composition's thermodynamics) holds.
 template <class EOS>
-PCD void reset_and_temp(double* S) {
-  constexpr int NS = EOS::NS;
+PCD void reset_and_temp(double* S, const typename EOS::Thermo& th) {
   {
     const double rho = S[URHO];
     const double rhoInv = 1.0 / rho;
@@ -140,10 +145,7 @@ PCD void reset_and_temp(double* S) {
   }
   const double rhoInv = 1.0 / S[URHO];
   const double e = S[UEINT] * rhoInv;
-  double Y[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) Y[s] = S[UFS + s] * rhoInv;
-  S[UTEMP] = EOS::EY2T(e, Y, S[UTEMP]);
+  S[UTEMP] = th.EY2T(e, S[UTEMP]);
 }
 
 // Fused per-cell step end: dU/dt from the fluxes, S_new = S_old + dt dU/dt
@@ -152,11 +154,11 @@ PCD void reset_and_temp(double* S) {
 // CFL partial minimum on the new state.
 template <class EOS>
 __global__ void __launch_bounds__(BLOCK)
-    k_update(Job j0, Sched sc, KArgs ka, DtState* ds) {
+    k_update(Sched sc, KArgs ka, DtState* ds) {
   constexpr int NS = EOS::NS;
   constexpr int NVAR = 7 + NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   double mydt = 1.7976931348623157e308;
@@ -169,8 +171,11 @@ __global__ void __launch_bounds__(BLOCK)
     double S[NVAR];
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) S[m] = J.U(i, j, k, m) + dt * h[m];
-    reset_and_temp<EOS>(S);
-    reset_and_temp<EOS>(S);
+    double Y[NS];
+    state_Y<NS>(S, Y);
+    const typename EOS::Thermo th(Y);
+    reset_and_temp<EOS>(S, th);
+    reset_and_temp<EOS>(S, th);
     bool bad = false;
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) {
@@ -178,7 +183,7 @@ __global__ void __launch_bounds__(BLOCK)
       bad |= !isfinite(S[m]);
     }
     if (bad) ds->bad = 1;
-    mydt = cell_estdt<EOS>(J.U, i, j, k, ka.dx);
+    mydt = cell_estdt<EOS>(S, th, ka.dx);
   }
   block_min_to(mydt, &ds->minbits);
 }
@@ -186,13 +191,22 @@ __global__ void __launch_bounds__(BLOCK)
 // estTimeStep partial minimum over the valid cells (no update)
 template <class EOS>
 __global__ void __launch_bounds__(BLOCK)
-    k_estdt(Job j0, Sched sc, KArgs ka, DtState* ds) {
+    k_estdt(Sched sc, KArgs ka, DtState* ds) {
+  constexpr int NS = EOS::NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   double mydt = 1.7976931348623157e308;
-  if (region_ijk(lo, n, local, i, j, k)) mydt = cell_estdt<EOS>(J.U, i, j, k, ka.dx);
+  if (region_ijk(lo, n, local, i, j, k)) {
+    double S[7 + NS];
+#pragma unroll
+    for (int m = 0; m < 7 + NS; ++m) S[m] = J.U(i, j, k, m);
+    double Y[NS];
+    state_Y<NS>(S, Y);
+    const typename EOS::Thermo th(Y);
+    mydt = cell_estdt<EOS>(S, th, ka.dx);
+  }
   block_min_to(mydt, &ds->minbits);
 }
 
@@ -227,10 +241,10 @@ __global__ void k_dt_finalize(DtState* ds, LevelArgs la, int init) {
 // primitive -> conservative (pc_initdata pattern; see workloads.py)
 template <class EOS>
 __global__ void __launch_bounds__(BLOCK)
-    k_prim_to_cons(Job j0, Sched sc, KArgs ka, V prim, int mode) {
+    k_prim_to_cons(Sched sc, KArgs ka, V prim, int mode) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job& J = job_of(j0, sc, local);
+  const Job J = load_job(sc, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   if (!region_ijk(lo, n, local, i, j, k)) return;
@@ -243,7 +257,8 @@ __global__ void __launch_bounds__(BLOCK)
     rho = prim(i, j, k, 0);
     const double p = prim(i, j, k, 1);
     e = EOS::RYP2E(rho, Y, p);
-    T = EOS::EY2T(e, Y, 1.0);
+    const typename EOS::Thermo th(Y);
+    T = th.EY2T(e, 1.0);
   } else {
     const double p = prim(i, j, k, 0);
     T = prim(i, j, k, 1);

@@ -2,11 +2,16 @@
 // and tracing, the approximate Riemann solver and the CTU transverse
 // corrections, as register-level functions of one cell or one face.
 //
-// Arithmetic follows the reference expressions term by term (the same
-// products, sums and roundings; no FMA contraction: built with
-// -ffp-contract=off), so results are reproducible against a CPU restatement
-// of the reference.  Data movement is this engine's own: states live in
-// registers (FaceState), stencils are gathered by the kernels.
+// Hydrodynamics arithmetic follows the reference expressions term by term
+// (the same products, sums and roundings; no FMA contraction: built with
+// -ffp-contract=off), so the gamma-law path reproduces a CPU restatement of
+// the reference bit for bit.  The mixture EOS is the exception: its NASA
+// polynomials are evaluated by Horner's rule in explicit FMAs from
+// compile-time coefficients pre-scaled by R/W, and the e -> T Newton
+// iteration runs on mass-weighted mixture coefficients formed once per call
+// (same functions, different rounding: agreement to ~1e-15 relative).
+// Data movement is this engine's own: states live in registers (FaceState),
+// stencils are gathered by the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -61,12 +66,6 @@ struct EosGamma {
   PCD static double RPY2Cs(double R, double P, const double* /*Y*/) {
     return sqrt(gamma * P / R);
   }
-  // RYP2E + RPY2Cs at one state
-  PCD static void RPY2ECs(double R, double P, const double* Y, double& E,
-                          double& Cs) {
-    E = RYP2E(R, Y, P);
-    Cs = RPY2Cs(R, P, Y);
-  }
   PCD static double EY2T(double E, const double* Y, double /*Tguess*/) {
     const double w = wbar(Y);
     const double poverrho = (gamma - 1.0) * E;
@@ -94,13 +93,26 @@ struct EosGamma {
     e += Y[0] * ei0;
     return e;
   }
+  // state functions of one composition (the interface of the mixture EOS)
+  struct Thermo {
+    double Y[1];
+    PCD explicit Thermo(const double* Y_) { Y[0] = Y_[0]; }
+    PCD double wbar() const { return EosGamma::wbar(Y); }
+    PCD double EY2T(double E, double Tg) const { return EosGamma::EY2T(E, Y, Tg); }
+    PCD double RTY2P(double R, double T) const { return EosGamma::RTY2P(R, T, Y); }
+    PCD double RTY2G(double R, double T) const { return EosGamma::RTY2G(R, T, Y); }
+    PCD double RTY2Cs(double R, double T) const { return EosGamma::RTY2Cs(R, T, Y); }
+    PCD void RTY2dpde_dpdre(double R, double T, double& dpde, double& dpdr_e) const {
+      EosGamma::RTY2dpde_dpdre(R, T, Y, dpde, dpdr_e);
+    }
+  };
 };
 
 // Ideal-gas mixture, 9 species (H2 O2 H2O H O OH HO2 H2O2 N2), NASA
 // 7-coefficient polynomials with T_mid = 1000 K (Li-Dryer thermo data).
-// Tables hold the coefficient combinations the formulas use:
-//   cv/R  = c0 + a1 T + a2 T^2 + a3 T^3 + a4 T^4          (c0 = a0 - 1)
-//   e/RT  = c0 + (a1/2) T + (a2/3) T^2 + (a3/4) T^3 + (a4/5) T^4 + a5/T
+// Tables hold a0..a5 of each species for T < 1000 K (lo) and T >= 1000 K
+// (hi):  cp/R = a0 + a1 T + a2 T^2 + a3 T^3 + a4 T^4,
+//        h/RT = a0 + a1 T/2 + a2 T^2/3 + a3 T^3/4 + a4 T^4/5 + a5/T.
 struct LiDryerTab {
   static constexpr double mw[9] = {2.01594,  31.9988,  18.01534,
                                    1.00797,  15.9994,  17.00737,
@@ -143,85 +155,77 @@ struct LiDryerTab {
        -9.227977e+02}};
 };
 
-// Coefficient combinations of species K as compile-time literals (constexpr
-// locals are evaluated by the compiler in IEEE double, the same roundings as
-// the run-time expressions they replace).
+// Species K's polynomials in one NASA range with every coefficient
+// pre-multiplied by RU / W_K (compile-time literals):
+//   cv_K(T) = v0 + v1 T + v2 T^2 + v3 T^3 + v4 T^4                erg/(g K)
+//   e_K(T)  = u0 + u1 T + u2 T^2 + u3 T^3 + u4 T^4 + u5 T^5       erg/g
+// v = RU/W (a0 - 1, a1, a2, a3, a4), u = RU/W (a5, a0 - 1, a1/2, a2/3,
+// a3/4, a4/5): e = RT (a0 - 1 + a1 T/2 + ... + a4 T^4/5 + a5/T) multiplied
+// through by T, so no division.  Horner's rule in fused multiply-adds.
 template <int K, bool LO>
 struct Nasa {
   static constexpr const double* a = LO ? LiDryerTab::lo[K] : LiDryerTab::hi[K];
-  static constexpr double c0 = a[0] - 1.0;
-  static constexpr double e1 = a[1] / 2.0, e2 = a[2] / 3.0, e3 = a[3] / 4.0,
-                          e4 = a[4] / 5.0, e5 = a[5];
-  static constexpr double c1 = a[1], c2 = a[2], c3 = a[3], c4 = a[4];
+  static constexpr double s = RU / LiDryerTab::mw[K];
   static constexpr double imw = 1.0 / LiDryerTab::mw[K];
-  // e_K / (R T) and cv_K / R
-  PCD static double eRT(double T, double T2, double T3, double T4,
-                        double invT) {
-    return c0 + e1 * T + e2 * T2 + e3 * T3 + e4 * T4 + e5 * invT;
+  static constexpr bool flat =
+      a[1] == 0.0 && a[2] == 0.0 && a[3] == 0.0 && a[4] == 0.0;
+  static constexpr double v0 = (a[0] - 1.0) * s, v1 = a[1] * s, v2 = a[2] * s,
+                          v3 = a[3] * s, v4 = a[4] * s;
+  static constexpr double u0 = a[5] * s, u1 = v0, u2 = a[1] / 2.0 * s,
+                          u3 = a[2] / 3.0 * s, u4 = a[3] / 4.0 * s,
+                          u5 = a[4] / 5.0 * s;
+  PCD static double cv(double T) {
+    if constexpr (flat) return v0;
+    return fma(T, fma(T, fma(T, fma(T, v4, v3), v2), v1), v0);
   }
-  PCD static double cvR(double T, double T2, double T3, double T4) {
-    return c0 + c1 * T + c2 * T2 + c3 * T3 + c4 * T4;
+  PCD static double e(double T) {
+    if constexpr (flat) return fma(T, u1, u0);
+    return fma(T, fma(T, fma(T, fma(T, fma(T, u5, u4), u3), u2), u1), u0);
   }
 };
 
+// e(T) = sum_m U[m] T^m and its derivative cv(T), Horner / FMA
+PCD double poly_e(const double* U, double T) {
+  return fma(T, fma(T, fma(T, fma(T, fma(T, U[5], U[4]), U[3]), U[2]), U[1]),
+             U[0]);
+}
+PCD double poly_cv(const double* U, double T) {
+  return fma(T, fma(T, fma(T, fma(T, 5.0 * U[5], 4.0 * U[4]), 3.0 * U[3]),
+                    2.0 * U[2]),
+             U[1]);
+}
+
 struct EosLiDryer {
   static constexpr int NS = 9;
-  using D = LiDryerTab;
+  static constexpr double TMID = 1000.0;
   using Seq = std::make_integer_sequence<int, 9>;
 
-  // sum_k Y_k e_k(T) (erg/g), CKUBMS; the sum runs k = 0..8 in order
+  // sum_k Y_k e_k(T) (erg/g) and sum_k Y_k cv_k(T), one range
   template <bool LO, int... K>
-  PCD static double ubms_seq(double T, const double* Y,
-                             std::integer_sequence<int, K...>) {
-    const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, invT = 1.0 / T;
-    const double RT = RU * T;
+  PCD static double e_seq(double T, const double* Y,
+                          std::integer_sequence<int, K...>) {
     double E = 0.0;
-    ((E += Y[K] * (Nasa<K, LO>::eRT(T, T2, T3, T4, invT) * RT *
-                   Nasa<K, LO>::imw)),
-     ...);
+    ((E = fma(Y[K], Nasa<K, LO>::e(T), E)), ...);
     return E;
   }
-  // sum_k Y_k cv_k(T), CKCVBS
   template <bool LO, int... K>
-  PCD static double cvbs_seq(double T, const double* Y,
-                             std::integer_sequence<int, K...>) {
-    const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  PCD static double cv_seq(double T, const double* Y,
+                           std::integer_sequence<int, K...>) {
     double cv = 0.0;
-    ((cv += Y[K] * (Nasa<K, LO>::cvR(T, T2, T3, T4) * RU * Nasa<K, LO>::imw)),
-     ...);
+    ((cv = fma(Y[K], Nasa<K, LO>::cv(T), cv)), ...);
     return cv;
   }
-  template <bool LO, int... K>
-  PCD static void ucv_seq(double T, const double* Y, double& E, double& cv,
-                          std::integer_sequence<int, K...>) {
-    const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, invT = 1.0 / T;
-    const double RT = RU * T;
-    E = 0.0;
-    cv = 0.0;
-    ((E += Y[K] * (Nasa<K, LO>::eRT(T, T2, T3, T4, invT) * RT *
-                   Nasa<K, LO>::imw),
-      cv += Y[K] * (Nasa<K, LO>::cvR(T, T2, T3, T4) * RU * Nasa<K, LO>::imw)),
-     ...);
-  }
   PCD static double ubms(double T, const double* Y) {
-    return (T < 1000.0) ? ubms_seq<true>(T, Y, Seq{})
-                        : ubms_seq<false>(T, Y, Seq{});
+    return (T < TMID) ? e_seq<true>(T, Y, Seq{}) : e_seq<false>(T, Y, Seq{});
   }
   PCD static double cvbs(double T, const double* Y) {
-    return (T < 1000.0) ? cvbs_seq<true>(T, Y, Seq{})
-                        : cvbs_seq<false>(T, Y, Seq{});
-  }
-  PCD static void ucv(double T, const double* Y, double& E, double& cv) {
-    if (T < 1000.0)
-      ucv_seq<true>(T, Y, E, cv, Seq{});
-    else
-      ucv_seq<false>(T, Y, E, cv, Seq{});
+    return (T < TMID) ? cv_seq<true>(T, Y, Seq{}) : cv_seq<false>(T, Y, Seq{});
   }
 
   template <int... K>
   PCD static double sum_y_imw(const double* Y, std::integer_sequence<int, K...>) {
     double s = 0.0;
-    ((s += Y[K] * Nasa<K, true>::imw), ...);
+    ((s = fma(Y[K], Nasa<K, true>::imw, s)), ...);
     return s;
   }
   PCD static double wbar(const double* Y) { return 1.0 / sum_y_imw(Y, Seq{}); }
@@ -237,72 +241,91 @@ struct EosLiDryer {
     const double G = (w * cv + RU) / (w * cv);
     return sqrt(G * P / R);
   }
-  PCD static void RPY2ECs(double R, double P, const double* Y, double& E,
-                          double& Cs) {
-    const double w = wbar(Y);
-    const double T = P * w / (R * RU);
-    double cv;
-    ucv(T, Y, E, cv);
-    const double G = (w * cv + RU) / (w * cv);
-    Cs = sqrt(G * P / R);
-  }
-  PCD static double RTY2P(double R, double T, const double* Y) {
-    return R * RU * T * sum_y_imw(Y, Seq{});
-  }
-  PCD static double RTY2Cs(double R, double T, const double* Y) {
-    const double P = RTY2P(R, T, Y);
-    const double w = wbar(Y);
-    const double cv = cvbs(T, Y);
-    const double G = (w * cv + RU) / (w * cv);
-    return sqrt(G * P / R);
-  }
-  PCD static double RTY2G(double, double T, const double* Y) {
-    const double w = wbar(Y);
-    const double cv = cvbs(T, Y);
-    return (w * cv + RU) / (w * cv);
-  }
-  PCD static void RTY2dpde_dpdre(double R, double T, const double* Y,
-                                 double& dpde, double& dpdr_e) {
-    const double w = wbar(Y);
-    const double cv = cvbs(T, Y);
-    dpde = R * RU / (w * cv);
-    dpdr_e = RU * T / w;
-  }
   PCD static double TY2E(double T, const double* Y) { return ubms(T, Y); }
-  // e -> T Newton (Fuego stopping rule: |dT| < 1e-12 or T + dT == T,
-  // |dT| clipped to 100 K, bracket [90, 4000] K, 200 iterations)
-  PCD static double EY2T(double E, const double* Y, double Tguess) {
-    const double tmin = 90.0, tmax = 4000.0;
-    const double emin = ubms_seq<true>(tmin, Y, Seq{});
-    const double emax = ubms_seq<false>(tmax, Y, Seq{});
-    if (E < emin) {
-      const double cv = cvbs_seq<true>(tmin, Y, Seq{});
-      return tmin - (emin - E) / cv;
-    }
-    if (E > emax) {
-      const double cv = cvbs_seq<false>(tmax, Y, Seq{});
-      return tmax - (emax - E) / cv;
-    }
-    double t1 = Tguess;
-    if (t1 < tmin || t1 > tmax)
-      t1 = tmin + (tmax - tmin) / (emax - emin) * (E - emin);
-    for (int it = 0; it < 200; ++it) {
-      double e1, cv;
-      ucv(t1, Y, e1, cv);
-      double dt = (E - e1) / cv;
-      if (dt > 100.0) {
-        dt = 100.0;
-      } else if (dt < -100.0) {
-        dt = -100.0;
-      } else if (fabs(dt) < 1.0e-12) {
-        break;
-      } else if (t1 + dt == t1) {
-        break;
-      }
-      t1 += dt;
-    }
-    return t1;
+
+  // mixture coefficients of one composition in one range
+  template <bool LO, int... K>
+  PCD static void mix_seq(const double* Y, double* U,
+                          std::integer_sequence<int, K...>) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) U[m] = 0.0;
+    ((U[0] = fma(Y[K], Nasa<K, LO>::u0, U[0]),
+      U[1] = fma(Y[K], Nasa<K, LO>::u1, U[1]),
+      U[2] = fma(Y[K], Nasa<K, LO>::u2, U[2]),
+      U[3] = fma(Y[K], Nasa<K, LO>::u3, U[3]),
+      U[4] = fma(Y[K], Nasa<K, LO>::u4, U[4]),
+      U[5] = fma(Y[K], Nasa<K, LO>::u5, U[5])),
+     ...);
   }
+
+  // Thermodynamics of one composition: the mass-weighted polynomial
+  // coefficients of both ranges are formed once, after which e(T) and cv(T)
+  // cost one Horner evaluation each (the e -> T Newton loop, and the state
+  // functions at the solved T).
+  struct Thermo {
+    double lo[6], hi[6];
+    double sy;  // sum_k Y_k / W_k = 1 / Wbar
+    PCD explicit Thermo(const double* Y) {
+      mix_seq<true>(Y, lo, Seq{});
+      mix_seq<false>(Y, hi, Seq{});
+      sy = sum_y_imw(Y, Seq{});
+    }
+    PCD double e(double T) const {
+      return (T < TMID) ? poly_e(lo, T) : poly_e(hi, T);
+    }
+    PCD double cv(double T) const {
+      return (T < TMID) ? poly_cv(lo, T) : poly_cv(hi, T);
+    }
+    PCD double wbar() const { return 1.0 / sy; }
+    PCD double RTY2P(double R, double T) const { return R * RU * T * sy; }
+    PCD double RTY2G(double, double T) const {
+      const double w = wbar();
+      const double c = cv(T);
+      return (w * c + RU) / (w * c);
+    }
+    PCD double RTY2Cs(double R, double T) const {
+      const double P = RTY2P(R, T);
+      const double G = RTY2G(R, T);
+      return sqrt(G * P / R);
+    }
+    PCD void RTY2dpde_dpdre(double R, double T, double& dpde,
+                            double& dpdr_e) const {
+      const double w = wbar();
+      dpde = R * RU / (w * cv(T));
+      dpdr_e = RU * T / w;
+    }
+    // e -> T Newton (Fuego stopping rule: |dT| < 1e-12 or T + dT == T,
+    // |dT| clipped to 100 K, bracket [90, 4000] K, 200 iterations; linear
+    // extrapolation outside the bracket)
+    PCD double EY2T(double E, double Tguess) const {
+      const double tmin = 90.0, tmax = 4000.0;
+      const double emin = poly_e(lo, tmin);
+      const double emax = poly_e(hi, tmax);
+      if (E < emin) return tmin - (emin - E) / poly_cv(lo, tmin);
+      if (E > emax) return tmax - (emax - E) / poly_cv(hi, tmax);
+      double t1 = Tguess;
+      if (t1 < tmin || t1 > tmax)
+        t1 = tmin + (tmax - tmin) / (emax - emin) * (E - emin);
+      for (int it = 0; it < 200; ++it) {
+        const bool l = t1 < TMID;
+        double c[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) c[m] = l ? lo[m] : hi[m];
+        double dt = (E - poly_e(c, t1)) / poly_cv(c, t1);
+        if (dt > 100.0) {
+          dt = 100.0;
+        } else if (dt < -100.0) {
+          dt = -100.0;
+        } else if (fabs(dt) < 1.0e-12) {
+          break;
+        } else if (t1 + dt == t1) {
+          break;
+        }
+        t1 += dt;
+      }
+      return t1;
+    }
+  };
 };
 
 // ---------------------------------------------------------------- states --
@@ -663,20 +686,21 @@ PCD RiemannOut<EOS> riemann(double rl, double ul, double vl, double v2l,
 
 // ------------------------------------------------------ cmpflx BCs (A9) --
 // Source/Godunov.H:378-418: wall/symmetry mirror or outflow copy at domain
-// faces; normal-frame states l=(rho, un, ut, utt, p), r likewise.
+// faces, on the normal-frame states (rho, un, ut, utt, p) of both sides.
 PCD void face_bc(int idx, int domlo, int domhi, int bclo, int bchi,
-                 double* l, double* r) {
+                 double& rl, double& ul, double& vl, double& v2l, double& pl,
+                 double& rr, double& ur, double& vr, double& v2r, double& pr) {
   if (idx == domlo) {
     if (bclo == BC_NOSLIPWALL || bclo == BC_SLIPWALL || bclo == BC_SYMMETRY) {
-      l[1] = -r[1]; l[2] = r[2]; l[3] = r[3]; l[4] = r[4]; l[0] = r[0];
+      ul = -ur; vl = vr; v2l = v2r; pl = pr; rl = rr;
     } else if (bclo == BC_OUTFLOW) {
-      l[1] = r[1]; l[2] = r[2]; l[3] = r[3]; l[4] = r[4]; l[0] = r[0];
+      ul = ur; vl = vr; v2l = v2r; pl = pr; rl = rr;
     }
   } else if (idx == domhi + 1) {
     if (bchi == BC_NOSLIPWALL || bchi == BC_SLIPWALL || bchi == BC_SYMMETRY) {
-      r[1] = -l[1]; r[2] = l[2]; r[3] = l[3]; r[4] = l[4]; r[0] = l[0];
+      ur = -ul; vr = vl; v2r = v2l; pr = pl; rr = rl;
     } else if (bchi == BC_OUTFLOW) {
-      r[1] = l[1]; r[2] = l[2]; r[3] = l[3]; r[4] = l[4]; r[0] = l[0];
+      ur = ul; vr = vl; v2r = v2l; pr = pl; rr = rl;
     }
   }
 }
@@ -696,15 +720,18 @@ PCD void cmpflx(const FaceState<EOS::NS>& ql, const FaceState<EOS::NS>& qr,
                 double cav, int idx, int domlo, int domhi, int bclo, int bchi,
                 Flux<EOS::NS>& F) {
   constexpr int NS = EOS::NS;
-  const double lu[3] = {ql.u, ql.v, ql.w};
-  const double ru[3] = {qr.u, qr.v, qr.w};
   constexpr int IU = DIR, IV = (DIR == 0) ? 1 : 0, IV2 = (DIR == 2) ? 1 : 2;
-  double l[5] = {ql.rho, lu[IU], lu[IV], lu[IV2], ql.p};
-  double r[5] = {qr.rho, ru[IU], ru[IV], ru[IV2], qr.p};
-  face_bc(idx, domlo, domhi, bclo, bchi, l, r);
-  const RiemannOut<EOS> o =
-      riemann<EOS>(l[0], l[1], l[2], l[3], l[4], ql.Y, r[0], r[1], r[2], r[3],
-                   r[4], qr.Y, cav);
+  auto comp = [](const FaceState<NS>& q, int c) {
+    return c == 0 ? q.u : (c == 1 ? q.v : q.w);
+  };
+  double rl = ql.rho, ul = comp(ql, IU), vl = comp(ql, IV), v2l = comp(ql, IV2),
+         pl = ql.p;
+  double rr = qr.rho, ur = comp(qr, IU), vr = comp(qr, IV), v2r = comp(qr, IV2),
+         pr = qr.p;
+  face_bc(idx, domlo, domhi, bclo, bchi, rl, ul, vl, v2l, pl, rr, ur, vr, v2r,
+          pr);
+  const RiemannOut<EOS> o = riemann<EOS>(rl, ul, vl, v2l, pl, ql.Y, rr, ur, vr,
+                                         v2r, pr, qr.Y, cav);
   F.f[0] = o.f_rho;
   F.f[1 + IU] = o.f_un;
   F.f[1 + IV] = o.f_ut;

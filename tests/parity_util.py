@@ -7,15 +7,20 @@ import oracle_bind as ob
 VALID = (slice(None), slice(4, -4), slice(4, -4), slice(4, -4))
 
 
-def compare(gpu, ref, tol=1e-12):
-    """Relative error of each component against that component's scale.
+def compare(gpu, ref, tol=1e-12, scale=None):
+    """Relative error of each component against that component's scale
+    (max |ref| of the component unless `scale` gives one per component; inf
+    when either side holds a non-finite value).
     Returns (max_err, n_not_bitwise, n_total)."""
     assert gpu.shape == ref.shape
-    scale = np.max(np.abs(ref.reshape(ref.shape[0], -1)), axis=1)
+    nb = int(np.count_nonzero(gpu != ref))
+    if not (np.isfinite(gpu).all() and np.isfinite(ref).all()):
+        return float("inf"), nb, gpu.size
+    if scale is None:
+        scale = np.max(np.abs(ref.reshape(ref.shape[0], -1)), axis=1)
     scale = np.where(scale > 0, scale, 1.0)
     d = np.abs(gpu - ref).reshape(ref.shape[0], -1)
     rel = np.max(d / scale[:, None]) if d.size else 0.0
-    nb = int(np.count_nonzero(gpu != ref))
     return float(rel), nb, gpu.size
 
 
@@ -78,3 +83,13 @@ def assert_parity(w, nsteps, prm_over=None, tol=1e-12, label=""):
     assert worst <= tol, worst
     assert dterr <= tol, (gdt, rdt)
     return worst, nbits
+
+
+def rate_scale(U, dt):
+    """Per-component scale for comparing a rate dU/dt: the error it makes in
+    the updated state U + dt dU/dt relative to max |U| of the component.  A
+    flux divergence cancels most of its face fluxes, so its own relative
+    rounding is ~|F|/|dF| times that of the fluxes; the updated state is what
+    the step keeps."""
+    s = np.max(np.abs(U.reshape(U.shape[0], -1)), axis=1)
+    return np.where(s > 0, s, 1.0) / dt

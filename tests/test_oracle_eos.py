@@ -142,3 +142,22 @@ def test_ideal_gas_relations_mixture():
         c = ob.eos(9, 1, rho, p, Y=Y)
         assert c == pytest.approx(np.sqrt(gam * p / rho), rel=1e-14)
         assert ob.eos(9, 4, rho, T, Y=Y) == pytest.approx(c, rel=1e-13)
+
+
+def test_horner_and_mixture_forms_agree_with_term_by_term():
+    """The Horner/FMA species energies against h/RT summed term by term
+    (e_k = (h_k/RT - 1) R T / W_k), and the mixture-coefficient cv against
+    the species-sum cv, on both sides of T_mid."""
+    rng = np.random.default_rng(11)
+    for T in (200.0, 650.0, 999.0, 1001.0, 1700.0, 3200.0):
+        ek = ob.eos(9, 7, T, nout=9)
+        h = ob.eos(9, 9, T, nout=9)
+        ref = (h - 1.0) * RU * T / MW
+        scale = np.abs(h) * RU * T / MW
+        assert np.all(np.abs(ek - ref) <= 1e-14 * scale), (T, ek - ref)
+        for _ in range(4):
+            Y = rng.random(9)
+            Y /= Y.sum()
+            cs = ob.eos(9, 8, T, Y=Y)
+            cm = ob.eos(9, 11, T, Y=Y)
+            assert cm == pytest.approx(cs, rel=1e-14)
