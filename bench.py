@@ -11,9 +11,11 @@ and e->T) + the next dt, for every box of the level, on the GPU.
 Prints ONE JSON line.  `value` = valid cells x timed steps / seconds / 1e6
 (Mcell-updates/s) with the state resident in HBM.  `roofline` is the
 dominant kernel against HBM bandwidth, from its stage-compulsory bytes and
-its HIP-event duration measured here; `hbm_frac_step` is the whole step
-against HBM on the reference's algorithmic bytes (2 * NVAR * 8 per
-cell-update).  `cpu_baseline` is the CPU oracle (this repository's C++
+its HIP-event duration measured here, with `traffic` (HBM bytes per launch
+from calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/); `hbm_frac_step`
+is the whole step against HBM on the reference's algorithmic bytes
+(2 * NVAR * 8 per cell-update) and `fp64_step` the whole step against FP64
+peak on the counted FP64 operations of one step (same profile).  `cpu_baseline` is the CPU oracle (this repository's C++
 restatement of the reference path, OpenMP over boxes) on one step of the
 same level, timed on this host.
 """
@@ -30,6 +32,7 @@ HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
 METRIC = ("Mcell-updates/s per hydro step + % HBM roofline; "
           "p99 ensemble-solve latency")
+PROFILE_SUMMARY = "premixed128_step_kernels.md"
 
 
 def stage_bytes(nspec, boxes):
@@ -68,6 +71,32 @@ def stage_bytes(nspec, boxes):
         add("update", v, 6 * N + 6)
         add("dt", 0, 0)
     return tot
+
+
+def profiled_traffic(summary, stage):
+    """HBM bytes per launch of `stage` from a committed rocprofv3 summary
+    (calibrated FETCH_SIZE + WRITE_SIZE, column `HBM MB`), or None."""
+    path = os.path.join(ROOT, "profiles", summary)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) > 6 and cols[1] == stage:
+            return float(cols[6]) * 1e6
+    return None
+
+
+def profiled_step_flops(summary):
+    """FP64 operations of one level step from the same summary (counted
+    SQ_INSTS_VALU_*_F64 x 64 lanes), or None."""
+    path = os.path.join(ROOT, "profiles", summary)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("Sum over one step"):
+            tok = line.split("FP64 ")[1].split(" GFLOP")[0]
+            return float(tok) * 1e9
+    return None
 
 
 def cpu_baseline(w, gpu_fabs, dt, steps, threads):
@@ -127,11 +156,15 @@ def main():
                 f"approximate Riemann solver, CFL 0.3")
 
     if args.copy_calibration_mib:
+        # two copies of known size (read N bytes, write N bytes) whose HBM
+        # counter readings calibrate FETCH_SIZE / WRITE_SIZE: the runtime's
+        # blit kernel and an elementwise multiply by one
         nel = args.copy_calibration_mib * (1 << 20) // 8
         src = torch.ones(nel, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
 
@@ -175,13 +208,24 @@ def main():
     dom = max(kt, key=lambda k: kt[k])
     dom_s = kt[dom] * 1e-3
     achieved = kbytes[dom] / dom_s
+    traffic = None
+    if args.workload == "premixed" and n == 128 and args.box == 32:
+        traffic = profiled_traffic(PROFILE_SUMMARY, dom)
     roof = {"bound": "hbm", "kernel": f"k_{dom.split('_')[0]} ({dom})",
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
+            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "traffic_source": (f"profiles/{PROFILE_SUMMARY}" if traffic else None),
             "bytes_per_launch": kbytes[dom], "ms_per_launch": kt[dom]}
     nvar = w.nvar
     step_bytes = cells * 2 * nvar * 8
     hbm_frac_step = step_bytes / (ms_step * 1e-3) / HBM_PEAK
+    fp64 = None
+    if traffic is not None:
+        fl = profiled_step_flops(PROFILE_SUMMARY)
+        if fl:
+            fp64 = {"flop_per_step": fl, "achieved_tflops": fl / (ms_step * 1e-3) / 1e12,
+                    "peak_tflops": FP64_PEAK / 1e12,
+                    "frac": fl / (ms_step * 1e-3) / FP64_PEAK}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -203,6 +247,7 @@ def main():
                    "parallelism": "single GPU, all boxes per launch"},
         "roofline": roof,
         "hbm_frac_step": hbm_frac_step,
+        "fp64_step": fp64,
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
         "cpu_baseline": cpu,
     }
