@@ -118,7 +118,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="premixed", choices=["premixed", "sod"])
+    ap.add_argument("--workload", default="premixed",
+                    choices=["premixed", "sod", "ragged"])
+    ap.add_argument("--nboxes", type=int, default=300,
+                    help="ragged workload: boxes in the table")
+    ap.add_argument("--ppm-type", type=int, default=1)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick every stage's launch shape on the level first")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="replay steps from a hipGraph of this many steps")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--box", type=int, default=32)
     ap.add_argument("--kernel-steps", type=int, default=3)
@@ -149,6 +157,15 @@ def main():
                 f"only, {n}^3 as {len(w.boxes)} boxes of {args.box}^3 + 4 ghosts, "
                 f"PPM, approximate Riemann solver, CFL 0.3, x/y periodic, "
                 f"z outflow")
+    elif args.workload == "ragged":
+        n = None
+        w = workloads.ragged(nboxes=args.nboxes, nsteps=args.steps,
+                             ppm_type=args.ppm_type)
+        desc = (f"ragged table of {len(w.boxes)} boxes, sides from "
+                f"{{8,16,24,32,48,64}} with probabilities (0.1,0.2,0.2,0.25,"
+                f"0.15,0.1), each its own periodic domain, premixed H2/air "
+                f"9-species generator, {'PPM' if args.ppm_type else 'PLM'}, "
+                f"approximate Riemann solver, CFL 0.3")
     else:
         n = args.n or 64
         w = workloads.sod(n=n, nsteps=args.steps)
@@ -177,6 +194,15 @@ def main():
     # warm-up (JIT-free: kernels are precompiled; this settles clocks/caches)
     lev.step(args.warmup)
     torch.cuda.synchronize()
+    plans = None
+    if args.autotune:
+        lev.autotune(reps=3)
+        plans = {k: [v[0] if isinstance(v[0], str) else list(v[0]), v[1], v[2],
+                     round(v[3], 4)] for k, v in lev.plans().items()}
+    if args.graph:
+        lev.set_graph(args.graph)
+        lev.step(args.graph)  # capture + one replay outside the timed region
+        torch.cuda.synchronize()
     fabs0 = None
     dt_cpu = None
     if not args.no_cpu_baseline:
@@ -209,7 +235,8 @@ def main():
     dom_s = kt[dom] * 1e-3
     achieved = kbytes[dom] / dom_s
     traffic = None
-    if args.workload == "premixed" and n == 128 and args.box == 32:
+    if (args.workload == "premixed" and n == 128 and args.box == 32
+            and not args.autotune):
         traffic = profiled_traffic(PROFILE_SUMMARY, dom)
     roof = {"bound": "hbm", "kernel": f"k_{dom.split('_')[0]} ({dom})",
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -244,7 +271,9 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": desc, "valid_cells": cells,
                    "boxes": len(w.boxes), "nspec": w.nspec,
-                   "parallelism": "single GPU, all boxes per launch"},
+                   "parallelism": "single GPU, all boxes per launch",
+                   "autotuned": bool(args.autotune), "graph_steps": args.graph,
+                   "plans": plans},
         "roofline": roof,
         "hbm_frac_step": hbm_frac_step,
         "fp64_step": fp64,

@@ -219,6 +219,32 @@ enum {
 int pc_level_enable_timing_v1(pc_level_t* lev, int on);
 /* Average ms per timed step of each kernel id (n <= PC_NKERN entries). */
 int pc_level_kernel_times_v1(pc_level_t* lev, double* ms, int n);
+/* Launch shapes.  Every stage of the level step is one launch over all
+ * boxes; a launch is a set of logical blocks of 256 threads, each a run of
+ * 256 cells of the stage's region or a 2^lx x 2^ly x 2^lz tile, dealt to
+ * workgroups (one per logical block, or fewer that loop: persistent).
+ * pc_level_autotune_v1 times a fixed candidate set per stage on the level as
+ * it stands (`reps` timed launches each, the update stage on a saved copy of
+ * the state) and keeps the fastest; state and dt are unchanged on return.
+ * Synchronous.  Replaces the tiling choice of amrex::ParallelFor /
+ * MFIter tiling in the reference (Source/Hydro.cpp:60-73); no counterpart
+ * to the timing itself. */
+int pc_level_autotune_v1(pc_level_t* lev, int reps, void* stream);
+/* shape[5] = {lx (-1: runs), ly, lz, workgroups, logical blocks} of stage
+ * `stage` (0..16: ctoprim, divu, trace, flux1 x/y/z, trans xy/xz/yx/yz/zx/zy,
+ * final x/y/z, update, ghost); *ms = its tuned time per launch (0 if never
+ * tuned; may be NULL). */
+int pc_level_plan_v1(const pc_level_t* lev, int stage, int* shape, double* ms);
+/* Set stage `stage`'s plan by hand (e.g. a shape saved from an earlier
+ * autotune): lx = -1 for runs, else a tile with lx + ly + lz = 8; wg_per_cu
+ * > 0 caps the workgroups at that many per compute unit (persistent).
+ * Synchronous. */
+int pc_level_set_plan_v1(pc_level_t* lev, int stage, int lx, int ly, int lz,
+                         int wg_per_cu);
+/* steps_per_graph > 0: pc_level_step_v1 captures that many consecutive
+ * steps into a hipGraph once (re-captured after autotuning) and replays it;
+ * leftover steps and timed steps launch directly.  0 turns graphs off. */
+int pc_level_set_graph_v1(pc_level_t* lev, int steps_per_graph);
 /* okToContinue (Source/PeleC.cpp:1310-1340): synchronous; returns
  * PC_ERR_NAN if any step produced NaN/Inf, PC_ERR_DT if dt < dt_cutoff. */
 int pc_level_status_v1(pc_level_t* lev, int* bad, int* small_dt);

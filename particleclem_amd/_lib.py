@@ -23,6 +23,8 @@ EXPORTS = [
     "pc_level_step_v1", "pc_level_fill_ghosts_v1", "pc_level_estdt_v1",
     "pc_level_enable_timing_v1", "pc_level_kernel_times_v1",
     "pc_level_status_v1", "pc_level_prim_to_cons_v1",
+    "pc_level_autotune_v1", "pc_level_plan_v1", "pc_level_set_plan_v1",
+    "pc_level_set_graph_v1",
 ]
 
 PC_OK = 0
@@ -30,6 +32,10 @@ KERNELS = ["ghost", "ctoprim", "divu", "trace_x", "trace_y", "trace_z",
            "flux1_x", "flux1_y", "flux1_z", "trans_xy", "trans_xz",
            "trans_yx", "trans_yz", "trans_zx", "trans_zy", "final_x",
            "final_y", "final_z", "update", "dt"]
+# stage plans (pc_level_plan_v1 stage ids)
+STAGES = ["ctoprim", "divu", "trace", "flux1_x", "flux1_y", "flux1_z",
+          "trans_xy", "trans_xz", "trans_yx", "trans_yz", "trans_zx",
+          "trans_zy", "final_x", "final_y", "final_z", "update", "ghost"]
 ERRORS = {-1: "PC_ERR_ARG", -2: "PC_ERR_GROW", -3: "PC_ERR_PPM_TYPE",
           -4: "PC_ERR_PLM_IORDER", -5: "PC_ERR_HIP", -6: "PC_ERR_NAN",
           -7: "PC_ERR_UNSUPPORTED", -8: "PC_ERR_NOMEM", -9: "PC_ERR_DT"}
@@ -75,6 +81,10 @@ def lib():
                                          Array4, Array4, d3, C.c_double,
                                          P(Params), P(Array4), P(Array4),
                                          Array4, vp]
+    L.pc_level_autotune_v1.argtypes = [vp, C.c_int, vp]
+    L.pc_level_plan_v1.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_double)]
+    L.pc_level_set_graph_v1.argtypes = [vp, C.c_int]
+    L.pc_level_set_plan_v1.argtypes = [vp] + [C.c_int] * 5
     L.pc_level_create_v1.argtypes = [vp, P(LevelDesc), P(Params), P(vp)]
     L.pc_level_destroy_v1.argtypes = [vp]
     L.pc_level_box_offset_v1.argtypes = [vp, C.c_int]

@@ -205,6 +205,11 @@ static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
   J.divu = fab_view(base + L.off_divu, L.node);
 }
 
+// Schedule of a launch over one job: one workgroup per block of cells.
+static Sched one_job_sched(const Job* j) {
+  return Sched{j, nullptr, nullptr, 0, -1, 0, 0};
+}
+
 // Stage job J for the kernels of one box-driver call on stream st.
 static int ctx_job(pc_ctx* c, const Job& J, hipStream_t st, Sched* sc,
                    int* slot) {
@@ -223,7 +228,7 @@ static int ctx_job(pc_ctx* c, const Job& J, hipStream_t st, Sched* sc,
   c->hjobs[s] = J;
   HIPCHK(c, hipMemcpyAsync(c->djobs + s, c->hjobs + s, sizeof(Job),
                            hipMemcpyHostToDevice, st));
-  *sc = Sched{c->djobs + s, nullptr, nullptr};
+  *sc = one_job_sched(c->djobs + s);
   *slot = s;
   return PC_OK;
 }
@@ -266,14 +271,27 @@ static long long region_npts(int st, const Job& J) {
 
 // launch plan of one stage over a job table
 struct StagePlan {
-  int nblocks = 0;
+  int nblocks = 0;          // logical blocks
+  int grid = 0;             // workgroups launched (< nblocks: persistent)
+  int lx = -1, ly = 0, lz = 0;  // tile shape (log2), lx = -1: runs
   int* blkmap = nullptr;    // device
   int* blkstart = nullptr;  // device
 };
 
-static int grid1(int st, const Job& J) {
-  return (int)((region_npts(st, J) + BLOCK - 1) / BLOCK);
+// logical blocks of one job for a stage and block shape
+static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz) {
+  int lo[3], n[3];
+  stage_region(st, J.lo, J.hi, lo, n);
+  if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return 0;
+  if (lx < 0) return ((long long)n[0] * n[1] * n[2] + BLOCK - 1) / BLOCK;
+  auto nt = [](int m, int l) { return (long long)((m + (1 << l) - 1) >> l); };
+  return nt(n[0], lx) * nt(n[1], ly) * nt(n[2], lz);
 }
+
+static int grid1(int st, const Job& J) {
+  return (int)stage_blocks(st, J, -1, 0, 0);
+}
+
 
 #define LAUNCH(kern, grid, stream, ...)                                      \
   do {                                                                       \
@@ -288,7 +306,7 @@ static void run_ctu(const Job& j0, const Sched& s1, const KArgs& ka,
                     int ppm_type, hipStream_t st) {
   auto G = [&](int stage) { return grid1(stage, j0); };
   auto S = [&](int) { return s1; };
-  LAUNCH(k_divu, G(ST_DIVU), st, S(ST_DIVU), ka);
+  LAUNCH(k_divu<>, G(ST_DIVU), st, S(ST_DIVU), ka);
   if (ppm_type == 1) {
     LAUNCH((k_trace_ppm<EOS, 0>), G(ST_TRACE), st, S(ST_TRACE), ka);
     LAUNCH((k_trace_ppm<EOS, 1>), G(ST_TRACE), st, S(ST_TRACE), ka);
@@ -635,21 +653,138 @@ struct pc_level {
   bool ev_made = false;
   double ktime[PC_NKERN];
   int nsteps_timed = 0;
+  std::vector<Job> hjobs;      // host copy of the job table (plans)
+  double tuned_ms[ST_COUNT] = {};
+  // hipGraph of gsteps consecutive steps, replayed by pc_level_step_v1
+  int gsteps = 0;
+  bool gvalid = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t cap = nullptr;
 };
 
+static void plan_free(StagePlan& P) {
+  if (P.blkmap) (void)hipFree(P.blkmap);
+  if (P.blkstart) (void)hipFree(P.blkstart);
+  P.blkmap = P.blkstart = nullptr;
+}
+
+// Launch plan of stage s over the level's boxes with block shape (lx,ly,lz)
+// and at most `cap` workgroups (0: one per logical block).
+static bool plan_make(pc_level* L, int s, int lx, int ly, int lz, int cap,
+                      StagePlan& P) {
+  const int nb = (int)L->hjobs.size();
+  std::vector<int> start(nb), map;
+  for (int b = 0; b < nb; ++b) {
+    start[b] = (int)map.size();
+    const long long n = stage_blocks(s, L->hjobs[b], lx, ly, lz);
+    for (long long k = 0; k < n; ++k) map.push_back(b);
+  }
+  P.nblocks = (int)map.size();
+  P.grid = (cap > 0 && cap < P.nblocks) ? cap : P.nblocks;
+  P.lx = lx;
+  P.ly = ly;
+  P.lz = lz;
+  P.blkmap = P.blkstart = nullptr;
+  if (hipMalloc(&P.blkmap, sizeof(int) * std::max<size_t>(map.size(), 1)) !=
+          hipSuccess ||
+      hipMalloc(&P.blkstart, sizeof(int) * start.size()) != hipSuccess) {
+    (void)hipGetLastError();
+    plan_free(P);
+    return false;
+  }
+  if (hipMemcpy(P.blkmap, map.data(), sizeof(int) * map.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(P.blkstart, start.data(), sizeof(int) * start.size(),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    plan_free(P);
+    return false;
+  }
+  return true;
+}
+
+static Sched plan_sched(const pc_level* L, const StagePlan& P) {
+  return Sched{L->jobs, P.blkmap, P.blkstart, P.nblocks, P.lx, P.ly, P.lz};
+}
+
+static void graph_drop(pc_level* L) {
+  if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
+  if (L->graph) (void)hipGraphDestroy(L->graph);
+  L->gexec = nullptr;
+  L->graph = nullptr;
+  L->gvalid = false;
+}
+
 static void level_free(pc_level* L) {
+  graph_drop(L);
+  if (L->cap) (void)hipStreamDestroy(L->cap);
   if (L->state) (void)hipFree(L->state);
   if (L->ws) (void)hipFree(L->ws);
   if (L->jobs) (void)hipFree(L->jobs);
   if (L->cellmap) (void)hipFree(L->cellmap);
   if (L->ds) (void)hipFree(L->ds);
-  for (int s = 0; s < ST_COUNT; ++s) {
-    if (L->plans[s].blkmap) (void)hipFree(L->plans[s].blkmap);
-    if (L->plans[s].blkstart) (void)hipFree(L->plans[s].blkstart);
-  }
+  for (int s = 0; s < ST_COUNT; ++s) plan_free(L->plans[s]);
   if (L->ev_made)
     for (int e = 0; e <= PC_NKERN; ++e) (void)hipEventDestroy(L->ev[e]);
   delete L;
+}
+
+// Launch every kernel of stage s once with schedule sc on `grid` workgroups
+// (sub >= 0: only that direction of the trace stage).  P: the persistent
+// instantiations, which loop over the logical blocks.
+template <class EOS, bool P>
+static void launch_stage_t(pc_level* L, int s, const Sched& sc, int grid,
+                           hipStream_t st, int sub) {
+  const KArgs& ka = L->ka;
+  auto tr = [&](int d) { return sub < 0 || sub == d; };
+  switch (s) {
+    case ST_GHOST: LAUNCH((k_ghost<EOS::NS, P>), grid, st, sc, ka, L->la); break;
+    case ST_CTOPRIM: LAUNCH((k_ctoprim<EOS, P>), grid, st, sc, ka); break;
+    case ST_DIVU: LAUNCH((k_divu<P>), grid, st, sc, ka); break;
+    case ST_TRACE:
+      if (L->prm.ppm_type == 1) {
+        if (tr(0)) LAUNCH((k_trace_ppm<EOS, 0, P>), grid, st, sc, ka);
+        if (tr(1)) LAUNCH((k_trace_ppm<EOS, 1, P>), grid, st, sc, ka);
+        if (tr(2)) LAUNCH((k_trace_ppm<EOS, 2, P>), grid, st, sc, ka);
+      } else {
+        if (tr(0)) LAUNCH((k_trace_plm<EOS, 0, P>), grid, st, sc, ka);
+        if (tr(1)) LAUNCH((k_trace_plm<EOS, 1, P>), grid, st, sc, ka);
+        if (tr(2)) LAUNCH((k_trace_plm<EOS, 2, P>), grid, st, sc, ka);
+      }
+      break;
+    case ST_FLUX1_X: LAUNCH((k_flux1<EOS, 0, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_Y: LAUNCH((k_flux1<EOS, 1, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_Z: LAUNCH((k_flux1<EOS, 2, P>), grid, st, sc, ka); break;
+    case ST_TRANS_XY: LAUNCH((k_trans<EOS, 0, 1, P>), grid, st, sc, ka); break;
+    case ST_TRANS_XZ: LAUNCH((k_trans<EOS, 0, 2, P>), grid, st, sc, ka); break;
+    case ST_TRANS_YX: LAUNCH((k_trans<EOS, 1, 0, P>), grid, st, sc, ka); break;
+    case ST_TRANS_YZ: LAUNCH((k_trans<EOS, 1, 2, P>), grid, st, sc, ka); break;
+    case ST_TRANS_ZX: LAUNCH((k_trans<EOS, 2, 0, P>), grid, st, sc, ka); break;
+    case ST_TRANS_ZY: LAUNCH((k_trans<EOS, 2, 1, P>), grid, st, sc, ka); break;
+    case ST_FINAL_X:
+      LAUNCH((k_final<EOS, 0, false, P>), grid, st, sc, ka);
+      if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0, P>), grid, st, sc, ka);
+      break;
+    case ST_FINAL_Y:
+      LAUNCH((k_final<EOS, 1, false, P>), grid, st, sc, ka);
+      if (L->lowo[1]) LAUNCH((k_final_loworder<EOS, 1, P>), grid, st, sc, ka);
+      break;
+    case ST_FINAL_Z:
+      LAUNCH((k_final<EOS, 2, false, P>), grid, st, sc, ka);
+      if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2, P>), grid, st, sc, ka);
+      break;
+    case ST_UPDATE: LAUNCH((k_update<EOS, P>), grid, st, sc, ka, L->ds); break;
+    default: break;
+  }
+}
+
+template <class EOS>
+static void launch_stage(pc_level* L, int s, const Sched& sc,
+                         const StagePlan& pl, hipStream_t st, int sub = -1) {
+  if (pl.grid < pl.nblocks)
+    launch_stage_t<EOS, true>(L, s, sc, pl.grid, st, sub);
+  else
+    launch_stage_t<EOS, false>(L, s, sc, pl.grid, st, sub);
 }
 
 extern "C" {
@@ -742,25 +877,14 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
     level_free(L);
     return fail(c, PC_ERR_HIP, "job upload failed");
   }
-  // per-stage launch plans
+  // per-stage launch plans: runs of BLOCK cells, one workgroup each
+  L->hjobs = jobs;
   for (int s = 0; s < ST_COUNT; ++s) {
-    std::vector<int> start(d->nboxes), map;
-    for (int b = 0; b < d->nboxes; ++b) {
-      start[b] = (int)map.size();
-      const int nb = grid1(s, jobs[b]);
-      for (int k = 0; k < nb; ++k) map.push_back(b);
-    }
-    L->plans[s].nblocks = (int)map.size();
-    if (!dalloc((void**)&L->plans[s].blkmap, sizeof(int) * std::max<size_t>(map.size(), 1)) ||
-        !dalloc((void**)&L->plans[s].blkstart, sizeof(int) * start.size())) {
+    if (!plan_make(L, s, -1, 0, 0, 0, L->plans[s])) {
       level_free(L);
       return fail(c, PC_ERR_NOMEM, "plan allocation failed");
     }
-    (void)hipMemcpy(L->plans[s].blkmap, map.data(), sizeof(int) * map.size(),
-                    hipMemcpyHostToDevice);
-    (void)hipMemcpy(L->plans[s].blkstart, start.data(), sizeof(int) * start.size(),
-                    hipMemcpyHostToDevice);
-    L->scs[s] = Sched{L->jobs, L->plans[s].blkmap, L->plans[s].blkstart};
+    L->scs[s] = plan_sched(L, L->plans[s]);
   }
   // ghost-fill cell map over the domain
   memset(&L->la, 0, sizeof(L->la));
@@ -884,13 +1008,10 @@ int pc_level_get_dt_v1(pc_level_t* L, double* dt) {
 }
 
 static int level_fill(pc_level* L, hipStream_t st) {
-  const int ns = L->ctx->ns;
-  Job dummy;
-  memset(&dummy, 0, sizeof(dummy));
-  if (ns == 1)
-    LAUNCH(k_ghost<1>, L->plans[ST_GHOST].nblocks, st, L->scs[ST_GHOST], L->ka, L->la);
+  if (L->ctx->ns == 1)
+    launch_stage<EosGamma>(L, ST_GHOST, L->scs[ST_GHOST], L->plans[ST_GHOST], st);
   else
-    LAUNCH(k_ghost<9>, L->plans[ST_GHOST].nblocks, st, L->scs[ST_GHOST], L->ka, L->la);
+    launch_stage<EosLiDryer>(L, ST_GHOST, L->scs[ST_GHOST], L->plans[ST_GHOST], st);
   HIPCHK(L->ctx, hipGetLastError());
   return PC_OK;
 }
@@ -904,9 +1025,11 @@ int pc_level_fill_ghosts_v1(pc_level_t* L, void* stream) {
 
 template <class EOS>
 static void level_estdt_launch(pc_level* L, hipStream_t st) {
-  Job dummy;
-  memset(&dummy, 0, sizeof(dummy));
-  LAUNCH(k_estdt<EOS>, L->plans[ST_UPDATE].nblocks, st, L->scs[ST_UPDATE], L->ka, L->ds);
+  const StagePlan& P = L->plans[ST_UPDATE];
+  if (P.grid < P.nblocks)
+    LAUNCH((k_estdt<EOS, true>), P.grid, st, L->scs[ST_UPDATE], L->ka, L->ds);
+  else
+    LAUNCH((k_estdt<EOS, false>), P.grid, st, L->scs[ST_UPDATE], L->ka, L->ds);
 }
 
 extern "C" {
@@ -959,70 +1082,27 @@ int pc_level_enable_timing_v1(pc_level_t* L, int on) {
 // host wait, so per-kernel device times accumulate in ktime[].
 template <class EOS>
 static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
-  Job d0;
-  memset(&d0, 0, sizeof(d0));
-  const int ppm = L->prm.ppm_type;
   const Sched* S = L->scs;
   const StagePlan* P = L->plans;
-  const KArgs& ka = L->ka;
   for (int s = 0; s < nsteps; ++s) {
     const bool tm = L->timing;
     int ne = 0;
     auto mark = [&]() {
       if (tm) (void)hipEventRecord(L->ev[ne++], st);
     };
-    mark();
-    level_fill(L, st);
-    mark();  // PC_K_GHOST
-    LAUNCH(k_ctoprim<EOS>, P[ST_CTOPRIM].nblocks, st, S[ST_CTOPRIM], ka);
-    mark();
-    LAUNCH(k_divu, P[ST_DIVU].nblocks, st, S[ST_DIVU], ka);
-    mark();
-    const int gT = P[ST_TRACE].nblocks;
-    if (ppm == 1) {
-      LAUNCH((k_trace_ppm<EOS, 0>), gT, st, S[ST_TRACE], ka);
+    auto run = [&](int stage, int sub = -1) {
+      launch_stage<EOS>(L, stage, S[stage], P[stage], st, sub);
       mark();
-      LAUNCH((k_trace_ppm<EOS, 1>), gT, st, S[ST_TRACE], ka);
-      mark();
-      LAUNCH((k_trace_ppm<EOS, 2>), gT, st, S[ST_TRACE], ka);
-      mark();
-    } else {
-      LAUNCH((k_trace_plm<EOS, 0>), gT, st, S[ST_TRACE], ka);
-      mark();
-      LAUNCH((k_trace_plm<EOS, 1>), gT, st, S[ST_TRACE], ka);
-      mark();
-      LAUNCH((k_trace_plm<EOS, 2>), gT, st, S[ST_TRACE], ka);
-      mark();
-    }
-    LAUNCH((k_flux1<EOS, 0>), P[ST_FLUX1_X].nblocks, st, S[ST_FLUX1_X], ka);
+    };
     mark();
-    LAUNCH((k_flux1<EOS, 1>), P[ST_FLUX1_Y].nblocks, st, S[ST_FLUX1_Y], ka);
-    mark();
-    LAUNCH((k_flux1<EOS, 2>), P[ST_FLUX1_Z].nblocks, st, S[ST_FLUX1_Z], ka);
-    mark();
-    LAUNCH((k_trans<EOS, 0, 1>), P[ST_TRANS_XY].nblocks, st, S[ST_TRANS_XY], ka);
-    mark();
-    LAUNCH((k_trans<EOS, 0, 2>), P[ST_TRANS_XZ].nblocks, st, S[ST_TRANS_XZ], ka);
-    mark();
-    LAUNCH((k_trans<EOS, 1, 0>), P[ST_TRANS_YX].nblocks, st, S[ST_TRANS_YX], ka);
-    mark();
-    LAUNCH((k_trans<EOS, 1, 2>), P[ST_TRANS_YZ].nblocks, st, S[ST_TRANS_YZ], ka);
-    mark();
-    LAUNCH((k_trans<EOS, 2, 0>), P[ST_TRANS_ZX].nblocks, st, S[ST_TRANS_ZX], ka);
-    mark();
-    LAUNCH((k_trans<EOS, 2, 1>), P[ST_TRANS_ZY].nblocks, st, S[ST_TRANS_ZY], ka);
-    mark();
-    LAUNCH((k_final<EOS, 0, false>), P[ST_FINAL_X].nblocks, st, S[ST_FINAL_X], ka);
-    mark();
-    LAUNCH((k_final<EOS, 1, false>), P[ST_FINAL_Y].nblocks, st, S[ST_FINAL_Y], ka);
-    mark();
-    LAUNCH((k_final<EOS, 2, false>), P[ST_FINAL_Z].nblocks, st, S[ST_FINAL_Z], ka);
-    if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0>), P[ST_FINAL_X].nblocks, st, S[ST_FINAL_X], ka);
-    if (L->lowo[1]) LAUNCH((k_final_loworder<EOS, 1>), P[ST_FINAL_Y].nblocks, st, S[ST_FINAL_Y], ka);
-    if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2>), P[ST_FINAL_Z].nblocks, st, S[ST_FINAL_Z], ka);
-    mark();
-    LAUNCH(k_update<EOS>, P[ST_UPDATE].nblocks, st, S[ST_UPDATE], ka, L->ds);
-    mark();
+    run(ST_GHOST);
+    run(ST_CTOPRIM);
+    run(ST_DIVU);
+    for (int d = 0; d < 3; ++d) run(ST_TRACE, d);
+    for (int x = ST_FLUX1_X; x <= ST_FLUX1_Z; ++x) run(x);
+    for (int x = ST_TRANS_XY; x <= ST_TRANS_ZY; ++x) run(x);
+    for (int x = ST_FINAL_X; x <= ST_FINAL_Z; ++x) run(x);
+    run(ST_UPDATE);
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 0);
     mark();
     if (tm) {
@@ -1039,13 +1119,181 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
   return PC_OK;
 }
 
+// Candidate block shapes of the autotuner: {lx, ly, lz, workgroups per CU}
+// (lx = -1: runs of BLOCK cells in i-fastest order; a non-zero last entry
+// makes the launch persistent with that many workgroups per compute unit).
+static const int kTuneCand[][4] = {
+    {-1, 0, 0, 0}, {-1, 0, 0, 2}, {-1, 0, 0, 4}, {6, 2, 0, 0}, {5, 3, 0, 0},
+    {5, 2, 1, 0},  {4, 4, 0, 0},  {4, 2, 2, 0},  {3, 3, 2, 0}};
+static constexpr int kNTune = sizeof(kTuneCand) / sizeof(kTuneCand[0]);
+
+// Time each candidate plan of every stage on the level as it stands (the
+// intermediates of one step are computed first; the update stage runs on a
+// saved copy of the state, restored before each launch) and keep the
+// fastest.  The state and dt are left as they were.
+template <class EOS>
+static int level_autotune(pc_level* L, int reps, hipStream_t st) {
+  pc_ctx* c = L->ctx;
+  graph_drop(L);
+  int ncu = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                  c->device < 0 ? 0 : c->device));
+  const size_t sbytes = sizeof(double) * (size_t)L->state_total;
+  double* saved = nullptr;
+  if (hipMalloc(&saved, sbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, PC_ERR_NOMEM, "autotune: no room for a state copy");
+  }
+  DtState ds0;
+  HIPCHK(c, hipMemcpy(&ds0, L->ds, sizeof(DtState), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpyAsync(saved, L->state, sbytes, hipMemcpyDeviceToDevice, st));
+  // intermediates of one step (no update)
+  for (int s : {ST_GHOST, ST_CTOPRIM, ST_DIVU, ST_TRACE, ST_FLUX1_X, ST_FLUX1_Y,
+                ST_FLUX1_Z, ST_TRANS_XY, ST_TRANS_XZ, ST_TRANS_YX, ST_TRANS_YZ,
+                ST_TRANS_ZX, ST_TRANS_ZY, ST_FINAL_X, ST_FINAL_Y, ST_FINAL_Z})
+    launch_stage<EOS>(L, s, L->scs[s], L->plans[s], st);
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int rc = PC_OK;
+  for (int s = 0; s < ST_COUNT && rc == PC_OK; ++s) {
+    double best = 1e300;
+    int bestc = -1;
+    StagePlan bestP;
+    for (int k = 0; k < kNTune && rc == PC_OK; ++k) {
+      StagePlan P;
+      const int cap = kTuneCand[k][3] * ncu;
+      if (!plan_make(L, s, kTuneCand[k][0], kTuneCand[k][1], kTuneCand[k][2],
+                     cap, P)) {
+        rc = fail(c, PC_ERR_NOMEM, "autotune: plan allocation failed");
+        break;
+      }
+      const Sched sc = plan_sched(L, P);
+      double tot = 0.0;
+      for (int r = 0; r <= reps; ++r) {  // r = 0 warms up
+        if (s == ST_UPDATE)
+          (void)hipMemcpyAsync(L->state, saved, sbytes, hipMemcpyDeviceToDevice, st);
+        (void)hipEventRecord(e0, st);
+        launch_stage<EOS>(L, s, sc, P, st);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          rc = fail(c, PC_ERR_HIP, "autotune: launch failed");
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) tot += ms;
+      }
+      const double avg = tot / reps;
+      if (rc == PC_OK && avg < best) {
+        if (bestc >= 0) plan_free(bestP);
+        best = avg;
+        bestc = k;
+        bestP = P;
+      } else {
+        plan_free(P);
+      }
+    }
+    if (bestc >= 0) {
+      plan_free(L->plans[s]);
+      L->plans[s] = bestP;
+      L->scs[s] = plan_sched(L, bestP);
+      L->tuned_ms[s] = best;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipMemcpyAsync(L->state, saved, sbytes, hipMemcpyDeviceToDevice, st);
+  (void)hipMemcpyAsync(L->ds, &ds0, sizeof(DtState), hipMemcpyHostToDevice, st);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(saved);
+  HIPCHK(c, hipGetLastError());
+  return rc;
+}
+
+template <class EOS>
+static int level_step_graph(pc_level* L, int nsteps, hipStream_t st) {
+  pc_ctx* c = L->ctx;
+  if (!L->gvalid) {
+    if (!L->cap) HIPCHK(c, hipStreamCreateWithFlags(&L->cap, hipStreamNonBlocking));
+    HIPCHK(c, hipStreamSynchronize(st));
+    HIPCHK(c, hipStreamBeginCapture(L->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = level_step_eos<EOS>(L, L->gsteps, L->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(L->cap, &g);
+    if (rc != PC_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess)
+      return fail(c, PC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    L->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&L->gexec, L->graph, nullptr, nullptr, 0));
+    L->gvalid = true;
+  }
+  for (int r = 0; r < nsteps / L->gsteps; ++r) HIPCHK(c, hipGraphLaunch(L->gexec, st));
+  return level_step_eos<EOS>(L, nsteps % L->gsteps, st);
+}
+
 extern "C" {
 
 int pc_level_step_v1(pc_level_t* L, int nsteps, void* stream) {
   if (!L || nsteps < 0) return PC_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  if (L->ctx->ns == 1) return level_step_eos<EosGamma>(L, nsteps, st);
-  return level_step_eos<EosLiDryer>(L, nsteps, st);
+  const bool g = L->gsteps > 0 && !L->timing;
+  if (L->ctx->ns == 1)
+    return g ? level_step_graph<EosGamma>(L, nsteps, st)
+             : level_step_eos<EosGamma>(L, nsteps, st);
+  return g ? level_step_graph<EosLiDryer>(L, nsteps, st)
+           : level_step_eos<EosLiDryer>(L, nsteps, st);
+}
+
+int pc_level_autotune_v1(pc_level_t* L, int reps, void* stream) {
+  if (!L || reps < 1) return PC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (L->ctx->ns == 1) return level_autotune<EosGamma>(L, reps, st);
+  return level_autotune<EosLiDryer>(L, reps, st);
+}
+
+int pc_level_plan_v1(const pc_level_t* L, int stage, int* shape, double* ms) {
+  if (!L || stage < 0 || stage >= ST_COUNT || !shape) return PC_ERR_ARG;
+  const StagePlan& P = L->plans[stage];
+  shape[0] = P.lx;
+  shape[1] = P.ly;
+  shape[2] = P.lz;
+  shape[3] = P.grid;
+  shape[4] = P.nblocks;
+  if (ms) *ms = L->tuned_ms[stage];
+  return PC_OK;
+}
+
+int pc_level_set_plan_v1(pc_level_t* L, int stage, int lx, int ly, int lz,
+                         int wg_per_cu) {
+  if (!L || stage < 0 || stage >= ST_COUNT || wg_per_cu < 0)
+    return fail(L ? L->ctx : nullptr, PC_ERR_ARG, "bad plan request");
+  if (lx >= 0 && (ly < 0 || lz < 0 || (1 << (lx + ly + lz)) != BLOCK))
+    return fail(L->ctx, PC_ERR_ARG, "tile 2^%d x 2^%d x 2^%d is not %d cells",
+                lx, ly, lz, BLOCK);
+  int ncu = 0;
+  HIPCHK(L->ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                       L->ctx->device < 0 ? 0 : L->ctx->device));
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  StagePlan P;
+  if (!plan_make(L, stage, lx < 0 ? -1 : lx, ly, lz, wg_per_cu * ncu, P))
+    return fail(L->ctx, PC_ERR_NOMEM, "plan allocation failed");
+  graph_drop(L);
+  plan_free(L->plans[stage]);
+  L->plans[stage] = P;
+  L->scs[stage] = plan_sched(L, P);
+  L->tuned_ms[stage] = 0.0;
+  return PC_OK;
+}
+
+int pc_level_set_graph_v1(pc_level_t* L, int steps_per_graph) {
+  if (!L || steps_per_graph < 0) return PC_ERR_ARG;
+  graph_drop(L);
+  L->gsteps = steps_per_graph;
+  return PC_OK;
 }
 
 int pc_level_kernel_times_v1(pc_level_t* L, double* ms, int n) {
@@ -1079,7 +1327,7 @@ int pc_level_prim_to_cons_v1(pc_level_t* L, int b, pc_array4_t prim, int mode,
   if ((rc = a4_view(L->ctx, prim, bx, 5 + L->ctx->ns, "prim", pv))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const int g = grid1(ST_UPDATE, J);
-  const Sched s0{L->jobs + b, nullptr, nullptr};
+  const Sched s0 = one_job_sched(L->jobs + b);
   if (L->ctx->ns == 1)
     LAUNCH(k_prim_to_cons<EosGamma>, g, st, s0, L->ka, pv, mode);
   else

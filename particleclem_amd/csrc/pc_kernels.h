@@ -75,11 +75,17 @@ PCD double get_dt(const KArgs& a) { return a.dtp ? *a.dtp : a.dt_val; }
 
 PCD int tslot(int d, int t) { return (t < d) ? t : t - 1; }
 
-// block -> (job, local thread index)
+// Logical block -> (job, cells).  A launch covers `nlog` logical blocks of
+// BLOCK threads (one workgroup per logical block, or fewer workgroups that
+// loop over them: a persistent launch).  A logical block is either a run of
+// BLOCK consecutive cells of the stage region in i-fastest order (lx == -1)
+// or a tile of 2^lx x 2^ly x 2^lz cells (lx + ly + lz = log2 BLOCK).
 struct Sched {
   const Job* jobs;
-  const int* blkmap;
-  const int* blkstart;
+  const int* blkmap;    // logical block -> job (null: a single job)
+  const int* blkstart;  // job -> its first logical block
+  int nlog;             // logical blocks (0: one per workgroup)
+  int lx, ly, lz;       // tile shape (log2), lx = -1 for runs
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
@@ -96,31 +102,47 @@ PCD int xcd_swizzle(int i, int n) {
 typedef const __attribute__((address_space(4))) Job CJob;
 typedef const __attribute__((address_space(4))) int CInt;
 
-PCD Job load_job(const Sched& s, int& local) {
+PCD int first_block() { return xcd_swizzle(blockIdx.x, gridDim.x); }
+PCD int nlogical(const Sched& s) { return s.nlog ? s.nlog : (int)gridDim.x; }
+
+PCD Job load_job(const Sched& s, int lb, int& local) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
   int b = 0, start = 0;
   if (s.blkmap) {
-    b = ((CInt*)s.blkmap)[blk];
+    b = ((CInt*)s.blkmap)[lb];
     start = ((CInt*)s.blkstart)[b];
   }
-  local = (blk - start) * BLOCK + threadIdx.x;
+  local = (lb - start) * BLOCK + threadIdx.x;
   return ((CJob*)s.jobs)[b];
 #else
   // the host compilation pass parses device code but never runs it
-  local = 0;
+  local = lb;
   return *s.jobs;
 #endif
 }
 
-PCD bool region_ijk(const int lo[3], const int n[3], int local, int& i,
-                    int& j, int& k) {
-  const int nxy = n[0] * n[1];
-  if (local >= nxy * n[2]) return false;
-  k = local / nxy;
-  const int r = local - k * nxy;
-  j = r / n[0];
-  i = r - j * n[0];
+// Cell (i,j,k) of thread `local` of a job over the region [lo, lo + n).
+PCD bool region_ijk(const Sched& s, const int lo[3], const int n[3], int local,
+                    int& i, int& j, int& k) {
+  if (s.lx < 0) {
+    const int nxy = n[0] * n[1];
+    if (local >= nxy * n[2]) return false;
+    k = local / nxy;
+    const int r = local - k * nxy;
+    j = r / n[0];
+    i = r - j * n[0];
+  } else {
+    const int bib = local / BLOCK, t = local % BLOCK;
+    const int tx = 1 << s.lx, ty = 1 << s.ly, tz = 1 << s.lz;
+    const int nt0 = (n[0] + tx - 1) >> s.lx, nt1 = (n[1] + ty - 1) >> s.ly;
+    const int b0 = bib % nt0, r = bib / nt0;
+    const int b1 = r % nt1, b2 = r / nt1;
+    i = (b0 << s.lx) + (t & (tx - 1));
+    j = (b1 << s.ly) + ((t >> s.lx) & (ty - 1));
+    k = (b2 << s.lz) + (t >> (s.lx + s.ly));
+    (void)tz;
+    if (i >= n[0] || j >= n[1] || k >= n[2]) return false;
+  }
   i += lo[0];
   j += lo[1];
   k += lo[2];
@@ -238,14 +260,13 @@ PCD void step(int& i, int& j, int& k, int s) {
 // ------------------------------------------------------------ K1 ctoprim --
 // Source/PeleCUtilities.H:242-320
 template <class EOS>
-__global__ void __launch_bounds__(BLOCK)
-    k_ctoprim(Sched sc, KArgs ka) {
+PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const V& u = J.U;
   const V& q = J.q;
   const V& qa = J.qa;
@@ -291,17 +312,26 @@ __global__ void __launch_bounds__(BLOCK)
     qa(i, j, k, QRSPEC) = RU / w;
   }
 }
+template <class EOS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_ctoprim(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_ctoprim<EOS>(sc, ka, lb_);
+  } else {
+    b_ctoprim<EOS>(sc, ka, first_block());
+  }
+}
 
 // ---------------------------------------------------------- K2 srctoprim --
 // Source/Hydro.H:14-68 (src: NVAR comps over the q box)
 template <int NS>
-__global__ void __launch_bounds__(BLOCK)
-    k_srctoprim(Sched sc, KArgs ka, V src) {
+PCD void b_srctoprim(const Sched& sc, const KArgs& ka, int lb_, V src) {
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const V& q = J.q;
   const V& qa = J.qa;
   const V& srcq = J.srcq;
@@ -329,15 +359,25 @@ __global__ void __launch_bounds__(BLOCK)
     srcq(i, j, k, QFS + s) =
         (src(i, j, k, UFS + s) - q(i, j, k, QFS + s) * srcrho) * rhoinv;
 }
+template <int NS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_srctoprim(Sched sc, KArgs ka, V src) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_srctoprim<NS>(sc, ka, lb_, src);
+  } else {
+    b_srctoprim<NS>(sc, ka, first_block(), src);
+  }
+}
 
 // --------------------------------------------------------------- K24 divu --
 // Source/Hydro.H:476-517 on the nodes of the valid box
-__global__ void __launch_bounds__(BLOCK) k_divu(Sched sc, KArgs ka) {
+PCD void b_divu(const Sched& sc, const KArgs& ka, int lb_) {
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_DIVU, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const V& q = J.q;
   const double ux =
       0.25 *
@@ -358,6 +398,16 @@ __global__ void __launch_bounds__(BLOCK) k_divu(Sched sc, KArgs ka) {
        q(i - 1, j - 1, k, QW) - q(i - 1, j - 1, k - 1, QW)) /
       ka.dx[2];
   J.divu(i, j, k, 0) = ux + vy + wz;
+}
+template <bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_divu(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_divu(sc, ka, lb_);
+  } else {
+    b_divu(sc, ka, first_block());
+  }
 }
 
 // flattening coefficient of cell (i,j,k): min over the three directions
@@ -384,14 +434,13 @@ PCD double flat_cell(const Job& J, int i, int j, int k) {
 // ------------------------------------------------------ K3-5 trace (PPM) --
 // Source/PPM.cpp:82-423 for one cell and one direction (no WENO).
 template <class EOS, int D>
-__global__ void __launch_bounds__(BLOCK)
-    k_trace_ppm(Sched sc, KArgs ka) {
+PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const V& q = J.q;
   const double dtdx = get_dt(ka) / ka.dx[D];
   constexpr int QUN = QU + D;
@@ -540,18 +589,27 @@ __global__ void __launch_bounds__(BLOCK)
   step<D>(i1, j1, k1, 1);
   store_trace<NS>(tr, i1, j1, k1, 0, M);
 }
+template <class EOS, int D, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_trace_ppm(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_trace_ppm<EOS, D>(sc, ka, lb_);
+  } else {
+    b_trace_ppm<EOS, D>(sc, ka, first_block());
+  }
+}
 
 // --------------------------------------------------- K3' trace (PLM) ----
 // Source/Godunov.cpp:139-180 (one direction per launch)
 template <class EOS, int D>
-__global__ void __launch_bounds__(BLOCK)
-    k_trace_plm(Sched sc, KArgs ka) {
+PCD void b_trace_plm(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const V& q = J.q;
   const double dtdx = get_dt(ka) / ka.dx[D];
   const double flat = ka.use_flattening ? flat_cell(J, i, j, k) : 1.0;
@@ -590,17 +648,26 @@ __global__ void __launch_bounds__(BLOCK)
   step<D>(i1, j1, k1, 1);
   store_trace<NS>(tr, i1, j1, k1, 0, M);
 }
+template <class EOS, int D, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_trace_plm(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_trace_plm<EOS, D>(sc, ka, lb_);
+  } else {
+    b_trace_plm<EOS, D>(sc, ka, first_block());
+  }
+}
 
 // -------------------------------------------------- K6-8 first fluxes ----
 template <class EOS, int D>
-__global__ void __launch_bounds__(BLOCK)
-    k_flux1(Sched sc, KArgs ka) {
+PCD void b_flux1(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_FLUX1_X + D, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   FaceState<NS> L, R;
   load_trace<NS>(J.tr[D], i, j, k, 0, L);
   load_trace<NS>(J.tr[D], i, j, k, 1, R);
@@ -611,6 +678,16 @@ __global__ void __launch_bounds__(BLOCK)
   Flux<NS> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
   store_flux<NS>(J.f1[D], i, j, k, F);
+}
+template <class EOS, int D, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_flux1(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_flux1<EOS, D>(sc, ka, lb_);
+  } else {
+    b_flux1<EOS, D>(sc, ka, first_block());
+  }
 }
 
 // flux differences across cell c in direction T from a compact flux array
@@ -633,14 +710,13 @@ PCD void dflux(const V& f, int i, int j, int k, double c, DFlux<NS>& D) {
 // pc_transdo on the two cells of a d-face, then pc_cmpflx
 // (Source/Godunov.cpp:246-384).
 template <class EOS, int D, int T>
-__global__ void __launch_bounds__(BLOCK)
-    k_trans(Sched sc, KArgs ka) {
+PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRANS_XY + 2 * D + tslot(D, T), J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const double cdtdx = 1.0 / 3.0 * get_dt(ka) / ka.dx[T];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
@@ -659,6 +735,16 @@ __global__ void __launch_bounds__(BLOCK)
   Flux<NS> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
   store_flux<NS>(J.ft[2 * D + tslot(D, T)], i, j, k, F);
+}
+template <class EOS, int D, int T, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_trans(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_trans<EOS, D, T>(sc, ka, lb_);
+  } else {
+    b_trans<EOS, D, T>(sc, ka, first_block());
+  }
 }
 
 // transdd flux differences across cell (i,j,k) for final direction D
@@ -827,16 +913,15 @@ PCD void final_tail(const Job& J, const KArgs& ka, int i, int j, int k,
 }
 
 template <class EOS, int D, bool SRC>
-__global__ void __launch_bounds__(BLOCK)
-    k_final(Sched sc, KArgs ka) {
+PCD void b_final(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   constexpr int T0 = (D == 0) ? 1 : 0;
   constexpr int T1 = (D == 2) ? 1 : 2;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   if (loworder_face(J, D, idx)) return;
   const double dt = get_dt(ka);
@@ -852,19 +937,28 @@ __global__ void __launch_bounds__(BLOCK)
   final_corr<NS, D, SRC>(J, i, j, k, hdt, c0, c1, s, R);
   final_tail<EOS, D>(J, ka, i, j, k, L, R);
 }
+template <class EOS, int D, bool SRC, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_final(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_final<EOS, D, SRC>(sc, ka, lb_);
+  } else {
+    b_final<EOS, D, SRC>(sc, ka, first_block());
+  }
+}
 
 // pc_low_order_boundary (Source/Godunov.cpp:7-55,440-468): at Hard/UserBC
 // domain faces the flux comes from first-order-in-time PLM states without
 // transverse terms.  Launched only when such a boundary exists.
 template <class EOS, int D>
-__global__ void __launch_bounds__(BLOCK)
-    k_final_loworder(Sched sc, KArgs ka) {
+PCD void b_final_loworder(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   if (!loworder_face(J, D, idx)) return;
   const double dtdx = get_dt(ka) / ka.dx[D];
@@ -874,6 +968,16 @@ __global__ void __launch_bounds__(BLOCK)
   plm_cell<EOS, D>(J, ka, dtdx, im, jm, km, unused, L);
   plm_cell<EOS, D>(J, ka, dtdx, i, j, k, R, unused);
   final_tail<EOS, D>(J, ka, i, j, k, L, R);
+}
+template <class EOS, int D, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_final_loworder(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_final_loworder<EOS, D>(sc, ka, lb_);
+  } else {
+    b_final_loworder<EOS, D>(sc, ka, first_block());
+  }
 }
 
 // pdivu (Source/Godunov.H:1033-1060) and the flux divergence of one cell
@@ -911,20 +1015,29 @@ PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
 
 // ----------------------------------------------------------- K28 consup --
 template <int NS>
-__global__ void __launch_bounds__(BLOCK)
-    k_consup(Sched sc, KArgs ka) {
+PCD void b_consup(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NVAR = 7 + NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   double h[NVAR];
 #pragma unroll
   for (int m = 0; m < NVAR; ++m) h[m] = J.upd(i, j, k, m);
   cell_divergence<NS>(J, ka, i, j, k, h);
 #pragma unroll
   for (int m = 0; m < NVAR; ++m) J.upd(i, j, k, m) = h[m];
+}
+template <int NS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_consup(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_consup<NS>(sc, ka, lb_);
+  } else {
+    b_consup<NS>(sc, ka, first_block());
+  }
 }
 
 }  // namespace pcd

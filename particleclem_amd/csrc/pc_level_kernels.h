@@ -40,14 +40,13 @@ PCD int ghost_src(int x, int lo, int hi, int bcl, int bch, int& refl) {
 }
 
 template <int NS>
-__global__ void __launch_bounds__(BLOCK)
-    k_ghost(Sched sc, KArgs ka, LevelArgs la) {
+PCD void b_ghost(const Sched& sc, const KArgs& ka, int lb_, LevelArgs la) {
   constexpr int NVAR = 7 + NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_GHOST, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   if (i >= J.lo[0] && i <= J.hi[0] && j >= J.lo[1] && j <= J.hi[1] &&
       k >= J.lo[2] && k <= J.hi[2])
     return;
@@ -78,6 +77,16 @@ __global__ void __launch_bounds__(BLOCK)
       if ((r[d] == 2 && mom) || (r[d] == 1 && m == UMX + d)) sg = -sg;
     }
     J.U(i, j, k, m) = sg * src(s0, s1, s2, m);
+  }
+}
+template <int NS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_ghost(Sched sc, KArgs ka, LevelArgs la) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_ghost<NS>(sc, ka, lb_, la);
+  } else {
+    b_ghost<NS>(sc, ka, first_block(), la);
   }
 }
 
@@ -153,16 +162,15 @@ PCD void reset_and_temp(double* S, const typename EOS::Thermo& th) {
 // (computeTemp in do_sdc_iteration and in post_timestep), NaN check and the
 // CFL partial minimum on the new state.
 template <class EOS>
-__global__ void __launch_bounds__(BLOCK)
-    k_update(Sched sc, KArgs ka, DtState* ds) {
+PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState* ds) {
   constexpr int NS = EOS::NS;
   constexpr int NVAR = 7 + NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   double mydt = 1.7976931348623157e308;
-  if (region_ijk(lo, n, local, i, j, k)) {
+  if (region_ijk(sc, lo, n, local, i, j, k)) {
     const double dt = get_dt(ka);
     double h[NVAR];
 #pragma unroll
@@ -185,20 +193,32 @@ __global__ void __launch_bounds__(BLOCK)
     if (bad) ds->bad = 1;
     mydt = cell_estdt<EOS>(S, th, ka.dx);
   }
-  block_min_to(mydt, &ds->minbits);
+  return mydt;
+}
+
+template <class EOS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_update(Sched sc, KArgs ka, DtState* ds) {
+  double m = 1.7976931348623157e308;
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      m = mn(m, b_update<EOS>(sc, ka, lb_, ds));
+  } else {
+    m = b_update<EOS>(sc, ka, first_block(), ds);
+  }
+  block_min_to(m, &ds->minbits);
 }
 
 // estTimeStep partial minimum over the valid cells (no update)
 template <class EOS>
-__global__ void __launch_bounds__(BLOCK)
-    k_estdt(Sched sc, KArgs ka, DtState* ds) {
+PCD double b_estdt(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   double mydt = 1.7976931348623157e308;
-  if (region_ijk(lo, n, local, i, j, k)) {
+  if (region_ijk(sc, lo, n, local, i, j, k)) {
     double S[7 + NS];
 #pragma unroll
     for (int m = 0; m < 7 + NS; ++m) S[m] = J.U(i, j, k, m);
@@ -207,7 +227,20 @@ __global__ void __launch_bounds__(BLOCK)
     const typename EOS::Thermo th(Y);
     mydt = cell_estdt<EOS>(S, th, ka.dx);
   }
-  block_min_to(mydt, &ds->minbits);
+  return mydt;
+}
+
+template <class EOS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_estdt(Sched sc, KArgs ka, DtState* ds) {
+  double m = 1.7976931348623157e308;
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      m = mn(m, b_estdt<EOS>(sc, ka, lb_));
+  } else {
+    m = b_estdt<EOS>(sc, ka, first_block());
+  }
+  block_min_to(m, &ds->minbits);
 }
 
 // estTimeStep (Source/PeleC.cpp:829-961) from the partial minimum, then
@@ -240,14 +273,13 @@ __global__ void k_dt_finalize(DtState* ds, LevelArgs la, int init) {
 
 // primitive -> conservative (pc_initdata pattern; see workloads.py)
 template <class EOS>
-__global__ void __launch_bounds__(BLOCK)
-    k_prim_to_cons(Sched sc, KArgs ka, V prim, int mode) {
+PCD void b_prim_to_cons(const Sched& sc, const KArgs& ka, int lb_, V prim, int mode) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, local);
+  const Job J = load_job(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
-  if (!region_ijk(lo, n, local, i, j, k)) return;
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   double Y[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) Y[s] = prim(i, j, k, 5 + s);
@@ -277,6 +309,16 @@ __global__ void __launch_bounds__(BLOCK)
   U(i, j, k, UTEMP) = T;
 #pragma unroll
   for (int s = 0; s < NS; ++s) U(i, j, k, UFS + s) = rho * Y[s];
+}
+template <class EOS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_prim_to_cons(Sched sc, KArgs ka, V prim, int mode) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_prim_to_cons<EOS>(sc, ka, lb_, prim, mode);
+  } else {
+    b_prim_to_cons<EOS>(sc, ka, first_block(), prim, mode);
+  }
 }
 
 }  // namespace pcd

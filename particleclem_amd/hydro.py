@@ -212,6 +212,34 @@ class DeviceLevel:
         rc = self.L.pc_level_status_v1(self.h, C.byref(bad), C.byref(small))
         return rc, bad.value, small.value
 
+    def autotune(self, reps=3, stream=None):
+        """Pick the fastest launch shape of every stage on this level
+        (synchronous; state and dt unchanged)."""
+        self._chk(self.L.pc_level_autotune_v1(self.h, int(reps), _stream_ptr(stream)))
+
+    def plans(self):
+        """{stage: (shape (lx, ly, lz) or 'runs', workgroups, logical
+        blocks, tuned ms)}"""
+        out = {}
+        for sid, name in enumerate(_lib.STAGES):
+            sh = (C.c_int * 5)()
+            ms = C.c_double()
+            self._chk(self.L.pc_level_plan_v1(self.h, sid, sh, C.byref(ms)))
+            shape = "runs" if sh[0] < 0 else (1 << sh[0], 1 << sh[1], 1 << sh[2])
+            out[name] = (shape, sh[3], sh[4], ms.value)
+        return out
+
+    def set_plan(self, stage, shape=None, wg_per_cu=0):
+        """Plan of one stage (name or id): shape None = runs of 256 cells,
+        else (lx, ly, lz) log2 tile extents summing to 8."""
+        sid = _lib.STAGES.index(stage) if isinstance(stage, str) else int(stage)
+        lx, ly, lz = (-1, 0, 0) if shape is None else shape
+        self._chk(self.L.pc_level_set_plan_v1(self.h, sid, lx, ly, lz, int(wg_per_cu)))
+
+    def set_graph(self, steps_per_graph):
+        """Replay steps from a hipGraph of `steps_per_graph` steps (0: off)."""
+        self._chk(self.L.pc_level_set_graph_v1(self.h, int(steps_per_graph)))
+
     def enable_timing(self, on=True):
         self._chk(self.L.pc_level_enable_timing_v1(self.h, 1 if on else 0))
 
