@@ -99,6 +99,81 @@ def profiled_step_flops(summary):
     return None
 
 
+def oracle_norms(w, gpu_fabs, host_fabs):
+    """L1/Linf relative errors in rho and T of GPU box states against the
+    oracle's (valid cells)."""
+    import numpy as np
+    acc = {q: [0.0, 0.0, 0.0, 0.0] for q in ("rho", "T")}
+    for g, h in zip(gpu_fabs, host_fabs):
+        for q, c in (("rho", 0), ("T", 6)):
+            a = g[c, 4:-4, 4:-4, 4:-4]
+            b = h[c, 4:-4, 4:-4, 4:-4]
+            d = np.abs(a - b)
+            v = acc[q]
+            v[0] += d.sum()
+            v[1] += np.abs(b).sum()
+            v[2] = max(v[2], d.max())
+            v[3] = max(v[3], np.abs(b).max())
+    return {q: {"L1": v[0] / v[1], "Linf": v[2] / v[3]} for q, v in acc.items()}
+
+
+def precision_check(ctx, w, steps, with_oracle):
+    """Fresh FP64 and mixed levels from the initial state, stepped in
+    lockstep to each checkpoint: error norms of the mixed state against the
+    FP64 state (and, at the first checkpoint when it is step 1, against the
+    CPU oracle), plus each level's wall time per step."""
+    import torch
+    from particleclem_amd.hydro import DeviceLevel
+    from particleclem_amd.precision import error_norms
+    l64 = DeviceLevel.from_workload(ctx, w)
+    l32 = DeviceLevel.from_workload(ctx, w)
+    l32.set_precision(True)
+    for x in (l64, l32):
+        x.init_dt()
+    torch.cuda.synchronize()
+    t64 = t32 = 0.0
+    done = 0
+    res = {}
+    for s in steps:
+        n = s - done
+        t0 = time.perf_counter()
+        l64.step(n)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        l32.step(n)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t64 += t1 - t0
+        t32 += t2 - t1
+        done = s
+        e = error_norms(l32, l64, w.nspec, len(w.boxes))
+        ys = [e[f"Y{k}"] for k in range(w.nspec)] if w.nspec > 1 else []
+        r = {"rho": e["rho"], "T": e["T"]}
+        if ys:
+            r["Y_max"] = {"L1": max(y["L1"] for y in ys),
+                          "Linf": max(y["Linf"] for y in ys)}
+            r["Y"] = {f"Y{k}": e[f"Y{k}"] for k in range(w.nspec)}
+        res[str(s)] = r
+        if with_oracle and s == 1:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_bind as ob
+            H = ob.HostLevel(w.nspec, w.boxes, w.domlo, w.domhi, w.bclo,
+                             w.bchi, w.dx, periodic_self=w.periodic_self)
+            H.set_prims(w)
+            prm = ob.params(**w.params)
+            H.step(prm, 1, H.estdt(prm) * prm.init_shrink, 0)
+            g = [l32.read_box(b) for b in range(len(w.boxes))]
+            r["vs_cpu_oracle"] = oracle_norms(w, g, [H.fab(b) for b in range(len(w.boxes))])
+    out = {"reference": "FP64 step on the GPU (bitwise equal to the CPU "
+                        "oracle in the parity tests); dt from each run's own "
+                        "CFL estimate",
+           "errors": res, "fp64_ms_per_step": t64 / done * 1e3,
+           "mixed_ms_per_step": t32 / done * 1e3, "mixed_speedup": t64 / t32}
+    for x in (l64, l32):
+        x.close()
+    return out
+
+
 def cpu_baseline(w, gpu_fabs, dt, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
@@ -127,6 +202,13 @@ def main():
                     help="pick every stage's launch shape on the level first")
     ap.add_argument("--graph", type=int, default=0,
                     help="replay steps from a hipGraph of this many steps")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed: FP32 reconstruction/limiters/Riemann and "
+                         "intermediates, FP64 EOS inversion and update")
+    ap.add_argument("--error-steps", default="",
+                    help="comma list of steps: afterwards step fresh FP64 and "
+                         "mixed levels from the initial state in lockstep and "
+                         "report error norms (and both step times) there")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--box", type=int, default=32)
     ap.add_argument("--kernel-steps", type=int, default=3)
@@ -187,6 +269,7 @@ def main():
 
     ctx = HydroContext(w.nspec)
     lev = DeviceLevel.from_workload(ctx, w)
+    lev.set_precision(args.precision == "mixed")
     lev.init_dt()
     torch.cuda.synchronize()
     cells = w.valid_cells()
@@ -268,7 +351,9 @@ def main():
         "metric": METRIC, "value": value, "unit": "Mcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64 (mixed)",
+        "data": "synthetic",
         "config": {"workload": desc, "valid_cells": cells,
                    "boxes": len(w.boxes), "nspec": w.nspec,
                    "parallelism": "single GPU, all boxes per launch",
@@ -280,8 +365,12 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
     lev.close()
+    if args.error_steps:
+        out["precision_check"] = precision_check(
+            ctx, w, sorted(int(x) for x in args.error_steps.split(",")),
+            not args.no_cpu_baseline)
+    print(json.dumps(out))
     ctx.close()
 
 

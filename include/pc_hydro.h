@@ -241,6 +241,17 @@ int pc_level_plan_v1(const pc_level_t* lev, int stage, int* shape, double* ms);
  * Synchronous. */
 int pc_level_set_plan_v1(pc_level_t* lev, int stage, int lx, int ly, int lz,
                          int wg_per_cu);
+/* Arithmetic of the level step.  0 (default): FP64 throughout, the
+ * reference's arithmetic.  1 (mixed): the primitives and every stage
+ * intermediate are stored in FP32 and the PPM/PLM reconstruction, limiters,
+ * characteristic tracing, transverse corrections and Riemann solves run in
+ * FP32 (with the EOS calls they make); the e -> T Newton inversion (ctoprim
+ * and computeTemp), the total-minus-kinetic internal energy, the final flux
+ * assembly (artificial viscosity, species normalisation, area) and the
+ * conservative update stay FP64, so sums of fluxes telescope exactly and
+ * conservation holds to FP64 round-off.  No reference counterpart (the
+ * reference computes in FP64 only).  Synchronous. */
+int pc_level_set_precision_v1(pc_level_t* lev, int mixed);
 /* steps_per_graph > 0: pc_level_step_v1 captures that many consecutive
  * steps into a hipGraph once (re-captured after autotuning) and replays it;
  * leftover steps and timed steps launch directly.  0 turns graphs off. */

@@ -24,7 +24,7 @@ EXPORTS = [
     "pc_level_enable_timing_v1", "pc_level_kernel_times_v1",
     "pc_level_status_v1", "pc_level_prim_to_cons_v1",
     "pc_level_autotune_v1", "pc_level_plan_v1", "pc_level_set_plan_v1",
-    "pc_level_set_graph_v1",
+    "pc_level_set_graph_v1", "pc_level_set_precision_v1",
 ]
 
 PC_OK = 0
@@ -84,6 +84,7 @@ def lib():
     L.pc_level_autotune_v1.argtypes = [vp, C.c_int, vp]
     L.pc_level_plan_v1.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_double)]
     L.pc_level_set_graph_v1.argtypes = [vp, C.c_int]
+    L.pc_level_set_precision_v1.argtypes = [vp, C.c_int]
     L.pc_level_set_plan_v1.argtypes = [vp] + [C.c_int] * 5
     L.pc_level_create_v1.argtypes = [vp, P(LevelDesc), P(Params), P(vp)]
     L.pc_level_destroy_v1.argtypes = [vp]

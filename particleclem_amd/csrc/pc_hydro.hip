@@ -654,6 +654,7 @@ struct pc_level {
   double ktime[PC_NKERN];
   int nsteps_timed = 0;
   std::vector<Job> hjobs;      // host copy of the job table (plans)
+  int mixed = 0;               // 1: FP32 reconstruction/Riemann/intermediates
   double tuned_ms[ST_COUNT] = {};
   // hipGraph of gsteps consecutive steps, replayed by pc_level_step_v1
   int gsteps = 0;
@@ -732,46 +733,46 @@ static void level_free(pc_level* L) {
 // Launch every kernel of stage s once with schedule sc on `grid` workgroups
 // (sub >= 0: only that direction of the trace stage).  P: the persistent
 // instantiations, which loop over the logical blocks.
-template <class EOS, bool P>
+template <class EOS, class FP, bool P>
 static void launch_stage_t(pc_level* L, int s, const Sched& sc, int grid,
                            hipStream_t st, int sub) {
   const KArgs& ka = L->ka;
   auto tr = [&](int d) { return sub < 0 || sub == d; };
   switch (s) {
     case ST_GHOST: LAUNCH((k_ghost<EOS::NS, P>), grid, st, sc, ka, L->la); break;
-    case ST_CTOPRIM: LAUNCH((k_ctoprim<EOS, P>), grid, st, sc, ka); break;
-    case ST_DIVU: LAUNCH((k_divu<P>), grid, st, sc, ka); break;
+    case ST_CTOPRIM: LAUNCH((k_ctoprim<EOS, FP, P>), grid, st, sc, ka); break;
+    case ST_DIVU: LAUNCH((k_divu<FP, P>), grid, st, sc, ka); break;
     case ST_TRACE:
       if (L->prm.ppm_type == 1) {
-        if (tr(0)) LAUNCH((k_trace_ppm<EOS, 0, P>), grid, st, sc, ka);
-        if (tr(1)) LAUNCH((k_trace_ppm<EOS, 1, P>), grid, st, sc, ka);
-        if (tr(2)) LAUNCH((k_trace_ppm<EOS, 2, P>), grid, st, sc, ka);
+        if (tr(0)) LAUNCH((k_trace_ppm<EOS, 0, FP, P>), grid, st, sc, ka);
+        if (tr(1)) LAUNCH((k_trace_ppm<EOS, 1, FP, P>), grid, st, sc, ka);
+        if (tr(2)) LAUNCH((k_trace_ppm<EOS, 2, FP, P>), grid, st, sc, ka);
       } else {
-        if (tr(0)) LAUNCH((k_trace_plm<EOS, 0, P>), grid, st, sc, ka);
-        if (tr(1)) LAUNCH((k_trace_plm<EOS, 1, P>), grid, st, sc, ka);
-        if (tr(2)) LAUNCH((k_trace_plm<EOS, 2, P>), grid, st, sc, ka);
+        if (tr(0)) LAUNCH((k_trace_plm<EOS, 0, FP, P>), grid, st, sc, ka);
+        if (tr(1)) LAUNCH((k_trace_plm<EOS, 1, FP, P>), grid, st, sc, ka);
+        if (tr(2)) LAUNCH((k_trace_plm<EOS, 2, FP, P>), grid, st, sc, ka);
       }
       break;
-    case ST_FLUX1_X: LAUNCH((k_flux1<EOS, 0, P>), grid, st, sc, ka); break;
-    case ST_FLUX1_Y: LAUNCH((k_flux1<EOS, 1, P>), grid, st, sc, ka); break;
-    case ST_FLUX1_Z: LAUNCH((k_flux1<EOS, 2, P>), grid, st, sc, ka); break;
-    case ST_TRANS_XY: LAUNCH((k_trans<EOS, 0, 1, P>), grid, st, sc, ka); break;
-    case ST_TRANS_XZ: LAUNCH((k_trans<EOS, 0, 2, P>), grid, st, sc, ka); break;
-    case ST_TRANS_YX: LAUNCH((k_trans<EOS, 1, 0, P>), grid, st, sc, ka); break;
-    case ST_TRANS_YZ: LAUNCH((k_trans<EOS, 1, 2, P>), grid, st, sc, ka); break;
-    case ST_TRANS_ZX: LAUNCH((k_trans<EOS, 2, 0, P>), grid, st, sc, ka); break;
-    case ST_TRANS_ZY: LAUNCH((k_trans<EOS, 2, 1, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_X: LAUNCH((k_flux1<EOS, 0, FP, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_Y: LAUNCH((k_flux1<EOS, 1, FP, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_Z: LAUNCH((k_flux1<EOS, 2, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_XY: LAUNCH((k_trans<EOS, 0, 1, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_XZ: LAUNCH((k_trans<EOS, 0, 2, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_YX: LAUNCH((k_trans<EOS, 1, 0, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_YZ: LAUNCH((k_trans<EOS, 1, 2, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_ZX: LAUNCH((k_trans<EOS, 2, 0, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_ZY: LAUNCH((k_trans<EOS, 2, 1, FP, P>), grid, st, sc, ka); break;
     case ST_FINAL_X:
-      LAUNCH((k_final<EOS, 0, false, P>), grid, st, sc, ka);
-      if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0, P>), grid, st, sc, ka);
+      LAUNCH((k_final<EOS, 0, false, FP, P>), grid, st, sc, ka);
+      if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0, FP, P>), grid, st, sc, ka);
       break;
     case ST_FINAL_Y:
-      LAUNCH((k_final<EOS, 1, false, P>), grid, st, sc, ka);
-      if (L->lowo[1]) LAUNCH((k_final_loworder<EOS, 1, P>), grid, st, sc, ka);
+      LAUNCH((k_final<EOS, 1, false, FP, P>), grid, st, sc, ka);
+      if (L->lowo[1]) LAUNCH((k_final_loworder<EOS, 1, FP, P>), grid, st, sc, ka);
       break;
     case ST_FINAL_Z:
-      LAUNCH((k_final<EOS, 2, false, P>), grid, st, sc, ka);
-      if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2, P>), grid, st, sc, ka);
+      LAUNCH((k_final<EOS, 2, false, FP, P>), grid, st, sc, ka);
+      if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2, FP, P>), grid, st, sc, ka);
       break;
     case ST_UPDATE: LAUNCH((k_update<EOS, P>), grid, st, sc, ka, L->ds); break;
     default: break;
@@ -781,10 +782,18 @@ static void launch_stage_t(pc_level* L, int s, const Sched& sc, int grid,
 template <class EOS>
 static void launch_stage(pc_level* L, int s, const Sched& sc,
                          const StagePlan& pl, hipStream_t st, int sub = -1) {
-  if (pl.grid < pl.nblocks)
-    launch_stage_t<EOS, true>(L, s, sc, pl.grid, st, sub);
-  else
-    launch_stage_t<EOS, false>(L, s, sc, pl.grid, st, sub);
+  const bool persist = pl.grid < pl.nblocks;
+  if (L->mixed) {
+    if (persist)
+      launch_stage_t<EOS, float, true>(L, s, sc, pl.grid, st, sub);
+    else
+      launch_stage_t<EOS, float, false>(L, s, sc, pl.grid, st, sub);
+  } else {
+    if (persist)
+      launch_stage_t<EOS, double, true>(L, s, sc, pl.grid, st, sub);
+    else
+      launch_stage_t<EOS, double, false>(L, s, sc, pl.grid, st, sub);
+  }
 }
 
 extern "C" {
@@ -1286,6 +1295,14 @@ int pc_level_set_plan_v1(pc_level_t* L, int stage, int lx, int ly, int lz,
   L->plans[stage] = P;
   L->scs[stage] = plan_sched(L, P);
   L->tuned_ms[stage] = 0.0;
+  return PC_OK;
+}
+
+int pc_level_set_precision_v1(pc_level_t* L, int mixed) {
+  if (!L || (mixed != 0 && mixed != 1)) return PC_ERR_ARG;
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  graph_drop(L);
+  L->mixed = mixed;
   return PC_OK;
 }
 

@@ -22,41 +22,51 @@ namespace pcd {
 
 constexpr int BLOCK = 256;
 
-// Global-memory double (address space 1): loads and stores through it are
-// global_load/global_store rather than flat accesses.
-typedef __attribute__((address_space(1))) double gdouble;
-
-// SoA view of one FAB (Array4 with 32-bit strides).
-struct V {
-  double* p;
+// SoA view of one FAB (Array4 with 32-bit strides) of element type T.
+// Element references are in the global address space (address space 1), so
+// loads and stores are global_load/global_store rather than flat accesses.
+template <class T>
+struct VT {
+  typedef __attribute__((address_space(1))) T gT;
+  T* p;
   int js, ks, ns;
   int b0, b1, b2;
-  PCD gdouble& operator()(int i, int j, int k, int n) const {
-    return ((gdouble*)p)[(i - b0) + (j - b1) * js + (k - b2) * ks + n * ns];
+  PCD gT& operator()(int i, int j, int k, int n) const {
+    return ((gT*)p)[(i - b0) + (j - b1) * js + (k - b2) * ks + n * ns];
   }
   PCD bool contains(int i, int j, int k, int e0, int e1, int e2) const {
     return i >= b0 && i < e0 && j >= b1 && j < e1 && k >= b2 && k < e2;
   }
 };
+using V = VT<double>;
 
 // Compact layouts of the stage intermediates.
 //   traces: comps [0, QT) = minus state (left of face), [QT, 2QT) = plus
 //           state; per state rho, u, v, w, p, rhoe, Y[NS]  (QT = 6 + NS)
 //   fluxes: rho, mx, my, mz, rhoE, p_gd, un_gd, rhoY[NS]   (FT = 7 + NS)
-struct Job {
+// FP is the element type of the primitives and of the stage intermediates
+// (double, or float in the mixed-precision step); the state, the final
+// fluxes, qec and divu are always double.  Both instantiations have the same
+// layout (views differ only in the pointee type), so one job table serves
+// either.
+template <class FP>
+struct JobT {
   int lo[3], hi[3];                 // valid box
   int dlo[3], dhi[3], bclo[3], bchi[3];
   int qe[3];                        // exclusive end of the q box (contains)
-  V U;                              // state with >= 4 ghosts (uin)
-  V q, qa, srcq;                    // primitives on grow(bx,4); srcq.p may be 0
-  V tr[3];                          // traces per direction
-  V f1[3];                          // first fluxes
-  V ft[6];                          // transverse fluxes, [2*d + slot(d,t)]
-  V flx[3], qec[3];                 // final fluxes (NVAR) and p_gd, un_gd
-  V area[3], vol;                   // p == 0: uniform geometry
-  V upd;                            // uout (dU/dt accumulator) or p == 0
-  V divu;                           // nodes [L, H+1]^3
+  VT<double> U;                     // state with >= 4 ghosts (uin)
+  VT<FP> q, qa, srcq;               // primitives on grow(bx,4); srcq.p may be 0
+  VT<FP> tr[3];                     // traces per direction
+  VT<FP> f1[3];                     // first fluxes
+  VT<FP> ft[6];                     // transverse fluxes, [2*d + slot(d,t)]
+  VT<double> flx[3], qec[3];        // final fluxes (NVAR) and p_gd, un_gd
+  VT<double> area[3], vol;          // p == 0: uniform geometry
+  VT<double> upd;                   // uout (dU/dt accumulator) or p == 0
+  VT<double> divu;                  // nodes [L, H+1]^3
 };
+using Job = JobT<double>;
+static_assert(sizeof(JobT<float>) == sizeof(JobT<double>),
+              "job layouts of both precisions must agree");
 
 struct KArgs {
   double dx[3];
@@ -99,13 +109,14 @@ PCD int xcd_swizzle(int i, int n) {
 
 // The job table is read-only during a launch: read it through the constant
 // address space so the (block-uniform) descriptor fields are scalar loads.
-typedef const __attribute__((address_space(4))) Job CJob;
 typedef const __attribute__((address_space(4))) int CInt;
 
 PCD int first_block() { return xcd_swizzle(blockIdx.x, gridDim.x); }
 PCD int nlogical(const Sched& s) { return s.nlog ? s.nlog : (int)gridDim.x; }
 
-PCD Job load_job(const Sched& s, int lb, int& local) {
+template <class FP = double>
+PCD JobT<FP> load_job(const Sched& s, int lb, int& local) {
+  typedef const __attribute__((address_space(4))) JobT<FP> CJob;
 #if defined(__HIP_DEVICE_COMPILE__)
   int b = 0, start = 0;
   if (s.blkmap) {
@@ -117,7 +128,7 @@ PCD Job load_job(const Sched& s, int lb, int& local) {
 #else
   // the host compilation pass parses device code but never runs it
   local = lb;
-  return *s.jobs;
+  return *(const JobT<FP>*)s.jobs;
 #endif
 }
 
@@ -209,9 +220,9 @@ __host__ __device__ inline void stage_region(int st, const int L[3],
 }
 
 // ----------------------------------------------------------------- loads --
-template <int NS>
-PCD void load_trace(const V& tr, int i, int j, int k, int side,
-                    FaceState<NS>& s) {
+template <int NS, class FP>
+PCD void load_trace(const VT<FP>& tr, int i, int j, int k, int side,
+                    FaceState<NS, FP>& s) {
   constexpr int QT = 6 + NS;
   const int o = side * QT;
   s.rho = tr(i, j, k, o + 0);
@@ -223,9 +234,9 @@ PCD void load_trace(const V& tr, int i, int j, int k, int side,
 #pragma unroll
   for (int n = 0; n < NS; ++n) s.Y[n] = tr(i, j, k, o + 6 + n);
 }
-template <int NS>
-PCD void store_trace(const V& tr, int i, int j, int k, int side,
-                     const FaceState<NS>& s) {
+template <int NS, class FP>
+PCD void store_trace(const VT<FP>& tr, int i, int j, int k, int side,
+                     const FaceState<NS, FP>& s) {
   constexpr int QT = 6 + NS;
   const int o = side * QT;
   tr(i, j, k, o + 0) = s.rho;
@@ -237,8 +248,8 @@ PCD void store_trace(const V& tr, int i, int j, int k, int side,
 #pragma unroll
   for (int n = 0; n < NS; ++n) tr(i, j, k, o + 6 + n) = s.Y[n];
 }
-template <int NS>
-PCD void store_flux(const V& f, int i, int j, int k, const Flux<NS>& F) {
+template <int NS, class FP>
+PCD void store_flux(const VT<FP>& f, int i, int j, int k, const Flux<NS, FP>& F) {
   f(i, j, k, 0) = F.f[0];
   f(i, j, k, 1) = F.f[1];
   f(i, j, k, 2) = F.f[2];
@@ -259,17 +270,17 @@ PCD void step(int& i, int& j, int& k, int s) {
 
 // ------------------------------------------------------------ K1 ctoprim --
 // Source/PeleCUtilities.H:242-320
-template <class EOS>
+template <class EOS, class FP>
 PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const V& u = J.U;
-  const V& q = J.q;
-  const V& qa = J.qa;
+  const auto& u = J.U;
+  const auto& q = J.q;
+  const auto& qa = J.qa;
   const double rho = u(i, j, k, URHO);
   const double rhoinv = 1.0 / rho;
   const double vx = u(i, j, k, UMX) * rhoinv;
@@ -312,14 +323,14 @@ PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
     qa(i, j, k, QRSPEC) = RU / w;
   }
 }
-template <class EOS, bool PERSIST = false>
+template <class EOS, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_ctoprim(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_ctoprim<EOS>(sc, ka, lb_);
+      b_ctoprim<EOS, FP>(sc, ka, lb_);
   } else {
-    b_ctoprim<EOS>(sc, ka, first_block());
+    b_ctoprim<EOS, FP>(sc, ka, first_block());
   }
 }
 
@@ -372,13 +383,14 @@ __global__ void __launch_bounds__(BLOCK)
 
 // --------------------------------------------------------------- K24 divu --
 // Source/Hydro.H:476-517 on the nodes of the valid box
+template <class FP>
 PCD void b_divu(const Sched& sc, const KArgs& ka, int lb_) {
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_DIVU, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const V& q = J.q;
+  const auto& q = J.q;
   const double ux =
       0.25 *
       (q(i, j, k, QU) - q(i - 1, j, k, QU) + q(i, j, k - 1, QU) -
@@ -399,21 +411,22 @@ PCD void b_divu(const Sched& sc, const KArgs& ka, int lb_) {
       ka.dx[2];
   J.divu(i, j, k, 0) = ux + vy + wz;
 }
-template <bool PERSIST = false>
+template <class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_divu(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_divu(sc, ka, lb_);
+      b_divu<FP>(sc, ka, lb_);
   } else {
-    b_divu(sc, ka, first_block());
+    b_divu<FP>(sc, ka, first_block());
   }
 }
 
 // flattening coefficient of cell (i,j,k): min over the three directions
-PCD double flat_cell(const Job& J, int i, int j, int k) {
-  const V& q = J.q;
-  double flat = 1.0;
+template <class FP>
+PCD FP flat_cell(const JobT<FP>& J, int i, int j, int k) {
+  const auto& q = J.q;
+  FP flat = FP(1.0);
 #pragma unroll
   for (int df = 0; df < 3; ++df) {
     int oi = (df == 0), oj = (df == 1), ok = (df == 2);
@@ -426,23 +439,23 @@ PCD double flat_cell(const Job& J, int i, int j, int k) {
     auto uq = [&](int m) {
       return q(i + m * oi, j + m * oj, k + m * ok, QU + df);
     };
-    flat = mn(flat, flatten_line(inside, pq, uq));
+    flat = mn(flat, flatten_line<FP>(inside, pq, uq));
   }
   return flat;
 }
 
 // ------------------------------------------------------ K3-5 trace (PPM) --
 // Source/PPM.cpp:82-423 for one cell and one direction (no WENO).
-template <class EOS, int D>
+template <class EOS, int D, class FP>
 PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const V& q = J.q;
-  const double dtdx = get_dt(ka) / ka.dx[D];
+  const auto& q = J.q;
+  const FP dtdx = get_dt(ka) / ka.dx[D];
   constexpr int QUN = QU + D;
   constexpr int QUT = (D == 0) ? QV : ((D == 1) ? QW : QU);
   constexpr int QUTT = (D == 0) ? QW : ((D == 1) ? QU : QV);
@@ -451,16 +464,16 @@ PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
     step<D>(ii, jj, kk, m);
     return q(ii, jj, kk, comp);
   };
-  double Yc[NS];
+  FP Yc[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) Yc[s] = q(i, j, k, QFS + s);
-  const double cc = EOS::RPY2Cs(q(i, j, k, QRHO), q(i, j, k, QPRES), Yc);
-  const double un = q(i, j, k, QUN);
-  const double flat = ka.use_flattening ? flat_cell(J, i, j, k) : 1.0;
-  const double spd0 = un - cc, spd1 = un, spd2 = un + cc;
-  double sm, sp, s6;
+  const FP cc = EOS::RPY2Cs(q(i, j, k, QRHO), q(i, j, k, QPRES), Yc);
+  const FP un = q(i, j, k, QUN);
+  const FP flat = ka.use_flattening ? flat_cell(J, i, j, k) : FP(1.0);
+  const FP spd0 = un - cc, spd1 = un, spd2 = un + cc;
+  FP sm, sp, s6;
   auto recon = [&](int comp) {
-    double s[5];
+    FP s[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) s[m] = at(m - 2, comp);
     ppm_reconstruct(s, flat, sm, sp);
@@ -468,35 +481,35 @@ PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
   };
   // integrals needed by the characteristic projection
   recon(QRHO);
-  const double Im_r0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const double Im_r1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const double Ip_r2 = ppm_ip(sm, sp, s6, spd2, dtdx);
-  const double Ip_r1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  const FP Im_r0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const FP Im_r1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const FP Ip_r2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  const FP Ip_r1 = ppm_ip(sm, sp, s6, spd1, dtdx);
   recon(QUN);
-  const double Im_u0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const double Im_u2 = ppm_im(sm, sp, s6, spd2, dtdx);
-  const double Ip_u2 = ppm_ip(sm, sp, s6, spd2, dtdx);
-  const double Ip_u0 = ppm_ip(sm, sp, s6, spd0, dtdx);
+  const FP Im_u0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const FP Im_u2 = ppm_im(sm, sp, s6, spd2, dtdx);
+  const FP Ip_u2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  const FP Ip_u0 = ppm_ip(sm, sp, s6, spd0, dtdx);
   recon(QPRES);
-  const double Im_p0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const double Im_p1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const double Im_p2 = ppm_im(sm, sp, s6, spd2, dtdx);
-  const double Ip_p0 = ppm_ip(sm, sp, s6, spd0, dtdx);
-  const double Ip_p1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  const double Ip_p2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  const FP Im_p0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const FP Im_p1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const FP Im_p2 = ppm_im(sm, sp, s6, spd2, dtdx);
+  const FP Ip_p0 = ppm_ip(sm, sp, s6, spd0, dtdx);
+  const FP Ip_p1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  const FP Ip_p2 = ppm_ip(sm, sp, s6, spd2, dtdx);
   recon(QREINT);
-  const double Im_e0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const double Im_e1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const double Ip_e2 = ppm_ip(sm, sp, s6, spd2, dtdx);
-  const double Ip_e1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  const FP Im_e0 = ppm_im(sm, sp, s6, spd0, dtdx);
+  const FP Im_e1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const FP Ip_e2 = ppm_ip(sm, sp, s6, spd2, dtdx);
+  const FP Ip_e1 = ppm_ip(sm, sp, s6, spd1, dtdx);
   recon(QUT);
-  const double Im_t1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const double Ip_t1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  const FP Im_t1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const FP Ip_t1 = ppm_ip(sm, sp, s6, spd1, dtdx);
   recon(QUTT);
-  const double Im_tt1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const double Ip_tt1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  FaceState<NS> P, M;
-  double YrefP[NS], YrefM[NS];
+  const FP Im_tt1 = ppm_im(sm, sp, s6, spd1, dtdx);
+  const FP Ip_tt1 = ppm_ip(sm, sp, s6, spd1, dtdx);
+  FaceState<NS, FP> P, M;
+  FP YrefP[NS], YrefM[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     recon(QFS + s);
@@ -505,116 +518,116 @@ PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
     YrefM[s] = ppm_ip(sm, sp, s6, spd2, dtdx);
     M.Y[s] = ppm_ip(sm, sp, s6, spd1, dtdx);
   }
-  double vel[3];
+  FP vel[3];
   // ---- plus state on face i (reference state: fastest wave to the left)
   {
-    double rho_ref = Im_r0;
-    const double un_ref = Im_u0;
-    double p_ref = Im_p0;
-    const double rhoe_g_ref = Im_e0;
-    rho_ref = mx(rho_ref, DMIN);
-    const double rho_ref_inv = 1.0 / rho_ref;
-    p_ref = mx(p_ref, DMIN);
-    const double cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefP);
-    const double csq_ref = cc_ref * cc_ref;
-    const double cc_ref_inv = 1.0 / cc_ref;
+    FP rho_ref = Im_r0;
+    const FP un_ref = Im_u0;
+    FP p_ref = Im_p0;
+    const FP rhoe_g_ref = Im_e0;
+    rho_ref = mx(rho_ref, rmin<FP>());
+    const FP rho_ref_inv = FP(1.0) / rho_ref;
+    p_ref = mx(p_ref, rmin<FP>());
+    const FP cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefP);
+    const FP csq_ref = cc_ref * cc_ref;
+    const FP cc_ref_inv = FP(1.0) / cc_ref;
     (void)rhoe_g_ref;
-    const double dum = un_ref - Im_u0;
-    const double dptotm = p_ref - Im_p0;
-    const double drho = rho_ref - Im_r1;
-    const double dptot = p_ref - Im_p1;
-    const double dup = un_ref - Im_u2;
-    const double dptotp = p_ref - Im_p2;
-    double alpham =
-        0.5 * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
-    double alphap =
-        0.5 * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
-    double alpha0r = drho - dptot / csq_ref;
-    alpham = un - cc > 0.0 ? 0.0 : -alpham;
-    alphap = un + cc > 0.0 ? 0.0 : -alphap;
-    alpha0r = un > 0.0 ? 0.0 : -alpha0r;
-    P.rho = mx(DMIN, rho_ref + alphap + alpham + alpha0r);
+    const FP dum = un_ref - Im_u0;
+    const FP dptotm = p_ref - Im_p0;
+    const FP drho = rho_ref - Im_r1;
+    const FP dptot = p_ref - Im_p1;
+    const FP dup = un_ref - Im_u2;
+    const FP dptotp = p_ref - Im_p2;
+    FP alpham =
+        FP(0.5) * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+    FP alphap =
+        FP(0.5) * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+    FP alpha0r = drho - dptot / csq_ref;
+    alpham = un - cc > FP(0.0) ? FP(0.0) : -alpham;
+    alphap = un + cc > FP(0.0) ? FP(0.0) : -alphap;
+    alpha0r = un > FP(0.0) ? FP(0.0) : -alpha0r;
+    P.rho = mx(rmin<FP>(), rho_ref + alphap + alpham + alpha0r);
     vel[D] = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
-    P.p = mx(DMIN, p_ref + (alphap + alpham) * csq_ref);
+    P.p = mx(rmin<FP>(), p_ref + (alphap + alpham) * csq_ref);
     vel[QUT - QU] = Im_t1;
     vel[QUTT - QU] = Im_tt1;
     P.u = vel[0];
     P.v = vel[1];
     P.w = vel[2];
-    const double eint = EOS::RYP2E(P.rho, P.Y, P.p);
+    const FP eint = EOS::RYP2E(P.rho, P.Y, P.p);
     P.rhoe = P.rho * eint;
   }
   // ---- minus state on face i+1 (fastest wave to the right)
   {
-    double rho_ref = Ip_r2;
-    const double un_ref = Ip_u2;
-    double p_ref = Ip_p2;
-    rho_ref = mx(rho_ref, DMIN);
-    const double rho_ref_inv = 1.0 / rho_ref;
-    p_ref = mx(p_ref, DMIN);
-    const double cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefM);
-    const double csq_ref = cc_ref * cc_ref;
-    const double cc_ref_inv = 1.0 / cc_ref;
-    const double dum = un_ref - Ip_u0;
-    const double dptotm = p_ref - Ip_p0;
-    const double drho = rho_ref - Ip_r1;
-    const double dptot = p_ref - Ip_p1;
-    const double dup = un_ref - Ip_u2;
-    const double dptotp = p_ref - Ip_p2;
-    double alpham =
-        0.5 * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
-    double alphap =
-        0.5 * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
-    double alpha0r = drho - dptot / csq_ref;
-    alpham = un - cc > 0.0 ? -alpham : 0.0;
-    alphap = un + cc > 0.0 ? -alphap : 0.0;
-    alpha0r = un > 0.0 ? -alpha0r : 0.0;
-    M.rho = mx(DMIN, rho_ref + alphap + alpham + alpha0r);
+    FP rho_ref = Ip_r2;
+    const FP un_ref = Ip_u2;
+    FP p_ref = Ip_p2;
+    rho_ref = mx(rho_ref, rmin<FP>());
+    const FP rho_ref_inv = FP(1.0) / rho_ref;
+    p_ref = mx(p_ref, rmin<FP>());
+    const FP cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefM);
+    const FP csq_ref = cc_ref * cc_ref;
+    const FP cc_ref_inv = FP(1.0) / cc_ref;
+    const FP dum = un_ref - Ip_u0;
+    const FP dptotm = p_ref - Ip_p0;
+    const FP drho = rho_ref - Ip_r1;
+    const FP dptot = p_ref - Ip_p1;
+    const FP dup = un_ref - Ip_u2;
+    const FP dptotp = p_ref - Ip_p2;
+    FP alpham =
+        FP(0.5) * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+    FP alphap =
+        FP(0.5) * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+    FP alpha0r = drho - dptot / csq_ref;
+    alpham = un - cc > FP(0.0) ? -alpham : FP(0.0);
+    alphap = un + cc > FP(0.0) ? -alphap : FP(0.0);
+    alpha0r = un > FP(0.0) ? -alpha0r : FP(0.0);
+    M.rho = mx(rmin<FP>(), rho_ref + alphap + alpham + alpha0r);
     vel[D] = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
-    M.p = mx(DMIN, p_ref + (alphap + alpham) * csq_ref);
+    M.p = mx(rmin<FP>(), p_ref + (alphap + alpham) * csq_ref);
     vel[QUT - QU] = Ip_t1;
     vel[QUTT - QU] = Ip_tt1;
     M.u = vel[0];
     M.v = vel[1];
     M.w = vel[2];
-    const double eint = EOS::RYP2E(M.rho, M.Y, M.p);
+    const FP eint = EOS::RYP2E(M.rho, M.Y, M.p);
     M.rhoe = M.rho * eint;
   }
   (void)Ip_e2;
   (void)Ip_e1;
   (void)Im_e1;
-  const V& tr = J.tr[D];
-  store_trace<NS>(tr, i, j, k, 1, P);
+  const auto& tr = J.tr[D];
+  store_trace<NS, FP>(tr, i, j, k, 1, P);
   int i1 = i, j1 = j, k1 = k;
   step<D>(i1, j1, k1, 1);
-  store_trace<NS>(tr, i1, j1, k1, 0, M);
+  store_trace<NS, FP>(tr, i1, j1, k1, 0, M);
 }
-template <class EOS, int D, bool PERSIST = false>
+template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_trace_ppm(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_trace_ppm<EOS, D>(sc, ka, lb_);
+      b_trace_ppm<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_trace_ppm<EOS, D>(sc, ka, first_block());
+    b_trace_ppm<EOS, D, FP>(sc, ka, first_block());
   }
 }
 
 // --------------------------------------------------- K3' trace (PLM) ----
 // Source/Godunov.cpp:139-180 (one direction per launch)
-template <class EOS, int D>
+template <class EOS, int D, class FP>
 PCD void b_trace_plm(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const V& q = J.q;
-  const double dtdx = get_dt(ka) / ka.dx[D];
-  const double flat = ka.use_flattening ? flat_cell(J, i, j, k) : 1.0;
+  const auto& q = J.q;
+  const FP dtdx = get_dt(ka) / ka.dx[D];
+  const FP flat = ka.use_flattening ? flat_cell(J, i, j, k) : FP(1.0);
   auto slope = [&](int comp) {
-    double s[5];
+    FP s[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
       int ii = i, jj = j, kk = k;
@@ -623,7 +636,7 @@ PCD void b_trace_plm(const Sched& sc, const KArgs& ka, int lb_) {
     }
     return plm_slope(s, flat, ka.plm_iorder);
   };
-  FaceState<NS> c, sl, P, M;
+  FaceState<NS, FP> c, sl, P, M;
   c.rho = q(i, j, k, QRHO);
   c.u = q(i, j, k, QU);
   c.v = q(i, j, k, QV);
@@ -642,57 +655,57 @@ PCD void b_trace_plm(const Sched& sc, const KArgs& ka, int lb_) {
     sl.Y[s] = slope(QFS + s);
   }
   plm_trace<NS>(D, c, sl, J.qa(i, j, k, QC), dtdx, P, M);
-  const V& tr = J.tr[D];
-  store_trace<NS>(tr, i, j, k, 1, P);
+  const auto& tr = J.tr[D];
+  store_trace<NS, FP>(tr, i, j, k, 1, P);
   int i1 = i, j1 = j, k1 = k;
   step<D>(i1, j1, k1, 1);
-  store_trace<NS>(tr, i1, j1, k1, 0, M);
+  store_trace<NS, FP>(tr, i1, j1, k1, 0, M);
 }
-template <class EOS, int D, bool PERSIST = false>
+template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_trace_plm(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_trace_plm<EOS, D>(sc, ka, lb_);
+      b_trace_plm<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_trace_plm<EOS, D>(sc, ka, first_block());
+    b_trace_plm<EOS, D, FP>(sc, ka, first_block());
   }
 }
 
 // -------------------------------------------------- K6-8 first fluxes ----
-template <class EOS, int D>
+template <class EOS, int D, class FP>
 PCD void b_flux1(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_FLUX1_X + D, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  FaceState<NS> L, R;
-  load_trace<NS>(J.tr[D], i, j, k, 0, L);
-  load_trace<NS>(J.tr[D], i, j, k, 1, R);
+  FaceState<NS, FP> L, R;
+  load_trace<NS, FP>(J.tr[D], i, j, k, 0, L);
+  load_trace<NS, FP>(J.tr[D], i, j, k, 1, R);
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
-  const double cav = 0.5 * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  const FP cav = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-  Flux<NS> F;
+  Flux<NS, FP> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
-  store_flux<NS>(J.f1[D], i, j, k, F);
+  store_flux<NS, FP>(J.f1[D], i, j, k, F);
 }
-template <class EOS, int D, bool PERSIST = false>
+template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_flux1(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_flux1<EOS, D>(sc, ka, lb_);
+      b_flux1<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_flux1<EOS, D>(sc, ka, first_block());
+    b_flux1<EOS, D, FP>(sc, ka, first_block());
   }
 }
 
 // flux differences across cell c in direction T from a compact flux array
-template <int NS, int T>
-PCD void dflux(const V& f, int i, int j, int k, double c, DFlux<NS>& D) {
+template <int NS, int T, class FP>
+PCD void dflux(const VT<FP>& f, int i, int j, int k, FP c, DFlux<NS, FP>& D) {
   int ip = i, jp = j, kp = k;
   step<T>(ip, jp, kp, 1);
 #pragma unroll
@@ -709,58 +722,58 @@ PCD void dflux(const V& f, int i, int j, int k, double c, DFlux<NS>& D) {
 // ------------------------------------- K9-16 transverse (d|t) fluxes ----
 // pc_transdo on the two cells of a d-face, then pc_cmpflx
 // (Source/Godunov.cpp:246-384).
-template <class EOS, int D, int T>
+template <class EOS, int D, int T, class FP>
 PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRANS_XY + 2 * D + tslot(D, T), J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const double cdtdx = 1.0 / 3.0 * get_dt(ka) / ka.dx[T];
+  const FP cdtdx = FP(1.0) / FP(3.0) * get_dt(ka) / ka.dx[T];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
-  FaceState<NS> s, L, R;
-  DFlux<NS> df;
+  FaceState<NS, FP> s, L, R;
+  DFlux<NS, FP> df;
   // minus side: cell f - e_D, state on its high face
-  load_trace<NS>(J.tr[D], i, j, k, 0, s);
-  dflux<NS, T>(J.f1[T], im, jm, km, cdtdx, df);
+  load_trace<NS, FP>(J.tr[D], i, j, k, 0, s);
+  dflux<NS, T, FP>(J.f1[T], im, jm, km, cdtdx, df);
   transdo_side<NS>(s, df, J.qa(im, jm, km, QGAMC), cdtdx, L);
   // plus side: cell f, state on its low face
-  load_trace<NS>(J.tr[D], i, j, k, 1, s);
-  dflux<NS, T>(J.f1[T], i, j, k, cdtdx, df);
+  load_trace<NS, FP>(J.tr[D], i, j, k, 1, s);
+  dflux<NS, T, FP>(J.f1[T], i, j, k, cdtdx, df);
   transdo_side<NS>(s, df, J.qa(i, j, k, QGAMC), cdtdx, R);
-  const double cav = 0.5 * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  const FP cav = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-  Flux<NS> F;
+  Flux<NS, FP> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
-  store_flux<NS>(J.ft[2 * D + tslot(D, T)], i, j, k, F);
+  store_flux<NS, FP>(J.ft[2 * D + tslot(D, T)], i, j, k, F);
 }
-template <class EOS, int D, int T, bool PERSIST = false>
+template <class EOS, int D, int T, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_trans(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_trans<EOS, D, T>(sc, ka, lb_);
+      b_trans<EOS, D, T, FP>(sc, ka, lb_);
   } else {
-    b_trans<EOS, D, T>(sc, ka, first_block());
+    b_trans<EOS, D, T, FP>(sc, ka, first_block());
   }
 }
 
 // transdd flux differences across cell (i,j,k) for final direction D
-template <int NS, int D, bool SRC>
-PCD void final_corr(const Job& J, int i, int j, int k, double hdt,
-                    double c0, double c1, const FaceState<NS>& s,
-                    FaceState<NS>& o) {
+template <int NS, int D, bool SRC, class FP>
+PCD void final_corr(const JobT<FP>& J, int i, int j, int k, FP hdt,
+                    FP c0, FP c1, const FaceState<NS, FP>& s,
+                    FaceState<NS, FP>& o) {
   constexpr int T0 = (D == 0) ? 1 : 0;
   constexpr int T1 = (D == 2) ? 1 : 2;
-  const V& fx = J.ft[2 * T0 + tslot(T0, T1)];  // F_{T0|T1} on T0-faces
-  const V& fy = J.ft[2 * T1 + tslot(T1, T0)];  // F_{T1|T0} on T1-faces
+  const auto& fx = J.ft[2 * T0 + tslot(T0, T1)];  // F_{T0|T1} on T0-faces
+  const auto& fy = J.ft[2 * T1 + tslot(T1, T0)];  // F_{T1|T0} on T1-faces
   int i0 = i, j0 = j, k0 = k;
   step<T0>(i0, j0, k0, 1);
   int i1 = i, j1 = j, k1 = k;
   step<T1>(i1, j1, k1, 1);
-  double dd[5], dds[NS];
+  FP dd[5], dds[NS];
 #pragma unroll
   for (int m = 0; m < 5; ++m)
     dd[m] = c0 * (fx(i0, j0, k0, m) - fx(i, j, k, m)) +
@@ -769,20 +782,20 @@ PCD void final_corr(const Job& J, int i, int j, int k, double hdt,
   for (int s2 = 0; s2 < NS; ++s2)
     dds[s2] = c0 * (fx(i0, j0, k0, 7 + s2) - fx(i, j, k, 7 + s2)) +
               c1 * (fy(i1, j1, k1, 7 + s2) - fy(i, j, k, 7 + s2));
-  const double gamc = J.qa(i, j, k, QGAMC);
-  const double pggpx = fx(i0, j0, k0, 5), pggmx = fx(i, j, k, 5);
-  const double ugpx = fx(i0, j0, k0, 6), ugmx = fx(i, j, k, 6);
-  const double dAupx = pggpx * ugpx - pggmx * ugmx;
-  const double pavx = 0.5 * (pggpx + pggmx);
-  const double dAux = ugpx - ugmx;
-  const double pggpy = fy(i1, j1, k1, 5), pggmy = fy(i, j, k, 5);
-  const double ugpy = fy(i1, j1, k1, 6), ugmy = fy(i, j, k, 6);
-  const double dAupy = pggpy * ugpy - pggmy * ugmy;
-  const double pavy = 0.5 * (pggpy + pggmy);
-  const double dAuy = ugpy - ugmy;
-  const double pxnew = c0 * (dAupx + pavx * dAux * (gamc - 1.0));
-  const double pynew = c1 * (dAupy + pavy * dAuy * (gamc - 1.0));
-  double src[6 + NS];
+  const FP gamc = J.qa(i, j, k, QGAMC);
+  const FP pggpx = fx(i0, j0, k0, 5), pggmx = fx(i, j, k, 5);
+  const FP ugpx = fx(i0, j0, k0, 6), ugmx = fx(i, j, k, 6);
+  const FP dAupx = pggpx * ugpx - pggmx * ugmx;
+  const FP pavx = FP(0.5) * (pggpx + pggmx);
+  const FP dAux = ugpx - ugmx;
+  const FP pggpy = fy(i1, j1, k1, 5), pggmy = fy(i, j, k, 5);
+  const FP ugpy = fy(i1, j1, k1, 6), ugmy = fy(i, j, k, 6);
+  const FP dAupy = pggpy * ugpy - pggmy * ugmy;
+  const FP pavy = FP(0.5) * (pggpy + pggmy);
+  const FP dAuy = ugpy - ugmy;
+  const FP pxnew = c0 * (dAupx + pavx * dAux * (gamc - FP(1.0)));
+  const FP pynew = c1 * (dAupy + pavy * dAuy * (gamc - FP(1.0)));
+  FP src[6 + NS];
   if (SRC) {
     src[0] = J.srcq(i, j, k, QRHO);
     src[1] = J.srcq(i, j, k, QU);
@@ -794,20 +807,20 @@ PCD void final_corr(const Job& J, int i, int j, int k, double hdt,
     for (int s2 = 0; s2 < NS; ++s2) src[6 + s2] = J.srcq(i, j, k, QFS + s2);
   } else {
 #pragma unroll
-    for (int m = 0; m < 6 + NS; ++m) src[m] = 0.0;
+    for (int m = 0; m < 6 + NS; ++m) src[m] = FP(0.0);
   }
   transdd_side<NS>(s, dd, dds, pxnew, pynew, src, hdt, o);
 }
 
 // PLM state of cell (i,j,k) in direction D (pc_low_order_boundary)
-template <class EOS, int D>
-PCD void plm_cell(const Job& J, const KArgs& ka, double dtdx, int i, int j,
-                  int k, FaceState<EOS::NS>& P, FaceState<EOS::NS>& M) {
+template <class EOS, int D, class FP>
+PCD void plm_cell(const JobT<FP>& J, const KArgs& ka, FP dtdx, int i, int j,
+                  int k, FaceState<EOS::NS, FP>& P, FaceState<EOS::NS, FP>& M) {
   constexpr int NS = EOS::NS;
-  const V& q = J.q;
-  const double flat = ka.use_flattening ? flat_cell(J, i, j, k) : 1.0;
+  const auto& q = J.q;
+  const FP flat = ka.use_flattening ? flat_cell(J, i, j, k) : FP(1.0);
   auto slope = [&](int comp) {
-    double s[5];
+    FP s[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
       int ii = i, jj = j, kk = k;
@@ -816,7 +829,7 @@ PCD void plm_cell(const Job& J, const KArgs& ka, double dtdx, int i, int j,
     }
     return plm_slope(s, flat, ka.plm_iorder);
   };
-  FaceState<NS> c, sl;
+  FaceState<NS, FP> c, sl;
   c.rho = q(i, j, k, QRHO);
   c.u = q(i, j, k, QU);
   c.v = q(i, j, k, QV);
@@ -840,15 +853,16 @@ PCD void plm_cell(const Job& J, const KArgs& ka, double dtdx, int i, int j,
 // pc_transdd + pc_cmpflx (Source/Godunov.cpp:392-438), then pc_artif_visc,
 // pc_norm_spec_flx and pc_ext_flx (Source/Hydro.cpp:380-413) on the face.
 // Faces of Hard/UserBC domain boundaries are left to k_final_loworder.
-PCD bool loworder_face(const Job& J, int D, int idx) {
+template <class FP>
+PCD bool loworder_face(const JobT<FP>& J, int D, int idx) {
   return (idx == J.dlo[D] && (J.bclo[D] == BC_INFLOW || J.bclo[D] == BC_USER)) ||
          (idx == J.dhi[D] + 1 &&
           (J.bchi[D] == BC_INFLOW || J.bchi[D] == BC_USER));
 }
 
-template <class EOS, int D>
-PCD void final_tail(const Job& J, const KArgs& ka, int i, int j, int k,
-                    const FaceState<EOS::NS>& L, const FaceState<EOS::NS>& R) {
+template <class EOS, int D, class FP>
+PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
+                    const FaceState<EOS::NS, FP>& L, const FaceState<EOS::NS, FP>& R) {
   constexpr int NS = EOS::NS;
   constexpr int NVAR = 7 + NS;
   constexpr int T0 = (D == 0) ? 1 : 0;
@@ -856,8 +870,8 @@ PCD void final_tail(const Job& J, const KArgs& ka, int i, int j, int k,
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-  const double cav = 0.5 * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
-  Flux<NS> F;
+  const FP cav = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  Flux<NS, FP> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
   // lab-frame NVAR layout (UTEMP flux starts at zero)
   double fl[NVAR];
@@ -872,7 +886,7 @@ PCD void final_tail(const Job& J, const KArgs& ka, int i, int j, int k,
   for (int s = 0; s < NS; ++s) fl[UFS + s] = F.f[6 + s];
   // pc_artif_visc (Source/Godunov.H:1067-1138)
   {
-    const V& dv = J.divu;
+    const auto& dv = J.divu;
     int a0 = i, b0 = j, e0 = k;
     step<T0>(a0, b0, e0, 1);
     int a1 = i, b1 = j, e1 = k;
@@ -905,78 +919,78 @@ PCD void final_tail(const Job& J, const KArgs& ka, int i, int j, int k,
   }
   // pc_ext_flx (Source/Hydro.H:854-865)
   const double area = J.area[D].p ? J.area[D](i, j, k, 0) : ka.area_u[D];
-  const V& fo = J.flx[D];
+  const auto& fo = J.flx[D];
 #pragma unroll
   for (int m = 0; m < NVAR; ++m) fo(i, j, k, m) = fl[m] * area;
   J.qec[D](i, j, k, 0) = F.pgd;
   J.qec[D](i, j, k, 1) = F.ugd;
 }
 
-template <class EOS, int D, bool SRC>
+template <class EOS, int D, bool SRC, class FP>
 PCD void b_final(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   constexpr int T0 = (D == 0) ? 1 : 0;
   constexpr int T1 = (D == 2) ? 1 : 2;
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   if (loworder_face(J, D, idx)) return;
-  const double dt = get_dt(ka);
-  const double hdt = 0.5 * dt;
-  const double c0 = 0.5 * dt / ka.dx[T0];
-  const double c1 = 0.5 * dt / ka.dx[T1];
+  const FP dt = get_dt(ka);
+  const FP hdt = FP(0.5) * dt;
+  const FP c0 = FP(0.5) * dt / ka.dx[T0];
+  const FP c1 = FP(0.5) * dt / ka.dx[T1];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
-  FaceState<NS> L, R, s;
-  load_trace<NS>(J.tr[D], i, j, k, 0, s);
-  final_corr<NS, D, SRC>(J, im, jm, km, hdt, c0, c1, s, L);
-  load_trace<NS>(J.tr[D], i, j, k, 1, s);
-  final_corr<NS, D, SRC>(J, i, j, k, hdt, c0, c1, s, R);
-  final_tail<EOS, D>(J, ka, i, j, k, L, R);
+  FaceState<NS, FP> L, R, s;
+  load_trace<NS, FP>(J.tr[D], i, j, k, 0, s);
+  final_corr<NS, D, SRC, FP>(J, im, jm, km, hdt, c0, c1, s, L);
+  load_trace<NS, FP>(J.tr[D], i, j, k, 1, s);
+  final_corr<NS, D, SRC, FP>(J, i, j, k, hdt, c0, c1, s, R);
+  final_tail<EOS, D, FP>(J, ka, i, j, k, L, R);
 }
-template <class EOS, int D, bool SRC, bool PERSIST = false>
+template <class EOS, int D, bool SRC, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_final(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_final<EOS, D, SRC>(sc, ka, lb_);
+      b_final<EOS, D, SRC, FP>(sc, ka, lb_);
   } else {
-    b_final<EOS, D, SRC>(sc, ka, first_block());
+    b_final<EOS, D, SRC, FP>(sc, ka, first_block());
   }
 }
 
 // pc_low_order_boundary (Source/Godunov.cpp:7-55,440-468): at Hard/UserBC
 // domain faces the flux comes from first-order-in-time PLM states without
 // transverse terms.  Launched only when such a boundary exists.
-template <class EOS, int D>
+template <class EOS, int D, class FP>
 PCD void b_final_loworder(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
-  const Job J = load_job(sc, lb_, local);
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_FINAL_X + D, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   if (!loworder_face(J, D, idx)) return;
-  const double dtdx = get_dt(ka) / ka.dx[D];
+  const FP dtdx = get_dt(ka) / ka.dx[D];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
-  FaceState<NS> L, R, unused;
-  plm_cell<EOS, D>(J, ka, dtdx, im, jm, km, unused, L);
-  plm_cell<EOS, D>(J, ka, dtdx, i, j, k, R, unused);
-  final_tail<EOS, D>(J, ka, i, j, k, L, R);
+  FaceState<NS, FP> L, R, unused;
+  plm_cell<EOS, D, FP>(J, ka, dtdx, im, jm, km, unused, L);
+  plm_cell<EOS, D, FP>(J, ka, dtdx, i, j, k, R, unused);
+  final_tail<EOS, D, FP>(J, ka, i, j, k, L, R);
 }
-template <class EOS, int D, bool PERSIST = false>
+template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_final_loworder(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_final_loworder<EOS, D>(sc, ka, lb_);
+      b_final_loworder<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_final_loworder<EOS, D>(sc, ka, first_block());
+    b_final_loworder<EOS, D, FP>(sc, ka, first_block());
   }
 }
 

@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <limits>
+#include <type_traits>
 #include <utility>
 
 #define PCD __device__ __forceinline__
@@ -46,6 +47,8 @@ constexpr double RU = 8.31446261815324e7;  // erg/(mol K)
 // amrex::min / amrex::max (std::min / std::max semantics)
 PCD double mn(double a, double b) { return (b < a) ? b : a; }
 PCD double mx(double a, double b) { return (a < b) ? b : a; }
+PCD float mn(float a, float b) { return (b < a) ? b : a; }
+PCD float mx(float a, float b) { return (a < b) ? b : a; }
 
 // ------------------------------------------------------------------- EOS --
 // Calorically perfect gas, gamma = 1.4, one species of molecular weight
@@ -55,16 +58,20 @@ struct EosGamma {
   static constexpr double gamma = 1.4;
   static constexpr double imw0 = 1.0 / 28.0134;
 
-  PCD static double wbar(const double* Y) {
-    double s = 0.0;
-    s += Y[0] * imw0;
-    return 1.0 / s;
+  template <class FP>
+  PCD static FP wbar(const FP* Y) {
+    FP s = FP(0.0);
+    s += Y[0] * FP(imw0);
+    return FP(1.0) / s;
   }
-  PCD static double RYP2E(double R, const double* /*Y*/, double P) {
-    return P / (R * (gamma - 1.0));
+  // e(rho, p) and c(rho, p) in the precision of their arguments
+  template <class FP>
+  PCD static FP RYP2E(FP R, const FP* /*Y*/, FP P) {
+    return P / (R * (FP(gamma) - FP(1.0)));
   }
-  PCD static double RPY2Cs(double R, double P, const double* /*Y*/) {
-    return sqrt(gamma * P / R);
+  template <class FP>
+  PCD static FP RPY2Cs(FP R, FP P, const FP* /*Y*/) {
+    return sqrt(FP(gamma) * P / R);
   }
   PCD static double EY2T(double E, const double* Y, double /*Tguess*/) {
     const double w = wbar(Y);
@@ -174,13 +181,17 @@ struct Nasa {
   static constexpr double u0 = a[5] * s, u1 = v0, u2 = a[1] / 2.0 * s,
                           u3 = a[2] / 3.0 * s, u4 = a[3] / 4.0 * s,
                           u5 = a[4] / 5.0 * s;
-  PCD static double cv(double T) {
-    if constexpr (flat) return v0;
-    return fma(T, fma(T, fma(T, fma(T, v4, v3), v2), v1), v0);
+  template <class FP>
+  PCD static FP cv(FP T) {
+    if constexpr (flat) return FP(v0);
+    return fma(T, fma(T, fma(T, fma(T, FP(v4), FP(v3)), FP(v2)), FP(v1)), FP(v0));
   }
-  PCD static double e(double T) {
-    if constexpr (flat) return fma(T, u1, u0);
-    return fma(T, fma(T, fma(T, fma(T, fma(T, u5, u4), u3), u2), u1), u0);
+  template <class FP>
+  PCD static FP e(FP T) {
+    if constexpr (flat) return fma(T, FP(u1), FP(u0));
+    return fma(T, fma(T, fma(T, fma(T, fma(T, FP(u5), FP(u4)), FP(u3)), FP(u2)),
+                      FP(u1)),
+               FP(u0));
   }
 };
 
@@ -200,45 +211,49 @@ struct EosLiDryer {
   static constexpr double TMID = 1000.0;
   using Seq = std::make_integer_sequence<int, 9>;
 
-  // sum_k Y_k e_k(T) (erg/g) and sum_k Y_k cv_k(T), one range
-  template <bool LO, int... K>
-  PCD static double e_seq(double T, const double* Y,
-                          std::integer_sequence<int, K...>) {
-    double E = 0.0;
+  // sum_k Y_k e_k(T) (erg/g) and sum_k Y_k cv_k(T), one range, in the
+  // precision FP of the arguments
+  template <bool LO, class FP, int... K>
+  PCD static FP e_seq(FP T, const FP* Y, std::integer_sequence<int, K...>) {
+    FP E = FP(0.0);
     ((E = fma(Y[K], Nasa<K, LO>::e(T), E)), ...);
     return E;
   }
-  template <bool LO, int... K>
-  PCD static double cv_seq(double T, const double* Y,
-                           std::integer_sequence<int, K...>) {
-    double cv = 0.0;
+  template <bool LO, class FP, int... K>
+  PCD static FP cv_seq(FP T, const FP* Y, std::integer_sequence<int, K...>) {
+    FP cv = FP(0.0);
     ((cv = fma(Y[K], Nasa<K, LO>::cv(T), cv)), ...);
     return cv;
   }
-  PCD static double ubms(double T, const double* Y) {
-    return (T < TMID) ? e_seq<true>(T, Y, Seq{}) : e_seq<false>(T, Y, Seq{});
+  template <class FP>
+  PCD static FP ubms(FP T, const FP* Y) {
+    return (T < FP(TMID)) ? e_seq<true>(T, Y, Seq{}) : e_seq<false>(T, Y, Seq{});
   }
-  PCD static double cvbs(double T, const double* Y) {
-    return (T < TMID) ? cv_seq<true>(T, Y, Seq{}) : cv_seq<false>(T, Y, Seq{});
+  template <class FP>
+  PCD static FP cvbs(FP T, const FP* Y) {
+    return (T < FP(TMID)) ? cv_seq<true>(T, Y, Seq{}) : cv_seq<false>(T, Y, Seq{});
   }
 
-  template <int... K>
-  PCD static double sum_y_imw(const double* Y, std::integer_sequence<int, K...>) {
-    double s = 0.0;
-    ((s = fma(Y[K], Nasa<K, true>::imw, s)), ...);
+  template <class FP, int... K>
+  PCD static FP sum_y_imw(const FP* Y, std::integer_sequence<int, K...>) {
+    FP s = FP(0.0);
+    ((s = fma(Y[K], FP(Nasa<K, true>::imw), s)), ...);
     return s;
   }
-  PCD static double wbar(const double* Y) { return 1.0 / sum_y_imw(Y, Seq{}); }
-  PCD static double RYP2E(double R, const double* Y, double P) {
-    const double w = wbar(Y);
-    const double T = P * w / (R * RU);
+  template <class FP>
+  PCD static FP wbar(const FP* Y) { return FP(1.0) / sum_y_imw(Y, Seq{}); }
+  template <class FP>
+  PCD static FP RYP2E(FP R, const FP* Y, FP P) {
+    const FP w = wbar(Y);
+    const FP T = P * w / (R * FP(RU));
     return ubms(T, Y);
   }
-  PCD static double RPY2Cs(double R, double P, const double* Y) {
-    const double w = wbar(Y);
-    const double T = P * w / (R * RU);
-    const double cv = cvbs(T, Y);
-    const double G = (w * cv + RU) / (w * cv);
+  template <class FP>
+  PCD static FP RPY2Cs(FP R, FP P, const FP* Y) {
+    const FP w = wbar(Y);
+    const FP T = P * w / (R * FP(RU));
+    const FP cv = cvbs(T, Y);
+    const FP G = (w * cv + FP(RU)) / (w * cv);
     return sqrt(G * P / R);
   }
   PCD static double TY2E(double T, const double* Y) { return ubms(T, Y); }
@@ -328,243 +343,259 @@ struct EosLiDryer {
   };
 };
 
+// Floors of the reference in precision FP: numeric_limits<FP>::min(), and
+// very_small_num (epsilon * 1e-100), which underflows in FP32 and becomes
+// FLT_MIN there.
+template <class FP>
+PCD constexpr FP rmin() { return std::numeric_limits<FP>::min(); }
+template <class FP>
+PCD constexpr FP vsmall() {
+  if constexpr (std::is_same<FP, double>::value) return very_small_num;
+  return std::numeric_limits<FP>::min();
+}
+
 // ---------------------------------------------------------------- states --
 // One primitive state on a face: lab-frame velocity, pressure, rho*e and
 // mass fractions.
-template <int NS>
+template <int NS, class FP = double>
 struct FaceState {
-  double rho, u, v, w, p, rhoe;
-  double Y[NS];
+  FP rho, u, v, w, p, rhoe;
+  FP Y[NS];
 };
 
 // ----------------------------------------------------- flattening (A6) --
-// Source/Godunov.H:188-260.  p(o) and un(o) give the pressure and the
+// Source/Godunov.H:188-FP(260.)  p(o) and un(o) give the pressure and the
 // velocity normal to `dir` at offset o in [-3, 3] along dir; `inside` is
 // whether both +-3 neighbours lie in the primitive box.
-template <class PF, class UF>
-PCD double flatten_line(bool inside, PF&& pq, UF&& unq) {
-  if (!inside) return 1.0;
-  const double small_pres = 1.e-200;
-  const double shktst = 0.33e0;
-  const double zcut1 = 0.75e0;
-  const double zcut2 = 0.85e0;
-  const double dzcut = 1.0 / (zcut2 - zcut1);
-  const double qm = pq(-1);
-  const double qp = pq(1);
-  const double qmun = unq(-1);
-  const double qpun = unq(1);
-  const double qmm = pq(-2);
-  const double qpp = pq(2);
-  double dp = qp - qm;
-  const int ishft = (dp > 0.0) ? 1 : -1;
-  double denom = mx(small_pres, fabs(qpp - qmm));
-  double zeta = fabs(dp) / denom;
-  const double z = mn(1.0, mx(0.0, dzcut * (zeta - zcut1)));
-  double tst = (qmun - qpun >= 0.0) ? 1.0 : 0.0;
-  double tmp = mn(qp, qm);
-  const double chi = (fabs(dp) / tmp) > shktst ? tst : 0.0;
-  const double qpshft = pq(1 - ishft);
-  const double qmshft = pq(-(1 + ishft));
-  const double qpunshft = unq(1 - ishft);
-  const double qmunshft = unq(-(1 + ishft));
-  const double qppshft = pq(2 - ishft);
-  const double qmmshft = pq(-(2 + ishft));
+template <class FP = double, class PF, class UF>
+PCD FP flatten_line(bool inside, PF&& pq, UF&& unq) {
+  if (!inside) return FP(1.0);
+  const FP small_pres = FP(1.e-200);
+  const FP shktst = FP(0.33e0);
+  const FP zcut1 = FP(0.75e0);
+  const FP zcut2 = FP(0.85e0);
+  const FP dzcut = FP(1.0) / (zcut2 - zcut1);
+  const FP qm = pq(-1);
+  const FP qp = pq(1);
+  const FP qmun = unq(-1);
+  const FP qpun = unq(1);
+  const FP qmm = pq(-2);
+  const FP qpp = pq(2);
+  FP dp = qp - qm;
+  const int ishft = (dp > FP(0.0)) ? 1 : -1;
+  FP denom = mx(small_pres, fabs(qpp - qmm));
+  FP zeta = fabs(dp) / denom;
+  const FP z = mn(FP(1.0), mx(FP(0.0), dzcut * (zeta - zcut1)));
+  FP tst = (qmun - qpun >= FP(0.0)) ? FP(1.0) : FP(0.0);
+  FP tmp = mn(qp, qm);
+  const FP chi = (fabs(dp) / tmp) > shktst ? tst : FP(0.0);
+  const FP qpshft = pq(1 - ishft);
+  const FP qmshft = pq(-(1 + ishft));
+  const FP qpunshft = unq(1 - ishft);
+  const FP qmunshft = unq(-(1 + ishft));
+  const FP qppshft = pq(2 - ishft);
+  const FP qmmshft = pq(-(2 + ishft));
   dp = qpshft - qmshft;
   denom = mx(small_pres, fabs(qppshft - qmmshft));
   zeta = fabs(dp) / denom;
-  const double z2 = mn(1.0, mx(0.0, dzcut * (zeta - zcut1)));
-  tst = (qmunshft - qpunshft >= 0.0) ? 1.0 : 0.0;
+  const FP z2 = mn(FP(1.0), mx(FP(0.0), dzcut * (zeta - zcut1)));
+  tst = (qmunshft - qpunshft >= FP(0.0)) ? FP(1.0) : FP(0.0);
   tmp = mn(qpshft, qmshft);
-  const double chi2 = (fabs(dp) / tmp) > shktst ? tst : 0.0;
-  return 1.0 - mx(chi2 * z2, chi * z);
+  const FP chi2 = (fabs(dp) / tmp) > shktst ? tst : FP(0.0);
+  return FP(1.0) - mx(chi2 * z2, chi * z);
 }
 
 // --------------------------------------------------------------- PPM (A7) --
 // Source/PPM.H:28-115: CW84 edge values, van Leer slopes, flattening blend,
 // Colella-Sekora parabola limiter.
-PCD void ppm_reconstruct(const double s[5], double flatn, double& sm,
-                         double& sp) {
-  double dsl = 2.0 * (s[1] - s[0]);
-  double dsr = 2.0 * (s[2] - s[1]);
-  double dsvl_l = 0.0;
-  if (dsl * dsr > 0.0) {
-    const double dsc = 0.5 * (s[2] - s[0]);
-    dsvl_l = copysign(1.0, dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
+template <class FP>
+PCD void ppm_reconstruct(const FP s[5], FP flatn, FP& sm,
+                         FP& sp) {
+  FP dsl = FP(2.0) * (s[1] - s[0]);
+  FP dsr = FP(2.0) * (s[2] - s[1]);
+  FP dsvl_l = FP(0.0);
+  if (dsl * dsr > FP(0.0)) {
+    const FP dsc = FP(0.5) * (s[2] - s[0]);
+    dsvl_l = copysign(FP(1.0), dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
   }
-  dsl = 2.0 * (s[2] - s[1]);
-  dsr = 2.0 * (s[3] - s[2]);
-  double dsvl_r = 0.0;
-  if (dsl * dsr > 0.0) {
-    const double dsc = 0.5 * (s[3] - s[1]);
-    dsvl_r = copysign(1.0, dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
+  dsl = FP(2.0) * (s[2] - s[1]);
+  dsr = FP(2.0) * (s[3] - s[2]);
+  FP dsvl_r = FP(0.0);
+  if (dsl * dsr > FP(0.0)) {
+    const FP dsc = FP(0.5) * (s[3] - s[1]);
+    dsvl_r = copysign(FP(1.0), dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
   }
-  sm = 0.5 * (s[2] + s[1]) - (1.0 / 6.0) * (dsvl_r - dsvl_l);
+  sm = FP(0.5) * (s[2] + s[1]) - (FP(1.0) / FP(6.0)) * (dsvl_r - dsvl_l);
   sm = mx(sm, mn(s[2], s[1]));
   sm = mn(sm, mx(s[2], s[1]));
   // the left slope of the right edge equals the right slope of the left edge
   dsvl_l = dsvl_r;
-  dsl = 2.0 * (s[3] - s[2]);
-  dsr = 2.0 * (s[4] - s[3]);
-  dsvl_r = 0.0;
-  if (dsl * dsr > 0.0) {
-    const double dsc = 0.5 * (s[4] - s[2]);
-    dsvl_r = copysign(1.0, dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
+  dsl = FP(2.0) * (s[3] - s[2]);
+  dsr = FP(2.0) * (s[4] - s[3]);
+  dsvl_r = FP(0.0);
+  if (dsl * dsr > FP(0.0)) {
+    const FP dsc = FP(0.5) * (s[4] - s[2]);
+    dsvl_r = copysign(FP(1.0), dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
   }
-  sp = 0.5 * (s[3] + s[2]) - (1.0 / 6.0) * (dsvl_r - dsvl_l);
+  sp = FP(0.5) * (s[3] + s[2]) - (FP(1.0) / FP(6.0)) * (dsvl_r - dsvl_l);
   sp = mx(sp, mn(s[3], s[2]));
   sp = mn(sp, mx(s[3], s[2]));
-  sm = flatn * sm + (1.0 - flatn) * s[2];
-  sp = flatn * sp + (1.0 - flatn) * s[2];
-  if ((sp - s[2]) * (s[2] - sm) <= 0.0) {
+  sm = flatn * sm + (FP(1.0) - flatn) * s[2];
+  sp = flatn * sp + (FP(1.0) - flatn) * s[2];
+  if ((sp - s[2]) * (s[2] - sm) <= FP(0.0)) {
     sp = s[2];
     sm = s[2];
-  } else if (fabs(sp - s[2]) >= 2.0 * fabs(sm - s[2])) {
-    sp = 3.0 * s[2] - 2.0 * sm;
-  } else if (fabs(sm - s[2]) >= 2.0 * fabs(sp - s[2])) {
-    sm = 3.0 * s[2] - 2.0 * sp;
+  } else if (fabs(sp - s[2]) >= FP(2.0) * fabs(sm - s[2])) {
+    sp = FP(3.0) * s[2] - FP(2.0) * sm;
+  } else if (fabs(sm - s[2]) >= FP(2.0) * fabs(sp - s[2])) {
+    sm = FP(3.0) * s[2] - FP(2.0) * sp;
   }
 }
 
 // One wave of Source/PPM.H:131-189: integral of the parabola over the
 // domain of dependence of a wave of speed `speed`, from the left (Im) or the
 // right (Ip) edge of the zone.
-PCD double ppm_im(double sm, double sp, double s6, double speed, double dtdx) {
-  const double sigma = fabs(speed) * dtdx;
-  if (speed <= 0.0)
-    return sm + 0.5 * sigma * (sp - sm + (1.0 - (2.0 / 3.0) * sigma) * s6);
+template <class FP>
+PCD FP ppm_im(FP sm, FP sp, FP s6, FP speed, FP dtdx) {
+  const FP sigma = fabs(speed) * dtdx;
+  if (speed <= FP(0.0))
+    return sm + FP(0.5) * sigma * (sp - sm + (FP(1.0) - (FP(2.0) / FP(3.0)) * sigma) * s6);
   return sm;
 }
-PCD double ppm_ip(double sm, double sp, double s6, double speed, double dtdx) {
-  const double sigma = fabs(speed) * dtdx;
-  if (speed <= 0.0) return sp;
-  return sp - 0.5 * sigma * (sp - sm - (1.0 - (2.0 / 3.0) * sigma) * s6);
+template <class FP>
+PCD FP ppm_ip(FP sm, FP sp, FP s6, FP speed, FP dtdx) {
+  const FP sigma = fabs(speed) * dtdx;
+  if (speed <= FP(0.0)) return sp;
+  return sp - FP(0.5) * sigma * (sp - sm - (FP(1.0) - (FP(2.0) / FP(3.0)) * sigma) * s6);
 }
-PCD double ppm_s6(double sm, double sp, double sc) {
-  return 6.0 * sc - 3.0 * (sm + sp);
+template <class FP>
+PCD FP ppm_s6(FP sm, FP sp, FP sc) {
+  return FP(6.0) * sc - FP(3.0) * (sm + sp);
 }
 
 // ---------------------------------------------------------------- PLM (A8) --
 // Source/PLM.H:21-77 on a 5-point line q[0..4] centred at q[2].
-PCD double plm_slope(const double q[5], double flat, int order) {
-  if (order == 1) return 0.0;
-  double dlft, drgt, dcen, dfm = 0.0, dfp = 0.0, dlim, dsgn, dtemp;
-  const double qc = q[2], qm2 = q[0], qm = q[1], qp = q[3], qp2 = q[4];
+template <class FP>
+PCD FP plm_slope(const FP q[5], FP flat, int order) {
+  if (order == 1) return FP(0.0);
+  FP dlft, drgt, dcen, dfm = FP(0.0), dfp = FP(0.0), dlim, dsgn, dtemp;
+  const FP qc = q[2], qm2 = q[0], qm = q[1], qp = q[3], qp2 = q[4];
   if (order == 4) {
     dlft = qm - qm2;
     drgt = qc - qm;
-    dcen = 0.5 * (dlft + drgt);
-    dsgn = copysign(1.0, dcen);
-    dlim = (dlft * drgt >= 0.0) ? 2.0 * mn(fabs(dlft), fabs(drgt)) : 0.0;
+    dcen = FP(0.5) * (dlft + drgt);
+    dsgn = copysign(FP(1.0), dcen);
+    dlim = (dlft * drgt >= FP(0.0)) ? FP(2.0) * mn(fabs(dlft), fabs(drgt)) : FP(0.0);
     dfm = dsgn * mn(dlim, fabs(dcen));
     dlft = qp - qc;
     drgt = qp2 - qp;
-    dcen = 0.5 * (dlft + drgt);
-    dsgn = copysign(1.0, dcen);
-    dlim = (dlft * drgt >= 0.0) ? 2.0 * mn(fabs(dlft), fabs(drgt)) : 0.0;
+    dcen = FP(0.5) * (dlft + drgt);
+    dsgn = copysign(FP(1.0), dcen);
+    dlim = (dlft * drgt >= FP(0.0)) ? FP(2.0) * mn(fabs(dlft), fabs(drgt)) : FP(0.0);
     dfp = dsgn * mn(dlim, fabs(dcen));
   }
   dlft = qc - qm;
   drgt = qp - qc;
-  dcen = 0.5 * (dlft + drgt);
-  dsgn = copysign(1.0, dcen);
-  dlim = (dlft * drgt >= 0.0) ? 2.0 * mn(fabs(dlft), fabs(drgt)) : 0.0;
-  dtemp = (order == 2) ? dcen : 4.0 / 3.0 * dcen - 1.0 / 6.0 * (dfp + dfm);
+  dcen = FP(0.5) * (dlft + drgt);
+  dsgn = copysign(FP(1.0), dcen);
+  dlim = (dlft * drgt >= FP(0.0)) ? FP(2.0) * mn(fabs(dlft), fabs(drgt)) : FP(0.0);
+  dtemp = (order == 2) ? dcen : FP(4.0) / FP(3.0) * dcen - FP(1.0) / FP(6.0) * (dfp + dfm);
   return flat * dsgn * mn(dlim, fabs(dtemp));
 }
 
-// Source/PLM.H:167-309.  Cell state c (rho, vel[3], p, rhoe, Y), slopes sl
+// Source/PLM.H:167-FP(309.)  Cell state c (rho, vel[3], p, rhoe, Y), slopes sl
 // of the same layout, sound speed cc; writes the right state at the low face
 // (qp) and the left state at the high face (qm) of the cell in direction dir.
-template <int NS>
-PCD void plm_trace(int dir, const FaceState<NS>& c, const FaceState<NS>& sl,
-                   double cc, double dtdx, FaceState<NS>& qp,
-                   FaceState<NS>& qm) {
+template <int NS, class FP>
+PCD void plm_trace(int dir, const FaceState<NS, FP>& c, const FaceState<NS, FP>& sl,
+                   FP cc, FP dtdx, FaceState<NS, FP>& qp,
+                   FaceState<NS, FP>& qm) {
   const int l1 = (dir == 0) ? 1 : 0;
   const int l2 = (dir == 2) ? 1 : 2;
-  const double cs = cc * cc;
-  const double rho = c.rho;
-  const double vel[3] = {c.u, c.v, c.w};
-  const double p = c.p;
-  const double rhoe = c.rhoe;
-  const double enth = ((rhoe + p) / rho) / cs;
-  const double drho = sl.rho;
-  const double dvel[3] = {sl.u, sl.v, sl.w};
-  const double dp = sl.p;
-  const double drhoe = sl.rhoe;
-  const double alpham = 0.5 * (dp / (rho * cc) - dvel[dir]) * rho / cc;
-  const double alphap = 0.5 * (dp / (rho * cc) + dvel[dir]) * rho / cc;
-  const double alpha0r = drho - dp / cs;
-  const double alpha0e = drhoe - dp * enth;
-  const double alpha0v = dvel[l1];
-  const double alpha0w = dvel[l2];
-  const double wv[3] = {vel[dir] - cc, vel[dir], vel[dir] + cc};
-  double out_vel[3];
+  const FP cs = cc * cc;
+  const FP rho = c.rho;
+  const FP vel[3] = {c.u, c.v, c.w};
+  const FP p = c.p;
+  const FP rhoe = c.rhoe;
+  const FP enth = ((rhoe + p) / rho) / cs;
+  const FP drho = sl.rho;
+  const FP dvel[3] = {sl.u, sl.v, sl.w};
+  const FP dp = sl.p;
+  const FP drhoe = sl.rhoe;
+  const FP alpham = FP(0.5) * (dp / (rho * cc) - dvel[dir]) * rho / cc;
+  const FP alphap = FP(0.5) * (dp / (rho * cc) + dvel[dir]) * rho / cc;
+  const FP alpha0r = drho - dp / cs;
+  const FP alpha0e = drhoe - dp * enth;
+  const FP alpha0v = dvel[l1];
+  const FP alpha0w = dvel[l2];
+  const FP wv[3] = {vel[dir] - cc, vel[dir], vel[dir] + cc};
+  FP out_vel[3];
   {
-    const double rho_ref = rho - 0.5 * (1.0 + dtdx * mn(wv[0], 0.0)) * drho;
-    double vel_ref[3];
+    const FP rho_ref = rho - FP(0.5) * (FP(1.0) + dtdx * mn(wv[0], FP(0.0))) * drho;
+    FP vel_ref[3];
 #pragma unroll
     for (int dim = 0; dim < 3; dim++)
-      vel_ref[dim] = vel[dim] - 0.5 * (1.0 + dtdx * mn(wv[0], 0.0)) * dvel[dim];
-    const double p_ref = p - 0.5 * (1.0 + dtdx * mn(wv[0], 0.0)) * dp;
-    const double rhoe_ref = rhoe - 0.5 * (1.0 + dtdx * mn(wv[0], 0.0)) * drhoe;
-    const double apright =
-        0.25 * dtdx * (wv[0] - wv[2]) * (1.0 - copysign(1.0, wv[2])) * alphap;
-    const double amright = 0.0;
-    const double azrright =
-        0.25 * dtdx * (wv[0] - wv[1]) * (1.0 - copysign(1.0, wv[1])) * alpha0r;
-    const double azeright =
-        0.25 * dtdx * (wv[0] - wv[1]) * (1.0 - copysign(1.0, wv[1])) * alpha0e;
-    const double azv1rght =
-        0.25 * dtdx * (wv[0] - wv[1]) * (1.0 - copysign(1.0, wv[1])) * alpha0v;
-    const double azw1rght =
-        0.25 * dtdx * (wv[0] - wv[1]) * (1.0 - copysign(1.0, wv[1])) * alpha0w;
-    qp.rho = mx(DMIN, rho_ref + apright + amright + azrright);
+      vel_ref[dim] = vel[dim] - FP(0.5) * (FP(1.0) + dtdx * mn(wv[0], FP(0.0))) * dvel[dim];
+    const FP p_ref = p - FP(0.5) * (FP(1.0) + dtdx * mn(wv[0], FP(0.0))) * dp;
+    const FP rhoe_ref = rhoe - FP(0.5) * (FP(1.0) + dtdx * mn(wv[0], FP(0.0))) * drhoe;
+    const FP apright =
+        FP(0.25) * dtdx * (wv[0] - wv[2]) * (FP(1.0) - copysign(FP(1.0), wv[2])) * alphap;
+    const FP amright = FP(0.0);
+    const FP azrright =
+        FP(0.25) * dtdx * (wv[0] - wv[1]) * (FP(1.0) - copysign(FP(1.0), wv[1])) * alpha0r;
+    const FP azeright =
+        FP(0.25) * dtdx * (wv[0] - wv[1]) * (FP(1.0) - copysign(FP(1.0), wv[1])) * alpha0e;
+    const FP azv1rght =
+        FP(0.25) * dtdx * (wv[0] - wv[1]) * (FP(1.0) - copysign(FP(1.0), wv[1])) * alpha0v;
+    const FP azw1rght =
+        FP(0.25) * dtdx * (wv[0] - wv[1]) * (FP(1.0) - copysign(FP(1.0), wv[1])) * alpha0w;
+    qp.rho = mx(rmin<FP>(), rho_ref + apright + amright + azrright);
     out_vel[dir] = vel_ref[dir] + (apright - amright) * cc / rho;
     out_vel[l1] = vel_ref[l1] + azv1rght;
     out_vel[l2] = vel_ref[l2] + azw1rght;
     qp.u = out_vel[0];
     qp.v = out_vel[1];
     qp.w = out_vel[2];
-    qp.p = mx(p_ref + (apright + amright) * cs, DMIN);
+    qp.p = mx(p_ref + (apright + amright) * cs, rmin<FP>());
     qp.rhoe = rhoe_ref + (apright + amright) * enth * cs + azeright;
   }
   {
-    const double rho_ref = rho + 0.5 * (1.0 - dtdx * mx(wv[2], 0.0)) * drho;
-    double vel_ref[3];
+    const FP rho_ref = rho + FP(0.5) * (FP(1.0) - dtdx * mx(wv[2], FP(0.0))) * drho;
+    FP vel_ref[3];
 #pragma unroll
     for (int dim = 0; dim < 3; dim++)
-      vel_ref[dim] = vel[dim] + 0.5 * (1.0 - dtdx * mx(wv[2], 0.0)) * dvel[dim];
-    const double p_ref = p + 0.5 * (1.0 - dtdx * mx(wv[2], 0.0)) * dp;
-    const double rhoe_ref = rhoe + 0.5 * (1.0 - dtdx * mx(wv[2], 0.0)) * drhoe;
-    const double apleft = 0.0;
-    const double amleft =
-        0.25 * dtdx * (wv[2] - wv[0]) * (1.0 + copysign(1.0, wv[0])) * alpham;
-    const double azrleft =
-        0.25 * dtdx * (wv[2] - wv[1]) * (1.0 + copysign(1.0, wv[1])) * alpha0r;
-    const double azeleft =
-        0.25 * dtdx * (wv[2] - wv[1]) * (1.0 + copysign(1.0, wv[1])) * alpha0e;
-    const double azv1left =
-        0.25 * dtdx * (wv[2] - wv[1]) * (1.0 + copysign(1.0, wv[1])) * alpha0v;
-    const double azw1left =
-        0.25 * dtdx * (wv[2] - wv[1]) * (1.0 + copysign(1.0, wv[1])) * alpha0w;
-    qm.rho = mx(rho_ref + apleft + amleft + azrleft, DMIN);
+      vel_ref[dim] = vel[dim] + FP(0.5) * (FP(1.0) - dtdx * mx(wv[2], FP(0.0))) * dvel[dim];
+    const FP p_ref = p + FP(0.5) * (FP(1.0) - dtdx * mx(wv[2], FP(0.0))) * dp;
+    const FP rhoe_ref = rhoe + FP(0.5) * (FP(1.0) - dtdx * mx(wv[2], FP(0.0))) * drhoe;
+    const FP apleft = FP(0.0);
+    const FP amleft =
+        FP(0.25) * dtdx * (wv[2] - wv[0]) * (FP(1.0) + copysign(FP(1.0), wv[0])) * alpham;
+    const FP azrleft =
+        FP(0.25) * dtdx * (wv[2] - wv[1]) * (FP(1.0) + copysign(FP(1.0), wv[1])) * alpha0r;
+    const FP azeleft =
+        FP(0.25) * dtdx * (wv[2] - wv[1]) * (FP(1.0) + copysign(FP(1.0), wv[1])) * alpha0e;
+    const FP azv1left =
+        FP(0.25) * dtdx * (wv[2] - wv[1]) * (FP(1.0) + copysign(FP(1.0), wv[1])) * alpha0v;
+    const FP azw1left =
+        FP(0.25) * dtdx * (wv[2] - wv[1]) * (FP(1.0) + copysign(FP(1.0), wv[1])) * alpha0w;
+    qm.rho = mx(rho_ref + apleft + amleft + azrleft, rmin<FP>());
     out_vel[dir] = vel_ref[dir] + (apleft - amleft) * cc / rho;
     out_vel[l1] = vel_ref[l1] + azv1left;
     out_vel[l2] = vel_ref[l2] + azw1left;
     qm.u = out_vel[0];
     qm.v = out_vel[1];
     qm.w = out_vel[2];
-    qm.p = mx(p_ref + (apleft + amleft) * cs, DMIN);
+    qm.p = mx(p_ref + (apleft + amleft) * cs, rmin<FP>());
     qm.rhoe = rhoe_ref + (apleft + amleft) * enth * cs + azeleft;
   }
-  const double vel_ad = vel[dir];
-  const double spzerom = (vel_ad > 0) ? -1.0 : vel_ad * dtdx;
-  const double spzerop = (vel_ad >= 0) ? vel_ad * dtdx : 1.0;
+  const FP vel_ad = vel[dir];
+  const FP spzerom = (vel_ad > 0) ? -FP(1.0) : vel_ad * dtdx;
+  const FP spzerop = (vel_ad >= 0) ? vel_ad * dtdx : FP(1.0);
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    const double acmprght = 0.5 * (-1.0 - spzerom) * sl.Y[s];
+    const FP acmprght = FP(0.5) * (-FP(1.0) - spzerom) * sl.Y[s];
     qp.Y[s] = c.Y[s] + acmprght;
-    const double acmpleft = 0.5 * (1.0 - spzerop) * sl.Y[s];
+    const FP acmpleft = FP(0.5) * (FP(1.0) - spzerop) * sl.Y[s];
     qm.Y[s] = c.Y[s] + acmpleft;
   }
 }
@@ -574,84 +605,84 @@ PCD void plm_trace(int dir, const FaceState<NS>& c, const FaceState<NS>& sl,
 // (rho, un, ut, utt, p, Y).  Outputs: ustar, the fluxes of rho, normal and
 // transverse momenta, rho E, rho e, rho Y_k (optional), and the Godunov
 // state (u_gd, ut_gd, utt_gd, p_gd, game_gd).
-template <class EOS>
+template <class EOS, class FP = double>
 struct RiemannOut {
-  double ustar, f_rho, f_un, f_ut, f_utt, f_eden, f_eint;
-  double u_gd, ut_gd, utt_gd, p_gd, game_gd;
+  FP ustar, f_rho, f_un, f_ut, f_utt, f_eden, f_eint;
+  FP u_gd, ut_gd, utt_gd, p_gd, game_gd;
 };
 
-template <class EOS>
-PCD RiemannOut<EOS> riemann(double rl, double ul, double vl, double v2l,
-                            double pl, const double* spl, double rr, double ur,
-                            double vr, double v2r, double pr, const double* spr,
-                            double cav) {
+template <class EOS, class FP>
+PCD RiemannOut<EOS, FP> riemann(FP rl, FP ul, FP vl, FP v2l,
+                            FP pl, const FP* spl, FP rr, FP ur,
+                            FP vr, FP v2r, FP pr, const FP* spr,
+                            FP cav) {
   constexpr int NS = EOS::NS;
-  RiemannOut<EOS> o;
-  const double wsmall = DMIN;
-  const double cl = EOS::RPY2Cs(rl, pl, spl);
-  const double cr = EOS::RPY2Cs(rr, pr, spr);
-  const double wl = mx(wsmall, cl * rl);
-  const double wr = mx(wsmall, cr * rr);
-  const double pstar =
-      mx(DMIN, ((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr));
-  double ustar = ((wl * ul + wr * ur) + (pl - pr)) / (wl + wr);
-  const bool up = ustar > 0.0;
-  double uo = up ? ul : ur;
-  double po = up ? pl : pr;
+  RiemannOut<EOS, FP> o;
+  const FP wsmall = rmin<FP>();
+  const FP cl = EOS::RPY2Cs(rl, pl, spl);
+  const FP cr = EOS::RPY2Cs(rr, pr, spr);
+  const FP wl = mx(wsmall, cl * rl);
+  const FP wr = mx(wsmall, cr * rr);
+  const FP pstar =
+      mx(rmin<FP>(), ((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr));
+  FP ustar = ((wl * ul + wr * ur) + (pl - pr)) / (wl + wr);
+  const bool up = ustar > FP(0.0);
+  FP uo = up ? ul : ur;
+  FP po = up ? pl : pr;
   const bool tiny =
-      fabs(ustar) < smallu * 0.5 * (fabs(ul) + fabs(ur)) || ustar == 0.0;
-  ustar = tiny ? 0.0 : ustar;
-  double rspo[NS];
-  double ro = 0.0;
+      fabs(ustar) < FP(smallu) * FP(0.5) * (fabs(ul) + fabs(ur)) || ustar == FP(0.0);
+  ustar = tiny ? FP(0.0) : ustar;
+  FP rspo[NS];
+  FP ro = FP(0.0);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const double a = up ? rl * spl[n] : rr * spr[n];
-    rspo[n] = tiny ? 0.5 * (rl * spl[n] + rr * spr[n]) : a;
+    const FP a = up ? rl * spl[n] : rr * spr[n];
+    rspo[n] = tiny ? FP(0.5) * (rl * spl[n] + rr * spr[n]) : a;
     ro += rspo[n];
   }
-  uo = tiny ? 0.5 * (ul + ur) : uo;
-  po = tiny ? 0.5 * (pl + pr) : po;
-  double gmf[NS];
+  uo = tiny ? FP(0.5) * (ul + ur) : uo;
+  po = tiny ? FP(0.5) * (pl + pr) : po;
+  FP gmf[NS];
 #pragma unroll
   for (int n = 0; n < NS; n++) gmf[n] = rspo[n] / ro;
-  const double co = EOS::RPY2Cs(ro, po, gmf);
-  const double drho = (pstar - po) / (co * co);
-  double rspstar[NS];
-  double rstar = 0.0;
+  const FP co = EOS::RPY2Cs(ro, po, gmf);
+  const FP drho = (pstar - po) / (co * co);
+  FP rspstar[NS];
+  FP rstar = FP(0.0);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const double spon = rspo[n] / ro;
-    rspstar[n] = mx(0.0, rspo[n] + drho * spon);
+    const FP spon = rspo[n] / ro;
+    rspstar[n] = mx(FP(0.0), rspo[n] + drho * spon);
     rstar += rspstar[n];
   }
 #pragma unroll
   for (int n = 0; n < NS; n++) gmf[n] = rspstar[n] / rstar;
-  const double cstar = EOS::RPY2Cs(rstar, pstar, gmf);
-  const double sgnm = copysign(1.0, ustar);
-  double spout = co - sgnm * uo;
-  double spin = cstar - sgnm * ustar;
-  const double ushock = 0.5 * (spin + spout);
+  const FP cstar = EOS::RPY2Cs(rstar, pstar, gmf);
+  const FP sgnm = copysign(FP(1.0), ustar);
+  FP spout = co - sgnm * uo;
+  FP spin = cstar - sgnm * ustar;
+  const FP ushock = FP(0.5) * (spin + spout);
   const bool rare = pstar < po;
   spout = rare ? spout : ushock;
   spin = rare ? spin : ushock;
-  const double scr =
-      (fabs(spout - spin) < very_small_num) ? small_num * cav : spout - spin;
-  const double frac = mx(0.0, mn(1.0, (1.0 + (spout + spin) / scr) * 0.5));
-  const bool upz = ustar > 0.0;
-  double qv1 = upz ? vl : vr;
-  double qv2 = upz ? v2l : v2r;
-  const bool zero = (ustar == 0.0);
-  qv1 = zero ? 0.5 * (vl + vr) : qv1;
-  qv2 = zero ? 0.5 * (v2l + v2r) : qv2;
-  double rspgd[NS];
-  double qiu = frac * ustar + (1.0 - frac) * uo;
-  double qp_ = frac * pstar + (1.0 - frac) * po;
-  const bool outside = (spout < 0.0);
-  const bool inside = (spin >= 0.0);
-  double rgd = 0.0;
+  const FP scr =
+      (fabs(spout - spin) < vsmall<FP>()) ? FP(small_num) * cav : spout - spin;
+  const FP frac = mx(FP(0.0), mn(FP(1.0), (FP(1.0) + (spout + spin) / scr) * FP(0.5)));
+  const bool upz = ustar > FP(0.0);
+  FP qv1 = upz ? vl : vr;
+  FP qv2 = upz ? v2l : v2r;
+  const bool zero = (ustar == FP(0.0));
+  qv1 = zero ? FP(0.5) * (vl + vr) : qv1;
+  qv2 = zero ? FP(0.5) * (v2l + v2r) : qv2;
+  FP rspgd[NS];
+  FP qiu = frac * ustar + (FP(1.0) - frac) * uo;
+  FP qp_ = frac * pstar + (FP(1.0) - frac) * po;
+  const bool outside = (spout < FP(0.0));
+  const bool inside = (spin >= FP(0.0));
+  FP rgd = FP(0.0);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    double v = frac * rspstar[n] + (1.0 - frac) * rspo[n];
+    FP v = frac * rspstar[n] + (FP(1.0) - frac) * rspo[n];
     v = outside ? rspo[n] : v;
     v = inside ? rspstar[n] : v;
     rspgd[n] = v;
@@ -663,17 +694,17 @@ PCD RiemannOut<EOS> riemann(double rl, double ul, double vl, double v2l,
   qp_ = inside ? pstar : qp_;
 #pragma unroll
   for (int n = 0; n < NS; n++) gmf[n] = rspgd[n] / rgd;
-  const double ge = EOS::RYP2E(rgd, gmf, qp_);
-  const double regd = rgd * ge;
-  o.game_gd = qp_ / regd + 1.0;
+  const FP ge = EOS::RYP2E(rgd, gmf, qp_);
+  const FP regd = rgd * ge;
+  o.game_gd = qp_ / regd + FP(1.0);
   qiu = 1 * qiu;
   o.f_rho = rgd * qiu;
   o.f_un = o.f_rho * qiu + qp_;
   o.f_ut = o.f_rho * qv1;
   o.f_utt = o.f_rho * qv2;
-  const double rhoetot =
-      regd + 0.5 * rgd * (qiu * qiu + qv1 * qv1 + qv2 * qv2) -
-      0.5 * rgd * 0.0 * 0.0 * 0.0 * 0.0;
+  const FP rhoetot =
+      regd + FP(0.5) * rgd * (qiu * qiu + qv1 * qv1 + qv2 * qv2) -
+      FP(0.5) * rgd * FP(0.0) * FP(0.0) * FP(0.0) * FP(0.0);
   o.f_eden = qiu * (rhoetot + qp_);
   o.f_eint = qiu * regd;
   o.ustar = ustar;
@@ -687,9 +718,10 @@ PCD RiemannOut<EOS> riemann(double rl, double ul, double vl, double v2l,
 // ------------------------------------------------------ cmpflx BCs (A9) --
 // Source/Godunov.H:378-418: wall/symmetry mirror or outflow copy at domain
 // faces, on the normal-frame states (rho, un, ut, utt, p) of both sides.
+template <class FP>
 PCD void face_bc(int idx, int domlo, int domhi, int bclo, int bchi,
-                 double& rl, double& ul, double& vl, double& v2l, double& pl,
-                 double& rr, double& ur, double& vr, double& v2r, double& pr) {
+                 FP& rl, FP& ul, FP& vl, FP& v2l, FP& pl,
+                 FP& rr, FP& ur, FP& vr, FP& v2r, FP& pr) {
   if (idx == domlo) {
     if (bclo == BC_NOSLIPWALL || bclo == BC_SLIPWALL || bclo == BC_SYMMETRY) {
       ul = -ur; vl = vr; v2l = v2r; pl = pr; rl = rr;
@@ -707,30 +739,30 @@ PCD void face_bc(int idx, int domlo, int domhi, int bclo, int bchi,
 
 // Flux of one face in lab-frame layout: [rho, mx, my, mz, rhoE, rhoe, Y...]
 // plus the Godunov pressure and normal velocity.
-template <int NS>
+template <int NS, class FP = double>
 struct Flux {
-  double f[6 + NS];  // URHO, UMX, UMY, UMZ, UEDEN, UEINT, species
-  double pgd, ugd;
+  FP f[6 + NS];  // URHO, UMX, UMY, UMZ, UEDEN, UEINT, species
+  FP pgd, ugd;
 };
 
 // pc_cmpflx (Source/Godunov.H:303-449): BC-aware Riemann solve and passive
 // upwinding by the mass flux.
-template <class EOS, int DIR>
-PCD void cmpflx(const FaceState<EOS::NS>& ql, const FaceState<EOS::NS>& qr,
-                double cav, int idx, int domlo, int domhi, int bclo, int bchi,
-                Flux<EOS::NS>& F) {
+template <class EOS, int DIR, class FP>
+PCD void cmpflx(const FaceState<EOS::NS, FP>& ql, const FaceState<EOS::NS, FP>& qr,
+                FP cav, int idx, int domlo, int domhi, int bclo, int bchi,
+                Flux<EOS::NS, FP>& F) {
   constexpr int NS = EOS::NS;
   constexpr int IU = DIR, IV = (DIR == 0) ? 1 : 0, IV2 = (DIR == 2) ? 1 : 2;
-  auto comp = [](const FaceState<NS>& q, int c) {
+  auto comp = [](const FaceState<NS, FP>& q, int c) {
     return c == 0 ? q.u : (c == 1 ? q.v : q.w);
   };
-  double rl = ql.rho, ul = comp(ql, IU), vl = comp(ql, IV), v2l = comp(ql, IV2),
+  FP rl = ql.rho, ul = comp(ql, IU), vl = comp(ql, IV), v2l = comp(ql, IV2),
          pl = ql.p;
-  double rr = qr.rho, ur = comp(qr, IU), vr = comp(qr, IV), v2r = comp(qr, IV2),
+  FP rr = qr.rho, ur = comp(qr, IU), vr = comp(qr, IV), v2r = comp(qr, IV2),
          pr = qr.p;
   face_bc(idx, domlo, domhi, bclo, bchi, rl, ul, vl, v2l, pl, rr, ur, vr, v2r,
           pr);
-  const RiemannOut<EOS> o = riemann<EOS>(rl, ul, vl, v2l, pl, ql.Y, rr, ur, vr,
+  const RiemannOut<EOS, FP> o = riemann<EOS>(rl, ul, vl, v2l, pl, ql.Y, rr, ur, vr,
                                          v2r, pr, qr.Y, cav);
   F.f[0] = o.f_rho;
   F.f[1 + IU] = o.f_un;
@@ -740,102 +772,102 @@ PCD void cmpflx(const FaceState<EOS::NS>& ql, const FaceState<EOS::NS>& qr,
   F.f[5] = o.f_eint;
   F.pgd = o.p_gd;
   F.ugd = o.u_gd;
-  const double flxrho = o.f_rho;
+  const FP flxrho = o.f_rho;
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const double qlv = ql.Y[n], qrv = qr.Y[n];
-    F.f[6 + n] = (o.ustar > 0.0)   ? flxrho * qlv
-                 : (o.ustar < 0.0) ? flxrho * qrv
-                                   : flxrho * 0.5 * (qlv + qrv);
+    const FP qlv = ql.Y[n], qrv = qr.Y[n];
+    F.f[6 + n] = (o.ustar > FP(0.0))   ? flxrho * qlv
+                 : (o.ustar < FP(0.0)) ? flxrho * qrv
+                                   : flxrho * FP(0.5) * (qlv + qrv);
   }
 }
 
 // ---------------------------------------------- transverse corrections --
 // Flux differences across one cell in a transverse direction:
 // d[0..4] = c*(F_hi - F_lo) of rho, mx, my, mz, rhoE; ds[k] of rho Y_k.
-template <int NS>
+template <int NS, class FP = double>
 struct DFlux {
-  double d[5];
-  double ds[NS];
-  double pgp, pgm, ugp, ugm;  // Godunov p and normal u on the two faces
+  FP d[5];
+  FP ds[NS];
+  FP pgp, pgm, ugp, ugm;  // Godunov p and normal u on the two faces
 };
 
 // pc_transdo, one side (Source/Godunov.H:455-633 with the passive part
 // :30-73): the dir-state `s` corrected by the transverse flux difference of
 // its cell; gamc = QGAMC of that cell.
-template <int NS>
-PCD void transdo_side(const FaceState<NS>& s, const DFlux<NS>& D, double gamc,
-                      double cdtdx, FaceState<NS>& o) {
-  const double flxrho = D.d[0], flxu = D.d[1], flxv = D.d[2], flxw = D.d[3],
+template <int NS, class FP>
+PCD void transdo_side(const FaceState<NS, FP>& s, const DFlux<NS, FP>& D, FP gamc,
+                      FP cdtdx, FaceState<NS, FP>& o) {
+  const FP flxrho = D.d[0], flxu = D.d[1], flxv = D.d[2], flxw = D.d[3],
                flxe = D.d[4];
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const double rr = s.rho;
-    const double rrnew = rr - flxrho;
-    const double compo = rr * s.Y[n] - D.ds[n];
+    const FP rr = s.rho;
+    const FP rrnew = rr - flxrho;
+    const FP compo = rr * s.Y[n] - D.ds[n];
     o.Y[n] = compo / rrnew;
   }
-  const double dAup = D.pgp * D.ugp - D.pgm * D.ugm;
-  const double pav = 0.5 * (D.pgp + D.pgm);
-  const double dAu = D.ugp - D.ugm;
-  const double rrr = s.rho;
-  double rur = s.u, rvr = s.v, rwr = s.w;
-  const double ekinr = 0.5 * rrr * (rur * rur + rvr * rvr + rwr * rwr);
+  const FP dAup = D.pgp * D.ugp - D.pgm * D.ugm;
+  const FP pav = FP(0.5) * (D.pgp + D.pgm);
+  const FP dAu = D.ugp - D.ugm;
+  const FP rrr = s.rho;
+  FP rur = s.u, rvr = s.v, rwr = s.w;
+  const FP ekinr = FP(0.5) * rrr * (rur * rur + rvr * rvr + rwr * rwr);
   rur *= rrr;
   rvr *= rrr;
   rwr *= rrr;
-  const double rer = s.rhoe + ekinr;
-  const double rrnewr = rrr - flxrho;
-  const double runewr = rur - flxu;
-  const double rvnewr = rvr - flxv;
-  const double rwnewr = rwr - flxw;
-  const double renewr = rer - flxe;
+  const FP rer = s.rhoe + ekinr;
+  const FP rrnewr = rrr - flxrho;
+  const FP runewr = rur - flxu;
+  const FP rvnewr = rvr - flxv;
+  const FP rwnewr = rwr - flxw;
+  const FP renewr = rer - flxe;
   o.rho = rrnewr;
   o.u = runewr / rrnewr;
   o.v = rvnewr / rrnewr;
   o.w = rwnewr / rrnewr;
-  const double rhoekinr =
-      0.5 * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
+  const FP rhoekinr =
+      FP(0.5) * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
   o.rhoe = renewr - rhoekinr;
-  const double pnewr = s.p - cdtdx * (dAup + pav * dAu * (gamc - 1.0));
-  o.p = mx(pnewr, very_small_num);
+  const FP pnewr = s.p - cdtdx * (dAup + pav * dAu * (gamc - FP(1.0)));
+  o.p = mx(pnewr, vsmall<FP>());
 }
 
 // pc_transdd, one side (Source/Godunov.H:639-845, passive :78-132): the
 // dir-state corrected by both transverse flux differences (D0 scaled by
 // cdtdx0, D1 by cdtdx1 when they were formed) plus hdt * srcQ.
-template <int NS>
-PCD void transdd_side(const FaceState<NS>& s, const double* dd,
-                      const double* dds, double pxnew, double pynew,
-                      const double* src /* rho,u,v,w,rhoe,p,Y */, double hdt,
-                      FaceState<NS>& o) {
-  const double flxrho = dd[0], flxu = dd[1], flxv = dd[2], flxw = dd[3],
+template <int NS, class FP>
+PCD void transdd_side(const FaceState<NS, FP>& s, const FP* dd,
+                      const FP* dds, FP pxnew, FP pynew,
+                      const FP* src /* rho,u,v,w,rhoe,p,Y */, FP hdt,
+                      FaceState<NS, FP>& o) {
+  const FP flxrho = dd[0], flxu = dd[1], flxv = dd[2], flxw = dd[3],
                flxe = dd[4];
-  const double rrr = s.rho;
-  const double rrnewr = rrr - flxrho;
+  const FP rrr = s.rho;
+  const FP rrnewr = rrr - flxrho;
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const double compo = rrr * s.Y[n] - dds[n];
+    const FP compo = rrr * s.Y[n] - dds[n];
     o.Y[n] = compo / rrnewr + hdt * src[6 + n];
   }
-  const double rur = rrr * s.u;
-  const double rvr = rrr * s.v;
-  const double rwr = rrr * s.w;
-  const double ekinr = 0.5 * (rur * rur + rvr * rvr + rwr * rwr) / rrr;
-  const double rer = s.rhoe + ekinr;
-  const double runewr = rur - flxu;
-  const double rvnewr = rvr - flxv;
-  const double rwnewr = rwr - flxw;
-  const double renewr = rer - flxe;
+  const FP rur = rrr * s.u;
+  const FP rvr = rrr * s.v;
+  const FP rwr = rrr * s.w;
+  const FP ekinr = FP(0.5) * (rur * rur + rvr * rvr + rwr * rwr) / rrr;
+  const FP rer = s.rhoe + ekinr;
+  const FP runewr = rur - flxu;
+  const FP rvnewr = rvr - flxv;
+  const FP rwnewr = rwr - flxw;
+  const FP renewr = rer - flxe;
   o.rho = rrnewr + hdt * src[0];
   o.u = runewr / rrnewr + hdt * src[1];
   o.v = rvnewr / rrnewr + hdt * src[2];
   o.w = rwnewr / rrnewr + hdt * src[3];
-  const double rhoekinr =
-      0.5 * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
+  const FP rhoekinr =
+      FP(0.5) * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
   o.rhoe = renewr - rhoekinr + hdt * src[4];
-  double p = s.p - pxnew - pynew + hdt * src[5];
-  o.p = mx(p, DMIN);
+  FP p = s.p - pxnew - pynew + hdt * src[5];
+  o.p = mx(p, rmin<FP>());
 }
 
 }  // namespace pcd

@@ -236,6 +236,18 @@ class DeviceLevel:
         lx, ly, lz = (-1, 0, 0) if shape is None else shape
         self._chk(self.L.pc_level_set_plan_v1(self.h, sid, lx, ly, lz, int(wg_per_cu)))
 
+    def set_precision(self, mixed):
+        """False: FP64 step (default); True: FP32 reconstruction, limiters,
+        Riemann solves and intermediates, FP64 EOS inversion and update."""
+        self._chk(self.L.pc_level_set_precision_v1(self.h, 1 if mixed else 0))
+
+    def valid_tensor(self, b, out=None):
+        """Box b's valid cells as a device tensor (NVAR, nz, ny, nx)."""
+        import torch
+        full = torch.empty(self.fab_shape(b), dtype=torch.float64, device="cuda")
+        self._chk(self.L.pc_level_read_box_v1(self.h, b, C.c_void_p(full.data_ptr())))
+        return full[:, 4:-4, 4:-4, 4:-4]
+
     def set_graph(self, steps_per_graph):
         """Replay steps from a hipGraph of `steps_per_graph` steps (0: off)."""
         self._chk(self.L.pc_level_set_graph_v1(self.h, int(steps_per_graph)))
