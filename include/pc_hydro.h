@@ -267,6 +267,50 @@ int pc_level_status_v1(pc_level_t* lev, int* bad, int* small_dt);
 int pc_level_prim_to_cons_v1(pc_level_t* lev, int b, pc_array4_t prim,
                              int mode, void* stream);
 
+/* ------------------------------------------------- batch (ensembles) --- */
+/* Independent requests resident on the device: each is one box that is its
+ * own domain, with its own boundary codes, cell size and dt (a shock tube or
+ * a premixed slab of an ensemble).  Memory comes from one arena of
+ * `budget_bytes` allocated at creation (admission control: pc_batch_add_v1
+ * fails with PC_ERR_NOMEM, without side effects, when the request's state and
+ * stage workspace do not fit).  pc_batch_step_v1 advances a set of requests
+ * by one step each with one launch per stage over all their boxes (the
+ * batched, continuous form of pc_level_step_v1); every request keeps its own
+ * dt (estTimeStep/computeNewDt per request).  The arithmetic of a request
+ * does not depend on which others share its steps.  No reference counterpart
+ * (the reference steps one AMR hierarchy per process). */
+typedef struct pc_batch pc_batch_t;
+/* mixed: as pc_level_set_precision_v1.  share = NULL: a new arena of
+ * budget_bytes; else draw on share's arena (one budget for batches of
+ * different mechanisms), which lives until its last batch is destroyed.
+ * max_slots: most requests alive in this batch. */
+int pc_batch_create_v1(pc_ctx_t* ctx, const pc_params_t* prm, int mixed,
+                       long long budget_bytes, pc_batch_t* share,
+                       int max_slots, pc_batch_t** out);
+int pc_batch_destroy_v1(pc_batch_t* batch);
+/* device bytes a request on box bx takes from the arena */
+long long pc_batch_slot_bytes_v1(const pc_batch_t* batch, pc_box_t bx);
+int pc_batch_memory_v1(const pc_batch_t* batch, long long* total,
+                       long long* used, long long* largest_free);
+/* Admit a request: box bx (its own domain), PC_BC_* codes per direction,
+ * cell size dx[3], primitive fields over bx as pc_level_prim_to_cons_v1
+ * (device memory, prim_mode 0 or 1).  Builds the conservative state and the
+ * request's initial dt (init_shrink * estTimeStep); synchronous.  *slot
+ * names the request in the other calls. */
+int pc_batch_add_v1(pc_batch_t* batch, pc_box_t bx, const int* bclo,
+                    const int* bchi, const double* dx, pc_array4_t prim,
+                    int prim_mode, void* stream, int* slot);
+/* One step of each listed request (distinct live slots); asynchronous. */
+int pc_batch_step_v1(pc_batch_t* batch, const int* slots, int n, void* stream);
+/* The request's whole state FAB (grow(bx,4) x NVAR, Array4 layout) into buf
+ * (host or device memory); synchronous. */
+int pc_batch_read_v1(pc_batch_t* batch, int slot, double* buf);
+/* dt of its next step and the NaN / dt_cutoff flags; synchronous. */
+int pc_batch_status_v1(pc_batch_t* batch, int slot, double* dt, int* bad,
+                       int* small_dt);
+/* Retire a request and return its memory (waits for `stream`). */
+int pc_batch_remove_v1(pc_batch_t* batch, int slot, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -207,7 +208,7 @@ static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
 
 // Schedule of a launch over one job: one workgroup per block of cells.
 static Sched one_job_sched(const Job* j) {
-  return Sched{j, nullptr, nullptr, 0, -1, 0, 0};
+  return Sched{j, nullptr, nullptr, 0, -1, 0, 0, 1};
 }
 
 // Stage job J for the kernels of one box-driver call on stream st.
@@ -363,11 +364,21 @@ static int finish_umdrv_job(pc_ctx* c, Job& J, const IBox& bx, double* ws,
   return PC_OK;
 }
 
+// uniform Cartesian geometry of a job (amrex::Geometry on a Cartesian grid:
+// face areas dx1*dx2 etc., volume dx0*dx1*dx2)
+static void set_geom(Job& J, const double* dx) {
+  for (int d = 0; d < 3; ++d) J.dx[d] = dx[d];
+  J.area_u[0] = dx[1] * dx[2];
+  J.area_u[1] = dx[2] * dx[0];
+  J.area_u[2] = dx[0] * dx[1];
+  J.vol_u = dx[0] * dx[1] * dx[2];
+}
+
 static KArgs umdrv_args(const double* dx, double dt, int plm_iorder,
                         int use_flattening, double difmag) {
   KArgs ka;
   memset(&ka, 0, sizeof(ka));
-  for (int d = 0; d < 3; ++d) ka.dx[d] = dx[d];
+  (void)dx;
   ka.difmag = difmag;
   ka.dt_val = dt;
   ka.dtp = nullptr;
@@ -557,6 +568,7 @@ int pc_umdrv_v1(pc_ctx_t* c, double /*time*/, pc_box_t bx, const int* domlo,
   const WsLayout L = ws_layout(b, c->ns, false, false, false);
   if ((rc = ctx_scratch(c, L.total * sizeof(double), st))) return rc;
   if ((rc = finish_umdrv_job(c, J, b, c->ws, flx, a, vol, uout))) return rc;
+  set_geom(J, dx);
   const KArgs ka = umdrv_args(dx, dt, plm_iorder, use_flattening, difmag);
   Sched sc;
   int slot;
@@ -606,6 +618,7 @@ int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
   for (int d = 0; d < 3; ++d) J.qe[d] = Lq.q.hi[d] + 1;
   if ((rc = finish_umdrv_job(c, J, b, c->ws + qpart, flx, a, vol, hyd_src)))
     return rc;
+  set_geom(J, dx);
   KArgs ka = umdrv_args(dx, dt, prm->plm_iorder, prm->use_flattening,
                         prm->difmag);
   // srctoprim reads qaux's dp/de and dp/drho, which ctoprim only writes
@@ -705,7 +718,8 @@ static bool plan_make(pc_level* L, int s, int lx, int ly, int lz, int cap,
 }
 
 static Sched plan_sched(const pc_level* L, const StagePlan& P) {
-  return Sched{L->jobs, P.blkmap, P.blkstart, P.nblocks, P.lx, P.ly, P.lz};
+  return Sched{L->jobs, P.blkmap, P.blkstart, P.nblocks, P.lx, P.ly, P.lz,
+               (int)L->hjobs.size()};
 }
 
 static void graph_drop(pc_level* L) {
@@ -733,8 +747,8 @@ static void level_free(pc_level* L) {
 // Launch every kernel of stage s once with schedule sc on `grid` workgroups
 // (sub >= 0: only that direction of the trace stage).  P: the persistent
 // instantiations, which loop over the logical blocks.
-template <class EOS, class FP, bool P>
-static void launch_stage_t(pc_level* L, int s, const Sched& sc, int grid,
+template <class EOS, class FP, bool P, class H>
+static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
                            hipStream_t st, int sub) {
   const KArgs& ka = L->ka;
   auto tr = [&](int d) { return sub < 0 || sub == d; };
@@ -779,8 +793,9 @@ static void launch_stage_t(pc_level* L, int s, const Sched& sc, int grid,
   }
 }
 
-template <class EOS>
-static void launch_stage(pc_level* L, int s, const Sched& sc,
+// H: the level or the batch (ka, la, lowo, ds, prm.ppm_type, mixed)
+template <class EOS, class H>
+static void launch_stage(H* L, int s, const Sched& sc,
                          const StagePlan& pl, hipStream_t st, int sub = -1) {
   const bool persist = pl.grid < pl.nblocks;
   if (L->mixed) {
@@ -869,6 +884,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
       J.bchi[dd] = d->periodic_self ? PC_BC_INTERIOR : d->bchi[dd];
       J.qe[dd] = bx.hi[dd] + 5;
     }
+    set_geom(J, d->dx);
     J.U = fab_view(L->state + L->off[b], growb(bx, 4));
     const WsLayout W = ws_layout(bx, ns, true, false, true);
     ws_fill_job(J, L->ws + wsoff[b], W, true, false, true);
@@ -941,13 +957,6 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   L->la.init_shrink = prm->init_shrink;
   L->la.dt_cutoff = prm->dt_cutoff;
   memset(&L->ka, 0, sizeof(L->ka));
-  for (int dd = 0; dd < 3; ++dd) L->ka.dx[dd] = d->dx[dd];
-  // amrex::Geometry on a Cartesian grid: face areas dx1*dx2 etc., volume
-  // dx0*dx1*dx2
-  L->ka.area_u[0] = d->dx[1] * d->dx[2];
-  L->ka.area_u[1] = d->dx[2] * d->dx[0];
-  L->ka.area_u[2] = d->dx[0] * d->dx[1];
-  L->ka.vol_u = d->dx[0] * d->dx[1] * d->dx[2];
   L->ka.difmag = prm->difmag;
   L->ka.dtp = &L->ds->dt;
   L->ka.plm_iorder = prm->plm_iorder;
@@ -1350,6 +1359,373 @@ int pc_level_prim_to_cons_v1(pc_level_t* L, int b, pc_array4_t prim, int mode,
   else
     LAUNCH(k_prim_to_cons<EosLiDryer>, g, st, s0, L->ka, pv, mode);
   HIPCHK(L->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+}  // extern "C"
+
+// ==================================================================== batch
+// Independent requests (one box each, its own domain, BCs, cell size and
+// dt) resident in one device arena.  A step advances any subset of them with
+// one launch per stage over all their boxes; the job table of the subset is
+// rebuilt only when the subset changes.
+
+// First-fit allocator over one device allocation (host-side bookkeeping).
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  int refs = 0;  // batches drawing on this arena
+  std::map<size_t, size_t> fr;  // free ranges: offset -> bytes
+  static size_t round(size_t b) { return (b + 255) & ~size_t(255); }
+  long long alloc(size_t bytes) {
+    bytes = round(bytes);
+    for (auto it = fr.begin(); it != fr.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, rest = it->second - bytes;
+      fr.erase(it);
+      if (rest) fr[off + bytes] = rest;
+      used += bytes;
+      return (long long)off;
+    }
+    return -1;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = round(bytes);
+    used -= bytes;
+    auto it = fr.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != fr.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      fr.erase(nx);
+    }
+    if (it != fr.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        fr.erase(it);
+      }
+    }
+  }
+  size_t largest() const {
+    size_t m = 0;
+    for (const auto& f : fr) m = std::max(m, f.second);
+    return m;
+  }
+};
+
+struct BatchSlot {
+  bool live = false;
+  IBox box;
+  size_t off = 0, bytes = 0;
+  long long nstate = 0;  // doubles of the state FAB
+  Job job;
+};
+
+struct pc_batch {
+  pc_ctx* ctx = nullptr;
+  pc_params_t prm;
+  int mixed = 0;
+  KArgs ka;
+  LevelArgs la;
+  int lowo[3] = {0, 0, 0};
+  DtState* ds = nullptr;  // device, one per slot
+  Arena* arena = nullptr;  // possibly shared with other batches
+  std::vector<BatchSlot> slots;
+  int max_slots = 0;
+  Job* djobs = nullptr;   // device job table of the stepped subset
+  int* dstart = nullptr;  // device [ST_COUNT][max_slots] first blocks
+  int* dids = nullptr;    // device slot ids of the stepped subset
+  Job* dinit = nullptr;   // device job of a request being admitted
+  std::vector<int> cur;   // the stepped subset the tables describe
+  StagePlan plans[ST_COUNT];
+  Sched scs[ST_COUNT];
+};
+
+static size_t batch_slot_bytes(const pc_batch* B, const IBox& bx) {
+  const int ns = B->ctx->ns;
+  const size_t st = (size_t)growb(bx, 4).npts() * (7 + ns);
+  const size_t ws = ws_layout(bx, ns, true, false, true).total;
+  return Arena::round(st * sizeof(double)) + ws * sizeof(double);
+}
+
+static void batch_free(pc_batch* B) {
+  if (B->arena && --B->arena->refs == 0) {
+    if (B->arena->base) (void)hipFree(B->arena->base);
+    delete B->arena;
+  }
+  if (B->ds) (void)hipFree(B->ds);
+  if (B->djobs) (void)hipFree(B->djobs);
+  if (B->dstart) (void)hipFree(B->dstart);
+  if (B->dids) (void)hipFree(B->dids);
+  if (B->dinit) (void)hipFree(B->dinit);
+  delete B;
+}
+
+// Job table, per-stage first blocks and slot ids of `ids` on the device.
+static int batch_tables(pc_batch* B, const std::vector<int>& ids,
+                        hipStream_t st) {
+  if (ids == B->cur) return PC_OK;
+  pc_ctx* c = B->ctx;
+  const int n = (int)ids.size();
+  std::vector<Job> jobs(n);
+  std::vector<int> start((size_t)ST_COUNT * B->max_slots, 0);
+  for (int k = 0; k < n; ++k) jobs[k] = B->slots[ids[k]].job;
+  for (int s = 0; s < ST_COUNT; ++s) {
+    int tot = 0;
+    for (int k = 0; k < n; ++k) {
+      start[(size_t)s * B->max_slots + k] = tot;
+      tot += (int)stage_blocks(s, jobs[k], -1, 0, 0);
+    }
+    B->plans[s].nblocks = B->plans[s].grid = tot;
+    B->plans[s].lx = -1;
+    B->scs[s] = Sched{B->djobs, nullptr, B->dstart + (size_t)s * B->max_slots,
+                      tot, -1, 0, 0, n};
+  }
+  // the previous step may still read the old tables
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipMemcpy(B->djobs, jobs.data(), sizeof(Job) * n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(B->dstart, start.data(), sizeof(int) * start.size(),
+                      hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(B->dids, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  B->cur = ids;
+  return PC_OK;
+}
+
+template <class EOS>
+static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
+  const Sched* S = B->scs;
+  const StagePlan* P = B->plans;
+  launch_stage<EOS>(B, ST_GHOST, S[ST_GHOST], P[ST_GHOST], st);
+  launch_stage<EOS>(B, ST_CTOPRIM, S[ST_CTOPRIM], P[ST_CTOPRIM], st);
+  launch_stage<EOS>(B, ST_DIVU, S[ST_DIVU], P[ST_DIVU], st);
+  launch_stage<EOS>(B, ST_TRACE, S[ST_TRACE], P[ST_TRACE], st);
+  for (int x = ST_FLUX1_X; x <= ST_FINAL_Z; ++x)
+    launch_stage<EOS>(B, x, S[x], P[x], st);
+  launch_stage<EOS>(B, ST_UPDATE, S[ST_UPDATE], P[ST_UPDATE], st);
+  hipLaunchKernelGGL(k_dt_finalize_list, dim3((n + 63) / 64), dim3(64), 0, st,
+                     B->ds, (const int*)B->dids, n, B->la, 0);
+}
+
+extern "C" {
+
+int pc_batch_create_v1(pc_ctx_t* c, const pc_params_t* prm, int mixed,
+                       long long budget_bytes, pc_batch_t* share,
+                       int max_slots, pc_batch_t** out) {
+  if (!c || !prm || !out || (budget_bytes <= 0 && !share) || max_slots <= 0 ||
+      (mixed != 0 && mixed != 1))
+    return fail(c, PC_ERR_ARG, "bad batch arguments");
+  int rc;
+  if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder, prm->use_hybrid_weno)))
+    return rc;
+  if (!prm->allow_small_energy || !prm->allow_negative_energy)
+    return fail(c, PC_ERR_UNSUPPORTED,
+                "allow_small_energy = allow_negative_energy = 1 required");
+  pc_batch* B = new pc_batch;
+  B->ctx = c;
+  B->prm = *prm;
+  B->mixed = mixed;
+  B->max_slots = max_slots;
+  B->slots.resize(max_slots);
+  memset(&B->ka, 0, sizeof(B->ka));
+  B->ka.difmag = prm->difmag;
+  B->ka.plm_iorder = prm->plm_iorder;
+  B->ka.use_flattening = prm->use_flattening;
+  memset(&B->la, 0, sizeof(B->la));
+  B->la.periodic_self = 2;
+  B->la.cfl = prm->cfl;
+  B->la.max_dt = prm->max_dt;
+  B->la.fixed_dt = prm->fixed_dt;
+  B->la.change_max = prm->change_max;
+  B->la.init_shrink = prm->init_shrink;
+  B->la.dt_cutoff = prm->dt_cutoff;
+  auto dalloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    return true;
+  };
+  if (share) {
+    B->arena = share->arena;
+    B->arena->refs++;
+  } else {
+    B->arena = new Arena;
+    B->arena->refs = 1;
+    B->arena->size = (size_t)budget_bytes;
+    if (!dalloc((void**)&B->arena->base, B->arena->size)) {
+      batch_free(B);
+      return fail(c, PC_ERR_NOMEM, "batch arena of %lld bytes failed",
+                  budget_bytes);
+    }
+    B->arena->fr[0] = B->arena->size;
+  }
+  if (!dalloc((void**)&B->ds, sizeof(DtState) * max_slots) ||
+      !dalloc((void**)&B->djobs, sizeof(Job) * max_slots) ||
+      !dalloc((void**)&B->dstart, sizeof(int) * ST_COUNT * max_slots) ||
+      !dalloc((void**)&B->dids, sizeof(int) * max_slots) ||
+      !dalloc((void**)&B->dinit, sizeof(Job))) {
+    batch_free(B);
+    return fail(c, PC_ERR_NOMEM, "batch tables allocation failed");
+  }
+  *out = B;
+  return PC_OK;
+}
+
+int pc_batch_destroy_v1(pc_batch_t* B) {
+  if (!B) return PC_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  batch_free(B);
+  return PC_OK;
+}
+
+long long pc_batch_slot_bytes_v1(const pc_batch_t* B, pc_box_t bx) {
+  if (!B || !box_ok(bx)) return -1;
+  return (long long)batch_slot_bytes(B, mkbox(bx.lo, bx.hi));
+}
+
+int pc_batch_memory_v1(const pc_batch_t* B, long long* total, long long* used,
+                       long long* largest_free) {
+  if (!B) return PC_ERR_ARG;
+  if (total) *total = (long long)B->arena->size;
+  if (used) *used = (long long)B->arena->used;
+  if (largest_free) *largest_free = (long long)B->arena->largest();
+  return PC_OK;
+}
+
+int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
+                    const int* bchi, const double* dx, pc_array4_t prim,
+                    int prim_mode, void* stream, int* slot_out) {
+  if (!B || !box_ok(bx) || !bclo || !bchi || !dx || !slot_out ||
+      (prim_mode != 0 && prim_mode != 1))
+    return fail(B ? B->ctx : nullptr, PC_ERR_ARG, "bad request");
+  pc_ctx* c = B->ctx;
+  hipStream_t st = (hipStream_t)stream;
+  int id = -1;
+  for (int k = 0; k < B->max_slots; ++k)
+    if (!B->slots[k].live) { id = k; break; }
+  if (id < 0) return fail(c, PC_ERR_NOMEM, "all %d request slots in use", B->max_slots);
+  const IBox box = mkbox(bx.lo, bx.hi);
+  int rc;
+  V pv;
+  if ((rc = a4_view(c, prim, box, 5 + c->ns, "prim", pv))) return rc;
+  const size_t bytes = batch_slot_bytes(B, box);
+  const long long off = B->arena->alloc(bytes);
+  if (off < 0)
+    return fail(c, PC_ERR_NOMEM, "request of %zu bytes does not fit the budget",
+                bytes);
+  BatchSlot& S = B->slots[id];
+  S.box = box;
+  S.off = (size_t)off;
+  S.bytes = bytes;
+  S.nstate = growb(box, 4).npts() * (7 + c->ns);
+  double* base = (double*)(B->arena->base + off);
+  Job& J = S.job;
+  memset(&J, 0, sizeof(J));
+  for (int d = 0; d < 3; ++d) {
+    J.lo[d] = J.dlo[d] = box.lo[d];
+    J.hi[d] = J.dhi[d] = box.hi[d];
+    J.bclo[d] = bclo[d];
+    J.bchi[d] = bchi[d];
+    J.qe[d] = box.hi[d] + 5;
+  }
+  set_geom(J, dx);
+  J.U = fab_view(base, growb(box, 4));
+  const WsLayout W = ws_layout(box, c->ns, true, false, true);
+  ws_fill_job(J, base + (Arena::round(S.nstate * sizeof(double)) / sizeof(double)),
+              W, true, false, true);
+  J.srcq = null_view();
+  for (int d = 0; d < 3; ++d) J.area[d] = null_view();
+  J.vol = null_view();
+  J.upd = null_view();
+  J.ds = B->ds + id;
+  for (int d = 0; d < 3; ++d) {
+    auto lo_order = [](int b) { return b == PC_BC_INFLOW || b == PC_BC_USERBC; };
+    if (lo_order(bclo[d]) || lo_order(bchi[d])) B->lowo[d] = 1;
+  }
+  // conservative state, then dt = init_shrink * estTimeStep of this request
+  DtState z;
+  z.dt = 0.0;
+  z.minbits = 0x7FEFFFFFFFFFFFFFull;
+  z.bad = 0;
+  z.small = 0;
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipMemcpy(B->dinit, &J, sizeof(Job), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(B->ds + id, &z, sizeof(z), hipMemcpyHostToDevice));
+  const Sched s0 = one_job_sched(B->dinit);
+  const int g = grid1(ST_UPDATE, J);
+  if (c->ns == 1) {
+    LAUNCH(k_prim_to_cons<EosGamma>, g, st, s0, B->ka, pv, prim_mode);
+    LAUNCH((k_estdt<EosGamma, false>), g, st, s0, B->ka, B->ds + id);
+  } else {
+    LAUNCH(k_prim_to_cons<EosLiDryer>, g, st, s0, B->ka, pv, prim_mode);
+    LAUNCH((k_estdt<EosLiDryer, false>), g, st, s0, B->ka, B->ds + id);
+  }
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, B->ds + id, B->la, 1);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  S.live = true;
+  *slot_out = id;
+  return PC_OK;
+}
+
+int pc_batch_step_v1(pc_batch_t* B, const int* slots, int n, void* stream) {
+  if (!B || n < 0 || (n > 0 && !slots)) return PC_ERR_ARG;
+  if (n == 0) return PC_OK;
+  pc_ctx* c = B->ctx;
+  std::vector<int> ids(slots, slots + n);
+  std::vector<char> seen(B->max_slots, 0);
+  for (int k = 0; k < n; ++k) {
+    const int id = ids[k];
+    if (id < 0 || id >= B->max_slots || !B->slots[id].live || seen[id])
+      return fail(c, PC_ERR_ARG, "slot %d is not a live, distinct request", id);
+    seen[id] = 1;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = batch_tables(B, ids, st))) return rc;
+  if (c->ns == 1)
+    batch_step_eos<EosGamma>(B, n, st);
+  else
+    batch_step_eos<EosLiDryer>(B, n, st);
+  HIPCHK(c, hipGetLastError());
+  return PC_OK;
+}
+
+int pc_batch_read_v1(pc_batch_t* B, int slot, double* buf) {
+  if (!B || !buf || slot < 0 || slot >= B->max_slots || !B->slots[slot].live)
+    return PC_ERR_ARG;
+  const BatchSlot& S = B->slots[slot];
+  HIPCHK(B->ctx, hipDeviceSynchronize());
+  HIPCHK(B->ctx, hipMemcpy(buf, B->arena->base + S.off, S.nstate * sizeof(double),
+                           hipMemcpyDefault));
+  return PC_OK;
+}
+
+int pc_batch_status_v1(pc_batch_t* B, int slot, double* dt, int* bad,
+                       int* small_dt) {
+  if (!B || slot < 0 || slot >= B->max_slots || !B->slots[slot].live)
+    return PC_ERR_ARG;
+  DtState s;
+  HIPCHK(B->ctx, hipDeviceSynchronize());
+  HIPCHK(B->ctx, hipMemcpy(&s, B->ds + slot, sizeof(s), hipMemcpyDeviceToHost));
+  if (dt) *dt = s.dt;
+  if (bad) *bad = s.bad;
+  if (small_dt) *small_dt = s.small;
+  return s.bad ? PC_ERR_NAN : (s.small ? PC_ERR_DT : PC_OK);
+}
+
+int pc_batch_remove_v1(pc_batch_t* B, int slot, void* stream) {
+  if (!B || slot < 0 || slot >= B->max_slots || !B->slots[slot].live)
+    return PC_ERR_ARG;
+  HIPCHK(B->ctx, hipStreamSynchronize((hipStream_t)stream));
+  BatchSlot& S = B->slots[slot];
+  B->arena->release(S.off, S.bytes);
+  S.live = false;
+  for (int id : B->cur)
+    if (id == slot) {
+      B->cur.clear();  // tables must be rebuilt
+      break;
+    }
   return PC_OK;
 }
 

@@ -49,11 +49,24 @@ using V = VT<double>;
 // fluxes, qec and divu are always double.  Both instantiations have the same
 // layout (views differ only in the pointee type), so one job table serves
 // either.
+// Device-resident time-step state of a level or of one request: dt of the
+// next step, the running minimum of dx/(c+|u|) (bits of a positive double),
+// and the NaN and dt_cutoff flags.
+struct DtState {
+  double dt;
+  unsigned long long minbits;
+  int bad;
+  int small;
+};
+
 template <class FP>
 struct JobT {
   int lo[3], hi[3];                 // valid box
   int dlo[3], dhi[3], bclo[3], bchi[3];
   int qe[3];                        // exclusive end of the q box (contains)
+  double dx[3];                     // cell size
+  double area_u[3], vol_u;          // uniform face areas / volume (no area/vol)
+  DtState* ds;                      // dt of this job's step (0: from KArgs)
   VT<double> U;                     // state with >= 4 ghosts (uin)
   VT<FP> q, qa, srcq;               // primitives on grow(bx,4); srcq.p may be 0
   VT<FP> tr[3];                     // traces per direction
@@ -69,9 +82,6 @@ static_assert(sizeof(JobT<float>) == sizeof(JobT<double>),
               "job layouts of both precisions must agree");
 
 struct KArgs {
-  double dx[3];
-  double area_u[3];
-  double vol_u;
   double difmag;
   double dt_val;
   const double* dtp;  // device dt (level path) or 0
@@ -81,7 +91,10 @@ struct KArgs {
   int full_prims;  // 1: write every q/qaux comp; 0: only what the step reads
 };
 
-PCD double get_dt(const KArgs& a) { return a.dtp ? *a.dtp : a.dt_val; }
+template <class J_>
+PCD double get_dt(const J_& J, const KArgs& a) {
+  return J.ds ? J.ds->dt : (a.dtp ? *a.dtp : a.dt_val);
+}
 
 PCD int tslot(int d, int t) { return (t < d) ? t : t - 1; }
 
@@ -92,10 +105,11 @@ PCD int tslot(int d, int t) { return (t < d) ? t : t - 1; }
 // or a tile of 2^lx x 2^ly x 2^lz cells (lx + ly + lz = log2 BLOCK).
 struct Sched {
   const Job* jobs;
-  const int* blkmap;    // logical block -> job (null: a single job)
-  const int* blkstart;  // job -> its first logical block
+  const int* blkmap;    // logical block -> job (null: search blkstart)
+  const int* blkstart;  // job -> its first logical block (null: one job)
   int nlog;             // logical blocks (0: one per workgroup)
   int lx, ly, lz;       // tile shape (log2), lx = -1 for runs
+  int njobs;            // jobs in blkstart (used without blkmap)
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
@@ -121,6 +135,16 @@ PCD JobT<FP> load_job(const Sched& s, int lb, int& local) {
   int b = 0, start = 0;
   if (s.blkmap) {
     b = ((CInt*)s.blkmap)[lb];
+    start = ((CInt*)s.blkstart)[b];
+  } else if (s.blkstart) {
+    // last job whose first block is <= lb (uniform: scalar loads)
+    int lo = 0, hi = s.njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (((CInt*)s.blkstart)[mid] <= lb) lo = mid;
+      else hi = mid - 1;
+    }
+    b = lo;
     start = ((CInt*)s.blkstart)[b];
   }
   local = (lb - start) * BLOCK + threadIdx.x;
@@ -396,19 +420,19 @@ PCD void b_divu(const Sched& sc, const KArgs& ka, int lb_) {
       (q(i, j, k, QU) - q(i - 1, j, k, QU) + q(i, j, k - 1, QU) -
        q(i - 1, j, k - 1, QU) + q(i, j - 1, k, QU) - q(i - 1, j - 1, k, QU) +
        q(i, j - 1, k - 1, QU) - q(i - 1, j - 1, k - 1, QU)) /
-      ka.dx[0];
+      J.dx[0];
   const double vy =
       0.25 *
       (q(i, j, k, QV) - q(i, j - 1, k, QV) + q(i, j, k - 1, QV) -
        q(i, j - 1, k - 1, QV) + q(i - 1, j, k, QV) - q(i - 1, j - 1, k, QV) +
        q(i - 1, j, k - 1, QV) - q(i - 1, j - 1, k - 1, QV)) /
-      ka.dx[1];
+      J.dx[1];
   const double wz =
       0.25 *
       (q(i, j, k, QW) - q(i, j, k - 1, QW) + q(i - 1, j, k, QW) -
        q(i - 1, j, k - 1, QW) + q(i, j - 1, k, QW) - q(i, j - 1, k - 1, QW) +
        q(i - 1, j - 1, k, QW) - q(i - 1, j - 1, k - 1, QW)) /
-      ka.dx[2];
+      J.dx[2];
   J.divu(i, j, k, 0) = ux + vy + wz;
 }
 template <class FP = double, bool PERSIST = false>
@@ -455,7 +479,7 @@ PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const auto& q = J.q;
-  const FP dtdx = get_dt(ka) / ka.dx[D];
+  const FP dtdx = get_dt(J, ka) / J.dx[D];
   constexpr int QUN = QU + D;
   constexpr int QUT = (D == 0) ? QV : ((D == 1) ? QW : QU);
   constexpr int QUTT = (D == 0) ? QW : ((D == 1) ? QU : QV);
@@ -624,7 +648,7 @@ PCD void b_trace_plm(const Sched& sc, const KArgs& ka, int lb_) {
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const auto& q = J.q;
-  const FP dtdx = get_dt(ka) / ka.dx[D];
+  const FP dtdx = get_dt(J, ka) / J.dx[D];
   const FP flat = ka.use_flattening ? flat_cell(J, i, j, k) : FP(1.0);
   auto slope = [&](int comp) {
     FP s[5];
@@ -730,7 +754,7 @@ PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRANS_XY + 2 * D + tslot(D, T), J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const FP cdtdx = FP(1.0) / FP(3.0) * get_dt(ka) / ka.dx[T];
+  const FP cdtdx = FP(1.0) / FP(3.0) * get_dt(J, ka) / J.dx[T];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   FaceState<NS, FP> s, L, R;
@@ -898,7 +922,7 @@ PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
                                     dv(a1, b1, e1, 0) + dv(a2, b2, e2, 0)));
     const bool at_bndry = (idx == J.dlo[D] && J.bclo[D] == BC_NOSLIPWALL) ||
                           (idx == J.dhi[D] + 1 && J.bchi[D] == BC_NOSLIPWALL);
-    const double dxd = ka.dx[D];
+    const double dxd = J.dx[D];
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) {
       if (m == UTEMP) continue;
@@ -918,7 +942,7 @@ PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
     for (int s = 0; s < NS; ++s) fl[UFS + s] *= fac;
   }
   // pc_ext_flx (Source/Hydro.H:854-865)
-  const double area = J.area[D].p ? J.area[D](i, j, k, 0) : ka.area_u[D];
+  const double area = J.area[D].p ? J.area[D](i, j, k, 0) : J.area_u[D];
   const auto& fo = J.flx[D];
 #pragma unroll
   for (int m = 0; m < NVAR; ++m) fo(i, j, k, m) = fl[m] * area;
@@ -938,10 +962,10 @@ PCD void b_final(const Sched& sc, const KArgs& ka, int lb_) {
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   if (loworder_face(J, D, idx)) return;
-  const FP dt = get_dt(ka);
+  const FP dt = get_dt(J, ka);
   const FP hdt = FP(0.5) * dt;
-  const FP c0 = FP(0.5) * dt / ka.dx[T0];
-  const FP c1 = FP(0.5) * dt / ka.dx[T1];
+  const FP c0 = FP(0.5) * dt / J.dx[T0];
+  const FP c1 = FP(0.5) * dt / J.dx[T1];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   FaceState<NS, FP> L, R, s;
@@ -975,7 +999,7 @@ PCD void b_final_loworder(const Sched& sc, const KArgs& ka, int lb_) {
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   if (!loworder_face(J, D, idx)) return;
-  const FP dtdx = get_dt(ka) / ka.dx[D];
+  const FP dtdx = get_dt(J, ka) / J.dx[D];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   FaceState<NS, FP> L, R, unused;
@@ -1001,9 +1025,9 @@ PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
                          double* h) {
   constexpr int NVAR = 7 + NS;
   auto A = [&](int d, int a, int b, int c) {
-    return J.area[d].p ? J.area[d](a, b, c, 0) : ka.area_u[d];
+    return J.area[d].p ? J.area[d](a, b, c, 0) : J.area_u[d];
   };
-  const double v = J.vol.p ? J.vol(i, j, k, 0) : ka.vol_u;
+  const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
   const V& q1 = J.qec[0];
   const V& q2 = J.qec[1];
   const V& q3 = J.qec[2];

@@ -7,13 +7,6 @@
 
 namespace pcd {
 
-// Device-resident time-step state.
-struct DtState {
-  double dt;                 // dt of the next step
-  unsigned long long minbits;  // running min of dx/(c+|u|) (positive double bits)
-  int bad;                   // NaN/Inf seen in the new state
-  int small;                 // dt fell below dt_cutoff
-};
 
 struct LevelArgs {
   int periodic_self;
@@ -50,12 +43,16 @@ PCD void b_ghost(const Sched& sc, const KArgs& ka, int lb_, LevelArgs la) {
   if (i >= J.lo[0] && i <= J.hi[0] && j >= J.lo[1] && j <= J.hi[1] &&
       k >= J.lo[2] && k <= J.hi[2])
     return;
+  // periodic_self: 0 = boxes tile the level domain, 1 = every box is its own
+  // periodic domain, 2 = every box is its own domain with its own BCs
   int D0[3], D1[3], bl[3], bh[3];
   for (int d = 0; d < 3; ++d) {
     D0[d] = la.periodic_self ? J.lo[d] : la.dlo[d];
     D1[d] = la.periodic_self ? J.hi[d] : la.dhi[d];
-    bl[d] = la.periodic_self ? BC_INTERIOR : la.bclo[d];
-    bh[d] = la.periodic_self ? BC_INTERIOR : la.bchi[d];
+    bl[d] = la.periodic_self == 1 ? BC_INTERIOR
+                                  : (la.periodic_self == 2 ? J.bclo[d] : la.bclo[d]);
+    bh[d] = la.periodic_self == 1 ? BC_INTERIOR
+                                  : (la.periodic_self == 2 ? J.bchi[d] : la.bchi[d]);
   }
   int r0, r1, r2;
   const int s0 = ghost_src(i, D0[0], D1[0], bl[0], bh[0], r0);
@@ -162,16 +159,17 @@ PCD void reset_and_temp(double* S, const typename EOS::Thermo& th) {
 // (computeTemp in do_sdc_iteration and in post_timestep), NaN check and the
 // CFL partial minimum on the new state.
 template <class EOS>
-PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState* ds) {
+PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState*& ds) {
   constexpr int NS = EOS::NS;
   constexpr int NVAR = 7 + NS;
   int local;
   const Job J = load_job(sc, lb_, local);
+  if (J.ds) ds = J.ds;  // this job's dt state (the same for the whole block)
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   double mydt = 1.7976931348623157e308;
   if (region_ijk(sc, lo, n, local, i, j, k)) {
-    const double dt = get_dt(ka);
+    const double dt = get_dt(J, ka);
     double h[NVAR];
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) h[m] = 0.0;
@@ -191,7 +189,7 @@ PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState* ds) {
       bad |= !isfinite(S[m]);
     }
     if (bad) ds->bad = 1;
-    mydt = cell_estdt<EOS>(S, th, ka.dx);
+    mydt = cell_estdt<EOS>(S, th, J.dx);
   }
   return mydt;
 }
@@ -211,10 +209,11 @@ __global__ void __launch_bounds__(BLOCK)
 
 // estTimeStep partial minimum over the valid cells (no update)
 template <class EOS>
-PCD double b_estdt(const Sched& sc, const KArgs& ka, int lb_) {
+PCD double b_estdt(const Sched& sc, const KArgs& ka, int lb_, DtState*& ds) {
   constexpr int NS = EOS::NS;
   int local;
   const Job J = load_job(sc, lb_, local);
+  if (J.ds) ds = J.ds;
   int lo[3], n[3], i, j, k;
   stage_region(ST_UPDATE, J.lo, J.hi, lo, n);
   double mydt = 1.7976931348623157e308;
@@ -225,7 +224,7 @@ PCD double b_estdt(const Sched& sc, const KArgs& ka, int lb_) {
     double Y[NS];
     state_Y<NS>(S, Y);
     const typename EOS::Thermo th(Y);
-    mydt = cell_estdt<EOS>(S, th, ka.dx);
+    mydt = cell_estdt<EOS>(S, th, J.dx);
   }
   return mydt;
 }
@@ -236,9 +235,9 @@ __global__ void __launch_bounds__(BLOCK)
   double m = 1.7976931348623157e308;
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      m = mn(m, b_estdt<EOS>(sc, ka, lb_));
+      m = mn(m, b_estdt<EOS>(sc, ka, lb_, ds));
   } else {
-    m = b_estdt<EOS>(sc, ka, first_block());
+    m = b_estdt<EOS>(sc, ka, first_block(), ds);
   }
   block_min_to(m, &ds->minbits);
 }
@@ -246,8 +245,7 @@ __global__ void __launch_bounds__(BLOCK)
 // estTimeStep (Source/PeleC.cpp:829-961) from the partial minimum, then
 // computeNewDt (:964-1034, single level, no stop_time) or, with init = 1,
 // initialTimeStep (:811-826).
-__global__ void k_dt_finalize(DtState* ds, LevelArgs la, int init) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+PCD void dt_finalize(DtState* ds, const LevelArgs& la, int init) {
   double est;
   if (la.fixed_dt > 0.0) {
     est = la.fixed_dt;
@@ -269,6 +267,18 @@ __global__ void k_dt_finalize(DtState* ds, LevelArgs la, int init) {
   ds->dt = dtn;
   if (dtn < la.dt_cutoff) ds->small = 1;
   ds->minbits = 0x7FEFFFFFFFFFFFFFull;
+}
+
+__global__ void k_dt_finalize(DtState* ds, LevelArgs la, int init) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  dt_finalize(ds, la, init);
+}
+
+// the same for the requests ids[0..n) of a batch, one thread each
+__global__ void k_dt_finalize_list(DtState* base, const int* ids, int n,
+                                   LevelArgs la, int init) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dt_finalize(base + ids[t], la, init);
 }
 
 // primitive -> conservative (pc_initdata pattern; see workloads.py)

@@ -1,0 +1,247 @@
+"""Ensemble serving on one GPU: requests (3D shock tubes and premixed slabs
+of different sizes and step counts) arrive over time, are admitted while
+the device memory budget allows, and advance together: every tick steps
+each admitted request once, with one launch per stage over all the requests
+of one mechanism (continuous batching: a request that arrives while others
+run joins at the next tick, a finished one leaves and frees its memory).
+
+Request fields are generated on the device from the request's own seed
+(torch's counter-based CUDA generator), so a request starts from the same
+bits whether it runs alone or in a mix.
+"""
+import ctypes as C
+import math
+import time
+from collections import deque
+
+import numpy as np
+
+from . import _lib
+from .ctypes_types import Box, make_box, make_params
+from .hydro import HydroContext, _stream_ptr, a4_from_tensor
+from .workloads import BC, burnt_h2_air, unburnt_h2_air
+
+check = _lib.check
+TUBE, SLAB = 0, 1          # request kinds: gamma-law shock tube, premixed slab
+NSPEC = {TUBE: 1, SLAB: 9}
+
+
+class Batch:
+    """A pc_batch_t: requests of one mechanism resident on the device."""
+
+    def __init__(self, ctx, params, mixed, budget_bytes=0, share=None,
+                 max_slots=512):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.prm = make_params(**params)
+        h = C.c_void_p()
+        check(self.L.pc_batch_create_v1(ctx.h, C.byref(self.prm), 1 if mixed else 0,
+                                        int(budget_bytes), share.h if share else None,
+                                        int(max_slots), C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.pc_batch_destroy_v1(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        return check(rc, self.ctx.h)
+
+    def slot_bytes(self, n):
+        return self.L.pc_batch_slot_bytes_v1(self.h, make_box([0] * 3, [n - 1] * 3))
+
+    def memory(self):
+        t, u, f = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        self._chk(self.L.pc_batch_memory_v1(self.h, C.byref(t), C.byref(u), C.byref(f)))
+        return t.value, u.value, f.value
+
+    def add(self, n, bclo, bchi, prim, prim_mode, stream=None):
+        slot = C.c_int()
+        dx = (C.c_double * 3)(*([1.0 / n] * 3))
+        self._chk(self.L.pc_batch_add_v1(
+            self.h, make_box([0] * 3, [n - 1] * 3), (C.c_int * 3)(*bclo),
+            (C.c_int * 3)(*bchi), dx, a4_from_tensor(prim, [0, 0, 0]),
+            int(prim_mode), _stream_ptr(stream), C.byref(slot)))
+        return slot.value
+
+    def step(self, slots, stream=None):
+        if not slots:
+            return
+        arr = (C.c_int * len(slots))(*slots)
+        self._chk(self.L.pc_batch_step_v1(self.h, arr, len(slots), _stream_ptr(stream)))
+
+    def read(self, slot, n):
+        out = np.empty((7 + self.ctx.nspec,) + (n + 8,) * 3)
+        self._chk(self.L.pc_batch_read_v1(self.h, slot, out.ctypes.data))
+        return out
+
+    def status(self, slot):
+        dt, bad, small = C.c_double(), C.c_int(), C.c_int()
+        rc = self.L.pc_batch_status_v1(self.h, slot, C.byref(dt), C.byref(bad),
+                                       C.byref(small))
+        return rc, dt.value, bad.value, small.value
+
+    def remove(self, slot, stream=None):
+        self._chk(self.L.pc_batch_remove_v1(self.h, slot, _stream_ptr(stream)))
+
+
+class Request:
+    def __init__(self, rid, kind, n, nsteps, ratio, phi, t_arrive, seed):
+        self.rid, self.kind, self.n, self.nsteps = rid, kind, n, nsteps
+        self.ratio, self.phi, self.t_arrive, self.seed = ratio, phi, t_arrive, seed
+        self.slot = None
+        self.done = 0
+        self.t_admit = self.t_done = None
+        self.result = None
+
+    @property
+    def cell_updates(self):
+        return self.n ** 3 * self.nsteps
+
+
+def request_stream(count, rate, seed=20261203, n_lo=32, n_hi=128, n_med=64,
+                   steps=(10, 200), ratio=(2.0, 20.0), phi=(0.5, 1.2)):
+    """Poisson arrivals at `rate` per second; half tubes, half slabs; sides
+    lognormal (median n_med, clipped to [n_lo, n_hi]); step counts, pressure
+    ratios and equivalence ratios uniform."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count)) if rate > 0 else np.zeros(count)
+    out = []
+    for r in range(count):
+        kind = TUBE if rng.random() < 0.5 else SLAB
+        n = int(np.clip(round(n_med * math.exp(0.5 * rng.standard_normal())), n_lo, n_hi))
+        out.append(Request(r, kind, n, int(rng.integers(steps[0], steps[1] + 1)),
+                           float(rng.uniform(*ratio)), float(rng.uniform(*phi)),
+                           float(t[r]), seed + 7919 * (r + 1)))
+    return out
+
+
+def request_fields(req):
+    """Primitive fields of a request on the device, and its BCs and mode.
+    Tube: rho, p = (1, 0.1 ratio) for x < 0.5 and (0.125, 0.1) beyond, rho
+    perturbed by a uniform 1e-3 relative noise; Outflow x, periodic y/z
+    (prim mode 0).  Slab: premixed H2/air at phi, T 300 -> 1800 K across a
+    tanh of width 8 cells in z, p = 1 atm, velocity (0, 0, 100) cm/s plus
+    N(0, 5 cm/s) noise; periodic x/y, Outflow z (prim mode 1)."""
+    import torch
+    n = req.n
+    g = torch.Generator(device="cuda")
+    g.manual_seed(req.seed)
+    x = (torch.arange(n, device="cuda", dtype=torch.float64) + 0.5) / n
+    if req.kind == TUBE:
+        P = torch.zeros((6, n, n, n), dtype=torch.float64, device="cuda")
+        left = (x <= 0.5).view(1, 1, n)
+        noise = torch.rand((n, n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+        P[0] = torch.where(left, 1.0, 0.125) * (1.0 + 1e-3 * noise)
+        P[1] = torch.where(left, 0.1 * req.ratio, 0.1).expand(n, n, n)
+        P[5] = 1.0
+        bc = [BC["outflow"], BC["interior"], BC["interior"]]
+        return P, bc, list(bc), 0
+    s = 0.5 * (1.0 + torch.tanh((x - 0.5) / (8.0 / n)))
+    Yu = torch.tensor(unburnt_h2_air(req.phi), device="cuda")
+    Yb = torch.tensor(burnt_h2_air(req.phi), device="cuda")
+    P = torch.empty((14, n, n, n), dtype=torch.float64, device="cuda")
+    P[0] = 1013250.0
+    P[1] = (300.0 + 1500.0 * s).view(n, 1, 1)
+    P[2:5] = torch.randn((3, n, n, n), generator=g, device="cuda", dtype=torch.float64) * 5.0
+    P[4] += 100.0
+    P[5:] = ((1.0 - s).view(1, n, 1, 1) * Yu.view(9, 1, 1, 1)
+             + s.view(1, n, 1, 1) * Yb.view(9, 1, 1, 1))
+    bc = [BC["interior"], BC["interior"], BC["outflow"]]
+    return P, bc, list(bc), 1
+
+
+class Server:
+    """The request loop.  `budget_frac` of the device memory is one arena
+    shared by the tube and slab batches; requests are admitted first come
+    first served while their state and workspace fit."""
+
+    def __init__(self, mixed=True, budget_frac=0.8, budget_bytes=None,
+                 max_slots=512, keep_results=False):
+        import torch
+        self.ctx = {TUBE: HydroContext(1), SLAB: HydroContext(9)}
+        if budget_bytes is None:
+            free, total = torch.cuda.mem_get_info()
+            budget_bytes = int(budget_frac * total)
+            budget_bytes = min(budget_bytes, int(0.95 * free))
+        self.budget = budget_bytes
+        params = dict(ppm_type=1, cfl=0.3)
+        self.batch = {}
+        self.batch[SLAB] = Batch(self.ctx[SLAB], params, mixed, budget_bytes,
+                                 max_slots=max_slots)
+        self.batch[TUBE] = Batch(self.ctx[TUBE], params, mixed,
+                                 share=self.batch[SLAB], max_slots=max_slots)
+        self.keep_results = keep_results
+
+    def close(self):
+        for b in self.batch.values():
+            b.close()
+        for c in self.ctx.values():
+            c.close()
+
+    def run(self, requests, realtime=True, on_tick=None):
+        """Serve `requests` (sorted by arrival).  realtime: arrivals follow
+        their timestamps on the wall clock; else all are queued at t = 0.
+        Returns the list of served requests with t_admit / t_done set."""
+        import torch
+        pending = deque(sorted(requests, key=lambda r: r.t_arrive))
+        waiting = deque()
+        active = []
+        served = []
+        t0 = time.perf_counter()
+        ticks = 0
+        while pending or waiting or active:
+            now = time.perf_counter() - t0
+            while pending and (not realtime or pending[0].t_arrive <= now):
+                waiting.append(pending.popleft())
+            while waiting:
+                r = waiting[0]
+                b = self.batch[r.kind]
+                if b.slot_bytes(r.n) > b.memory()[2]:
+                    break
+                P, bclo, bchi, mode = request_fields(r)
+                r.slot = b.add(r.n, bclo, bchi, P, mode)
+                del P
+                r.t_admit = time.perf_counter() - t0
+                waiting.popleft()
+                active.append(r)
+            if not active:
+                if pending and realtime:
+                    time.sleep(max(0.0, pending[0].t_arrive - (time.perf_counter() - t0)))
+                continue
+            for kind, b in self.batch.items():
+                b.step([r.slot for r in active if r.kind == kind])
+            ticks += 1
+            for r in active:
+                r.done += 1
+            finished = [r for r in active if r.done == r.nsteps]
+            if finished:
+                torch.cuda.synchronize()
+                t = time.perf_counter() - t0
+                for r in finished:
+                    r.t_done = t
+                    b = self.batch[r.kind]
+                    if self.keep_results:
+                        r.result = b.read(r.slot, r.n)
+                    r.status = b.status(r.slot)[0]
+                    b.remove(r.slot)
+                    served.append(r)
+                active = [r for r in active if r.done < r.nsteps]
+            if on_tick:
+                on_tick(ticks, active)
+        self.wall = time.perf_counter() - t0
+        self.ticks = ticks
+        return served
+
+
+def summarize(served, wall):
+    lat = np.array([r.t_done - r.t_arrive for r in served])
+    cells = sum(r.cell_updates for r in served)
+    return {"requests": len(served), "wall_s": wall,
+            "mcell_updates_per_s": cells / wall / 1e6,
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                           "p99": float(np.percentile(lat, 99) * 1e3),
+                           "mean": float(lat.mean() * 1e3)},
+            "queue_ms": {"p50": float(np.percentile([r.t_admit - r.t_arrive for r in served], 50) * 1e3),
+                         "p99": float(np.percentile([r.t_admit - r.t_arrive for r in served], 99) * 1e3)}}

@@ -40,12 +40,14 @@ def unburnt_h2_air(phi):
 
 
 def burnt_h2_air(phi):
-    """Complete-combustion products of a lean phi H2/air mixture with small
-    radical pools so that every species is present ("equilibrium-like")."""
+    """Complete-combustion products of a phi H2/air mixture (excess O2 when
+    lean, excess H2 when rich) with small radical pools so that every
+    species is present ("equilibrium-like")."""
     X = np.zeros(9)
-    X[2] = 2.0 * phi           # H2O
-    X[1] = max(1.0 - phi, 0.0)  # excess O2
-    X[8] = 3.76                 # N2
+    X[2] = 2.0 * min(phi, 1.0)       # H2O
+    X[1] = max(1.0 - phi, 0.0)       # excess O2
+    X[0] = 2.0 * max(phi - 1.0, 0.0)  # excess H2
+    X[8] = 3.76                      # N2
     X = X / X.sum()
     X += np.array([1e-4, 0.0, 0.0, 1e-5, 1e-5, 1e-3, 1e-6, 1e-7, 0.0])
     return mole_to_mass(X / X.sum())

@@ -1,0 +1,83 @@
+"""Ensemble serving: requests of different sizes, mechanisms and step
+counts batched into shared launches.  A request's result must not depend on
+the mix it ran in (bitwise), the batch path must equal the level stepper on
+the same problem (which is bitwise the oracle), and admission must respect
+the memory budget."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from particleclem_amd import serving as sv
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(count, seed=11):
+    return sv.request_stream(count, rate=0, seed=seed, n_lo=8, n_hi=20,
+                             n_med=12, steps=(3, 9))
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_each_result_same_alone_and_in_the_mix(cuda, mixed):
+    reqs = small_stream(8)
+    srv = sv.Server(mixed=mixed, budget_bytes=4 << 30, keep_results=True)
+    mix = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    srv.close()
+    assert len(mix) == len(reqs)
+    for r in reqs:
+        one = sv.Request(r.rid, r.kind, r.n, r.nsteps, r.ratio, r.phi, 0.0, r.seed)
+        s1 = sv.Server(mixed=mixed, budget_bytes=1 << 30, keep_results=True)
+        got = s1.run([one], realtime=False)[0]
+        s1.close()
+        assert mix[r.rid].status == 0 and got.status == 0
+        assert np.array_equal(got.result, mix[r.rid].result), (r.rid, r.kind, r.n)
+    print(f"mixed={mixed}: {len(reqs)} requests bitwise equal alone and in the mix")
+
+
+def test_batch_equals_level_stepper(cuda):
+    """One tube and one slab request through the FP64 batch against
+    pc_level_* on the same box, BCs and fields."""
+    import torch
+    from particleclem_amd.hydro import DeviceLevel, HydroContext, a4_from_tensor
+    reqs = [sv.Request(0, sv.TUBE, 12, 5, 10.0, 0.8, 0.0, 5),
+            sv.Request(1, sv.SLAB, 10, 4, 2.0, 1.1, 0.0, 6)]
+    srv = sv.Server(mixed=False, budget_bytes=1 << 30, keep_results=True)
+    out = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    srv.close()
+    for r in reqs:
+        P, bclo, bchi, mode = sv.request_fields(r)
+        ctx = HydroContext(sv.NSPEC[r.kind])
+        lev = DeviceLevel(ctx, [([0, 0, 0], [r.n - 1] * 3)], [0, 0, 0], [r.n - 1] * 3,
+                          bclo, bchi, [1.0 / r.n] * 3, params=dict(ppm_type=1, cfl=0.3))
+        lev._chk(lev.L.pc_level_prim_to_cons_v1(lev.h, 0, a4_from_tensor(P, [0, 0, 0]),
+                                                 mode, None))
+        lev.init_dt()
+        lev.step(r.nsteps)
+        ref = lev.read_box(0)
+        nd = int(np.count_nonzero(ref[:, 4:-4, 4:-4, 4:-4] != out[r.rid].result[:, 4:-4, 4:-4, 4:-4]))
+        print(f"request {r.rid} kind {r.kind}: {nd} values differ from the level stepper")
+        assert nd == 0
+        lev.close()
+        ctx.close()
+
+
+def test_admission_under_a_tight_budget(cuda):
+    reqs = small_stream(10, seed=3)
+    probe = sv.Server(mixed=True, budget_bytes=64 << 20)
+    per = max(probe.batch[r.kind].slot_bytes(r.n) for r in reqs)
+    probe.close()
+    budget = int(2.5 * per)
+    srv = sv.Server(mixed=True, budget_bytes=budget)
+    peak = [0]
+
+    def tick(_, active):
+        used = srv.batch[sv.SLAB].memory()[1]
+        peak[0] = max(peak[0], used)
+        assert used <= budget
+
+    done = srv.run(reqs, realtime=False, on_tick=tick)
+    srv.close()
+    assert len(done) == len(reqs) and all(r.status == 0 for r in done)
+    print(f"budget {budget >> 20} MiB, peak use {peak[0] >> 20} MiB, "
+          f"{len(done)} requests served")
