@@ -227,6 +227,7 @@ static int ctx_job(pc_ctx* c, const Job& J, hipStream_t st, Sched* sc,
   c->jnext = (c->jnext + 1) % pc_ctx::RING;
   if (c->jev_used[s]) HIPCHK(c, hipEventSynchronize(c->jev[s]));
   c->hjobs[s] = J;
+  c->hjobs[s].dtp = &c->djobs[s].dt_val;  // the call's dt travels in its slot
   HIPCHK(c, hipMemcpyAsync(c->djobs + s, c->hjobs + s, sizeof(Job),
                            hipMemcpyHostToDevice, st));
   *sc = one_job_sched(c->djobs + s);
@@ -379,9 +380,8 @@ static KArgs umdrv_args(const double* dx, double dt, int plm_iorder,
   KArgs ka;
   memset(&ka, 0, sizeof(ka));
   (void)dx;
+  (void)dt;
   ka.difmag = difmag;
-  ka.dt_val = dt;
-  ka.dtp = nullptr;
   ka.plm_iorder = plm_iorder;
   ka.use_flattening = use_flattening;
   return ka;
@@ -569,6 +569,7 @@ int pc_umdrv_v1(pc_ctx_t* c, double /*time*/, pc_box_t bx, const int* domlo,
   if ((rc = ctx_scratch(c, L.total * sizeof(double), st))) return rc;
   if ((rc = finish_umdrv_job(c, J, b, c->ws, flx, a, vol, uout))) return rc;
   set_geom(J, dx);
+  J.dt_val = dt;
   const KArgs ka = umdrv_args(dx, dt, plm_iorder, use_flattening, difmag);
   Sched sc;
   int slot;
@@ -619,6 +620,7 @@ int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
   if ((rc = finish_umdrv_job(c, J, b, c->ws + qpart, flx, a, vol, hyd_src)))
     return rc;
   set_geom(J, dx);
+  J.dt_val = dt;
   KArgs ka = umdrv_args(dx, dt, prm->plm_iorder, prm->use_flattening,
                         prm->difmag);
   // srctoprim reads qaux's dp/de and dp/drho, which ctoprim only writes
@@ -885,6 +887,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
       J.qe[dd] = bx.hi[dd] + 5;
     }
     set_geom(J, d->dx);
+    J.dtp = &L->ds->dt;
     J.U = fab_view(L->state + L->off[b], growb(bx, 4));
     const WsLayout W = ws_layout(bx, ns, true, false, true);
     ws_fill_job(J, L->ws + wsoff[b], W, true, false, true);
@@ -958,7 +961,6 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   L->la.dt_cutoff = prm->dt_cutoff;
   memset(&L->ka, 0, sizeof(L->ka));
   L->ka.difmag = prm->difmag;
-  L->ka.dtp = &L->ds->dt;
   L->ka.plm_iorder = prm->plm_iorder;
   L->ka.use_flattening = prm->use_flattening;
   DtState z;
@@ -1638,6 +1640,7 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
   J.vol = null_view();
   J.upd = null_view();
   J.ds = B->ds + id;
+  J.dtp = &B->ds[id].dt;
   for (int d = 0; d < 3; ++d) {
     auto lo_order = [](int b) { return b == PC_BC_INFLOW || b == PC_BC_USERBC; };
     if (lo_order(bclo[d]) || lo_order(bchi[d])) B->lowo[d] = 1;

@@ -66,7 +66,9 @@ struct JobT {
   int qe[3];                        // exclusive end of the q box (contains)
   double dx[3];                     // cell size
   double area_u[3], vol_u;          // uniform face areas / volume (no area/vol)
-  DtState* ds;                      // dt of this job's step (0: from KArgs)
+  const double* dtp;                // device address of this job's dt
+  double dt_val;                    // the dt of a box-driver call (dtp -> here)
+  DtState* ds;                      // dt state the CFL minimum goes to (or 0)
   VT<double> U;                     // state with >= 4 ghosts (uin)
   VT<FP> q, qa, srcq;               // primitives on grow(bx,4); srcq.p may be 0
   VT<FP> tr[3];                     // traces per direction
@@ -83,8 +85,6 @@ static_assert(sizeof(JobT<float>) == sizeof(JobT<double>),
 
 struct KArgs {
   double difmag;
-  double dt_val;
-  const double* dtp;  // device dt (level path) or 0
   int plm_iorder;
   int use_flattening;
   int nblocks;
@@ -92,8 +92,8 @@ struct KArgs {
 };
 
 template <class J_>
-PCD double get_dt(const J_& J, const KArgs& a) {
-  return J.ds ? J.ds->dt : (a.dtp ? *a.dtp : a.dt_val);
+PCD double get_dt(const J_& J, const KArgs&) {
+  return *(const __attribute__((address_space(1))) double*)J.dtp;
 }
 
 PCD int tslot(int d, int t) { return (t < d) ? t : t - 1; }
@@ -315,7 +315,7 @@ PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
   double Y[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    double y = u(i, j, k, UFS + s) / rho;
+    double y = div_by((double)u(i, j, k, UFS + s), rho, rhoinv);
     if ((-1e-4 * DEPS < y) && (y < 1e-4 * DEPS)) y = 0.0;
     Y[s] = y;
   }

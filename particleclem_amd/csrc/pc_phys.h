@@ -343,6 +343,17 @@ struct EosLiDryer {
   };
 };
 
+// Quotient a / b given rb = 1 / b (both correctly rounded): Markstein's
+// correction of a * rb, q0 = a*rb, q = q0 + rb*(a - b*q0) with the remainder
+// exact by FMA.  It equals the correctly rounded a / b whenever neither
+// overflows nor underflows, so one division serves every quotient by the
+// same divisor (the species fractions of a state) at three operations each.
+template <class FP>
+PCD FP div_by(FP a, FP b, FP rb) {
+  const FP q0 = a * rb;
+  return fma(fma(-q0, b, a), rb, q0);
+}
+
 // Floors of the reference in precision FP: numeric_limits<FP>::min(), and
 // very_small_num (epsilon * 1e-100), which underflows in FP32 and becomes
 // FLT_MIN there.
@@ -643,20 +654,22 @@ PCD RiemannOut<EOS, FP> riemann(FP rl, FP ul, FP vl, FP v2l,
   uo = tiny ? FP(0.5) * (ul + ur) : uo;
   po = tiny ? FP(0.5) * (pl + pr) : po;
   FP gmf[NS];
+  const FP rro = FP(1.0) / ro;
 #pragma unroll
-  for (int n = 0; n < NS; n++) gmf[n] = rspo[n] / ro;
+  for (int n = 0; n < NS; n++) gmf[n] = div_by(rspo[n], ro, rro);
   const FP co = EOS::RPY2Cs(ro, po, gmf);
   const FP drho = (pstar - po) / (co * co);
   FP rspstar[NS];
   FP rstar = FP(0.0);
 #pragma unroll
   for (int n = 0; n < NS; n++) {
-    const FP spon = rspo[n] / ro;
+    const FP spon = gmf[n];  // rspo[n] / ro
     rspstar[n] = mx(FP(0.0), rspo[n] + drho * spon);
     rstar += rspstar[n];
   }
+  const FP rrstar = FP(1.0) / rstar;
 #pragma unroll
-  for (int n = 0; n < NS; n++) gmf[n] = rspstar[n] / rstar;
+  for (int n = 0; n < NS; n++) gmf[n] = div_by(rspstar[n], rstar, rrstar);
   const FP cstar = EOS::RPY2Cs(rstar, pstar, gmf);
   const FP sgnm = copysign(FP(1.0), ustar);
   FP spout = co - sgnm * uo;
@@ -692,8 +705,9 @@ PCD RiemannOut<EOS, FP> riemann(FP rl, FP ul, FP vl, FP v2l,
   qp_ = outside ? po : qp_;
   qiu = inside ? ustar : qiu;
   qp_ = inside ? pstar : qp_;
+  const FP rrgd = FP(1.0) / rgd;
 #pragma unroll
-  for (int n = 0; n < NS; n++) gmf[n] = rspgd[n] / rgd;
+  for (int n = 0; n < NS; n++) gmf[n] = div_by(rspgd[n], rgd, rrgd);
   const FP ge = EOS::RYP2E(rgd, gmf, qp_);
   const FP regd = rgd * ge;
   o.game_gd = qp_ / regd + FP(1.0);
@@ -718,21 +732,33 @@ PCD RiemannOut<EOS, FP> riemann(FP rl, FP ul, FP vl, FP v2l,
 // ------------------------------------------------------ cmpflx BCs (A9) --
 // Source/Godunov.H:378-418: wall/symmetry mirror or outflow copy at domain
 // faces, on the normal-frame states (rho, un, ut, utt, p) of both sides.
+// The empty asm statement ends the left-face branch differently from the
+// right-face one, so the compiler does not merge the two symmetric bodies
+// into one block that addresses the states through pointers (that would put
+// all ten values in scratch memory).
 template <class FP>
 PCD void face_bc(int idx, int domlo, int domhi, int bclo, int bchi,
                  FP& rl, FP& ul, FP& vl, FP& v2l, FP& pl,
                  FP& rr, FP& ur, FP& vr, FP& v2r, FP& pr) {
+  auto wall = [](int b) {
+    return b == BC_NOSLIPWALL || b == BC_SLIPWALL || b == BC_SYMMETRY;
+  };
   if (idx == domlo) {
-    if (bclo == BC_NOSLIPWALL || bclo == BC_SLIPWALL || bclo == BC_SYMMETRY) {
-      ul = -ur; vl = vr; v2l = v2r; pl = pr; rl = rr;
-    } else if (bclo == BC_OUTFLOW) {
-      ul = ur; vl = vr; v2l = v2r; pl = pr; rl = rr;
+    if (wall(bclo) || bclo == BC_OUTFLOW) {
+      ul = wall(bclo) ? -ur : ur;
+      vl = vr;
+      v2l = v2r;
+      pl = pr;
+      rl = rr;
+      asm volatile("");
     }
   } else if (idx == domhi + 1) {
-    if (bchi == BC_NOSLIPWALL || bchi == BC_SLIPWALL || bchi == BC_SYMMETRY) {
-      ur = -ul; vr = vl; v2r = v2l; pr = pl; rr = rl;
-    } else if (bchi == BC_OUTFLOW) {
-      ur = ul; vr = vl; v2r = v2l; pr = pl; rr = rl;
+    if (wall(bchi) || bchi == BC_OUTFLOW) {
+      ur = wall(bchi) ? -ul : ul;
+      vr = vl;
+      v2r = v2l;
+      pr = pl;
+      rr = rl;
     }
   }
 }
@@ -800,12 +826,13 @@ PCD void transdo_side(const FaceState<NS, FP>& s, const DFlux<NS, FP>& D, FP gam
                       FP cdtdx, FaceState<NS, FP>& o) {
   const FP flxrho = D.d[0], flxu = D.d[1], flxv = D.d[2], flxw = D.d[3],
                flxe = D.d[4];
+  const FP rrnew = s.rho - flxrho;
+  const FP rrinv = FP(1.0) / rrnew;
 #pragma unroll
   for (int n = 0; n < NS; n++) {
     const FP rr = s.rho;
-    const FP rrnew = rr - flxrho;
     const FP compo = rr * s.Y[n] - D.ds[n];
-    o.Y[n] = compo / rrnew;
+    o.Y[n] = div_by(compo, rrnew, rrinv);
   }
   const FP dAup = D.pgp * D.ugp - D.pgm * D.ugm;
   const FP pav = FP(0.5) * (D.pgp + D.pgm);
@@ -823,11 +850,11 @@ PCD void transdo_side(const FaceState<NS, FP>& s, const DFlux<NS, FP>& D, FP gam
   const FP rwnewr = rwr - flxw;
   const FP renewr = rer - flxe;
   o.rho = rrnewr;
-  o.u = runewr / rrnewr;
-  o.v = rvnewr / rrnewr;
-  o.w = rwnewr / rrnewr;
-  const FP rhoekinr =
-      FP(0.5) * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
+  o.u = div_by(runewr, rrnewr, rrinv);
+  o.v = div_by(rvnewr, rrnewr, rrinv);
+  o.w = div_by(rwnewr, rrnewr, rrinv);
+  const FP rhoekinr = div_by(
+      FP(0.5) * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr), rrnewr, rrinv);
   o.rhoe = renewr - rhoekinr;
   const FP pnewr = s.p - cdtdx * (dAup + pav * dAu * (gamc - FP(1.0)));
   o.p = mx(pnewr, vsmall<FP>());
@@ -845,10 +872,11 @@ PCD void transdd_side(const FaceState<NS, FP>& s, const FP* dd,
                flxe = dd[4];
   const FP rrr = s.rho;
   const FP rrnewr = rrr - flxrho;
+  const FP rrinv = FP(1.0) / rrnewr;
 #pragma unroll
   for (int n = 0; n < NS; n++) {
     const FP compo = rrr * s.Y[n] - dds[n];
-    o.Y[n] = compo / rrnewr + hdt * src[6 + n];
+    o.Y[n] = div_by(compo, rrnewr, rrinv) + hdt * src[6 + n];
   }
   const FP rur = rrr * s.u;
   const FP rvr = rrr * s.v;
@@ -860,11 +888,11 @@ PCD void transdd_side(const FaceState<NS, FP>& s, const FP* dd,
   const FP rwnewr = rwr - flxw;
   const FP renewr = rer - flxe;
   o.rho = rrnewr + hdt * src[0];
-  o.u = runewr / rrnewr + hdt * src[1];
-  o.v = rvnewr / rrnewr + hdt * src[2];
-  o.w = rwnewr / rrnewr + hdt * src[3];
-  const FP rhoekinr =
-      FP(0.5) * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
+  o.u = div_by(runewr, rrnewr, rrinv) + hdt * src[1];
+  o.v = div_by(rvnewr, rrnewr, rrinv) + hdt * src[2];
+  o.w = div_by(rwnewr, rrnewr, rrinv) + hdt * src[3];
+  const FP rhoekinr = div_by(
+      FP(0.5) * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr), rrnewr, rrinv);
   o.rhoe = renewr - rhoekinr + hdt * src[4];
   FP p = s.p - pxnew - pynew + hdt * src[5];
   o.p = mx(p, rmin<FP>());
