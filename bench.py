@@ -213,18 +213,35 @@ def serve(args):
     print(json.dumps(out))
 
 
-def cpu_baseline(w, gpu_fabs, dt, steps, threads):
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(w, gpu_fabs, dt, steps, threads, reps=3):
+    """The oracle stepping the same level from the same state and dt:
+    median over `reps` timings of `steps` steps each."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     H = ob.HostLevel(w.nspec, w.boxes, w.domlo, w.domhi, w.bclo, w.bchi, w.dx,
                      periodic_self=w.periodic_self)
-    for b, f in enumerate(gpu_fabs):
-        H.fab(b)[...] = f
     prm = ob.params(**w.params)
-    t0 = time.perf_counter()
-    H.step(prm, steps, dt, threads)
-    t1 = time.perf_counter()
-    return w.valid_cells() * steps / (t1 - t0) / 1e6, t1 - t0
+    times = []
+    for _ in range(reps):
+        for b, f in enumerate(gpu_fabs):
+            H.fab(b)[...] = f
+        t0 = time.perf_counter()
+        H.step(prm, steps, dt, threads)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    return w.valid_cells() * steps / t / 1e6, t
 
 
 def main():
@@ -386,11 +403,14 @@ def main():
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
         v, s = cpu_baseline(w, fabs0, dt_cpu, args.cpu_steps, threads)
+        model, ncpu = host_cpu()
         cpu = {"value": v, "unit": "Mcell-updates/s", "cores": threads,
                "kind": "port",
                "sample": f"{args.cpu_steps} full step(s) of the same level "
-                         f"({cells} cells) from the GPU run's state, "
-                         f"{s:.1f} s, OpenMP over boxes"}
+                         f"({cells} cells) from the GPU run's state and dt, "
+                         f"median of 3 timings ({s:.2f} s), OpenMP over "
+                         f"boxes, {threads} threads",
+               "host_cpu": model, "host_logical_cpus": ncpu}
 
     out = {
         "metric": METRIC, "value": value, "unit": "Mcell-updates/s",
