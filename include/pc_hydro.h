@@ -56,7 +56,8 @@ enum {
   PC_ERR_NAN = -6,         /* NaN/Inf found in the state (okToContinue)     */
   PC_ERR_UNSUPPORTED = -7, /* option outside what this build implements     */
   PC_ERR_NOMEM = -8,       /* device allocation failed                      */
-  PC_ERR_DT = -9           /* dt fell below dt_cutoff (okToContinue)        */
+  PC_ERR_DT = -9,          /* dt fell below dt_cutoff (okToContinue)        */
+  PC_ERR_MOL_IORDER = -10  /* do_mol with mol_iorder not 1 or 2             */
 };
 
 /* Equation of state / mechanism (compile-time NUM_SPECIES in the reference:
@@ -95,6 +96,14 @@ typedef struct pc_params {
   int allow_negative_energy;       /* must be 1 in this build    (1)         */
   int dual_energy_update_E_from_e; /*                            (1)         */
   double dual_energy_eta2;         /*                            (1e-4)      */
+  /* Method-of-lines hydro (Source/Params/_cpp_parameters:62,106,109): a
+   * level with do_mol = 1 advances by do_mol_advance's two-stage Heun step
+   * on PLM-MOL fluxes instead of the CTU step.  As in the reference,
+   * mol_iorder is only validated (1 or 2); the MOL slopes are taken when
+   * plm_iorder != 1 (Source/Diffusion.cpp:371-373 passes plm_iorder). */
+  int do_mol;          /*                                        (0)         */
+  int mol_iorder;      /* 1 or 2                                 (2)         */
+  int use_laxf_flux;   /* 1: Lax-Friedrichs flux in MOL          (0)         */
 } pc_params_t;
 
 void pc_params_default(pc_params_t* p);
@@ -144,6 +153,19 @@ int pc_umdrv_v1(pc_ctx_t* ctx, double time, pc_box_t bx, const int* domlo,
                 int use_hybrid_weno, int weno_scheme, double difmag,
                 const pc_array4_t* flx, const pc_array4_t* a, pc_array4_t vol,
                 double* cflLoc, void* stream);
+
+/* getMOLSrcTerm for one regular box with diffusion off (Source/Diffusion.cpp:
+ * 90-98,136-148,226-240,349-374,509-519,654): primitives of sarr (>= 4 ghost
+ * cells), the MOL hydro fluxes of pc_compute_hyp_mol_flux (Source/MOL.cpp:
+ * 6-190; slopes when prm->plm_iorder != 1, Lax-Friedrichs when
+ * prm->use_laxf_flux) and molsrc = -div(F)/vol on bx (overwritten).  flx[d]
+ * (surroundingNodes(bx,d), NVAR comps) receives the extensive fluxes, or
+ * flx may be NULL; a[d] and vol as pc_umdrv_v1.  prm->mol_iorder must be 1
+ * or 2 (PC_ERR_MOL_IORDER); do_mol itself is not read. */
+int pc_mol_src_box_v1(pc_ctx_t* ctx, pc_box_t bx, pc_array4_t sarr,
+                      pc_array4_t molsrc, const pc_params_t* prm,
+                      const pc_array4_t* flx, const pc_array4_t* a,
+                      pc_array4_t vol, void* stream);
 
 /* One box of construct_hydro_source (Source/Hydro.cpp:73-252): K1 ctoprim
  * and K2 srctoprim on grow(bx,4) into context scratch, then pc_umdrv.
@@ -197,7 +219,9 @@ int pc_level_set_dt_v1(pc_level_t* lev, double dt);
 int pc_level_get_dt_v1(pc_level_t* lev, double* dt);
 /* dt = init_shrink * estTimeStep(state) (or fixed_dt), on the device. */
 int pc_level_init_dt_v1(pc_level_t* lev, void* stream);
-/* Advance nsteps Godunov steps (asynchronous on `stream`). */
+/* Advance nsteps steps (asynchronous on `stream`): Godunov steps, or with
+ * prm->do_mol = 1 at creation the two-stage MOL steps of do_mol_advance
+ * (Source/Advance.cpp:49-227, hydro only: no reactions or other sources). */
 int pc_level_step_v1(pc_level_t* lev, int nsteps, void* stream);
 /* Fill the 4 ghost cells of every box (asynchronous). */
 int pc_level_fill_ghosts_v1(pc_level_t* lev, void* stream);
@@ -213,6 +237,9 @@ enum {
   PC_K_TRACE_Z, PC_K_FLUX1_X, PC_K_FLUX1_Y, PC_K_FLUX1_Z, PC_K_TRANS_XY,
   PC_K_TRANS_XZ, PC_K_TRANS_YX, PC_K_TRANS_YZ, PC_K_TRANS_ZX, PC_K_TRANS_ZY,
   PC_K_FINAL_X, PC_K_FINAL_Y, PC_K_FINAL_Z, PC_K_UPDATE, PC_K_DT,
+  /* method-of-lines step (do_mol = 1): ghost fill and primitives count
+   * under PC_K_GHOST / PC_K_CTOPRIM (both stages), then */
+  PC_K_MOL_X, PC_K_MOL_Y, PC_K_MOL_Z, PC_K_MOL_UPD1, PC_K_MOL_UPD2,
   PC_NKERN
 };
 /* on = 1 also resets the accumulated times. */

@@ -1252,6 +1252,222 @@ struct Hydro {
     return 0;
   }
 
+  // ============================================================ MOL hydro
+  // Normal / transverse velocity components and momentum components of a
+  // direction (Source/MOL.cpp:37-48).
+  static void mol_idx(int dir, int* q_idx, int* f_idx) {
+    const int b0 = dir == 0, b1 = dir == 1, b2 = dir == 2;
+    q_idx[0] = b0 * QU + b1 * QV + b2 * QW;
+    q_idx[1] = b0 * QV + b1 * QU + b2 * QU;
+    q_idx[2] = b0 * QW + b1 * QW + b2 * QV;
+    f_idx[0] = b0 * UMX + b1 * UMY + b2 * UMZ;
+    f_idx[1] = b0 * UMY + b1 * UMX + b2 * UMX;
+    f_idx[2] = b0 * UMZ + b1 * UMZ + b2 * UMY;
+  }
+
+  // ------------------------------------------------- Source/MOL.H:31-120
+  // (regular cells: both neighbours connected, not covered)
+  static void mol_slope(int i, int j, int k, int dir, const int* q_idx,
+                        const A4& q, const A4& qaux, const A4& dq) {
+    const IV iv = ivec(i, j, k);
+    const IV ivm = shift(iv, dir, -1), ivp = shift(iv, dir, 1);
+    double dlft[QVAR], drgt[QVAR];
+    for (int n = 0; n < QVAR; ++n) dlft[n] = drgt[n] = 0.0;
+    dlft[0] = 0.5 * (at(q, iv, QPRES) - at(q, ivm, QPRES)) / at(qaux, iv, QC) -
+              0.5 * at(q, iv, QRHO) * (at(q, iv, q_idx[0]) - at(q, ivm, q_idx[0]));
+    dlft[1] = 0.5 * (at(q, iv, QPRES) - at(q, ivm, QPRES)) / at(qaux, iv, QC) +
+              0.5 * at(q, iv, QRHO) * (at(q, iv, q_idx[0]) - at(q, ivm, q_idx[0]));
+    dlft[2] = at(q, iv, q_idx[1]) - at(q, ivm, q_idx[1]);
+    dlft[3] = at(q, iv, q_idx[2]) - at(q, ivm, q_idx[2]);
+    drgt[0] = 0.5 * (at(q, ivp, QPRES) - at(q, iv, QPRES)) / at(qaux, iv, QC) -
+              0.5 * at(q, iv, QRHO) * (at(q, ivp, q_idx[0]) - at(q, iv, q_idx[0]));
+    drgt[1] = 0.5 * (at(q, ivp, QPRES) - at(q, iv, QPRES)) / at(qaux, iv, QC) +
+              0.5 * at(q, iv, QRHO) * (at(q, ivp, q_idx[0]) - at(q, iv, q_idx[0]));
+    drgt[2] = at(q, ivp, q_idx[1]) - at(q, iv, q_idx[1]);
+    drgt[3] = at(q, ivp, q_idx[2]) - at(q, iv, q_idx[2]);
+    for (int n = 0; n < NS; n++) {
+      dlft[QFS + n] = at(q, iv, QRHO) * at(q, iv, QFS + n) -
+                      at(q, ivm, QRHO) * at(q, ivm, QFS + n) -
+                      at(q, iv, QFS + n) * (at(q, iv, QPRES) - at(q, ivm, QPRES)) /
+                          (at(qaux, iv, QC) * at(qaux, iv, QC));
+      drgt[QFS + n] = at(q, ivp, QRHO) * at(q, ivp, QFS + n) -
+                      at(q, iv, QRHO) * at(q, iv, QFS + n) -
+                      at(q, iv, QFS + n) * (at(q, ivp, QPRES) - at(q, iv, QPRES)) /
+                          (at(qaux, iv, QC) * at(qaux, iv, QC));
+    }
+    for (int n = 0; n < QVAR; n++) {
+      const double dcen = 0.5 * (dlft[n] + drgt[n]);
+      const double dlim = dlft[n] * drgt[n] >= 0.0
+                              ? 2.0 * mn(std::abs(dlft[n]), std::abs(drgt[n]))
+                              : 0.0;
+      at(dq, iv, n) = std::copysign(1.0, dcen) * mn(dlim, std::abs(dcen));
+    }
+  }
+
+  // ------------------------------------------------- Source/Riemann.H:203-305
+  // (no rotating frame: omega = rad = 0)
+  static void laxfriedrich_flux(double rl, double ul, double vl, double v2l,
+                                double pl, const double* spl, double rr,
+                                double ur, double vr, double v2r, double pr,
+                                const double* spr, double& ustar,
+                                double& maxeigval, double& uflx_rho,
+                                double* uflx_rhoY, double& uflx_u,
+                                double& uflx_v, double& uflx_w,
+                                double& uflx_eden, double& uflx_eint) {
+    const double wsmall = DMIN;
+    const double omegal = 0.0, radl = 0.0, omegar = 0.0, radr = 0.0;
+    double cl = 0.0, cr = 0.0, el = 0.0, er = 0.0;
+    double mfl[NS], mfr[NS];
+    for (int n = 0; n < NS; n++) {
+      mfl[n] = spl[n];
+      mfr[n] = spr[n];
+    }
+    EOS::RPY2Cs(rl, pl, mfl, cl);
+    EOS::RPY2Cs(rr, pr, mfr, cr);
+    EOS::RYP2E(rl, mfl, pl, el);
+    EOS::RYP2E(rr, mfr, pr, er);
+    const double r_elt = rl * (el + 0.5 * (ul * ul + vl * vl + v2l * v2l) -
+                               0.5 * omegal * omegal * radl * radl);
+    const double r_ert = rr * (er + 0.5 * (ur * ur + vr * vr + v2r * v2r) -
+                               0.5 * omegar * omegar * radr * radr);
+    const double wl = mx(wsmall, cl * rl);
+    const double wr = mx(wsmall, cr * rr);
+    ustar = ((wl * ul + wr * ur) + (pl - pr)) / (wl + wr);
+    const bool mask =
+        std::abs(ustar) < smallu * 0.5 * (std::abs(ul) + std::abs(ur)) ||
+        ustar == 0.0;
+    ustar = mask ? 0.0 : ustar;
+    const double chalf = 0.5 * (cr + cl);
+    const double max_wavespd = std::abs(ustar) + chalf;
+    maxeigval = max_wavespd;
+    uflx_rho = 0.0;
+    for (int n = 0; n < NS; n++) {
+      uflx_rhoY[n] = 0.5 * ((rl * ul * spl[n] + rr * ur * spr[n]) -
+                            max_wavespd * (rr * spr[n] - rl * spl[n]));
+      uflx_rho += uflx_rhoY[n];
+    }
+    uflx_u = 0.5 * ((rl * ul * ul + pl + rr * ur * ur + pr) -
+                    max_wavespd * (rr * ur - rl * ul));
+    uflx_v = 0.5 * ((rl * vl * ul + rr * vr * ur) - max_wavespd * (rr * vr - rl * vl));
+    uflx_w = 0.5 * ((rl * v2l * ul + rr * v2r * ur) -
+                    max_wavespd * (rr * v2r - rl * v2l));
+    uflx_eden = 0.5 * (((r_elt + pl) * ul + (r_ert + pr) * ur) -
+                       max_wavespd * (r_ert - r_elt));
+    uflx_eint = 0.5 * ((rl * el * ul + rr * er * ur) - max_wavespd * (rr * er - rl * el));
+  }
+
+  // ------------------------------------------------- Source/MOL.cpp:6-190
+  // pc_compute_hyp_mol_flux (regular cells, no rotating frame, no advected or
+  // linear scalars): flx[dir] += flux * area on the faces of grow(cbox,dir,-1)
+  static void hyp_mol_flux(const Box& cbox, const A4& q, const A4& qaux,
+                           const A4* flx, const A4* area, int mol_iorder,
+                           bool use_laxf_flux) {
+    const int bc_test_val = 1;
+    for (int dir = 0; dir < 3; dir++) {
+      Fab dq_fab(cbox, QVAR);
+      const A4 dq = dq_fab.a;
+      ParallelFor(cbox, [&](int i, int j, int k) {
+        for (int n = 0; n < QVAR; ++n) dq(i, j, k, n) = 0.0;  // setV
+      });
+      int q_idx[3], f_idx[3];
+      mol_idx(dir, q_idx, f_idx);
+      if (mol_iorder != 1) {
+        ParallelFor(cbox, [&](int i, int j, int k) {
+          mol_slope(i, j, k, dir, q_idx, q, qaux, dq);
+        });
+      }
+      Box tbox = cbox;
+      tbox.lo[dir] += 1;
+      tbox.hi[dir] -= 1;
+      const Box ebox = surroundingNodes(tbox, dir);
+      ParallelFor(ebox, [&](int i, int j, int k) {
+        const IV iv = ivec(i, j, k);
+        const IV ivm = shift(iv, dir, -1);
+        // face states in the normal frame: rho, un, ut1, ut2, p, Y
+        double l[5], r[5], spl[NS], spr[NS];
+        l[1] = at(q, ivm, q_idx[0]) + 0.5 * ((at(dq, ivm, 1) - at(dq, ivm, 0)) / at(q, ivm, QRHO));
+        l[4] = at(q, ivm, QPRES) + 0.5 * (at(dq, ivm, 0) + at(dq, ivm, 1)) * at(qaux, ivm, QC);
+        l[2] = at(q, ivm, q_idx[1]) + 0.5 * at(dq, ivm, 2);
+        l[3] = at(q, ivm, q_idx[2]) + 0.5 * at(dq, ivm, 3);
+        l[0] = 0.0;
+        for (int n = 0; n < NS; n++) {
+          spl[n] = at(q, ivm, QFS + n) * at(q, ivm, QRHO) +
+                   0.5 * (at(dq, ivm, QFS + n) +
+                          at(q, ivm, QFS + n) * (at(dq, ivm, 0) + at(dq, ivm, 1)) /
+                              at(qaux, ivm, QC));
+          l[0] += spl[n];
+        }
+        for (int n = 0; n < NS; n++) spl[n] = spl[n] / l[0];
+        r[1] = at(q, iv, q_idx[0]) - 0.5 * ((at(dq, iv, 1) - at(dq, iv, 0)) / at(q, iv, QRHO));
+        r[4] = at(q, iv, QPRES) - 0.5 * (at(dq, iv, 0) + at(dq, iv, 1)) * at(qaux, iv, QC);
+        r[2] = at(q, iv, q_idx[1]) - 0.5 * at(dq, iv, 2);
+        r[3] = at(q, iv, q_idx[2]) - 0.5 * at(dq, iv, 3);
+        r[0] = 0.0;
+        for (int n = 0; n < NS; n++) {
+          spr[n] = at(q, iv, QFS + n) * at(q, iv, QRHO) -
+                   0.5 * (at(dq, iv, QFS + n) +
+                          at(q, iv, QFS + n) * (at(dq, iv, 0) + at(dq, iv, 1)) /
+                              at(qaux, iv, QC));
+          r[0] += spr[n];
+        }
+        for (int n = 0; n < NS; n++) spr[n] = spr[n] / r[0];
+        const double cavg = 0.5 * (at(qaux, iv, QC) + at(qaux, ivm, QC));
+        double flux_tmp[NVAR];
+        for (int n = 0; n < NVAR; ++n) flux_tmp[n] = 0.0;
+        double ustar = 0.0;
+        if (!use_laxf_flux) {
+          double qint_iu = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+          riemann(l[0], l[1], l[2], l[3], l[4], spl, r[0], r[1], r[2], r[3],
+                  r[4], spr, bc_test_val, cavg, ustar, flux_tmp[URHO],
+                  &flux_tmp[UFS], flux_tmp[f_idx[0]], flux_tmp[f_idx[1]],
+                  flux_tmp[f_idx[2]], flux_tmp[UEDEN], flux_tmp[UEINT],
+                  qint_iu, t1, t2, t3, t4);
+        } else {
+          double maxeigval = 0.0;
+          laxfriedrich_flux(l[0], l[1], l[2], l[3], l[4], spl, r[0], r[1],
+                            r[2], r[3], r[4], spr, ustar, maxeigval,
+                            flux_tmp[URHO], &flux_tmp[UFS], flux_tmp[f_idx[0]],
+                            flux_tmp[f_idx[1]], flux_tmp[f_idx[2]],
+                            flux_tmp[UEDEN], flux_tmp[UEINT]);
+        }
+        flux_tmp[UTEMP] = 0.0;
+        for (int ivar = 0; ivar < NVAR; ivar++)
+          at(flx[dir], iv, ivar) += flux_tmp[ivar] * area[dir](i, j, k);
+      });
+    }
+  }
+
+  // getMOLSrcTerm for one regular box with diffusion, filtering, EB and
+  // refluxing off (Source/Diffusion.cpp:90-98,136-148,226-240,349-374,
+  // 509-519,654): primitives on grow(vbox,4), face fluxes zeroed on
+  // surroundingNodes(grow(vbox,3),d) (the diffusive ones are all cleared by
+  // :332-346 when no diffuse_* switch is on), the MOL hydro fluxes added,
+  // then pc_flux_div (Source/Diffterm.H:470-487) on vbox into src.
+  static void mol_src(const Box& vbox, const A4& S, const A4& src,
+                      const A4* area, const A4& vol, int mol_iorder,
+                      bool use_laxf_flux, const A4* flx_out = nullptr) {
+    const Box gbox = grow(vbox, 4), cbox = grow(vbox, 3);
+    Fab q(gbox, QVAR), qa(gbox, NQAUX);
+    ParallelFor(gbox, [&](int i, int j, int k) { ctoprim(i, j, k, S, q.a, qa.a); });
+    Fab f0(surroundingNodes(cbox, 0), NVAR), f1(surroundingNodes(cbox, 1), NVAR),
+        f2(surroundingNodes(cbox, 2), NVAR);
+    for (Fab* f : {&f0, &f1, &f2}) std::fill(f->v.begin(), f->v.end(), 0.0);
+    const A4 flx[3] = {f0.a, f1.a, f2.a};
+    hyp_mol_flux(cbox, q.a, qa.a, flx, area, mol_iorder, use_laxf_flux);
+    ParallelFor(vbox, [&](int i, int j, int k) {
+      for (int n = 0; n < NVAR; ++n)
+        src(i, j, k, n) =
+            -(flx[0](i + 1, j, k, n) - flx[0](i, j, k, n) + flx[1](i, j + 1, k, n) -
+              flx[1](i, j, k, n) + flx[2](i, j, k + 1, n) - flx[2](i, j, k, n)) /
+            vol(i, j, k);
+    });
+    if (flx_out)
+      for (int d = 0; d < 3; ++d)
+        ParallelFor(surroundingNodes(vbox, d), [&](int i, int j, int k) {
+          for (int n = 0; n < NVAR; ++n) flx_out[d](i, j, k, n) = flx[d](i, j, k, n);
+        });
+  }
+
   // ------------------------------------- Source/PeleCUtilities.H:70-84, 227-236
   static void rst_int_e_default(int i, int j, int k, const A4& S) {
     const double rho = S(i, j, k, URHO);

@@ -68,6 +68,7 @@ struct HostLevel {
   std::vector<long long> off;
   Box dom;
   int bclo[3], bchi[3];
+  double dx[3];
   bool self;
   std::vector<int> cellmap;  // domain cell -> box id (domain mode)
   long long total = 0;
@@ -85,6 +86,7 @@ HostLevel make_level(int nvar, const pc_level_desc_t* d) {
     L.dom.hi[dd] = d->domhi[dd];
     L.bclo[dd] = d->bclo[dd];
     L.bchi[dd] = d->bchi[dd];
+    L.dx[dd] = d->dx[dd];
   }
   long long o = 0;
   for (int b = 0; b < d->nboxes; ++b) {
@@ -179,6 +181,81 @@ double estdt_level(const HostLevel& L, const pc_params_t& prm,
   return estdt;
 }
 
+// Uniform-geometry face areas and cell volumes of a box (Cartesian), the
+// areas over the faces of grow(bx,3) that the MOL flux loop visits (the
+// reference's area MultiFab carries ghost cells).
+struct BoxGeom {
+  Fab a0, a1, a2, vol;
+  BoxGeom(const Box& bx, const double* dx)
+      : a0(surroundingNodes(grow(bx, 3), 0), 1),
+        a1(surroundingNodes(grow(bx, 3), 1), 1),
+        a2(surroundingNodes(grow(bx, 3), 2), 1), vol(bx, 1) {
+    std::fill(a0.v.begin(), a0.v.end(), dx[1] * dx[2]);
+    std::fill(a1.v.begin(), a1.v.end(), dx[2] * dx[0]);
+    std::fill(a2.v.begin(), a2.v.end(), dx[0] * dx[1]);
+    std::fill(vol.v.begin(), vol.v.end(), dx[0] * dx[1] * dx[2]);
+  }
+};
+
+// do_mol_advance (Source/Advance.cpp:49-227) for a hydro-only single level
+// (no reactions or other sources, mol_iters = 1): MOL source at t^n,
+// U* = 1.0 U^n + dt S^n, computeTemp; MOL source of U* (ghosts filled again),
+// U^{n+1} = 0.5 U* + 0.5 U^n, += (0.5 dt) S*, computeTemp; then
+// post_timestep's computeTemp (Source/PeleC.cpp:1108-1110) and computeNewDt.
+template <class EOS>
+int mol_level_step_t(HostLevel& L, const pc_params_t& prm, double* arena,
+                     int nsteps, double* dtp) {
+  using H = Hydro<EOS>;
+  const int nb = (int)L.boxes.size();
+  const int iorder = prm.plm_iorder;  // Source/Diffusion.cpp:371-373
+  const bool laxf = prm.use_laxf_flux != 0;
+  double dt = *dtp;
+  std::vector<std::vector<double>> sold(nb);
+  for (int s = 0; s < nsteps; ++s) {
+    for (int stage = 0; stage < 2; ++stage) {
+      fill_ghosts(L, arena);
+#pragma omp parallel for schedule(dynamic)
+      for (int b = 0; b < nb; ++b) {
+        const Box bx = L.boxes[b];
+        const A4 S = L.fab(arena, b);
+        BoxGeom g(bx, L.dx);
+        const A4 a[3] = {g.a0.a, g.a1.a, g.a2.a};
+        Fab src(bx, H::NVAR);
+        H::mol_src(bx, S, src.a, a, g.vol.a, iorder, laxf);
+        if (stage == 0) {
+          sold[b].resize((size_t)bx.npts() * H::NVAR);
+          const A4 O = make_a4(sold[b].data(), bx, H::NVAR);
+          ParallelFor(bx, [&](int i, int j, int k) {
+            for (int n = 0; n < H::NVAR; ++n) {
+              O(i, j, k, n) = S(i, j, k, n);
+              S(i, j, k, n) = 1.0 * S(i, j, k, n) + dt * src.a(i, j, k, n);
+            }
+            H::rst_int_e_default(i, j, k, S);
+            H::cmpTemp(i, j, k, S);
+          });
+        } else {
+          const A4 O = make_a4(sold[b].data(), bx, H::NVAR);
+          ParallelFor(bx, [&](int i, int j, int k) {
+            for (int n = 0; n < H::NVAR; ++n) {
+              S(i, j, k, n) = 0.5 * S(i, j, k, n) + 0.5 * O(i, j, k, n);
+              S(i, j, k, n) += (0.5 * dt) * src.a(i, j, k, n);
+            }
+            H::rst_int_e_default(i, j, k, S);
+            H::cmpTemp(i, j, k, S);
+            H::rst_int_e_default(i, j, k, S);
+            H::cmpTemp(i, j, k, S);
+          });
+        }
+      }
+    }
+    double dtn = estdt_level<EOS>(L, prm, arena, L.dx);
+    if (prm.fixed_dt <= 0.0) dtn = mn(dtn, prm.change_max * dt);
+    dt = dtn;
+  }
+  *dtp = dt;
+  return 0;
+}
+
 template <class EOS>
 int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
                  double* arena, int nsteps, double* dtp, int nthreads) {
@@ -198,6 +275,7 @@ int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
     blo[d] = desc->bclo[d]; bhi[d] = desc->bchi[d];
   }
   double dt = *dtp;
+  if (prm.do_mol) return mol_level_step_t<EOS>(L, prm, arena, nsteps, dtp);
   for (int s = 0; s < nsteps; ++s) {
     fill_ghosts(L, arena);
     int err = 0;
@@ -329,6 +407,36 @@ int riemann_point(const double* L5, const double* spl, const double* R5,
   return 0;
 }
 
+template <class EOS>
+int laxf_point(const double* L5, const double* spl, const double* R5,
+               const double* spr, double* o) {
+  Hydro<EOS>::laxfriedrich_flux(L5[0], L5[1], L5[2], L5[3], L5[4], spl, R5[0],
+                                R5[1], R5[2], R5[3], R5[4], spr, o[0], o[1],
+                                o[2], o + 8, o[3], o[4], o[5], o[6], o[7]);
+  return 0;
+}
+
+template <class EOS>
+int mol_src_box_t(const Box& bx, const A4& S, const A4& src,
+                  const pc_params_t& prm, const A4* flx, const A4* a,
+                  const A4& vol) {
+  // The flux loop visits the faces of grow(bx,3); only those of bx enter
+  // the result.  The caller's areas cover surroundingNodes(bx,d): extend
+  // them with NaN, so a flux outside bx could not reach the output unseen.
+  Fab ext[3] = {Fab(surroundingNodes(grow(bx, 3), 0), 1),
+                Fab(surroundingNodes(grow(bx, 3), 1), 1),
+                Fab(surroundingNodes(grow(bx, 3), 2), 1)};
+  A4 ae[3];
+  for (int d = 0; d < 3; ++d) {
+    ae[d] = ext[d].a;
+    ParallelFor(surroundingNodes(bx, d),
+                [&](int i, int j, int k) { ae[d](i, j, k) = a[d](i, j, k); });
+  }
+  Hydro<EOS>::mol_src(bx, S, src, ae, vol, prm.plm_iorder,
+                      prm.use_laxf_flux != 0, flx);
+  return 0;
+}
+
 }  // namespace
 
 #define DISPATCH(eos, call)                                   \
@@ -357,6 +465,9 @@ void pc_ref_params_default(pc_params_t* p) {
   p->allow_negative_energy = 1;
   p->dual_energy_update_E_from_e = 1;
   p->dual_energy_eta2 = 1.0e-4;
+  p->do_mol = 0;
+  p->mol_iorder = 2;
+  p->use_laxf_flux = 0;
 }
 
 int pc_ref_ctoprim(int eos, pc_box_t qbx, pc_array4_t u, pc_array4_t q,
@@ -452,6 +563,9 @@ int pc_ref_level_step(int eos, const pc_level_desc_t* desc,
   if (prm->use_hybrid_weno || !prm->allow_small_energy ||
       !prm->allow_negative_energy)
     return PC_ERR_UNSUPPORTED;
+  // Source/PeleC.cpp:388-392
+  if (prm->do_mol && prm->mol_iorder != 1 && prm->mol_iorder != 2)
+    return PC_ERR_MOL_IORDER;
   int rc = 0;
   DISPATCH(eos, rc = level_step_t<E>(desc, *prm, arena, nsteps, dt, nthreads));
   return rc;
@@ -489,6 +603,35 @@ int pc_ref_ppm_int_profile(double sm, double sp, double sc, double u, double c,
                            double dtdx, double* Ip3, double* Im3) {
   Hydro<GammaLaw>::ppm_int_profile(sm, sp, sc, u, c, dtdx, Ip3, Im3);
   return 0;
+}
+
+int pc_ref_mol_src_box(int eos, pc_box_t bx, pc_array4_t sarr,
+                       pc_array4_t molsrc, const pc_params_t* prm,
+                       const pc_array4_t* flx, const pc_array4_t* a,
+                       pc_array4_t vol) {
+  const Box b = to_box(bx);
+  if (!covers(sarr, grow(b, 4))) return PC_ERR_GROW;
+  if (!covers(molsrc, b) || !covers(vol, b)) return PC_ERR_ARG;
+  A4 F[3];
+  for (int d = 0; d < 3; ++d) {
+    if (!covers(a[d], surroundingNodes(b, d))) return PC_ERR_ARG;
+    if (flx) {
+      if (!covers(flx[d], surroundingNodes(b, d))) return PC_ERR_ARG;
+      F[d] = to_a4(flx[d]);
+    }
+  }
+  const A4 AA[3] = {to_a4(a[0]), to_a4(a[1]), to_a4(a[2])};
+  int rc = 0;
+  DISPATCH(eos, rc = mol_src_box_t<E>(b, to_a4(sarr), to_a4(molsrc), *prm,
+                                      flx ? F : nullptr, AA, to_a4(vol)));
+  return rc;
+}
+
+int pc_ref_laxf(int eos, const double* left5, const double* spl,
+                const double* right5, const double* spr, double* out) {
+  int rc = 0;
+  DISPATCH(eos, rc = laxf_point<E>(left5, spl, right5, spr, out));
+  return rc;
 }
 
 double pc_ref_plm_slope_line(const double* q5, double flat, int order) {

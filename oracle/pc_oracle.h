@@ -74,6 +74,20 @@ int pc_ref_ppm_int_profile(double sm, double sp, double sc, double u, double c,
                            double dtdx, double* Ip3, double* Im3);
 double pc_ref_plm_slope_line(const double* q5, double flat, int order);
 
+/* getMOLSrcTerm for one box, hydro only: molsrc (NVAR comps on bx) =
+ * -div(F)/vol of the MOL fluxes of sarr (>= 4 ghosts); flx[d] (NULL, or on
+ * surroundingNodes(bx,d)) receives the extensive fluxes.  Slopes iff
+ * prm->plm_iorder != 1; prm->use_laxf_flux selects Lax-Friedrichs. */
+int pc_ref_mol_src_box(int eos, pc_box_t bx, pc_array4_t sarr,
+                       pc_array4_t molsrc, const pc_params_t* prm,
+                       const pc_array4_t* flx, const pc_array4_t* a,
+                       pc_array4_t vol);
+/* laxfriedrich_flux of Source/Riemann.H:205-307: out[0..] = ustar,
+ * maxeigval, uflx_rho, uflx_u, uflx_v, uflx_w, uflx_eden, uflx_eint,
+ * uflx_rhoY[NS] */
+int pc_ref_laxf(int eos, const double* left5, const double* spl,
+                const double* right5, const double* spr, double* out);
+
 #ifdef __cplusplus
 }
 #endif

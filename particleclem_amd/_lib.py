@@ -16,7 +16,8 @@ EXPORTS = [
     "pc_params_default", "pc_ctx_create", "pc_ctx_destroy",
     "pc_ctx_num_species", "pc_ctx_nvar", "pc_ctx_qvar", "pc_last_error",
     "pc_ctoprim_v1", "pc_srctoprim_v1", "pc_umdrv_v1",
-    "pc_hydro_source_box_v1", "pc_level_create_v1", "pc_level_destroy_v1",
+    "pc_hydro_source_box_v1", "pc_mol_src_box_v1", "pc_level_create_v1",
+    "pc_level_destroy_v1",
     "pc_level_box_offset_v1", "pc_level_box_size_v1", "pc_level_state_ptr_v1",
     "pc_level_state_size_v1", "pc_level_write_box_v1", "pc_level_read_box_v1",
     "pc_level_set_dt_v1", "pc_level_get_dt_v1", "pc_level_init_dt_v1",
@@ -34,14 +35,17 @@ PC_OK = 0
 KERNELS = ["ghost", "ctoprim", "divu", "trace_x", "trace_y", "trace_z",
            "flux1_x", "flux1_y", "flux1_z", "trans_xy", "trans_xz",
            "trans_yx", "trans_yz", "trans_zx", "trans_zy", "final_x",
-           "final_y", "final_z", "update", "dt"]
+           "final_y", "final_z", "update", "dt", "mol_x", "mol_y", "mol_z",
+           "mol_update1", "mol_update2"]
 # stage plans (pc_level_plan_v1 stage ids)
 STAGES = ["ctoprim", "divu", "trace", "flux1_x", "flux1_y", "flux1_z",
           "trans_xy", "trans_xz", "trans_yx", "trans_yz", "trans_zx",
-          "trans_zy", "final_x", "final_y", "final_z", "update", "ghost"]
+          "trans_zy", "final_x", "final_y", "final_z", "update", "ghost",
+          "mol_prim", "mol_x", "mol_y", "mol_z", "mol_update"]
 ERRORS = {-1: "PC_ERR_ARG", -2: "PC_ERR_GROW", -3: "PC_ERR_PPM_TYPE",
           -4: "PC_ERR_PLM_IORDER", -5: "PC_ERR_HIP", -6: "PC_ERR_NAN",
-          -7: "PC_ERR_UNSUPPORTED", -8: "PC_ERR_NOMEM", -9: "PC_ERR_DT"}
+          -7: "PC_ERR_UNSUPPORTED", -8: "PC_ERR_NOMEM", -9: "PC_ERR_DT",
+          -10: "PC_ERR_MOL_IORDER"}
 
 _lib = None
 
@@ -84,6 +88,8 @@ def lib():
                                          Array4, Array4, d3, C.c_double,
                                          P(Params), P(Array4), P(Array4),
                                          Array4, vp]
+    L.pc_mol_src_box_v1.argtypes = [vp, Box, Array4, Array4, P(Params),
+                                    P(Array4), P(Array4), Array4, vp]
     L.pc_level_autotune_v1.argtypes = [vp, C.c_int, vp]
     L.pc_level_plan_v1.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_double)]
     L.pc_level_set_graph_v1.argtypes = [vp, C.c_int]

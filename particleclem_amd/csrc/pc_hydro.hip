@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "../../include/pc_hydro.h"
-#include "pc_level_kernels.h"
+#include "pc_mol_kernels.h"
 
 using namespace pcd;
 
@@ -102,7 +102,8 @@ static int a4_view(pc_ctx* c, const pc_array4_t& a, const IBox& need, int nc,
   for (int d = 0; d < 3; ++d)
     if (need.lo[d] < a.begin[d] || need.hi[d] >= a.end[d])
       return fail(c, (nc > 0 && strcmp(name, "flx") && strcmp(name, "a") &&
-                      strcmp(name, "vol") && strcmp(name, "uout"))
+                      strcmp(name, "vol") && strcmp(name, "uout") &&
+                      strcmp(name, "molsrc"))
                          ? PC_ERR_GROW
                          : PC_ERR_ARG,
                   "%s does not cover the required box in dir %d", name, d);
@@ -132,11 +133,14 @@ static int a4_view(pc_ctx* c, const pc_array4_t& a, const IBox& need, int nc,
 struct WsLayout {
   IBox q, tr[3], f1[3], ft[6], fl[3], node;
   size_t off_q = 0, off_qa = 0, off_srcq = 0, off_tr[3], off_f1[3], off_ft[6],
-         off_fl[3], off_qec[3], off_divu = 0, total = 0;
+         off_fl[3], off_qec[3], off_divu = 0, off_sold = 0, total = 0;
+  bool mol = false;
 };
 
+// mol: the method-of-lines step's workspace instead of the CTU one:
+// primitives, the three face-flux arrays and U^n of the valid box.
 static WsLayout ws_layout(const IBox& bx, int ns, bool with_q, bool with_src,
-                          bool with_flx) {
+                          bool with_flx, bool mol = false) {
   WsLayout L;
   const int QVAR = 8 + ns, NVAR = 7 + ns, QT = 6 + ns, FT = 7 + ns;
   size_t o = 0;
@@ -150,6 +154,19 @@ static WsLayout ws_layout(const IBox& bx, int ns, bool with_q, bool with_src,
     L.off_q = take(L.q.npts() * QVAR);
     L.off_qa = take(L.q.npts() * NQAUX);
     if (with_src) L.off_srcq = take(L.q.npts() * QVAR);
+  }
+  if (mol) {
+    L.mol = true;
+    for (int d = 0; d < 3; ++d) {
+      IBox b = bx;
+      b.hi[d] += 1;
+      L.fl[d] = b;
+      if (with_flx) L.off_fl[d] = take(b.npts() * NVAR);
+    }
+    L.off_sold = take(bx.npts() * NVAR);
+    L.node = bx;
+    L.total = o;
+    return L;
   }
   for (int d = 0; d < 3; ++d) {
     IBox b = growb(bx, 1);
@@ -196,6 +213,17 @@ static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
     J.qa = fab_view(base + L.off_qa, L.q);
     J.srcq = with_src ? fab_view(base + L.off_srcq, L.q) : null_view();
   }
+  if (L.mol) {
+    for (int d = 0; d < 3; ++d) {
+      if (with_flx) J.flx[d] = fab_view(base + L.off_fl[d], L.fl[d]);
+      J.tr[d] = J.f1[d] = J.qec[d] = null_view();
+    }
+    for (int s = 0; s < 6; ++s) J.ft[s] = null_view();
+    J.divu = null_view();
+    J.sold = fab_view(base + L.off_sold, L.node);
+    return;
+  }
+  J.sold = null_view();
   for (int d = 0; d < 3; ++d) {
     J.tr[d] = fab_view(base + L.off_tr[d], L.tr[d]);
     J.f1[d] = fab_view(base + L.off_f1[d], L.f1[d]);
@@ -413,6 +441,9 @@ void pc_params_default(pc_params_t* p) {
   p->allow_negative_energy = 1;
   p->dual_energy_update_E_from_e = 1;
   p->dual_energy_eta2 = 1.0e-4;
+  p->do_mol = 0;
+  p->mol_iorder = 2;
+  p->use_laxf_flux = 0;
 }
 
 int pc_ctx_create(int eos_kind, pc_ctx_t** out) {
@@ -520,6 +551,16 @@ static int check_opts(pc_ctx* c, int ppm_type, int plm_iorder,
   if (plm_iorder != 1 && plm_iorder != 2 && plm_iorder != 4)
     return fail(c, PC_ERR_PLM_IORDER, "plm_iorder must be 1, 2 or 4");
   if (use_hybrid_weno)
+    return fail(c, PC_ERR_UNSUPPORTED, "use_hybrid_weno is not implemented");
+  return PC_OK;
+}
+
+// Source/PeleC.cpp:384-392 with do_mol = 1: only mol_iorder is checked
+// (ppm_type and plm_iorder are not; plm_iorder != 1 switches the slopes on)
+static int check_mol_opts(pc_ctx* c, const pc_params_t* prm) {
+  if (prm->mol_iorder != 1 && prm->mol_iorder != 2)
+    return fail(c, PC_ERR_MOL_IORDER, "PeleC::mol_iorder must be 1, or 2.");
+  if (prm->use_hybrid_weno)
     return fail(c, PC_ERR_UNSUPPORTED, "use_hybrid_weno is not implemented");
   return PC_OK;
 }
@@ -645,6 +686,74 @@ int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
 
 }  // extern "C"
 
+// primitives, MOL fluxes and the divergence of one box-driver job
+template <class EOS>
+static void launch_mol_src(const Job& J, const Sched& sc, const KArgs& ka,
+                           bool laxf, hipStream_t st) {
+  LAUNCH((k_ctoprim<EOS, double, false, ST_MOL_PRIM>), grid1(ST_MOL_PRIM, J), st,
+         sc, ka);
+  if (laxf) {
+    LAUNCH((k_mol_flux<EOS, 0, true>), grid1(ST_MOL_X, J), st, sc, ka);
+    LAUNCH((k_mol_flux<EOS, 1, true>), grid1(ST_MOL_Y, J), st, sc, ka);
+    LAUNCH((k_mol_flux<EOS, 2, true>), grid1(ST_MOL_Z, J), st, sc, ka);
+  } else {
+    LAUNCH((k_mol_flux<EOS, 0, false>), grid1(ST_MOL_X, J), st, sc, ka);
+    LAUNCH((k_mol_flux<EOS, 1, false>), grid1(ST_MOL_Y, J), st, sc, ka);
+    LAUNCH((k_mol_flux<EOS, 2, false>), grid1(ST_MOL_Z, J), st, sc, ka);
+  }
+  LAUNCH(k_mol_div<EOS::NS>, grid1(ST_MOL_UPD, J), st, sc, ka);
+}
+
+extern "C" {
+
+int pc_mol_src_box_v1(pc_ctx_t* c, pc_box_t bx, pc_array4_t sarr,
+                      pc_array4_t molsrc, const pc_params_t* prm,
+                      const pc_array4_t* flx, const pc_array4_t* a,
+                      pc_array4_t vol, void* stream) {
+  if (!c || !prm || !box_ok(bx) || !a)
+    return fail(c, PC_ERR_ARG, "null argument or empty box");
+  int rc;
+  if ((rc = check_mol_opts(c, prm))) return rc;
+  const IBox b = mkbox(bx.lo, bx.hi);
+  const IBox g4 = growb(b, 4);
+  const int NVAR = 7 + c->ns;
+  Job J;
+  memset(&J, 0, sizeof(J));
+  for (int d = 0; d < 3; ++d) { J.lo[d] = b.lo[d]; J.hi[d] = b.hi[d]; }
+  if ((rc = a4_view(c, sarr, g4, NVAR, "sarr", J.U))) return rc;
+  if ((rc = a4_view(c, molsrc, b, NVAR, "molsrc", J.upd))) return rc;
+  if ((rc = a4_view(c, vol, b, 1, "vol", J.vol))) return rc;
+  for (int d = 0; d < 3; ++d) {
+    IBox fb = b;
+    fb.hi[d] += 1;
+    if ((rc = a4_view(c, a[d], fb, 1, "a", J.area[d]))) return rc;
+    if (flx && (rc = a4_view(c, flx[d], fb, NVAR, "flx", J.flx[d]))) return rc;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const WsLayout W = ws_layout(b, c->ns, true, false, flx == nullptr, true);
+  if ((rc = ctx_scratch(c, W.total * sizeof(double), st))) return rc;
+  J.q = fab_view(c->ws + W.off_q, W.q);
+  J.qa = fab_view(c->ws + W.off_qa, W.q);
+  for (int d = 0; d < 3; ++d) {
+    J.qe[d] = W.q.hi[d] + 1;
+    if (!flx) J.flx[d] = fab_view(c->ws + W.off_fl[d], W.fl[d]);
+  }
+  KArgs ka;
+  memset(&ka, 0, sizeof(ka));
+  ka.plm_iorder = prm->plm_iorder;
+  Sched sc;
+  int slot;
+  if ((rc = ctx_job(c, J, st, &sc, &slot))) return rc;
+  if (c->eos == PC_EOS_GAMMALAW)
+    launch_mol_src<EosGamma>(J, sc, ka, prm->use_laxf_flux != 0, st);
+  else
+    launch_mol_src<EosLiDryer>(J, sc, ka, prm->use_laxf_flux != 0, st);
+  HIPCHK(c, hipGetLastError());
+  return ctx_job_done(c, slot, st);
+}
+
+}  // extern "C"
+
 // ==================================================================== level
 struct pc_level {
   pc_ctx* ctx = nullptr;
@@ -670,6 +779,8 @@ struct pc_level {
   int nsteps_timed = 0;
   std::vector<Job> hjobs;      // host copy of the job table (plans)
   int mixed = 0;               // 1: FP32 reconstruction/Riemann/intermediates
+  bool mol = false;            // method-of-lines step (do_mol)
+  bool laxf = false;           // MOL with the Lax-Friedrichs flux
   double tuned_ms[ST_COUNT] = {};
   // hipGraph of gsteps consecutive steps, replayed by pc_level_step_v1
   int gsteps = 0;
@@ -791,6 +902,25 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
       if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2, FP, P>), grid, st, sc, ka);
       break;
     case ST_UPDATE: LAUNCH((k_update<EOS, P>), grid, st, sc, ka, L->ds); break;
+    case ST_MOL_PRIM:
+      LAUNCH((k_ctoprim<EOS, FP, P, ST_MOL_PRIM>), grid, st, sc, ka);
+      break;
+    case ST_MOL_X:
+      if (L->laxf) LAUNCH((k_mol_flux<EOS, 0, true, FP, P>), grid, st, sc, ka);
+      else LAUNCH((k_mol_flux<EOS, 0, false, FP, P>), grid, st, sc, ka);
+      break;
+    case ST_MOL_Y:
+      if (L->laxf) LAUNCH((k_mol_flux<EOS, 1, true, FP, P>), grid, st, sc, ka);
+      else LAUNCH((k_mol_flux<EOS, 1, false, FP, P>), grid, st, sc, ka);
+      break;
+    case ST_MOL_Z:
+      if (L->laxf) LAUNCH((k_mol_flux<EOS, 2, true, FP, P>), grid, st, sc, ka);
+      else LAUNCH((k_mol_flux<EOS, 2, false, FP, P>), grid, st, sc, ka);
+      break;
+    case ST_MOL_UPD:  // sub: the Heun stage (0 or 1)
+      if (sub == 0) LAUNCH((k_mol_update<EOS, 0, P>), grid, st, sc, ka, L->ds);
+      else LAUNCH((k_mol_update<EOS, 1, P>), grid, st, sc, ka, L->ds);
+      break;
     default: break;
   }
 }
@@ -820,13 +950,19 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   if (!c || !d || !prm || !out || d->nboxes <= 0 || !d->boxes)
     return fail(c, PC_ERR_ARG, "bad level description");
   int rc;
-  if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder, prm->use_hybrid_weno)))
+  if (prm->do_mol) {
+    if ((rc = check_mol_opts(c, prm))) return rc;
+  } else if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder,
+                              prm->use_hybrid_weno))) {
     return rc;
+  }
   if (!prm->allow_small_energy || !prm->allow_negative_energy)
     return fail(c, PC_ERR_UNSUPPORTED,
                 "allow_small_energy = allow_negative_energy = 1 required");
   *out = nullptr;
   pc_level* L = new pc_level();
+  L->mol = prm->do_mol != 0;
+  L->laxf = prm->use_laxf_flux != 0;
   L->ctx = c;
   L->prm = *prm;
   const int ns = c->ns, NVAR = 7 + ns;
@@ -852,7 +988,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
     L->boxes.push_back(bx);
     L->off.push_back(o);
     o += growb(bx, 4).npts() * NVAR;
-    const WsLayout W = ws_layout(bx, ns, true, false, true);
+    const WsLayout W = ws_layout(bx, ns, true, false, true, L->mol);
     wsoff.push_back(wso);
     wso += W.total;
   }
@@ -889,7 +1025,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
     set_geom(J, d->dx);
     J.dtp = &L->ds->dt;
     J.U = fab_view(L->state + L->off[b], growb(bx, 4));
-    const WsLayout W = ws_layout(bx, ns, true, false, true);
+    const WsLayout W = ws_layout(bx, ns, true, false, true, L->mol);
     ws_fill_job(J, L->ws + wsoff[b], W, true, false, true);
     J.srcq = null_view();
     for (int dd = 0; dd < 3; ++dd) J.area[dd] = null_view();
@@ -1097,9 +1233,11 @@ int pc_level_enable_timing_v1(pc_level_t* L, int on) {
 
 }  // extern "C"
 
-// One Godunov step of the whole level: 21 launches, each covering every box.
-// With timing on, an event follows every launch and the step ends with a
-// host wait, so per-kernel device times accumulate in ktime[].
+// One step of the whole level, each launch covering every box: the
+// Godunov step (21 launches) or the MOL step (two Heun stages of ghost fill,
+// primitives, three flux launches and an update; 13 launches).  With timing
+// on, an event follows every launch and the step ends with a host wait, so
+// per-kernel device times accumulate in ktime[] under the kernel's id.
 template <class EOS>
 static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
   const Sched* S = L->scs;
@@ -1107,30 +1245,46 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
   for (int s = 0; s < nsteps; ++s) {
     const bool tm = L->timing;
     int ne = 0;
-    auto mark = [&]() {
-      if (tm) (void)hipEventRecord(L->ev[ne++], st);
+    int kid[PC_NKERN + 1];
+    auto mark = [&](int id) {
+      if (tm) {
+        kid[ne] = id;
+        (void)hipEventRecord(L->ev[ne++], st);
+      }
     };
-    auto run = [&](int stage, int sub = -1) {
+    auto run = [&](int stage, int id, int sub = -1) {
       launch_stage<EOS>(L, stage, S[stage], P[stage], st, sub);
-      mark();
+      mark(id);
     };
-    mark();
-    run(ST_GHOST);
-    run(ST_CTOPRIM);
-    run(ST_DIVU);
-    for (int d = 0; d < 3; ++d) run(ST_TRACE, d);
-    for (int x = ST_FLUX1_X; x <= ST_FLUX1_Z; ++x) run(x);
-    for (int x = ST_TRANS_XY; x <= ST_TRANS_ZY; ++x) run(x);
-    for (int x = ST_FINAL_X; x <= ST_FINAL_Z; ++x) run(x);
-    run(ST_UPDATE);
+    mark(-1);
+    if (L->mol) {
+      for (int h = 0; h < 2; ++h) {
+        run(ST_GHOST, PC_K_GHOST);
+        run(ST_MOL_PRIM, PC_K_CTOPRIM);
+        for (int d = 0; d < 3; ++d) run(ST_MOL_X + d, PC_K_MOL_X + d);
+        run(ST_MOL_UPD, h == 0 ? PC_K_MOL_UPD1 : PC_K_MOL_UPD2, h);
+      }
+    } else {
+      run(ST_GHOST, PC_K_GHOST);
+      run(ST_CTOPRIM, PC_K_CTOPRIM);
+      run(ST_DIVU, PC_K_DIVU);
+      for (int d = 0; d < 3; ++d) run(ST_TRACE, PC_K_TRACE_X + d, d);
+      for (int x = ST_FLUX1_X; x <= ST_FLUX1_Z; ++x)
+        run(x, PC_K_FLUX1_X + (x - ST_FLUX1_X));
+      for (int x = ST_TRANS_XY; x <= ST_TRANS_ZY; ++x)
+        run(x, PC_K_TRANS_XY + (x - ST_TRANS_XY));
+      for (int x = ST_FINAL_X; x <= ST_FINAL_Z; ++x)
+        run(x, PC_K_FINAL_X + (x - ST_FINAL_X));
+      run(ST_UPDATE, PC_K_UPDATE);
+    }
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 0);
-    mark();
+    mark(PC_K_DT);
     if (tm) {
       (void)hipEventSynchronize(L->ev[ne - 1]);
-      for (int e = 0; e + 1 < ne && e < PC_NKERN; ++e) {
+      for (int e = 0; e + 1 < ne; ++e) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, L->ev[e], L->ev[e + 1]);
-        L->ktime[e] += ms;
+        L->ktime[kid[e + 1]] += ms;
       }
       L->nsteps_timed++;
     }
@@ -1167,16 +1321,28 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
   DtState ds0;
   HIPCHK(c, hipMemcpy(&ds0, L->ds, sizeof(DtState), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpyAsync(saved, L->state, sbytes, hipMemcpyDeviceToDevice, st));
-  // intermediates of one step (no update)
-  for (int s : {ST_GHOST, ST_CTOPRIM, ST_DIVU, ST_TRACE, ST_FLUX1_X, ST_FLUX1_Y,
-                ST_FLUX1_Z, ST_TRANS_XY, ST_TRANS_XZ, ST_TRANS_YX, ST_TRANS_YZ,
-                ST_TRANS_ZX, ST_TRANS_ZY, ST_FINAL_X, ST_FINAL_Y, ST_FINAL_Z})
-    launch_stage<EOS>(L, s, L->scs[s], L->plans[s], st);
+  // intermediates of one step (the MOL first-stage update also leaves U^n in
+  // the workspace for the second-stage kernel; the state is restored after)
+  std::vector<int> tune;
+  if (L->mol) {
+    for (int s : {ST_GHOST, ST_MOL_PRIM, ST_MOL_X, ST_MOL_Y, ST_MOL_Z})
+      launch_stage<EOS>(L, s, L->scs[s], L->plans[s], st);
+    launch_stage<EOS>(L, ST_MOL_UPD, L->scs[ST_MOL_UPD], L->plans[ST_MOL_UPD], st, 0);
+    HIPCHK(c, hipMemcpyAsync(L->state, saved, sbytes, hipMemcpyDeviceToDevice, st));
+    tune = {ST_GHOST, ST_MOL_PRIM, ST_MOL_X, ST_MOL_Y, ST_MOL_Z, ST_MOL_UPD};
+  } else {
+    for (int s : {ST_GHOST, ST_CTOPRIM, ST_DIVU, ST_TRACE, ST_FLUX1_X, ST_FLUX1_Y,
+                  ST_FLUX1_Z, ST_TRANS_XY, ST_TRANS_XZ, ST_TRANS_YX, ST_TRANS_YZ,
+                  ST_TRANS_ZX, ST_TRANS_ZY, ST_FINAL_X, ST_FINAL_Y, ST_FINAL_Z})
+      launch_stage<EOS>(L, s, L->scs[s], L->plans[s], st);
+    for (int s = 0; s <= ST_GHOST; ++s) tune.push_back(s);
+  }
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   int rc = PC_OK;
-  for (int s = 0; s < ST_COUNT && rc == PC_OK; ++s) {
+  for (int s : tune) {
+    if (rc != PC_OK) break;
     double best = 1e300;
     int bestc = -1;
     StagePlan bestP;
@@ -1191,7 +1357,7 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
       const Sched sc = plan_sched(L, P);
       double tot = 0.0;
       for (int r = 0; r <= reps; ++r) {  // r = 0 warms up
-        if (s == ST_UPDATE)
+        if (s == ST_UPDATE || s == ST_MOL_UPD)
           (void)hipMemcpyAsync(L->state, saved, sbytes, hipMemcpyDeviceToDevice, st);
         (void)hipEventRecord(e0, st);
         launch_stage<EOS>(L, s, sc, P, st);
@@ -1427,6 +1593,7 @@ struct pc_batch {
   pc_ctx* ctx = nullptr;
   pc_params_t prm;
   int mixed = 0;
+  bool laxf = false;      // (the batch steps the Godunov path only)
   KArgs ka;
   LevelArgs la;
   int lowo[3] = {0, 0, 0};
@@ -1517,6 +1684,8 @@ int pc_batch_create_v1(pc_ctx_t* c, const pc_params_t* prm, int mixed,
       (mixed != 0 && mixed != 1))
     return fail(c, PC_ERR_ARG, "bad batch arguments");
   int rc;
+  if (prm->do_mol)
+    return fail(c, PC_ERR_UNSUPPORTED, "batches step the Godunov path (do_mol = 0)");
   if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder, prm->use_hybrid_weno)))
     return rc;
   if (!prm->allow_small_energy || !prm->allow_negative_energy)

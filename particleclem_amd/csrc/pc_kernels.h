@@ -15,6 +15,10 @@
 //   trans_(d|t)  d-faces [L, H+1] x t in [L, H] x t' in [L-1, H+1]
 //   final_d      d-faces [L, H+1] x transverse [L, H]
 //   update       cells [L, H]
+// and of the method-of-lines step:
+//   mol_prim     cells grow(bx,2)
+//   mol_d        d-faces [L, H+1] x transverse [L, H]
+//   mol_update   cells [L, H]
 #pragma once
 #include "pc_phys.h"
 
@@ -78,6 +82,7 @@ struct JobT {
   VT<double> area[3], vol;          // p == 0: uniform geometry
   VT<double> upd;                   // uout (dU/dt accumulator) or p == 0
   VT<double> divu;                  // nodes [L, H+1]^3
+  VT<double> sold;                  // MOL: U^n on the valid box (or p == 0)
 };
 using Job = JobT<double>;
 static_assert(sizeof(JobT<float>) == sizeof(JobT<double>),
@@ -194,6 +199,9 @@ enum Stage {
   ST_FINAL_X, ST_FINAL_Y, ST_FINAL_Z,
   ST_UPDATE,
   ST_GHOST,
+  ST_MOL_PRIM,
+  ST_MOL_X, ST_MOL_Y, ST_MOL_Z,
+  ST_MOL_UPD,
   ST_COUNT
 };
 
@@ -233,6 +241,14 @@ __host__ __device__ inline void stage_region(int st, const int L[3],
     }
     case ST_FINAL_X: case ST_FINAL_Y: case ST_FINAL_Z: {
       const int D = st - ST_FINAL_X;
+      set(D, L[D], H[D] + 1);
+      break;
+    }
+    case ST_MOL_PRIM:
+      for (int d = 0; d < 3; ++d) set(d, L[d] - 2, H[d] + 2);
+      break;
+    case ST_MOL_X: case ST_MOL_Y: case ST_MOL_Z: {
+      const int D = st - ST_MOL_X;
       set(D, L[D], H[D] + 1);
       break;
     }
@@ -294,13 +310,14 @@ PCD void step(int& i, int& j, int& k, int s) {
 
 // ------------------------------------------------------------ K1 ctoprim --
 // Source/PeleCUtilities.H:242-320
-template <class EOS, class FP>
+// (REG: the region's stage, ST_CTOPRIM or the MOL step's ST_MOL_PRIM)
+template <class EOS, class FP, int REG>
 PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
   const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
-  stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
+  stage_region(REG, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const auto& u = J.U;
   const auto& q = J.q;
@@ -347,14 +364,15 @@ PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
     qa(i, j, k, QRSPEC) = RU / w;
   }
 }
-template <class EOS, class FP = double, bool PERSIST = false>
+template <class EOS, class FP = double, bool PERSIST = false,
+          int REG = ST_CTOPRIM>
 __global__ void __launch_bounds__(BLOCK)
     k_ctoprim(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_ctoprim<EOS, FP>(sc, ka, lb_);
+      b_ctoprim<EOS, FP, REG>(sc, ka, lb_);
   } else {
-    b_ctoprim<EOS, FP>(sc, ka, first_block());
+    b_ctoprim<EOS, FP, REG>(sc, ka, first_block());
   }
 }
 

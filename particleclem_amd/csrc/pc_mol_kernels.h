@@ -1,0 +1,267 @@
+// Method-of-lines hydro stage kernels for gfx950.
+//
+// The MOL step (do_mol_advance) is a two-stage Heun integration of
+// dU/dt = -div(F)/V with fluxes from limited piecewise-linear face states
+// and the same approximate Riemann solver as the Godunov path (or the
+// Lax-Friedrichs flux).  Per stage the launches are
+//   ghost fill -> primitives on grow(bx,2) -> one flux launch per direction
+//   (d-faces [L, H+1] x transverse [L, H]) -> one update launch on [L, H].
+// A flux thread recomputes the slopes of the two cells of its face from
+// the primitives instead of reading a stored slope array: the slope of a
+// cell costs ~60 flops on values that sit in L2, storing it would cost
+// (4 + NS) words of HBM traffic per cell and direction.
+#pragma once
+#include "pc_level_kernels.h"
+
+namespace pcd {
+
+// The monotonised-central limiter of mol_slope (Source/MOL.H:105-113).
+template <class FP>
+PCD FP mol_lim(FP l, FP r) {
+  const FP dcen = FP(0.5) * (l + r);
+  const FP dlim = l * r >= FP(0.0) ? FP(2.0) * mn(fabs(l), fabs(r)) : FP(0.0);
+  return copysign(FP(1.0), dcen) * mn(dlim, fabs(dcen));
+}
+
+// Face state extrapolated half a cell from cell (i,j,k) in direction D:
+// the slope of the cell (Source/MOL.H:31-120, regular cell; zero when
+// `slopes` is off, as mol_iorder == 1 leaves dq at its setV value) and the
+// state of Source/MOL.cpp:61-104.  LEFT: qtempl of face c+1/2 (+ half a
+// slope); otherwise qtempr of face c-1/2 (- half a slope).  Outputs in the
+// normal frame: rho, un, ut (first transverse), utt (second), p, Y.
+template <int NS, int D, bool LEFT, class FP>
+PCD void mol_face_state(const VT<FP>& q, const VT<FP>& qa, bool slopes, int i,
+                        int j, int k, FP& rho, FP& un, FP& ut, FP& utt, FP& p,
+                        FP* Y) {
+  constexpr int IU = QU + D;
+  constexpr int IV = (D == 0) ? QV : QU;
+  constexpr int IV2 = (D == 2) ? QV : QW;
+  int im = i, jm = j, km = k, ip = i, jp = j, kp = k;
+  step<D>(im, jm, km, -1);
+  step<D>(ip, jp, kp, 1);
+  const FP rc = q(i, j, k, QRHO);
+  const FP cc = qa(i, j, k, QC);
+  const FP pc = q(i, j, k, QPRES);
+  const FP uc = q(i, j, k, IU), vc = q(i, j, k, IV), wc = q(i, j, k, IV2);
+  const FP rcc = FP(1.0) / cc;
+  FP d0 = FP(0.0), d1 = FP(0.0), d2 = FP(0.0), d3 = FP(0.0);
+  FP dY[NS];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) dY[n] = FP(0.0);
+  if (slopes) {
+    const FP dpl = pc - q(im, jm, km, QPRES);
+    const FP dpr = q(ip, jp, kp, QPRES) - pc;
+    const FP dul = uc - q(im, jm, km, IU);
+    const FP dur = q(ip, jp, kp, IU) - uc;
+    const FP al = div_by(FP(0.5) * dpl, cc, rcc);
+    const FP ar = div_by(FP(0.5) * dpr, cc, rcc);
+    const FP hr = FP(0.5) * rc;
+    d0 = mol_lim(al - hr * dul, ar - hr * dur);
+    d1 = mol_lim(al + hr * dul, ar + hr * dur);
+    d2 = mol_lim(vc - q(im, jm, km, IV), q(ip, jp, kp, IV) - vc);
+    d3 = mol_lim(wc - q(im, jm, km, IV2), q(ip, jp, kp, IV2) - wc);
+    const FP c2 = cc * cc;
+    const FP rc2 = FP(1.0) / c2;
+    const FP rm = q(im, jm, km, QRHO), rp = q(ip, jp, kp, QRHO);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const FP yc = q(i, j, k, QFS + n);
+      const FP l = rc * yc - rm * q(im, jm, km, QFS + n) - div_by(yc * dpl, c2, rc2);
+      const FP r = rp * q(ip, jp, kp, QFS + n) - rc * yc - div_by(yc * dpr, c2, rc2);
+      dY[n] = mol_lim(l, r);
+    }
+  }
+  const FP dsum = d0 + d1;
+  FP rY[NS];
+  FP rs = FP(0.0);
+  if constexpr (LEFT) {
+    un = uc + FP(0.5) * ((d1 - d0) / rc);
+    p = pc + FP(0.5) * dsum * cc;
+    ut = vc + FP(0.5) * d2;
+    utt = wc + FP(0.5) * d3;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const FP yc = q(i, j, k, QFS + n);
+      rY[n] = yc * rc + FP(0.5) * (dY[n] + div_by(yc * dsum, cc, rcc));
+      rs += rY[n];
+    }
+  } else {
+    un = uc - FP(0.5) * ((d1 - d0) / rc);
+    p = pc - FP(0.5) * dsum * cc;
+    ut = vc - FP(0.5) * d2;
+    utt = wc - FP(0.5) * d3;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const FP yc = q(i, j, k, QFS + n);
+      rY[n] = yc * rc - FP(0.5) * (dY[n] + div_by(yc * dsum, cc, rcc));
+      rs += rY[n];
+    }
+  }
+  const FP rrs = FP(1.0) / rs;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) Y[n] = div_by(rY[n], rs, rrs);
+  rho = rs;
+}
+
+// pc_compute_hyp_mol_flux for the d-faces of one box (Source/MOL.cpp:6-190,
+// regular cells, no rotating frame, no advected or linear scalars):
+// face (i,j,k) between cells (i,j,k)-e_D and (i,j,k); the extensive flux
+// goes to flx[D] (the flux register starts from zero: 0 + F * area).
+template <class EOS, int D, bool LAXF, class FP>
+PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_MOL_X + D, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  const bool slopes = ka.plm_iorder != 1;  // Source/Diffusion.cpp:371-373
+  int im = i, jm = j, km = k;
+  step<D>(im, jm, km, -1);
+  FP rl, ul, vl, v2l, pl, rr, ur, vr, v2r, pr;
+  FP Yl[NS], Yr[NS];
+  mol_face_state<NS, D, true, FP>(J.q, J.qa, slopes, im, jm, km, rl, ul, vl,
+                                  v2l, pl, Yl);
+  mol_face_state<NS, D, false, FP>(J.q, J.qa, slopes, i, j, k, rr, ur, vr,
+                                   v2r, pr, Yr);
+  RiemannOut<EOS, FP> o;
+  if constexpr (LAXF) {
+    o = laxf<EOS, FP>(rl, ul, vl, v2l, pl, Yl, rr, ur, vr, v2r, pr, Yr);
+  } else {
+    const FP cavg = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+    o = riemann<EOS, FP>(rl, ul, vl, v2l, pl, Yl, rr, ur, vr, v2r, pr, Yr, cavg);
+  }
+  constexpr int MU = UMX + D;
+  constexpr int MT = (D == 0) ? UMY : UMX;
+  constexpr int MTT = (D == 2) ? UMY : UMZ;
+  const double area = J.area[D].p ? J.area[D](i, j, k, 0) : J.area_u[D];
+  const auto& fo = J.flx[D];
+  fo(i, j, k, URHO) = 0.0 + (double)o.f_rho * area;
+  fo(i, j, k, MU) = 0.0 + (double)o.f_un * area;
+  fo(i, j, k, MT) = 0.0 + (double)o.f_ut * area;
+  fo(i, j, k, MTT) = 0.0 + (double)o.f_utt * area;
+  fo(i, j, k, UEDEN) = 0.0 + (double)o.f_eden * area;
+  fo(i, j, k, UEINT) = 0.0 + (double)o.f_eint * area;
+  fo(i, j, k, UTEMP) = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = 0.0 + (double)o.f_Y[s] * area;
+}
+template <class EOS, int D, bool LAXF, class FP = double, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_mol_flux(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_mol_flux<EOS, D, LAXF, FP>(sc, ka, lb_);
+  } else {
+    b_mol_flux<EOS, D, LAXF, FP>(sc, ka, first_block());
+  }
+}
+
+// pc_flux_div (Source/Diffterm.H:470-487): -div(F)/V of one cell.
+template <int NS>
+PCD void mol_divergence(const Job& J, int i, int j, int k, double* D) {
+  constexpr int NVAR = 7 + NS;
+  const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
+  const V& f0 = J.flx[0];
+  const V& f1 = J.flx[1];
+  const V& f2 = J.flx[2];
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m)
+    D[m] = -(f0(i + 1, j, k, m) - f0(i, j, k, m) + f1(i, j + 1, k, m) -
+             f1(i, j, k, m) + f2(i, j, k + 1, m) - f2(i, j, k, m)) /
+           v;
+}
+
+// The MOL source of one box into J.upd (getMOLSrcTerm's copy of Dterm to
+// MOLSrcTerm, Source/Diffusion.cpp:654: overwritten, not accumulated).
+template <int NS>
+PCD void b_mol_div(const Sched& sc, const KArgs& ka, int lb_) {
+  int local;
+  const Job J = load_job(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_MOL_UPD, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  double D[7 + NS];
+  mol_divergence<NS>(J, i, j, k, D);
+#pragma unroll
+  for (int m = 0; m < 7 + NS; ++m) J.upd(i, j, k, m) = D[m];
+}
+template <int NS, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_mol_div(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_mol_div<NS>(sc, ka, lb_);
+  } else {
+    b_mol_div<NS>(sc, ka, first_block());
+  }
+}
+
+// One Heun stage of do_mol_advance on a cell (Source/Advance.cpp:110-176):
+// STAGE 0: U* = 1.0 U^n + dt S(U^n) (U^n kept in J.sold), computeTemp;
+// STAGE 1: U^{n+1} = 0.5 U* + 0.5 U^n, += (0.5 dt) S(U*), computeTemp and
+// post_timestep's computeTemp, NaN check and the CFL partial minimum.
+template <class EOS, int STAGE>
+PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
+                        DtState*& ds) {
+  constexpr int NS = EOS::NS;
+  constexpr int NVAR = 7 + NS;
+  int local;
+  const Job J = load_job(sc, lb_, local);
+  if (J.ds) ds = J.ds;
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_MOL_UPD, J.lo, J.hi, lo, n);
+  double mydt = 1.7976931348623157e308;
+  if (region_ijk(sc, lo, n, local, i, j, k)) {
+    const double dt = get_dt(J, ka);
+    double D[NVAR];
+    mol_divergence<NS>(J, i, j, k, D);
+    double S[NVAR];
+    if constexpr (STAGE == 0) {
+#pragma unroll
+      for (int m = 0; m < NVAR; ++m) {
+        const double u = J.U(i, j, k, m);
+        J.sold(i, j, k, m) = u;
+        S[m] = 1.0 * u + dt * D[m];
+      }
+    } else {
+      const double hdt = 0.5 * dt;
+#pragma unroll
+      for (int m = 0; m < NVAR; ++m) {
+        S[m] = 0.5 * J.U(i, j, k, m) + 0.5 * J.sold(i, j, k, m);
+        S[m] += hdt * D[m];
+      }
+    }
+    double Y[NS];
+    state_Y<NS>(S, Y);
+    const typename EOS::Thermo th(Y);
+    reset_and_temp<EOS>(S, th);
+    if constexpr (STAGE == 1) reset_and_temp<EOS>(S, th);
+    bool bad = false;
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m) {
+      J.U(i, j, k, m) = S[m];
+      bad |= !isfinite(S[m]);
+    }
+    if constexpr (STAGE == 1) {
+      if (bad) ds->bad = 1;
+      mydt = cell_estdt<EOS>(S, th, J.dx);
+    }
+  }
+  return mydt;
+}
+
+template <class EOS, int STAGE, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_mol_update(Sched sc, KArgs ka, DtState* ds) {
+  double m = 1.7976931348623157e308;
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      m = mn(m, b_mol_update<EOS, STAGE>(sc, ka, lb_, ds));
+  } else {
+    m = b_mol_update<EOS, STAGE>(sc, ka, first_block(), ds);
+  }
+  if constexpr (STAGE == 1) block_min_to(m, &ds->minbits);
+}
+
+}  // namespace pcd

@@ -620,6 +620,7 @@ template <class EOS, class FP = double>
 struct RiemannOut {
   FP ustar, f_rho, f_un, f_ut, f_utt, f_eden, f_eint;
   FP u_gd, ut_gd, utt_gd, p_gd, game_gd;
+  FP f_Y[EOS::NS];  // uflx_rhoY (read by the MOL flux; dead code otherwise)
 };
 
 template <class EOS, class FP>
@@ -721,11 +722,55 @@ PCD RiemannOut<EOS, FP> riemann(FP rl, FP ul, FP vl, FP v2l,
       FP(0.5) * rgd * FP(0.0) * FP(0.0) * FP(0.0) * FP(0.0);
   o.f_eden = qiu * (rhoetot + qp_);
   o.f_eint = qiu * regd;
+#pragma unroll
+  for (int n = 0; n < NS; n++) o.f_Y[n] = rspgd[n] * qiu;
   o.ustar = ustar;
   o.u_gd = qiu;
   o.ut_gd = qv1;
   o.utt_gd = qv2;
   o.p_gd = qp_;
+  return o;
+}
+
+// laxfriedrich_flux (Source/Riemann.H:205-307) without a rotating frame:
+// the Rusanov flux with wave speed |u*| + (cl + cr)/2.  Same output record
+// as riemann(); only ustar and the fluxes are set.
+template <class EOS, class FP>
+PCD RiemannOut<EOS, FP> laxf(FP rl, FP ul, FP vl, FP v2l, FP pl, const FP* spl,
+                             FP rr, FP ur, FP vr, FP v2r, FP pr, const FP* spr) {
+  constexpr int NS = EOS::NS;
+  RiemannOut<EOS, FP> o;
+  const FP wsmall = rmin<FP>();
+  const FP cl = EOS::RPY2Cs(rl, pl, spl);
+  const FP cr = EOS::RPY2Cs(rr, pr, spr);
+  const FP el = EOS::RYP2E(rl, spl, pl);
+  const FP er = EOS::RYP2E(rr, spr, pr);
+  const FP r_elt = rl * (el + FP(0.5) * (ul * ul + vl * vl + v2l * v2l) -
+                         FP(0.5) * FP(0.0) * FP(0.0) * FP(0.0) * FP(0.0));
+  const FP r_ert = rr * (er + FP(0.5) * (ur * ur + vr * vr + v2r * v2r) -
+                         FP(0.5) * FP(0.0) * FP(0.0) * FP(0.0) * FP(0.0));
+  const FP wl = mx(wsmall, cl * rl);
+  const FP wr = mx(wsmall, cr * rr);
+  FP ustar = ((wl * ul + wr * ur) + (pl - pr)) / (wl + wr);
+  const bool tiny =
+      fabs(ustar) < FP(smallu) * FP(0.5) * (fabs(ul) + fabs(ur)) || ustar == FP(0.0);
+  ustar = tiny ? FP(0.0) : ustar;
+  const FP chalf = FP(0.5) * (cr + cl);
+  const FP a = fabs(ustar) + chalf;
+  o.ustar = ustar;
+  FP frho = FP(0.0);
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    o.f_Y[n] = FP(0.5) * ((rl * ul * spl[n] + rr * ur * spr[n]) -
+                          a * (rr * spr[n] - rl * spl[n]));
+    frho += o.f_Y[n];
+  }
+  o.f_rho = frho;
+  o.f_un = FP(0.5) * ((rl * ul * ul + pl + rr * ur * ur + pr) - a * (rr * ur - rl * ul));
+  o.f_ut = FP(0.5) * ((rl * vl * ul + rr * vr * ur) - a * (rr * vr - rl * vl));
+  o.f_utt = FP(0.5) * ((rl * v2l * ul + rr * v2r * ur) - a * (rr * v2r - rl * v2l));
+  o.f_eden = FP(0.5) * (((r_elt + pl) * ul + (r_ert + pr) * ur) - a * (r_ert - r_elt));
+  o.f_eint = FP(0.5) * ((rl * el * ul + rr * er * ur) - a * (rr * er - rl * el));
   return o;
 }
 

@@ -43,6 +43,9 @@ class Params(C.Structure):
         ("allow_negative_energy", C.c_int),
         ("dual_energy_update_E_from_e", C.c_int),
         ("dual_energy_eta2", C.c_double),
+        ("do_mol", C.c_int),
+        ("mol_iorder", C.c_int),
+        ("use_laxf_flux", C.c_int),
     ]
 
 
@@ -66,7 +69,7 @@ DEFAULTS = dict(
     weno_scheme=1, difmag=0.1, cfl=0.8, init_shrink=1.0, change_max=1.1,
     max_dt=1.0e200, fixed_dt=-1.0, dt_cutoff=0.0, allow_small_energy=1,
     allow_negative_energy=1, dual_energy_update_E_from_e=1,
-    dual_energy_eta2=1.0e-4,
+    dual_energy_eta2=1.0e-4, do_mol=0, mol_iorder=2, use_laxf_flux=0,
 )
 
 

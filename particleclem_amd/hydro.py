@@ -110,6 +110,18 @@ class HydroContext:
             float(dt), C.byref(prm), fl, aa, a4_from_tensor(*vol),
             _stream_ptr(stream)))
 
+    def mol_src_box(self, bx, sarr, molsrc, params, a, vol, flx=None,
+                    stream=None):
+        """getMOLSrcTerm of one box (pc_mol_src_box_v1): molsrc = -div(F)/vol
+        of the MOL fluxes of sarr; flx (three (tensor, lo) pairs) or None."""
+        aa = (Array4 * 3)(*[a4_from_tensor(*x) for x in a])
+        fl = (Array4 * 3)(*[a4_from_tensor(*f) for f in flx]) if flx else None
+        prm = params if not isinstance(params, dict) else make_params(**params)
+        self._chk(self.L.pc_mol_src_box_v1(
+            self.h, make_box(*bx), a4_from_tensor(*sarr),
+            a4_from_tensor(*molsrc), C.byref(prm), fl, aa,
+            a4_from_tensor(*vol), _stream_ptr(stream)))
+
 
 class DeviceLevel:
     """One AMR level (a box table) resident on the GPU (pc_level_t)."""

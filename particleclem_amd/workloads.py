@@ -214,3 +214,34 @@ def ragged(nboxes=300, seed=DEFAULT_SEED, nsteps=100, phi=0.8,
     return Workload("ragged", 9, boxes, [0, 0, 0], [0, 0, 0], bc, bc,
                     [1.0 / 32] * 3, 1, prims, periodic_self=True,
                     params=dict(ppm_type=ppm_type, cfl=cfl), nsteps=nsteps)
+
+
+def density_wave(n=32, ny=4, nz=4, amp=0.2, u=1.0, p=1.0, cfl=0.3,
+                 nsteps=10, max_grid=None):
+    """Smooth periodic advection, gamma-law gas: rho = 1 + amp sin(2 pi x),
+    uniform velocity u along x and pressure p, periodic in every direction.
+    The exact solution is the initial profile translated by u t, so the
+    error after a fixed time measures the order of the scheme."""
+    dx = [1.0 / n] * 3
+    domhi = [n - 1, ny - 1, nz - 1]
+    mg = max_grid or max(n, ny, nz)
+    boxes = []
+    for k0 in range(0, nz, mg):
+        for j0 in range(0, ny, mg):
+            for i0 in range(0, n, mg):
+                boxes.append(([i0, j0, k0], [min(i0 + mg, n) - 1,
+                                             min(j0 + mg, ny) - 1,
+                                             min(k0 + mg, nz) - 1]))
+    prims = []
+    for lo, hi in boxes:
+        xs = _box_centres(lo, hi, dx)
+        X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")[2]
+        P = np.zeros((6,) + X.shape)
+        P[0] = 1.0 + amp * np.sin(2.0 * np.pi * X)
+        P[1] = p
+        P[2] = u
+        P[5] = 1.0
+        prims.append(np.ascontiguousarray(P))
+    bc = [BC["interior"]] * 3
+    return Workload("density_wave", 1, boxes, [0, 0, 0], domhi, bc, list(bc),
+                    dx, 0, prims, params=dict(cfl=cfl), nsteps=nsteps)

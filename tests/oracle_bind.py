@@ -52,6 +52,9 @@ def lib():
     L.pc_ref_ppm_int_profile.argtypes = [C.c_double] * 6 + [d3, d3]
     L.pc_ref_plm_slope_line.argtypes = [d3, C.c_double, C.c_int]
     L.pc_ref_plm_slope_line.restype = C.c_double
+    L.pc_ref_mol_src_box.argtypes = [C.c_int, Box, Array4, Array4, P(Params),
+                                     P(Array4), P(Array4), Array4]
+    L.pc_ref_laxf.argtypes = [C.c_int, d3, d3, d3, d3, d3]
     _lib = L
     return L
 
@@ -91,6 +94,48 @@ def riemann(kind, left5, spl, right5, spr, cav=1.0):
     keys = ["ustar", "f_rho", "f_un", "f_ut", "f_utt", "f_eden", "f_eint",
             "u_gd", "ut_gd", "utt_gd", "p_gd", "game_gd"]
     return dict(zip(keys, out))
+
+
+def laxf(kind, left5, spl, right5, spr):
+    out = np.zeros(8 + kind)
+    l5 = np.ascontiguousarray(left5, dtype=np.float64)
+    r5 = np.ascontiguousarray(right5, dtype=np.float64)
+    sl = np.ascontiguousarray(spl, dtype=np.float64)
+    sr = np.ascontiguousarray(spr, dtype=np.float64)
+    assert lib().pc_ref_laxf(kind, dptr(l5), dptr(sl), dptr(r5), dptr(sr),
+                             dptr(out)) == 0
+    keys = ["ustar", "maxeig", "f_rho", "f_un", "f_ut", "f_utt", "f_eden",
+            "f_eint"]
+    d = dict(zip(keys, out[:8]))
+    d["f_rhoY"] = out[8:].copy()
+    return d
+
+
+def mol_src_box(kind, S, lo, hi, dx, prm, want_flux=False):
+    """MOL source term of one box: S is the (NVAR, nz+8, ny+8, nx+8) state
+    with 4 ghosts around [lo, hi]; returns molsrc on the box (and the three
+    extensive face-flux arrays when want_flux)."""
+    nvar = 7 + kind
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    glo = [x - 4 for x in lo]
+    src = np.zeros((nvar, n[2], n[1], n[0]))
+    vol = np.full((1, n[2], n[1], n[0]), dx[0] * dx[1] * dx[2])
+    area_v = [dx[1] * dx[2], dx[2] * dx[0], dx[0] * dx[1]]
+    areas, flx = [], []
+    for d in range(3):
+        m = list(n)
+        m[d] += 1
+        areas.append(np.full((1, m[2], m[1], m[0]), area_v[d]))
+        flx.append(np.zeros((nvar, m[2], m[1], m[0])))
+    A = (Array4 * 3)(*[a4_from_numpy(a, lo) for a in areas])
+    F = (Array4 * 3)(*[a4_from_numpy(f, lo) for f in flx])
+    rc = lib().pc_ref_mol_src_box(kind, make_box(lo, hi), a4_from_numpy(S, glo),
+                                  a4_from_numpy(src, lo), C.byref(prm),
+                                  F if want_flux else None, A,
+                                  a4_from_numpy(vol, lo))
+    assert rc == 0, rc
+    return (src, flx) if want_flux else src
 
 
 def params(**kw):
