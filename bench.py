@@ -18,6 +18,10 @@ is the whole step against HBM on the reference's algorithmic bytes
 peak on the counted FP64 operations of one step (same profile).  `cpu_baseline` is the CPU oracle (this repository's C++
 restatement of the reference path, OpenMP over boxes) on one step of the
 same level, timed on this host.
+
+`--mol` runs the method-of-lines step instead (do_mol = 1: two Heun stages
+of ghost fill, primitives, PLM-MOL fluxes and update per step; `--laxf`
+for the Lax-Friedrichs flux); the oracle baseline then runs the same step.
 """
 import argparse
 import json
@@ -33,9 +37,12 @@ FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
 METRIC = ("Mcell-updates/s per hydro step + % HBM roofline; "
           "p99 ensemble-solve latency")
 PROFILE_SUMMARY = "premixed128_step_kernels.md"
+PROFILE_SUMMARY_MOL = "premixed128_mol_step_kernels.md"
+# launches per step of the kernels the MOL step runs in both Heun stages
+MOL_LAUNCHES = {"ghost": 2, "ctoprim": 2, "mol_x": 2, "mol_y": 2, "mol_z": 2}
 
 
-def stage_bytes(nspec, boxes):
+def stage_bytes(nspec, boxes, mol=False):
     """Compulsory bytes of one launch of each kernel over the level: every
     input element read once and every output element written once on the
     kernel's region (stencil re-reads are not counted)."""
@@ -49,6 +56,21 @@ def stage_bytes(nspec, boxes):
         n = [hi[d] - lo[d] + 1 for d in range(3)]
         g4 = (n[0] + 8) * (n[1] + 8) * (n[2] + 8)
         v = n[0] * n[1] * n[2]
+        if mol:
+            # ghost fill; primitives on grow(bx,2); per d-face the primitives
+            # the slopes read (rho, u, v, w, p, Y, c) of one cell and the
+            # NVAR fluxes written; the updates read U, the fluxes (and U^n)
+            # and write U (and U^n)
+            add("ghost", g4 - v, 2 * N)
+            add("ctoprim", (n[0] + 4) * (n[1] + 4) * (n[2] + 4), N + Q + 6)
+            for d, a in enumerate("xyz"):
+                f = list(n)
+                f[d] += 1
+                add("mol_" + a, f[0] * f[1] * f[2], 6 + nspec + N)
+            add("mol_update1", v, 6 * N)
+            add("mol_update2", v, 6 * N)
+            add("dt", 0, 0)
+            continue
         g1 = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
         add("ghost", g4 - v, 2 * N)
         add("ctoprim", g4, N + Q + 6)
@@ -258,6 +280,10 @@ def main():
     ap.add_argument("--nboxes", type=int, default=300,
                     help="ragged workload: boxes in the table")
     ap.add_argument("--ppm-type", type=int, default=1)
+    ap.add_argument("--mol", action="store_true",
+                    help="method-of-lines step (do_mol = 1) instead of CTU")
+    ap.add_argument("--laxf", action="store_true",
+                    help="MOL with the Lax-Friedrichs flux (use_laxf_flux = 1)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick every stage's launch shape on the level first")
     ap.add_argument("--graph", type=int, default=0,
@@ -272,7 +298,7 @@ def main():
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--box", type=int, default=32)
     ap.add_argument("--kernel-steps", type=int, default=3)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (the "
@@ -315,6 +341,13 @@ def main():
         w = workloads.sod(n=n, nsteps=args.steps)
         desc = (f"3D Sod, gamma-law gas, {n}^3 single box + 4 ghosts, PPM, "
                 f"approximate Riemann solver, CFL 0.3")
+
+    if args.mol:
+        w.params.update(do_mol=1, use_laxf_flux=int(args.laxf))
+        desc = desc.replace("PPM, " if "PPM, " in desc else "PLM, ", "")
+        desc += (", method of lines (do_mol: PLM-MOL face states, two-stage "
+                 f"Heun), {'Lax-Friedrichs flux' if args.laxf else 'approximate Riemann solver'}")
+        desc = desc.replace(", approximate Riemann solver, CFL", ", CFL")
 
     if args.copy_calibration_mib:
         # two copies of known size (read N bytes, write N bytes) whose HBM
@@ -375,25 +408,30 @@ def main():
     torch.cuda.synchronize()
     kt = lev.kernel_times()
     lev.enable_timing(False)
-    kbytes = stage_bytes(w.nspec, w.boxes)
+    kbytes = stage_bytes(w.nspec, w.boxes, mol=args.mol)
+    launches = MOL_LAUNCHES if args.mol else {}
     dom = max(kt, key=lambda k: kt[k])
-    dom_s = kt[dom] * 1e-3
-    achieved = kbytes[dom] / dom_s
+    dom_ms = kt[dom] / launches.get(dom, 1)   # average launch duration
+    achieved = kbytes[dom] / (dom_ms * 1e-3)
+    summary = PROFILE_SUMMARY_MOL if args.mol else PROFILE_SUMMARY
     traffic = None
     if (args.workload == "premixed" and n == 128 and args.box == 32
-            and not args.autotune):
-        traffic = profiled_traffic(PROFILE_SUMMARY, dom)
-    roof = {"bound": "hbm", "kernel": f"k_{dom.split('_')[0]} ({dom})",
+            and not args.autotune and not args.laxf):
+        traffic = profiled_traffic(summary, dom)
+    kname = ("k_mol_update" if dom.startswith("mol_update") else
+             "k_mol_flux" if dom.startswith("mol_") else "k_" + dom.split("_")[0])
+    roof = {"bound": "hbm", "kernel": f"{kname} ({dom})",
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "traffic_source": (f"profiles/{PROFILE_SUMMARY}" if traffic else None),
-            "bytes_per_launch": kbytes[dom], "ms_per_launch": kt[dom]}
+            "traffic_source": (f"profiles/{summary}" if traffic else None),
+            "bytes_per_launch": kbytes[dom], "ms_per_launch": dom_ms,
+            "launches_per_step": launches.get(dom, 1)}
     nvar = w.nvar
     step_bytes = cells * 2 * nvar * 8
     hbm_frac_step = step_bytes / (ms_step * 1e-3) / HBM_PEAK
     fp64 = None
     if traffic is not None:
-        fl = profiled_step_flops(PROFILE_SUMMARY)
+        fl = profiled_step_flops(summary)
         if fl:
             fp64 = {"flop_per_step": fl, "achieved_tflops": fl / (ms_step * 1e-3) / 1e12,
                     "peak_tflops": FP64_PEAK / 1e12,

@@ -3,9 +3,12 @@
   HydroContext.ctoprim / srctoprim / umdrv / hydro_source_box
       -> pc_ctoprim, pc_srctoprim, pc_umdrv (Source/Hydro.H:910-935) and one
          box of PeleC::construct_hydro_source (Source/Hydro.cpp:73-252)
+  HydroContext.mol_src_box
+      -> one box of PeleC::getMOLSrcTerm, hydro only (Source/Diffusion.cpp:90-654)
   DeviceLevel.step / init_dt / estdt / fill_ghosts
       -> do_sdc_iteration's hydro path + computeTemp + computeNewDt
-         (Source/Advance.cpp:266-416, Source/PeleC.cpp:811-1034)
+         (Source/Advance.cpp:266-416, Source/PeleC.cpp:811-1034), or with
+         do_mol=1 do_mol_advance (Source/Advance.cpp:49-227)
 
 Device arrays are torch tensors (HIP memory) shaped (ncomp, nz, ny, nx),
 i.e. AMReX FAB layout; `lo` gives the index of element [.., 0, 0, 0].
