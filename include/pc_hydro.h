@@ -57,7 +57,8 @@ enum {
   PC_ERR_UNSUPPORTED = -7, /* option outside what this build implements     */
   PC_ERR_NOMEM = -8,       /* device allocation failed                      */
   PC_ERR_DT = -9,          /* dt fell below dt_cutoff (okToContinue)        */
-  PC_ERR_MOL_IORDER = -10  /* do_mol with mol_iorder not 1 or 2             */
+  PC_ERR_MOL_IORDER = -10, /* do_mol with mol_iorder not 1 or 2             */
+  PC_ERR_IO = -11          /* a file could not be written                   */
 };
 
 /* Equation of state / mechanism (compile-time NUM_SPECIES in the reference:
@@ -228,6 +229,31 @@ int pc_level_fill_ghosts_v1(pc_level_t* lev, void* stream);
 /* estTimeStep over the valid cells of the level: cfl*min(dx/(c+|u|)),
  * limited by max_dt (Source/PeleC.cpp:829-961).  Synchronous. */
 int pc_level_estdt_v1(pc_level_t* lev, double* dt_out, void* stream);
+/* The level as an AMReX plotfile directory `dir` (Header, Level_0/Cell_H,
+ * Level_0/Cell_D_00000; the format PeleCAmr::writePlotFileDoit writes
+ * through amrex::WriteMultiLevelPlotfile, Source/PeleCAmr.cpp:225-267), for
+ * fcompare or any plotfile reader.  vars: space-separated names, NULL/"" for
+ * every state component (amr.plot_vars' default); state names density xmom
+ * ymom zmom rho_E rho_e Temp rho_<species>; derived (Source/Derive.cpp)
+ * pressure kineng soundspeed MachNumber magvort logden Y(<species>) (or
+ * massfrac for all) x_velocity y_velocity z_velocity magvel; ALL for every
+ * one.  Variables are written in the reference's order (state components,
+ * then derived in derive-list order).  Species: H2 O2 H2O H O OH HO2 H2O2 N2
+ * (LiDryer), N2 (gamma law).  prob_lo: physical corner of the domain (NULL:
+ * origin); time and step label the plotfile.  Synchronous (derives on the
+ * device on `stream`, then writes). */
+int pc_level_write_plotfile_v1(pc_level_t* lev, const char* dir,
+                               const char* vars, double time, int step,
+                               const double* prob_lo, void* stream);
+
+/* The same format from host arrays: data[b] holds nvars components of box
+ * boxes[b] (AMReX FAB layout, i fastest, component slowest). */
+int pc_write_plotfile_v1(const char* dir, int nvars, const char* const* names,
+                         int nboxes, const pc_box_t* boxes,
+                         const double* const* data, const int* domlo,
+                         const int* domhi, const double* prob_lo,
+                         const double* dx, double time, int step);
+
 /* Per-kernel device time: with timing on, every launch of a step is
  * bracketed by HIP events on the step's stream and the step waits for its
  * last event (so timed steps do not overlap the host).  Kernel ids, in

@@ -437,6 +437,77 @@ int mol_src_box_t(const Box& bx, const A4& S, const A4& src,
   return 0;
 }
 
+// Plot variables of Source/Derive.cpp on the valid cells of one box (U with
+// at least one filled ghost cell for magvort).  codes: 0 state comp,
+// 1 comp / rho (pc_derdividebyrho), 2 magvel (:10-31), 3 kineng (:109-128),
+// 4 pressure (:531-555), 5 soundspeed (:446-469), 6 MachNumber (:498-529),
+// 7 magvort (:201-271, regular cells), 8 logden (:181-198).
+template <class EOS>
+int derive_t(const Box& bx, const A4& U, const int* codes, const int* comps,
+             int nplot, const A4& out, const double* dx) {
+  constexpr int NS = EOS::NS;
+  auto vel = [&](int i, int j, int k, int c) {
+    const double rhoInv = 1.0 / U(i, j, k, URHO);
+    return U(i, j, k, UMX + c) * rhoInv;
+  };
+  ParallelFor(bx, [&](int i, int j, int k) {
+    const double rho = U(i, j, k, URHO);
+    const double rhoInv = 1.0 / rho;
+    const double T = U(i, j, k, UTEMP);
+    double Y[NS];
+    for (int n = 0; n < NS; ++n) Y[n] = U(i, j, k, UFS + n) * rhoInv;
+    for (int m = 0; m < nplot; ++m) {
+      const int c = comps[m];
+      double v = 0.0;
+      switch (codes[m]) {
+        case 0: v = U(i, j, k, c); break;
+        case 1: v = U(i, j, k, c) / U(i, j, k, URHO); break;
+        case 2: {
+          const double d1 = U(i, j, k, UMX) * rhoInv, d2 = U(i, j, k, UMY) * rhoInv,
+                       d3 = U(i, j, k, UMZ) * rhoInv;
+          v = std::sqrt((d1 * d1) + (d2 * d2) + (d3 * d3));
+          break;
+        }
+        case 3: {
+          const double x = U(i, j, k, UMX) * U(i, j, k, UMX),
+                       y = U(i, j, k, UMY) * U(i, j, k, UMY),
+                       z = U(i, j, k, UMZ) * U(i, j, k, UMZ);
+          v = 0.5 / U(i, j, k, URHO) * (x + y + z);
+          break;
+        }
+        case 4: EOS::RTY2P(rho, T, Y, v); break;
+        case 5: EOS::RTY2Cs(rho, T, Y, v); break;
+        case 6: {
+          double cs;
+          EOS::RTY2Cs(rho, T, Y, cs);
+          const double x = U(i, j, k, UMX) * U(i, j, k, UMX),
+                       y = U(i, j, k, UMY) * U(i, j, k, UMY),
+                       z = U(i, j, k, UMZ) * U(i, j, k, UMZ);
+          v = std::sqrt(x + y + z) / U(i, j, k, URHO) / cs;
+          break;
+        }
+        case 7: {
+          const double wi = 0.5, wj = 0.5, wk = 0.5;  // get_weight(i-1, i+1)
+          const double vx = wi * (vel(i + 1, j, k, 1) - vel(i - 1, j, k, 1)) / dx[0];
+          const double uy = wj * (vel(i, j + 1, k, 0) - vel(i, j - 1, k, 0)) / dx[1];
+          const double v3 = vx - uy;
+          const double wx = wi * (vel(i + 1, j, k, 2) - vel(i - 1, j, k, 2)) / dx[0];
+          const double wy = wj * (vel(i, j + 1, k, 2) - vel(i, j - 1, k, 2)) / dx[1];
+          const double uz = wk * (vel(i, j, k + 1, 0) - vel(i, j, k - 1, 0)) / dx[2];
+          const double vz = wk * (vel(i, j, k + 1, 1) - vel(i, j, k - 1, 1)) / dx[2];
+          const double v1 = wy - vz, v2 = uz - wx;
+          v = std::sqrt(0. + v3 * v3 + v1 * v1 + v2 * v2);
+          break;
+        }
+        case 8: v = std::log10(U(i, j, k, URHO)); break;
+        default: return;
+      }
+      out(i, j, k, m) = v;
+    }
+  });
+  return 0;
+}
+
 }  // namespace
 
 #define DISPATCH(eos, call)                                   \
@@ -624,6 +695,18 @@ int pc_ref_mol_src_box(int eos, pc_box_t bx, pc_array4_t sarr,
   int rc = 0;
   DISPATCH(eos, rc = mol_src_box_t<E>(b, to_a4(sarr), to_a4(molsrc), *prm,
                                       flx ? F : nullptr, AA, to_a4(vol)));
+  return rc;
+}
+
+int pc_ref_derive(int eos, pc_box_t bx, pc_array4_t u, const int* codes,
+                  const int* comps, int nplot, pc_array4_t out,
+                  const double* dx) {
+  const Box b = to_box(bx);
+  if (!covers(u, grow(b, 1)) || !covers(out, b) || nplot <= 0) return PC_ERR_ARG;
+  for (int m = 0; m < nplot; ++m)
+    if (codes[m] < 0 || codes[m] > 8) return PC_ERR_ARG;
+  int rc = 0;
+  DISPATCH(eos, rc = derive_t<E>(b, to_a4(u), codes, comps, nplot, to_a4(out), dx));
   return rc;
 }
 

@@ -82,6 +82,12 @@ int pc_ref_mol_src_box(int eos, pc_box_t bx, pc_array4_t sarr,
                        pc_array4_t molsrc, const pc_params_t* prm,
                        const pc_array4_t* flx, const pc_array4_t* a,
                        pc_array4_t vol);
+/* Plot variables of Source/Derive.cpp on the cells of bx (u covering
+ * grow(bx,1)): codes 0 state comp `comps[m]`, 1 comps[m] / rho, 2 magvel,
+ * 3 kineng, 4 pressure, 5 soundspeed, 6 MachNumber, 7 magvort, 8 logden. */
+int pc_ref_derive(int eos, pc_box_t bx, pc_array4_t u, const int* codes,
+                  const int* comps, int nplot, pc_array4_t out,
+                  const double* dx);
 /* laxfriedrich_flux of Source/Riemann.H:205-307: out[0..] = ustar,
  * maxeigval, uflx_rho, uflx_u, uflx_v, uflx_w, uflx_eden, uflx_eint,
  * uflx_rhoY[NS] */

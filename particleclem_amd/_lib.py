@@ -22,6 +22,7 @@ EXPORTS = [
     "pc_level_state_size_v1", "pc_level_write_box_v1", "pc_level_read_box_v1",
     "pc_level_set_dt_v1", "pc_level_get_dt_v1", "pc_level_init_dt_v1",
     "pc_level_step_v1", "pc_level_fill_ghosts_v1", "pc_level_estdt_v1",
+    "pc_level_write_plotfile_v1", "pc_write_plotfile_v1",
     "pc_level_enable_timing_v1", "pc_level_kernel_times_v1",
     "pc_level_status_v1", "pc_level_prim_to_cons_v1",
     "pc_level_autotune_v1", "pc_level_plan_v1", "pc_level_set_plan_v1",
@@ -45,7 +46,7 @@ STAGES = ["ctoprim", "divu", "trace", "flux1_x", "flux1_y", "flux1_z",
 ERRORS = {-1: "PC_ERR_ARG", -2: "PC_ERR_GROW", -3: "PC_ERR_PPM_TYPE",
           -4: "PC_ERR_PLM_IORDER", -5: "PC_ERR_HIP", -6: "PC_ERR_NAN",
           -7: "PC_ERR_UNSUPPORTED", -8: "PC_ERR_NOMEM", -9: "PC_ERR_DT",
-          -10: "PC_ERR_MOL_IORDER"}
+          -10: "PC_ERR_MOL_IORDER", -11: "PC_ERR_IO"}
 
 _lib = None
 
@@ -90,6 +91,11 @@ def lib():
                                          Array4, vp]
     L.pc_mol_src_box_v1.argtypes = [vp, Box, Array4, Array4, P(Params),
                                     P(Array4), P(Array4), Array4, vp]
+    L.pc_level_write_plotfile_v1.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double,
+                                             C.c_int, d3, vp]
+    L.pc_write_plotfile_v1.argtypes = [C.c_char_p, C.c_int, P(C.c_char_p), C.c_int,
+                                       P(Box), P(P(C.c_double)), i3, i3, d3, d3,
+                                       C.c_double, C.c_int]
     L.pc_level_autotune_v1.argtypes = [vp, C.c_int, vp]
     L.pc_level_plan_v1.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_double)]
     L.pc_level_set_graph_v1.argtypes = [vp, C.c_int]

@@ -8,11 +8,14 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 
 #include "../../include/pc_hydro.h"
+#include "pc_derive_kernels.h"
 #include "pc_mol_kernels.h"
+#include "pc_plotfile.h"
 
 using namespace pcd;
 
@@ -1495,6 +1498,189 @@ int pc_level_kernel_times_v1(pc_level_t* L, double* ms, int n) {
   for (int s = 0; s < n && s < PC_NKERN; ++s)
     ms[s] = L->nsteps_timed ? L->ktime[s] / L->nsteps_timed : 0.0;
   return PC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- plotfiles --
+// Plot variables this writer derives, in the order the reference's plotfile
+// lists them: state components in component order (Source/Setup.cpp:292-360),
+// then derived quantities in derive-list registration order (:440-560).
+struct PlotVar {
+  std::string name;
+  int code, comp;
+};
+
+static const char* species_name(int ns, int s) {
+  // LiDryer mechanism order; the single gamma-law species carries N2's
+  // molecular weight
+  static const char* ld[9] = {"H2", "O2", "H2O", "H", "O", "OH", "HO2", "H2O2", "N2"};
+  return ns == 9 ? ld[s] : "N2";
+}
+
+static std::vector<PlotVar> plot_catalog(int ns) {
+  std::vector<PlotVar> v;
+  const char* st[7] = {"density", "xmom", "ymom", "zmom", "rho_E", "rho_e", "Temp"};
+  for (int c = 0; c < 7; ++c) v.push_back({st[c], DV_STATE, c});
+  for (int s = 0; s < ns; ++s)
+    v.push_back({std::string("rho_") + species_name(ns, s), DV_STATE, UFS + s});
+  v.push_back({"pressure", DV_PRESSURE, 0});
+  v.push_back({"kineng", DV_KINENG, 0});
+  v.push_back({"soundspeed", DV_SOUNDSPEED, 0});
+  v.push_back({"MachNumber", DV_MACH, 0});
+  v.push_back({"magvort", DV_MAGVORT, 0});
+  v.push_back({"logden", DV_LOGDEN, 0});
+  for (int s = 0; s < ns; ++s)
+    v.push_back({std::string("Y(") + species_name(ns, s) + ")", DV_DIVRHO, UFS + s});
+  v.push_back({"x_velocity", DV_DIVRHO, UMX});
+  v.push_back({"y_velocity", DV_DIVRHO, UMY});
+  v.push_back({"z_velocity", DV_DIVRHO, UMZ});
+  v.push_back({"magvel", DV_MAGVEL, 0});
+  return v;
+}
+
+extern "C" {
+
+int pc_level_write_plotfile_v1(pc_level_t* L, const char* dir, const char* vars,
+                               double time, int step, const double* prob_lo,
+                               void* stream) {
+  if (!L || !dir || !*dir) return PC_ERR_ARG;
+  pc_ctx* c = L->ctx;
+  const int ns = c->ns;
+  const std::vector<PlotVar> cat = plot_catalog(ns);
+  std::vector<char> sel(cat.size(), 0);
+  if (!vars || !*vars) {
+    for (int k = 0; k < 7 + ns; ++k) sel[k] = 1;  // every state component
+  } else {
+    std::istringstream in(vars);
+    std::string tok;
+    while (in >> tok) {
+      bool hit = false;
+      for (size_t k = 0; k < cat.size(); ++k) {
+        const bool spec = cat[k].name.compare(0, 2, "Y(") == 0;
+        if (tok == "ALL" || tok == cat[k].name || (tok == "massfrac" && spec)) {
+          sel[k] = 1;
+          hit = true;
+        }
+      }
+      if (!hit) return fail(c, PC_ERR_ARG, "unknown plot variable '%s'", tok.c_str());
+    }
+  }
+  DeriveArgs da;
+  memset(&da, 0, sizeof(da));
+  pcplot::PlotMeta meta;
+  bool vort = false;
+  for (size_t k = 0; k < cat.size(); ++k) {
+    if (!sel[k]) continue;
+    if (da.nplot == MAX_PLOT) return fail(c, PC_ERR_ARG, "more than %d plot variables", MAX_PLOT);
+    da.code[da.nplot] = cat[k].code;
+    da.comp[da.nplot] = cat[k].comp;
+    da.nplot++;
+    meta.names.push_back(cat[k].name);
+    vort |= cat[k].code == DV_MAGVORT;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if (vort && (rc = level_fill(L, st))) return rc;  // magvort reads one ghost cell
+  const int nb = (int)L->boxes.size();
+  std::vector<long long> off(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) off[b + 1] = off[b] + L->boxes[b].npts() * da.nplot;
+  double* dout = nullptr;
+  Job* djobs = nullptr;
+  StagePlan P;
+  auto cleanup = [&]() {
+    if (dout) (void)hipFree(dout);
+    if (djobs) (void)hipFree(djobs);
+    plan_free(P);
+  };
+  if (hipMalloc(&dout, sizeof(double) * off[nb]) != hipSuccess ||
+      hipMalloc(&djobs, sizeof(Job) * nb) != hipSuccess) {
+    (void)hipGetLastError();
+    cleanup();
+    return fail(c, PC_ERR_NOMEM, "plotfile staging allocation failed");
+  }
+  std::vector<Job> jobs = L->hjobs;
+  for (int b = 0; b < nb; ++b) jobs[b].upd = fab_view(dout + off[b], L->boxes[b]);
+  if (hipMemcpyAsync(djobs, jobs.data(), sizeof(Job) * nb, hipMemcpyHostToDevice,
+                     st) != hipSuccess ||
+      !plan_make(L, ST_UPDATE, -1, 0, 0, 0, P)) {
+    cleanup();
+    return fail(c, PC_ERR_HIP, "plotfile job table upload failed");
+  }
+  const Sched sc{djobs, P.blkmap, P.blkstart, P.nblocks, -1, 0, 0, nb};
+  if (ns == 1) LAUNCH(k_derive<EosGamma>, P.nblocks, st, sc, L->ka, da);
+  else LAUNCH(k_derive<EosLiDryer>, P.nblocks, st, sc, L->ka, da);
+  std::vector<double> host((size_t)off[nb]);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(host.data(), dout, sizeof(double) * off[nb], hipMemcpyDeviceToHost) !=
+          hipSuccess) {
+    cleanup();
+    return fail(c, PC_ERR_HIP, "plot variable derivation failed");
+  }
+  cleanup();
+  // the level's domain (or the bounding box of self-periodic boxes)
+  for (int d = 0; d < 3; ++d) {
+    meta.domlo[d] = L->la.dlo[d];
+    meta.domhi[d] = L->la.dhi[d];
+    if (L->la.periodic_self) {
+      meta.domlo[d] = L->boxes[0].lo[d];
+      meta.domhi[d] = L->boxes[0].hi[d];
+      for (const IBox& b : L->boxes) {
+        meta.domlo[d] = std::min(meta.domlo[d], b.lo[d]);
+        meta.domhi[d] = std::max(meta.domhi[d], b.hi[d]);
+      }
+    }
+    meta.prob_lo[d] = prob_lo ? prob_lo[d] : 0.0;
+    meta.dx[d] = L->hjobs[0].dx[d];
+  }
+  meta.time = time;
+  meta.step = step;
+  std::vector<pcplot::PlotBox> pb(nb);
+  for (int b = 0; b < nb; ++b) {
+    for (int d = 0; d < 3; ++d) {
+      pb[b].lo[d] = L->boxes[b].lo[d];
+      pb[b].hi[d] = L->boxes[b].hi[d];
+    }
+    pb[b].data = host.data() + off[b];
+  }
+  std::string msg;
+  if (pcplot::write_plotfile(dir, meta, pb, msg) != 0)
+    return fail(c, PC_ERR_IO, "%s", msg.c_str());
+  return PC_OK;
+}
+
+int pc_write_plotfile_v1(const char* dir, int nvars, const char* const* names,
+                         int nboxes, const pc_box_t* boxes,
+                         const double* const* data, const int* domlo,
+                         const int* domhi, const double* prob_lo,
+                         const double* dx, double time, int step) {
+  if (!dir || !*dir || nvars <= 0 || !names || nboxes <= 0 || !boxes || !data ||
+      !domlo || !domhi || !dx)
+    return PC_ERR_ARG;
+  pcplot::PlotMeta meta;
+  for (int v = 0; v < nvars; ++v) {
+    if (!names[v] || !*names[v]) return PC_ERR_ARG;
+    meta.names.push_back(names[v]);
+  }
+  for (int d = 0; d < 3; ++d) {
+    meta.domlo[d] = domlo[d];
+    meta.domhi[d] = domhi[d];
+    meta.prob_lo[d] = prob_lo ? prob_lo[d] : 0.0;
+    meta.dx[d] = dx[d];
+  }
+  meta.time = time;
+  meta.step = step;
+  std::vector<pcplot::PlotBox> pb(nboxes);
+  for (int b = 0; b < nboxes; ++b) {
+    if (!box_ok(boxes[b]) || !data[b]) return PC_ERR_ARG;
+    for (int d = 0; d < 3; ++d) {
+      pb[b].lo[d] = boxes[b].lo[d];
+      pb[b].hi[d] = boxes[b].hi[d];
+    }
+    pb[b].data = data[b];
+  }
+  std::string msg;
+  return pcplot::write_plotfile(dir, meta, pb, msg) == 0 ? PC_OK : PC_ERR_IO;
 }
 
 int pc_level_status_v1(pc_level_t* L, int* bad, int* small_dt) {

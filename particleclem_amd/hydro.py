@@ -267,6 +267,17 @@ class DeviceLevel:
         """Replay steps from a hipGraph of `steps_per_graph` steps (0: off)."""
         self._chk(self.L.pc_level_set_graph_v1(self.h, int(steps_per_graph)))
 
+    def write_plotfile(self, path, variables=None, time=0.0, step=0,
+                       prob_lo=(0.0, 0.0, 0.0), stream=None):
+        """AMReX plotfile directory `path` of the level (pc_level_write_plotfile_v1):
+        variables = list/str of plot variable names (None: the state)."""
+        if isinstance(variables, (list, tuple)):
+            variables = " ".join(variables)
+        v = variables.encode() if variables else None
+        self._chk(self.L.pc_level_write_plotfile_v1(
+            self.h, str(path).encode(), v, float(time), int(step), _d3(prob_lo),
+            _stream_ptr(stream)))
+
     def enable_timing(self, on=True):
         self._chk(self.L.pc_level_enable_timing_v1(self.h, 1 if on else 0))
 

@@ -55,6 +55,7 @@ def lib():
     L.pc_ref_mol_src_box.argtypes = [C.c_int, Box, Array4, Array4, P(Params),
                                      P(Array4), P(Array4), Array4]
     L.pc_ref_laxf.argtypes = [C.c_int, d3, d3, d3, d3, d3]
+    L.pc_ref_derive.argtypes = [C.c_int, Box, Array4, i3, i3, C.c_int, Array4, d3]
     _lib = L
     return L
 
