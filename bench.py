@@ -74,7 +74,9 @@ def stage_bytes(nspec, boxes, mol=False):
         g1 = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
         add("ghost", g4 - v, 2 * N)
         add("ctoprim", g4, N + Q + 6)
-        add("divu", (n[0] + 1) * (n[1] + 1) * (n[2] + 1), 4)
+        # divu on the nodes (4 velocity reads, one write) and the flattening
+        # coefficient on grow(bx,1) (p and the velocity read, one write)
+        add("divu", (n[0] + 2) * (n[1] + 2) * (n[2] + 2), 6)
         for d, a in enumerate("xyz"):
             t = [x + 2 for x in n]
             add("trace_" + a, g1, QT + 2 * QT)

@@ -134,9 +134,10 @@ static int a4_view(pc_ctx* c, const pc_array4_t& a, const IBox& need, int nc,
 // Sizes (doubles) of one box's intermediates; `full` adds q/qaux/srcq (the
 // box driver computes them) and the final fluxes (level path).
 struct WsLayout {
-  IBox q, tr[3], f1[3], ft[6], fl[3], node;
+  IBox q, tr[3], f1[3], ft[6], fl[3], node, g1;
   size_t off_q = 0, off_qa = 0, off_srcq = 0, off_tr[3], off_f1[3], off_ft[6],
-         off_fl[3], off_qec[3], off_divu = 0, off_sold = 0, total = 0;
+         off_fl[3], off_qec[3], off_divu = 0, off_sold = 0, off_flat = 0,
+         total = 0;
   bool mol = false;
 };
 
@@ -205,6 +206,8 @@ static WsLayout ws_layout(const IBox& bx, int ns, bool with_q, bool with_src,
   L.node = bx;
   for (int d = 0; d < 3; ++d) L.node.hi[d] += 1;
   L.off_divu = take(L.node.npts());
+  L.g1 = growb(bx, 1);
+  L.off_flat = take(L.g1.npts());
   L.total = o;
   return L;
 }
@@ -222,11 +225,12 @@ static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
       J.tr[d] = J.f1[d] = J.qec[d] = null_view();
     }
     for (int s = 0; s < 6; ++s) J.ft[s] = null_view();
-    J.divu = null_view();
+    J.divu = J.flat = null_view();
     J.sold = fab_view(base + L.off_sold, L.node);
     return;
   }
   J.sold = null_view();
+  J.flat = fab_view(base + L.off_flat, L.g1);
   for (int d = 0; d < 3; ++d) {
     J.tr[d] = fab_view(base + L.off_tr[d], L.tr[d]);
     J.f1[d] = fab_view(base + L.off_f1[d], L.f1[d]);

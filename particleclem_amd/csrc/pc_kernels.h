@@ -9,7 +9,8 @@
 // Stage regions (bx = [L, H] the valid box) are the dependency cone of the
 // valid outputs, smaller than the reference's grow-2 temporaries:
 //   ctoprim      cells grow(bx,4)
-//   divu         nodes [L, H+1]^3
+//   divu         [L-1, H+1]^3: divu on the nodes [L, H+1]^3, the flattening
+//                coefficient on the cells grow(bx,1)
 //   trace_d      cells grow(bx,1); writes qm on face c+1 and qp on face c
 //   flux1_d      d-faces [L, H+1] x transverse [L-1, H+1]
 //   trans_(d|t)  d-faces [L, H+1] x t in [L, H] x t' in [L-1, H+1]
@@ -83,6 +84,7 @@ struct JobT {
   VT<double> upd;                   // uout (dU/dt accumulator) or p == 0
   VT<double> divu;                  // nodes [L, H+1]^3
   VT<double> sold;                  // MOL: U^n on the valid box (or p == 0)
+  VT<FP> flat;                      // flattening coefficient on grow(bx,1)
 };
 using Job = JobT<double>;
 static_assert(sizeof(JobT<float>) == sizeof(JobT<double>),
@@ -217,7 +219,7 @@ __host__ __device__ inline void stage_region(int st, const int L[3],
       for (int d = 0; d < 3; ++d) set(d, L[d] - 4, H[d] + 4);
       break;
     case ST_DIVU:
-      for (int d = 0; d < 3; ++d) set(d, L[d], H[d] + 1);
+      for (int d = 0; d < 3; ++d) set(d, L[d] - 1, H[d] + 1);
       break;
     case ST_TRACE:
       for (int d = 0; d < 3; ++d) set(d, L[d] - 1, H[d] + 1);
@@ -426,12 +428,19 @@ __global__ void __launch_bounds__(BLOCK)
 // --------------------------------------------------------------- K24 divu --
 // Source/Hydro.H:476-517 on the nodes of the valid box
 template <class FP>
+PCD FP flat_cell(const JobT<FP>& J, int i, int j, int k);
+
+// Also the flattening coefficient of the cell (i,j,k) for the traces of all
+// three directions (the reference recomputes it in each direction's trace).
+template <class FP>
 PCD void b_divu(const Sched& sc, const KArgs& ka, int lb_) {
   int local;
   const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
   stage_region(ST_DIVU, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  if (ka.use_flattening) J.flat(i, j, k, 0) = flat_cell(J, i, j, k);
+  if (i < J.lo[0] || j < J.lo[1] || k < J.lo[2]) return;
   const auto& q = J.q;
   const double ux =
       0.25 *
@@ -511,7 +520,7 @@ PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
   for (int s = 0; s < NS; ++s) Yc[s] = q(i, j, k, QFS + s);
   const FP cc = EOS::RPY2Cs(q(i, j, k, QRHO), q(i, j, k, QPRES), Yc);
   const FP un = q(i, j, k, QUN);
-  const FP flat = ka.use_flattening ? flat_cell(J, i, j, k) : FP(1.0);
+  const FP flat = ka.use_flattening ? FP(J.flat(i, j, k, 0)) : FP(1.0);
   const FP spd0 = un - cc, spd1 = un, spd2 = un + cc;
   FP sm, sp, s6;
   auto recon = [&](int comp) {
@@ -667,7 +676,7 @@ PCD void b_trace_plm(const Sched& sc, const KArgs& ka, int lb_) {
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const auto& q = J.q;
   const FP dtdx = get_dt(J, ka) / J.dx[D];
-  const FP flat = ka.use_flattening ? flat_cell(J, i, j, k) : FP(1.0);
+  const FP flat = ka.use_flattening ? FP(J.flat(i, j, k, 0)) : FP(1.0);
   auto slope = [&](int comp) {
     FP s[5];
 #pragma unroll
@@ -860,7 +869,7 @@ PCD void plm_cell(const JobT<FP>& J, const KArgs& ka, FP dtdx, int i, int j,
                   int k, FaceState<EOS::NS, FP>& P, FaceState<EOS::NS, FP>& M) {
   constexpr int NS = EOS::NS;
   const auto& q = J.q;
-  const FP flat = ka.use_flattening ? flat_cell(J, i, j, k) : FP(1.0);
+  const FP flat = ka.use_flattening ? FP(J.flat(i, j, k, 0)) : FP(1.0);
   auto slope = [&](int comp) {
     FP s[5];
 #pragma unroll
