@@ -205,7 +205,7 @@ def serve(args):
     p50/p99 latency (arrival to result ready, queueing included)."""
     from particleclem_amd import serving as sv
     reqs = sv.request_stream(args.requests, args.rate)
-    srv = sv.Server(mixed=(args.precision == "mixed"))
+    srv = sv.Server(mixed=(args.precision == "mixed"), mol=args.mol)
     warm = sv.request_stream(4, 0, seed=1, n_lo=16, n_hi=16, n_med=16, steps=(3, 3))
     srv.run(warm, realtime=False)        # kernel loading, allocator warm-up
     served = srv.run(reqs, realtime=True)
@@ -226,8 +226,9 @@ def serve(args):
                                f"(gamma-law, pressure ratio 2-20) and half "
                                f"premixed H2/air slabs (9 species, phi 0.5-1.2), "
                                f"sides lognormal in [32, 128] (median 64), 10-200 "
-                               f"steps, PPM, CFL 0.3, continuous batching under "
-                               f"an 80% HBM budget ({budget / 2**30:.0f} GiB)",
+                               f"steps, {'MOL' if args.mol else 'PPM'}, CFL 0.3, "
+                               f"continuous batching under an 80% HBM budget "
+                               f"({budget / 2**30:.0f} GiB)",
                    "requests": len(served), "cell_updates": cells,
                    "failed": bad},
         "latency_ms": summ["latency_ms"], "queue_ms": summ["queue_ms"],

@@ -336,7 +336,8 @@ typedef struct pc_batch pc_batch_t;
 /* mixed: as pc_level_set_precision_v1.  share = NULL: a new arena of
  * budget_bytes; else draw on share's arena (one budget for batches of
  * different mechanisms), which lives until its last batch is destroyed.
- * max_slots: most requests alive in this batch. */
+ * max_slots: most requests alive in this batch.  prm->do_mol = 1 makes the
+ * batch step the method-of-lines step of a level (both Heun stages). */
 int pc_batch_create_v1(pc_ctx_t* ctx, const pc_params_t* prm, int mixed,
                        long long budget_bytes, pc_batch_t* share,
                        int max_slots, pc_batch_t** out);

@@ -1783,7 +1783,8 @@ struct pc_batch {
   pc_ctx* ctx = nullptr;
   pc_params_t prm;
   int mixed = 0;
-  bool laxf = false;      // (the batch steps the Godunov path only)
+  bool mol = false;       // method-of-lines steps (do_mol)
+  bool laxf = false;      // MOL with the Lax-Friedrichs flux
   KArgs ka;
   LevelArgs la;
   int lowo[3] = {0, 0, 0};
@@ -1803,7 +1804,7 @@ struct pc_batch {
 static size_t batch_slot_bytes(const pc_batch* B, const IBox& bx) {
   const int ns = B->ctx->ns;
   const size_t st = (size_t)growb(bx, 4).npts() * (7 + ns);
-  const size_t ws = ws_layout(bx, ns, true, false, true).total;
+  const size_t ws = ws_layout(bx, ns, true, false, true, B->mol).total;
   return Arena::round(st * sizeof(double)) + ws * sizeof(double);
 }
 
@@ -1854,6 +1855,16 @@ template <class EOS>
 static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
   const Sched* S = B->scs;
   const StagePlan* P = B->plans;
+  if (B->mol) {
+    for (int h = 0; h < 2; ++h) {
+      for (int x : {ST_GHOST, ST_MOL_PRIM, ST_MOL_X, ST_MOL_Y, ST_MOL_Z})
+        launch_stage<EOS>(B, x, S[x], P[x], st);
+      launch_stage<EOS>(B, ST_MOL_UPD, S[ST_MOL_UPD], P[ST_MOL_UPD], st, h);
+    }
+    hipLaunchKernelGGL(k_dt_finalize_list, dim3((n + 63) / 64), dim3(64), 0, st,
+                       B->ds, (const int*)B->dids, n, B->la, 0);
+    return;
+  }
   launch_stage<EOS>(B, ST_GHOST, S[ST_GHOST], P[ST_GHOST], st);
   launch_stage<EOS>(B, ST_CTOPRIM, S[ST_CTOPRIM], P[ST_CTOPRIM], st);
   launch_stage<EOS>(B, ST_DIVU, S[ST_DIVU], P[ST_DIVU], st);
@@ -1874,10 +1885,12 @@ int pc_batch_create_v1(pc_ctx_t* c, const pc_params_t* prm, int mixed,
       (mixed != 0 && mixed != 1))
     return fail(c, PC_ERR_ARG, "bad batch arguments");
   int rc;
-  if (prm->do_mol)
-    return fail(c, PC_ERR_UNSUPPORTED, "batches step the Godunov path (do_mol = 0)");
-  if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder, prm->use_hybrid_weno)))
+  if (prm->do_mol) {
+    if ((rc = check_mol_opts(c, prm))) return rc;
+  } else if ((rc = check_opts(c, prm->ppm_type, prm->plm_iorder,
+                              prm->use_hybrid_weno))) {
     return rc;
+  }
   if (!prm->allow_small_energy || !prm->allow_negative_energy)
     return fail(c, PC_ERR_UNSUPPORTED,
                 "allow_small_energy = allow_negative_energy = 1 required");
@@ -1885,6 +1898,8 @@ int pc_batch_create_v1(pc_ctx_t* c, const pc_params_t* prm, int mixed,
   B->ctx = c;
   B->prm = *prm;
   B->mixed = mixed;
+  B->mol = prm->do_mol != 0;
+  B->laxf = prm->use_laxf_flux != 0;
   B->max_slots = max_slots;
   B->slots.resize(max_slots);
   memset(&B->ka, 0, sizeof(B->ka));
@@ -1991,7 +2006,7 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
   }
   set_geom(J, dx);
   J.U = fab_view(base, growb(box, 4));
-  const WsLayout W = ws_layout(box, c->ns, true, false, true);
+  const WsLayout W = ws_layout(box, c->ns, true, false, true, B->mol);
   ws_fill_job(J, base + (Arena::round(S.nstate * sizeof(double)) / sizeof(double)),
               W, true, false, true);
   J.srcq = null_view();

@@ -158,7 +158,7 @@ class Server:
     first served while their state and workspace fit."""
 
     def __init__(self, mixed=True, budget_frac=0.8, budget_bytes=None,
-                 max_slots=512, keep_results=False):
+                 max_slots=512, keep_results=False, mol=False):
         import torch
         self.ctx = {TUBE: HydroContext(1), SLAB: HydroContext(9)}
         if budget_bytes is None:
@@ -166,7 +166,7 @@ class Server:
             budget_bytes = int(budget_frac * total)
             budget_bytes = min(budget_bytes, int(0.95 * free))
         self.budget = budget_bytes
-        params = dict(ppm_type=1, cfl=0.3)
+        params = dict(ppm_type=1, cfl=0.3, do_mol=1 if mol else 0)
         self.batch = {}
         self.batch[SLAB] = Batch(self.ctx[SLAB], params, mixed, budget_bytes,
                                  max_slots=max_slots)

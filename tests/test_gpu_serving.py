@@ -62,6 +62,41 @@ def test_batch_equals_level_stepper(cuda):
         ctx.close()
 
 
+def test_mol_batch_equals_mol_level_and_mix_independence(cuda):
+    """MOL requests (both Heun stages per step) through the FP64 batch:
+    equal to the MOL level stepper on the same box, BCs and fields, and the
+    same alone and in the mix."""
+    from particleclem_amd.hydro import DeviceLevel, HydroContext, a4_from_tensor
+    reqs = [sv.Request(0, sv.TUBE, 12, 5, 10.0, 0.8, 0.0, 5),
+            sv.Request(1, sv.SLAB, 10, 4, 2.0, 1.1, 0.0, 6),
+            sv.Request(2, sv.SLAB, 8, 3, 2.0, 0.6, 0.0, 7)]
+    srv = sv.Server(mixed=False, budget_bytes=1 << 30, keep_results=True, mol=True)
+    out = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    srv.close()
+    for r in reqs:
+        P, bclo, bchi, mode = sv.request_fields(r)
+        ctx = HydroContext(sv.NSPEC[r.kind])
+        lev = DeviceLevel(ctx, [([0, 0, 0], [r.n - 1] * 3)], [0, 0, 0], [r.n - 1] * 3,
+                          bclo, bchi, [1.0 / r.n] * 3,
+                          params=dict(ppm_type=1, cfl=0.3, do_mol=1))
+        lev._chk(lev.L.pc_level_prim_to_cons_v1(lev.h, 0, a4_from_tensor(P, [0, 0, 0]),
+                                                 mode, None))
+        lev.init_dt()
+        lev.step(r.nsteps)
+        ref = lev.read_box(0)
+        V = (slice(None), slice(4, -4), slice(4, -4), slice(4, -4))
+        nd = int(np.count_nonzero(ref[V] != out[r.rid].result[V]))
+        print(f"MOL request {r.rid} kind {r.kind}: {nd} values differ from the level stepper")
+        assert out[r.rid].status == 0 and nd == 0
+        lev.close()
+        ctx.close()
+        s1 = sv.Server(mixed=False, budget_bytes=1 << 30, keep_results=True, mol=True)
+        one = sv.Request(r.rid, r.kind, r.n, r.nsteps, r.ratio, r.phi, 0.0, r.seed)
+        got = s1.run([one], realtime=False)[0]
+        s1.close()
+        assert np.array_equal(got.result, out[r.rid].result)
+
+
 def test_admission_under_a_tight_budget(cuda):
     reqs = small_stream(10, seed=3)
     probe = sv.Server(mixed=True, budget_bytes=64 << 20)
