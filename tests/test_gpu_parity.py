@@ -31,3 +31,26 @@ def test_sod_plm(cuda):
 def test_premixed_9species(cuda):
     w = wl.premixed(n=32, box=16, nsteps=2)
     assert_parity(w, 2, label="premixed 32^3 / 16^3 boxes")
+
+
+@pytest.mark.parametrize("over", [dict(ppm_type=0, plm_iorder=1),
+                                  dict(ppm_type=0, plm_iorder=2),
+                                  dict(ppm_type=1, use_flattening=0),
+                                  dict(ppm_type=0, use_flattening=0),
+                                  dict(ppm_type=1, difmag=0.0)],
+                         ids=["plm1", "plm2", "ppm-noflat", "plm-noflat", "nodifmag"])
+def test_option_variants_premixed(cuda, over):
+    """The reference's other hydro options on the 9-species level: PLM
+    orders 1 and 2, no flattening, no artificial viscosity."""
+    w = wl.premixed(n=16, box=8, nsteps=2)
+    assert_parity(w, 2, over, label=f"premixed 16^3 {over}")
+
+
+def test_odd_boxes_walls_both_ends(cuda):
+    """Boxes of odd, unequal sides on a domain with slip and no-slip walls
+    (reflecting ghost fill and the wall Riemann faces)."""
+    w = wl.sod(n=21, ny=13, nz=7, max_grid=11, nsteps=2, xbc="slipwall")
+    w.bclo = [4, 5, 3]
+    w.bchi = [5, 4, 3]
+    assert len(w.boxes) > 2
+    assert_parity(w, 3, label="sod 21x13x7, walls")
