@@ -320,6 +320,47 @@ int pc_level_status_v1(pc_level_t* lev, int* bad, int* small_dt);
 int pc_level_prim_to_cons_v1(pc_level_t* lev, int b, pc_array4_t prim,
                              int mode, void* stream);
 
+/* ---------------------------------------- flux register, average-down --- */
+/* Refluxing between a coarse level and a level refined by `ratio`, for
+ * regular Cartesian boxes: the operations PeleC::update_flux_registers
+ * (CrseAdd / FineAdd per box, Source/PeleC.cpp:1343-1394, called from
+ * Source/Hydro.cpp:280-288) and PeleC::reflux / avgDown (:1396-1437) perform
+ * through AMReX's flux register.  Register faces are the coarse faces on the
+ * boundary of the refined region whose other ("outside") coarse cell is not
+ * refined (faces on a non-periodic domain boundary are not register faces;
+ * across a periodic one the outside cell is the periodic image).  Each face
+ * accumulates crse_add's scale * coarse flux plus fine_add's scale * (sum of
+ * its ratio^2 fine fluxes); fluxes are extensive (x area), as pc_umdrv_v1 and
+ * pc_hydro_source_box_v1 return them.  With crse scale -dt_c and fine scale
+ * +dt_f (per fine step), reflux adds -reg / V to an outside cell below the
+ * face and +reg / V to one above it.  Fine boxes must be aligned to ratio.
+ * All calls are stream-ordered; a box id is the index in the create call's
+ * list, bx that box (its flux views cover surroundingNodes(bx, d)). */
+typedef struct pc_fluxreg pc_fluxreg_t;
+int pc_fluxreg_create_v1(pc_ctx_t* ctx, const pc_box_t* crse_boxes, int ncrse,
+                         const pc_box_t* fine_boxes, int nfine,
+                         const int* crse_domlo, const int* crse_domhi,
+                         const int* periodic, int ratio, pc_fluxreg_t** out);
+int pc_fluxreg_destroy_v1(pc_fluxreg_t* reg);
+/* Number of register faces (or -1). */
+int pc_fluxreg_nfaces_v1(const pc_fluxreg_t* reg);
+/* Zero the register (a fill kernel: safe inside a captured graph). */
+int pc_fluxreg_reset_v1(pc_fluxreg_t* reg, void* stream);
+int pc_fluxreg_crse_add_v1(pc_fluxreg_t* reg, int crse_box, pc_box_t bx,
+                           const pc_array4_t* flx, double scale, void* stream);
+int pc_fluxreg_fine_add_v1(pc_fluxreg_t* reg, int fine_box, pc_box_t bx,
+                           const pc_array4_t* flx, double scale, void* stream);
+/* state: coarse box crse_box's state (covering bx); vol.p = NULL: uniform
+ * volume vol_u. */
+int pc_fluxreg_reflux_v1(pc_fluxreg_t* reg, int crse_box, pc_box_t bx,
+                         pc_array4_t state, pc_array4_t vol, double vol_u,
+                         void* stream);
+/* crse = (1 / ratio^3) * sum of the ratio^3 fine cells over crse_region
+ * (conserved state, uniform volumes; the fine view covers the refined
+ * region). */
+int pc_average_down_v1(pc_ctx_t* ctx, pc_box_t crse_region, pc_array4_t crse,
+                       pc_array4_t fine, int ratio, void* stream);
+
 /* ------------------------------------------------- batch (ensembles) --- */
 /* Independent requests resident on the device: each is one box that is its
  * own domain, with its own boundary codes, cell size and dt (a shock tube or

@@ -727,3 +727,234 @@ double pc_ref_plm_slope_line(const double* q5, double flat, int order) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ flux register
+// Dense restatement of the coarse/fine flux register (see include/pc_hydro.h,
+// pc_fluxreg_*): one register value per coarse face of the domain and
+// component, a face marked when exactly one of its two cells is refined and
+// the other exists (in the domain or as a periodic image).
+namespace {
+struct RefFluxReg {
+  int nc, ratio;
+  Box dom;
+  int periodic[3];
+  std::vector<Box> cboxes, fc;  // coarse boxes, coarsened fine boxes
+  // per direction: faces [domlo, domhi+1] x cells transverse; side: 0 none,
+  // +1 outside cell above the face, -1 below
+  std::vector<signed char> side[3];
+  std::vector<double> reg[3];
+  long nface(int d) const {
+    long n = 1;
+    for (int e = 0; e < 3; ++e) n *= (dom.hi[e] - dom.lo[e] + 1) + (e == d ? 1 : 0);
+    return n;
+  }
+  long fid(int d, const int* f) const {
+    long id = 0, m = 1;
+    for (int e = 0; e < 3; ++e) {
+      id += (f[e] - dom.lo[e]) * m;
+      m *= (dom.hi[e] - dom.lo[e] + 1) + (e == d ? 1 : 0);
+    }
+    return id;
+  }
+  bool covered(const int* x) const {
+    for (const Box& b : fc)
+      if (x[0] >= b.lo[0] && x[0] <= b.hi[0] && x[1] >= b.lo[1] && x[1] <= b.hi[1] &&
+          x[2] >= b.lo[2] && x[2] <= b.hi[2])
+        return true;
+    return false;
+  }
+  // the cell x moved into the domain through a periodic direction d, or
+  // false when it lies outside a non-periodic one
+  bool wrap(int d, int* x) const {
+    if (x[d] >= dom.lo[d] && x[d] <= dom.hi[d]) return true;
+    if (!periodic[d]) return false;
+    const int L = dom.hi[d] - dom.lo[d] + 1;
+    x[d] += x[d] < dom.lo[d] ? L : -L;
+    return true;
+  }
+};
+
+bool in_box(const Box& b, const int* x) {
+  return x[0] >= b.lo[0] && x[0] <= b.hi[0] && x[1] >= b.lo[1] && x[1] <= b.hi[1] &&
+         x[2] >= b.lo[2] && x[2] <= b.hi[2];
+}
+}  // namespace
+
+extern "C" {
+
+void* pc_ref_fluxreg_create(int nc, const pc_box_t* crse_boxes, int ncrse,
+                            const pc_box_t* fine_boxes, int nfine,
+                            const int* domlo, const int* domhi,
+                            const int* periodic, int ratio) {
+  RefFluxReg* R = new RefFluxReg();
+  R->nc = nc;
+  R->ratio = ratio;
+  for (int d = 0; d < 3; ++d) {
+    R->dom.lo[d] = domlo[d];
+    R->dom.hi[d] = domhi[d];
+    R->periodic[d] = periodic[d];
+  }
+  for (int b = 0; b < ncrse; ++b) R->cboxes.push_back(to_box(crse_boxes[b]));
+  for (int b = 0; b < nfine; ++b) {
+    Box c;
+    for (int d = 0; d < 3; ++d) {
+      c.lo[d] = fine_boxes[b].lo[d] / ratio;
+      c.hi[d] = (fine_boxes[b].hi[d] + 1) / ratio - 1;
+    }
+    R->fc.push_back(c);
+  }
+  for (int d = 0; d < 3; ++d) {
+    R->side[d].assign(R->nface(d), 0);
+    R->reg[d].assign(R->nface(d) * nc, 0.0);
+    Box fb = R->dom;
+    fb.hi[d] += 1;
+    // in a periodic direction the face domhi+1 is the face domlo
+    if (R->periodic[d]) fb.hi[d] -= 1;
+    ParallelFor(fb, [&](int i, int j, int k) {
+      int a[3] = {i, j, k}, b[3] = {i, j, k};
+      b[d] -= 1;  // the cell below the face
+      const bool ia = R->wrap(d, a), ib = R->wrap(d, b);
+      if (!ia || !ib) return;
+      const bool ca = R->covered(a), cb = R->covered(b);
+      const int f[3] = {i, j, k};
+      if (ca && !cb) R->side[d][R->fid(d, f)] = -1;
+      if (cb && !ca) R->side[d][R->fid(d, f)] = +1;
+    });
+  }
+  return R;
+}
+
+void pc_ref_fluxreg_destroy(void* r) { delete (RefFluxReg*)r; }
+
+int pc_ref_fluxreg_nfaces(void* r) {
+  const RefFluxReg* R = (const RefFluxReg*)r;
+  int n = 0;
+  for (int d = 0; d < 3; ++d)
+    for (signed char s : R->side[d]) n += s != 0;
+  return n;
+}
+
+// reg += scale * flux of coarse box bx on the register faces whose outside
+// cell is in bx
+int pc_ref_fluxreg_crse_add(void* r, pc_box_t bx, const pc_array4_t* flx,
+                            double scale) {
+  RefFluxReg* R = (RefFluxReg*)r;
+  const Box b = to_box(bx);
+  for (int d = 0; d < 3; ++d) {
+    const A4 F = to_a4(flx[d]);
+    Box fb = R->dom;
+    fb.hi[d] += R->periodic[d] ? 0 : 1;
+    ParallelFor(fb, [&](int i, int j, int k) {
+      const int f[3] = {i, j, k};
+      const long id = R->fid(d, f);
+      const int s = R->side[d][id];
+      if (!s) return;
+      int out[3] = {i, j, k};
+      if (s < 0) out[d] -= 1;
+      R->wrap(d, out);
+      if (!in_box(b, out)) return;
+      int face[3] = {out[0], out[1], out[2]};  // in the box's index space
+      if (s < 0) face[d] += 1;
+      for (int n = 0; n < R->nc; ++n)
+        R->reg[d][id * R->nc + n] += scale * F(face[0], face[1], face[2], n);
+    });
+  }
+  return 0;
+}
+
+// reg += scale * (sum of the ratio^2 fine fluxes of fine box bx)
+int pc_ref_fluxreg_fine_add(void* r, pc_box_t bx, const pc_array4_t* flx,
+                            double scale) {
+  RefFluxReg* R = (RefFluxReg*)r;
+  Box c;  // bx coarsened
+  for (int d = 0; d < 3; ++d) {
+    c.lo[d] = bx.lo[d] / R->ratio;
+    c.hi[d] = (bx.hi[d] + 1) / R->ratio - 1;
+  }
+  for (int d = 0; d < 3; ++d) {
+    const A4 F = to_a4(flx[d]);
+    const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
+    Box fb = R->dom;
+    fb.hi[d] += R->periodic[d] ? 0 : 1;
+    ParallelFor(fb, [&](int i, int j, int k) {
+      const int f[3] = {i, j, k};
+      const long id = R->fid(d, f);
+      const int s = R->side[d][id];
+      if (!s) return;
+      // the refined cell next to the face, in bx's unwrapped frame
+      for (int img = -1; img <= 1; ++img) {
+        if (img != 0 && !R->periodic[d]) continue;
+        int in[3] = {i, j, k};
+        in[d] += img * (R->dom.hi[d] - R->dom.lo[d] + 1);
+        const int face_d = in[d];
+        if (s > 0) in[d] -= 1;  // outside above: refined cell below
+        if (!in_box(c, in)) continue;
+        for (int n = 0; n < R->nc; ++n) {
+          double sum = 0.0;
+          for (int bb = 0; bb < R->ratio; ++bb)
+            for (int aa = 0; aa < R->ratio; ++aa) {
+              int x[3];
+              x[d] = face_d * R->ratio;
+              x[t0] = in[t0] * R->ratio + aa;
+              x[t1] = in[t1] * R->ratio + bb;
+              sum += F(x[0], x[1], x[2], n);
+            }
+          R->reg[d][id * R->nc + n] += scale * sum;
+        }
+      }
+    });
+  }
+  return 0;
+}
+
+// U += +reg/V (cell above its register face) or -reg/V (below) on the
+// outside cells in bx, per cell in direction order, low face first
+int pc_ref_fluxreg_reflux(void* r, pc_box_t bx, pc_array4_t state, double vol) {
+  RefFluxReg* R = (RefFluxReg*)r;
+  const Box b = to_box(bx);
+  const A4 U = to_a4(state);
+  ParallelFor(b, [&](int i, int j, int k) {
+    for (int n = 0; n < R->nc; ++n) {
+      double u = U(i, j, k, n);
+      bool touched = false;
+      for (int d = 0; d < 3; ++d)
+        for (int hi = 0; hi <= 1; ++hi) {
+          int f[3] = {i, j, k};
+          f[d] += hi;  // the cell's low (hi = 0) or high face
+          if (R->periodic[d] && f[d] == R->dom.hi[d] + 1) f[d] = R->dom.lo[d];
+          if (f[d] < R->dom.lo[d] || f[d] > R->dom.hi[d] + 1) continue;
+          const long id = R->fid(d, f);
+          const int s = R->side[d][id];
+          if (hi == 0 && s == +1) {
+            u += R->reg[d][id * R->nc + n] / vol;
+            touched = true;
+          }
+          if (hi == 1 && s == -1) {
+            u += -R->reg[d][id * R->nc + n] / vol;
+            touched = true;
+          }
+        }
+      if (touched) U(i, j, k, n) = u;
+    }
+  });
+  return 0;
+}
+
+int pc_ref_average_down(int nc, pc_box_t crse_region, pc_array4_t crse,
+                        pc_array4_t fine, int ratio) {
+  const A4 C = to_a4(crse), F = to_a4(fine);
+  const double volfrac = 1.0 / (double)(ratio * ratio * ratio);
+  ParallelFor(to_box(crse_region), [&](int i, int j, int k) {
+    for (int n = 0; n < nc; ++n) {
+      double c = 0.0;
+      for (int kk = 0; kk < ratio; ++kk)
+        for (int jj = 0; jj < ratio; ++jj)
+          for (int ii = 0; ii < ratio; ++ii)
+            c += F(ratio * i + ii, ratio * j + jj, ratio * k + kk, n);
+      C(i, j, k, n) = volfrac * c;
+    }
+  });
+  return 0;
+}
+
+}  // extern "C"

@@ -88,6 +88,22 @@ int pc_ref_mol_src_box(int eos, pc_box_t bx, pc_array4_t sarr,
 int pc_ref_derive(int eos, pc_box_t bx, pc_array4_t u, const int* codes,
                   const int* comps, int nplot, pc_array4_t out,
                   const double* dx);
+/* Coarse/fine flux register and average-down (the pc_fluxreg_* and
+ * pc_average_down_v1 semantics), dense restatement; boxes identify
+ * themselves by their index ranges. */
+void* pc_ref_fluxreg_create(int nc, const pc_box_t* crse_boxes, int ncrse,
+                            const pc_box_t* fine_boxes, int nfine,
+                            const int* domlo, const int* domhi,
+                            const int* periodic, int ratio);
+void pc_ref_fluxreg_destroy(void* reg);
+int pc_ref_fluxreg_nfaces(void* reg);
+int pc_ref_fluxreg_crse_add(void* reg, pc_box_t bx, const pc_array4_t* flx,
+                            double scale);
+int pc_ref_fluxreg_fine_add(void* reg, pc_box_t bx, const pc_array4_t* flx,
+                            double scale);
+int pc_ref_fluxreg_reflux(void* reg, pc_box_t bx, pc_array4_t state, double vol);
+int pc_ref_average_down(int nc, pc_box_t crse_region, pc_array4_t crse,
+                        pc_array4_t fine, int ratio);
 /* laxfriedrich_flux of Source/Riemann.H:205-307: out[0..] = ustar,
  * maxeigval, uflx_rho, uflx_u, uflx_v, uflx_w, uflx_eden, uflx_eint,
  * uflx_rhoY[NS] */

@@ -22,7 +22,10 @@ EXPORTS = [
     "pc_level_state_size_v1", "pc_level_write_box_v1", "pc_level_read_box_v1",
     "pc_level_set_dt_v1", "pc_level_get_dt_v1", "pc_level_init_dt_v1",
     "pc_level_step_v1", "pc_level_fill_ghosts_v1", "pc_level_estdt_v1",
-    "pc_level_write_plotfile_v1", "pc_write_plotfile_v1",
+    "pc_level_write_plotfile_v1", "pc_write_plotfile_v1", "pc_fluxreg_create_v1",
+    "pc_fluxreg_destroy_v1", "pc_fluxreg_nfaces_v1", "pc_fluxreg_reset_v1",
+    "pc_fluxreg_crse_add_v1", "pc_fluxreg_fine_add_v1", "pc_fluxreg_reflux_v1",
+    "pc_average_down_v1",
     "pc_level_enable_timing_v1", "pc_level_kernel_times_v1",
     "pc_level_status_v1", "pc_level_prim_to_cons_v1",
     "pc_level_autotune_v1", "pc_level_plan_v1", "pc_level_set_plan_v1",
@@ -93,6 +96,15 @@ def lib():
                                     P(Array4), P(Array4), Array4, vp]
     L.pc_level_write_plotfile_v1.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double,
                                              C.c_int, d3, vp]
+    L.pc_fluxreg_create_v1.argtypes = [vp, P(Box), C.c_int, P(Box), C.c_int, i3, i3, i3,
+                                       C.c_int, P(vp)]
+    L.pc_fluxreg_destroy_v1.argtypes = [vp]
+    L.pc_fluxreg_nfaces_v1.argtypes = [vp]
+    L.pc_fluxreg_reset_v1.argtypes = [vp, vp]
+    L.pc_fluxreg_crse_add_v1.argtypes = [vp, C.c_int, Box, P(Array4), C.c_double, vp]
+    L.pc_fluxreg_fine_add_v1.argtypes = [vp, C.c_int, Box, P(Array4), C.c_double, vp]
+    L.pc_fluxreg_reflux_v1.argtypes = [vp, C.c_int, Box, Array4, Array4, C.c_double, vp]
+    L.pc_average_down_v1.argtypes = [vp, Box, Array4, Array4, C.c_int, vp]
     L.pc_write_plotfile_v1.argtypes = [C.c_char_p, C.c_int, P(C.c_char_p), C.c_int,
                                        P(Box), P(P(C.c_double)), i3, i3, d3, d3,
                                        C.c_double, C.c_int]

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <map>
 #include <sstream>
 #include <string>
@@ -14,6 +15,7 @@
 
 #include "../../include/pc_hydro.h"
 #include "pc_derive_kernels.h"
+#include "pc_fluxreg.h"
 #include "pc_mol_kernels.h"
 #include "pc_plotfile.h"
 
@@ -1650,6 +1652,294 @@ int pc_level_write_plotfile_v1(pc_level_t* L, const char* dir, const char* vars,
   std::string msg;
   if (pcplot::write_plotfile(dir, meta, pb, msg) != 0)
     return fail(c, PC_ERR_IO, "%s", msg.c_str());
+  return PC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------- flux register --
+struct pc_fluxreg {
+  pc_ctx* ctx = nullptr;
+  int ratio = 2, nc = 0;
+  int ncrse = 0, nfine = 0;
+  int nfaces = 0;
+  RegFace* faces = nullptr;  // device
+  double* reg = nullptr;     // device, nfaces * nc
+  // per coarse box: its faces (outside cell in the box) and reflux cells;
+  // per fine box: its faces
+  std::vector<int> c_first, c_count, cell_first, cell_count, f_first, f_count;
+  int* cids = nullptr;        // device face ids grouped by coarse box
+  int* fids = nullptr;        // device face ids grouped by fine box
+  RegCell* cells = nullptr;   // device reflux cells grouped by coarse box
+  int* cell_faces = nullptr;  // device
+};
+
+static void fluxreg_free(pc_fluxreg* R) {
+  for (void* p : {(void*)R->faces, (void*)R->reg, (void*)R->cids, (void*)R->fids,
+                  (void*)R->cells, (void*)R->cell_faces})
+    if (p) (void)hipFree(p);
+  delete R;
+}
+
+template <class T>
+static bool upload(T** dst, const std::vector<T>& v) {
+  *dst = nullptr;
+  if (hipMalloc(dst, sizeof(T) * std::max<size_t>(v.size(), 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return v.empty() ||
+         hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) ==
+             hipSuccess;
+}
+
+extern "C" {
+
+int pc_fluxreg_create_v1(pc_ctx_t* c, const pc_box_t* crse_boxes, int ncrse,
+                         const pc_box_t* fine_boxes, int nfine,
+                         const int* crse_domlo, const int* crse_domhi,
+                         const int* periodic, int ratio, pc_fluxreg_t** out) {
+  if (!c || !crse_boxes || ncrse <= 0 || !fine_boxes || nfine <= 0 ||
+      !crse_domlo || !crse_domhi || !periodic || ratio < 2 || !out)
+    return fail(c, PC_ERR_ARG, "bad flux register arguments");
+  *out = nullptr;
+  std::vector<IBox> cb(ncrse), fc(nfine);  // coarse boxes, coarsened fine boxes
+  for (int b = 0; b < ncrse; ++b) cb[b] = mkbox(crse_boxes[b].lo, crse_boxes[b].hi);
+  for (int b = 0; b < nfine; ++b)
+    for (int d = 0; d < 3; ++d) {
+      const int lo = fine_boxes[b].lo[d], hi = fine_boxes[b].hi[d];
+      if (hi < lo || lo % ratio != 0 || (hi + 1) % ratio != 0 || lo < 0)
+        return fail(c, PC_ERR_ARG, "fine box %d is not aligned to the ratio %d", b, ratio);
+      fc[b].lo[d] = lo / ratio;
+      fc[b].hi[d] = (hi + 1) / ratio - 1;
+    }
+  auto in = [](const IBox& b, const int* x) {
+    return x[0] >= b.lo[0] && x[0] <= b.hi[0] && x[1] >= b.lo[1] && x[1] <= b.hi[1] &&
+           x[2] >= b.lo[2] && x[2] <= b.hi[2];
+  };
+  auto covered = [&](const int* x) {
+    for (const IBox& b : fc)
+      if (in(b, x)) return true;
+    return false;
+  };
+  auto owner = [&](const int* x) {
+    for (int b = 0; b < ncrse; ++b)
+      if (in(cb[b], x)) return b;
+    return -1;
+  };
+  std::vector<RegFace> faces;
+  std::vector<int> face_crse, face_fine;
+  for (int b = 0; b < nfine; ++b)
+    for (int d = 0; d < 3; ++d)
+      for (int side = -1; side <= 1; side += 2) {
+        const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
+        const int I = side < 0 ? fc[b].lo[d] : fc[b].hi[d] + 1;  // coarse face
+        for (int K = fc[b].lo[t1]; K <= fc[b].hi[t1]; ++K)
+          for (int J = fc[b].lo[t0]; J <= fc[b].hi[t0]; ++J) {
+            int out_cell[3];
+            out_cell[d] = side < 0 ? I - 1 : I;
+            out_cell[t0] = J;
+            out_cell[t1] = K;
+            if (covered(out_cell)) continue;  // a face between fine boxes
+            int shift = 0;                    // periodic image of the cell
+            if (out_cell[d] < crse_domlo[d] || out_cell[d] > crse_domhi[d]) {
+              if (!periodic[d]) continue;     // a domain face: no coarse cell
+              const int L = crse_domhi[d] - crse_domlo[d] + 1;
+              shift = out_cell[d] < crse_domlo[d] ? L : -L;
+              out_cell[d] += shift;
+            }
+            const int ob = owner(out_cell);
+            if (ob < 0)
+              return fail(c, PC_ERR_ARG, "coarse cell next to fine box %d is on no coarse box", b);
+            RegFace F;
+            F.d = d;
+            int fx[3];
+            fx[d] = I + shift;
+            fx[t0] = J;
+            fx[t1] = K;
+            F.i = fx[0];
+            F.j = fx[1];
+            F.k = fx[2];
+            F.side = side < 0 ? -1 : 1;
+            int ff[3];
+            ff[d] = I * ratio;
+            ff[t0] = J * ratio;
+            ff[t1] = K * ratio;
+            F.fi = ff[0];
+            F.fj = ff[1];
+            F.fk = ff[2];
+            faces.push_back(F);
+            face_crse.push_back(ob);
+            face_fine.push_back(b);
+          }
+      }
+  pc_fluxreg* R = new pc_fluxreg();
+  R->ctx = c;
+  R->ratio = ratio;
+  R->nc = 7 + c->ns;
+  R->ncrse = ncrse;
+  R->nfine = nfine;
+  R->nfaces = (int)faces.size();
+  std::vector<int> cids, fids, cell_faces;
+  std::vector<RegCell> cells;
+  R->c_first.assign(ncrse, 0);
+  R->c_count.assign(ncrse, 0);
+  R->cell_first.assign(ncrse, 0);
+  R->cell_count.assign(ncrse, 0);
+  for (int b = 0; b < ncrse; ++b) {
+    R->c_first[b] = (int)cids.size();
+    R->cell_first[b] = (int)cells.size();
+    std::map<std::array<int, 3>, std::vector<int>> by_cell;
+    for (int f = 0; f < R->nfaces; ++f) {
+      if (face_crse[f] != b) continue;
+      cids.push_back(f);
+      const RegFace& F = faces[f];
+      std::array<int, 3> x = {F.i, F.j, F.k};
+      if (F.side < 0) x[F.d] -= 1;
+      by_cell[{x[2], x[1], x[0]}].push_back(f);
+    }
+    R->c_count[b] = (int)cids.size() - R->c_first[b];
+    for (auto& kv : by_cell) {
+      // a cell's faces in direction order, its low face before its high face
+      std::stable_sort(kv.second.begin(), kv.second.end(), [&](int a, int b2) {
+        const RegFace &A = faces[a], &B = faces[b2];
+        return A.d != B.d ? A.d < B.d : A.side > B.side;
+      });
+      RegCell C;
+      C.k = kv.first[0];
+      C.j = kv.first[1];
+      C.i = kv.first[2];
+      C.first = (int)cell_faces.size();
+      C.count = (int)kv.second.size();
+      for (int f : kv.second) cell_faces.push_back(f);
+      cells.push_back(C);
+    }
+    R->cell_count[b] = (int)cells.size() - R->cell_first[b];
+  }
+  R->f_first.assign(nfine, 0);
+  R->f_count.assign(nfine, 0);
+  for (int b = 0; b < nfine; ++b) {
+    R->f_first[b] = (int)fids.size();
+    for (int f = 0; f < R->nfaces; ++f)
+      if (face_fine[f] == b) fids.push_back(f);
+    R->f_count[b] = (int)fids.size() - R->f_first[b];
+  }
+  if (!upload(&R->faces, faces) || !upload(&R->cids, cids) || !upload(&R->fids, fids) ||
+      !upload(&R->cells, cells) || !upload(&R->cell_faces, cell_faces) ||
+      hipMalloc(&R->reg, sizeof(double) * std::max<size_t>((size_t)R->nfaces * R->nc, 1)) !=
+          hipSuccess ||
+      hipMemset(R->reg, 0, sizeof(double) * std::max<size_t>((size_t)R->nfaces * R->nc, 1)) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    fluxreg_free(R);
+    return fail(c, PC_ERR_NOMEM, "flux register allocation failed");
+  }
+  *out = R;
+  return PC_OK;
+}
+
+int pc_fluxreg_destroy_v1(pc_fluxreg_t* R) {
+  if (!R) return PC_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  fluxreg_free(R);
+  return PC_OK;
+}
+
+int pc_fluxreg_nfaces_v1(const pc_fluxreg_t* R) { return R ? R->nfaces : -1; }
+
+int pc_fluxreg_reset_v1(pc_fluxreg_t* R, void* stream) {
+  if (!R) return PC_ERR_ARG;
+  // a fill kernel rather than a memset node, so the call can be captured
+  const size_t n = std::max<size_t>((size_t)R->nfaces * R->nc, 1);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, R->reg, (long long)n, 0.0);
+  HIPCHK(R->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+static int fluxreg_views(pc_fluxreg* R, const IBox& b, const pc_array4_t* flx,
+                         V* f) {
+  int rc;
+  for (int d = 0; d < 3; ++d) {
+    IBox fb = b;
+    fb.hi[d] += 1;
+    if ((rc = a4_view(R->ctx, flx[d], fb, R->nc, "flx", f[d]))) return rc;
+  }
+  return PC_OK;
+}
+
+int pc_fluxreg_crse_add_v1(pc_fluxreg_t* R, int crse_box, pc_box_t bx,
+                           const pc_array4_t* flx, double scale, void* stream) {
+  if (!R || !flx || crse_box < 0 || crse_box >= R->ncrse || !box_ok(bx))
+    return fail(R ? R->ctx : nullptr, PC_ERR_ARG, "bad crse_add arguments");
+  V f[3];
+  int rc;
+  if ((rc = fluxreg_views(R, mkbox(bx.lo, bx.hi), flx, f))) return rc;
+  const int n = R->c_count[crse_box];
+  if (n > 0)
+    hipLaunchKernelGGL(k_fluxreg_crse, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, R->faces, R->cids + R->c_first[crse_box],
+                       n, R->reg, R->nc, f[0], f[1], f[2], scale);
+  HIPCHK(R->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+int pc_fluxreg_fine_add_v1(pc_fluxreg_t* R, int fine_box, pc_box_t bx,
+                           const pc_array4_t* flx, double scale, void* stream) {
+  if (!R || !flx || fine_box < 0 || fine_box >= R->nfine || !box_ok(bx))
+    return fail(R ? R->ctx : nullptr, PC_ERR_ARG, "bad fine_add arguments");
+  V f[3];
+  int rc;
+  if ((rc = fluxreg_views(R, mkbox(bx.lo, bx.hi), flx, f))) return rc;
+  const int n = R->f_count[fine_box];
+  if (n > 0)
+    hipLaunchKernelGGL(k_fluxreg_fine, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, R->faces, R->fids + R->f_first[fine_box],
+                       n, R->reg, R->nc, f[0], f[1], f[2], R->ratio, scale);
+  HIPCHK(R->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+int pc_fluxreg_reflux_v1(pc_fluxreg_t* R, int crse_box, pc_box_t bx,
+                         pc_array4_t state, pc_array4_t vol, double vol_u,
+                         void* stream) {
+  if (!R || crse_box < 0 || crse_box >= R->ncrse || !box_ok(bx))
+    return fail(R ? R->ctx : nullptr, PC_ERR_ARG, "bad reflux arguments");
+  const IBox b = mkbox(bx.lo, bx.hi);
+  V U, vv = null_view();
+  int rc;
+  if ((rc = a4_view(R->ctx, state, b, R->nc, "uout", U))) return rc;
+  if (vol.p && (rc = a4_view(R->ctx, vol, b, 1, "vol", vv))) return rc;
+  const int n = R->cell_count[crse_box];
+  if (n > 0)
+    hipLaunchKernelGGL(k_fluxreg_reflux, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, R->faces, R->cells + R->cell_first[crse_box],
+                       R->cell_faces, n, R->reg, R->nc, U, vv, vol_u);
+  HIPCHK(R->ctx, hipGetLastError());
+  return PC_OK;
+}
+
+int pc_average_down_v1(pc_ctx_t* c, pc_box_t crse_region, pc_array4_t crse,
+                       pc_array4_t fine, int ratio, void* stream) {
+  if (!c || !box_ok(crse_region) || ratio < 1)
+    return fail(c, PC_ERR_ARG, "bad average_down arguments");
+  const IBox cbx = mkbox(crse_region.lo, crse_region.hi);
+  IBox fbx;
+  for (int d = 0; d < 3; ++d) {
+    fbx.lo[d] = cbx.lo[d] * ratio;
+    fbx.hi[d] = (cbx.hi[d] + 1) * ratio - 1;
+  }
+  V C, F;
+  int rc;
+  const int nc = 7 + c->ns;
+  if ((rc = a4_view(c, crse, cbx, nc, "uout", C))) return rc;
+  if ((rc = a4_view(c, fine, fbx, nc, "uout", F))) return rc;
+  const long long n = cbx.npts();
+  hipLaunchKernelGGL(k_average_down, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, C, F, cbx.lo[0], cbx.lo[1], cbx.lo[2],
+                     cbx.hi[0] - cbx.lo[0] + 1, cbx.hi[1] - cbx.lo[1] + 1,
+                     cbx.hi[2] - cbx.lo[2] + 1, ratio, nc);
+  HIPCHK(c, hipGetLastError());
   return PC_OK;
 }
 

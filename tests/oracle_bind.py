@@ -56,6 +56,15 @@ def lib():
                                      P(Array4), P(Array4), Array4]
     L.pc_ref_laxf.argtypes = [C.c_int, d3, d3, d3, d3, d3]
     L.pc_ref_derive.argtypes = [C.c_int, Box, Array4, i3, i3, C.c_int, Array4, d3]
+    L.pc_ref_fluxreg_create.argtypes = [C.c_int, P(Box), C.c_int, P(Box), C.c_int, i3,
+                                        i3, i3, C.c_int]
+    L.pc_ref_fluxreg_create.restype = C.c_void_p
+    L.pc_ref_fluxreg_destroy.argtypes = [C.c_void_p]
+    L.pc_ref_fluxreg_nfaces.argtypes = [C.c_void_p]
+    L.pc_ref_fluxreg_crse_add.argtypes = [C.c_void_p, Box, P(Array4), C.c_double]
+    L.pc_ref_fluxreg_fine_add.argtypes = [C.c_void_p, Box, P(Array4), C.c_double]
+    L.pc_ref_fluxreg_reflux.argtypes = [C.c_void_p, Box, Array4, C.c_double]
+    L.pc_ref_average_down.argtypes = [C.c_int, Box, Array4, Array4, C.c_int]
     _lib = L
     return L
 
