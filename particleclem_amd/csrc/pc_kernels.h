@@ -45,6 +45,29 @@ struct VT {
 };
 using V = VT<double>;
 
+// The same view with the component offset added to the (wave-uniform) base
+// and the cell offset a 32-bit byte offset (never negative: every access lies
+// in the view's box): loads take the SGPR-base + VGPR-offset form and all
+// components of a cell share one offset register instead of a 64-bit
+// address each.  Each component then holds its own SGPR base, so this pays
+// only in kernels with few arrays (the transverse kernels: 171 -> 168
+// VGPRs, 2 -> 3 waves per SIMD); with many arrays the bases spill.
+template <class T>
+struct VS : VT<T> {
+  PCD typename VT<T>::gT& operator()(int i, int j, int k, int n) const {
+    typedef __attribute__((address_space(1))) char gC;
+    const unsigned off = (unsigned)((i - this->b0) + (j - this->b1) * this->js +
+                                    (k - this->b2) * this->ks) *
+                         (unsigned)sizeof(T);
+    return *(typename VT<T>::gT*)((gC*)((typename VT<T>::gT*)this->p +
+                                        (long)n * this->ns) + off);
+  }
+};
+template <class T>
+PCD VS<T> split(const VT<T>& v) {
+  return VS<T>{v};
+}
+
 // Compact layouts of the stage intermediates.
 //   traces: comps [0, QT) = minus state (left of face), [QT, 2QT) = plus
 //           state; per state rho, u, v, w, p, rhoe, Y[NS]  (QT = 6 + NS)
@@ -262,8 +285,8 @@ __host__ __device__ inline void stage_region(int st, const int L[3],
 }
 
 // ----------------------------------------------------------------- loads --
-template <int NS, class FP>
-PCD void load_trace(const VT<FP>& tr, int i, int j, int k, int side,
+template <int NS, class FP, class VW>
+PCD void load_trace(const VW& tr, int i, int j, int k, int side,
                     FaceState<NS, FP>& s) {
   constexpr int QT = 6 + NS;
   const int o = side * QT;
@@ -290,8 +313,8 @@ PCD void store_trace(const VT<FP>& tr, int i, int j, int k, int side,
 #pragma unroll
   for (int n = 0; n < NS; ++n) tr(i, j, k, o + 6 + n) = s.Y[n];
 }
-template <int NS, class FP>
-PCD void store_flux(const VT<FP>& f, int i, int j, int k, const Flux<NS, FP>& F) {
+template <int NS, class FP, class VW>
+PCD void store_flux(const VW& f, int i, int j, int k, const Flux<NS, FP>& F) {
   f(i, j, k, 0) = F.f[0];
   f(i, j, k, 1) = F.f[1];
   f(i, j, k, 2) = F.f[2];
@@ -755,8 +778,8 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // flux differences across cell c in direction T from a compact flux array
-template <int NS, int T, class FP>
-PCD void dflux(const VT<FP>& f, int i, int j, int k, FP c, DFlux<NS, FP>& D) {
+template <int NS, int T, class FP, class VW>
+PCD void dflux(const VW& f, int i, int j, int k, FP c, DFlux<NS, FP>& D) {
   int ip = i, jp = j, kp = k;
   step<T>(ip, jp, kp, 1);
 #pragma unroll
@@ -786,19 +809,20 @@ PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   step<D>(im, jm, km, -1);
   FaceState<NS, FP> s, L, R;
   DFlux<NS, FP> df;
+  const VS<FP> tr = split(J.tr[D]), f1 = split(J.f1[T]), qa = split(J.qa);
   // minus side: cell f - e_D, state on its high face
-  load_trace<NS, FP>(J.tr[D], i, j, k, 0, s);
-  dflux<NS, T, FP>(J.f1[T], im, jm, km, cdtdx, df);
-  transdo_side<NS>(s, df, J.qa(im, jm, km, QGAMC), cdtdx, L);
+  load_trace<NS, FP>(tr, i, j, k, 0, s);
+  dflux<NS, T, FP>(f1, im, jm, km, cdtdx, df);
+  transdo_side<NS>(s, df, qa(im, jm, km, QGAMC), cdtdx, L);
   // plus side: cell f, state on its low face
-  load_trace<NS, FP>(J.tr[D], i, j, k, 1, s);
-  dflux<NS, T, FP>(J.f1[T], i, j, k, cdtdx, df);
-  transdo_side<NS>(s, df, J.qa(i, j, k, QGAMC), cdtdx, R);
-  const FP cav = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  load_trace<NS, FP>(tr, i, j, k, 1, s);
+  dflux<NS, T, FP>(f1, i, j, k, cdtdx, df);
+  transdo_side<NS>(s, df, qa(i, j, k, QGAMC), cdtdx, R);
+  const FP cav = FP(0.5) * (qa(i, j, k, QC) + qa(im, jm, km, QC));
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   Flux<NS, FP> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
-  store_flux<NS, FP>(J.ft[2 * D + tslot(D, T)], i, j, k, F);
+  store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T)]), i, j, k, F);
 }
 template <class EOS, int D, int T, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
