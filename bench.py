@@ -36,8 +36,15 @@ HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
 METRIC = ("Mcell-updates/s per hydro step + % HBM roofline; "
           "p99 ensemble-solve latency")
-PROFILE_SUMMARY = "premixed128_step_kernels.md"
-PROFILE_SUMMARY_MOL = "premixed128_mol_step_kernels.md"
+# committed rocprofv3 summaries of the 128^3 premixed level step, by
+# (method of lines, mixed precision)
+PROFILE_SUMMARIES = {
+    (False, False): "premixed128_step_kernels.md",
+    (False, True): "premixed128_mixed_step_kernels.md",
+    (True, False): "premixed128_mol_step_kernels.md",
+    (True, True): "premixed128_mol_mixed_step_kernels.md",
+}
+FP32_PEAK = 157.3e12
 # launches per step of the kernels the MOL step runs in both Heun stages
 MOL_LAUNCHES = {"ghost": 2, "ctoprim": 2, "mol_x": 2, "mol_y": 2, "mol_z": 2}
 
@@ -110,15 +117,15 @@ def profiled_traffic(summary, stage):
     return None
 
 
-def profiled_step_flops(summary):
-    """FP64 operations of one level step from the same summary (counted
-    SQ_INSTS_VALU_*_F64 x 64 lanes), or None."""
+def profiled_step_flops(summary, kind="FP64"):
+    """FP64 (or FP32) operations of one level step from the same summary
+    (counted SQ_INSTS_VALU_*_F64 / _F32 x 64 lanes), or None."""
     path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("Sum over one step"):
-            tok = line.split("FP64 ")[1].split(" GFLOP")[0]
+            tok = line.split(kind + " ")[1].split(" GFLOP")[0]
             return float(tok) * 1e9
     return None
 
@@ -289,6 +296,9 @@ def main():
                     help="MOL with the Lax-Friedrichs flux (use_laxf_flux = 1)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick every stage's launch shape on the level first")
+    ap.add_argument("--branches", type=int, default=1, choices=[0, 1],
+                    help="1: the three directions' stages on three streams in "
+                         "the timed steps (kernel timing steps run on one)")
     ap.add_argument("--graph", type=int, default=0,
                     help="replay steps from a hipGraph of this many steps")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
@@ -368,6 +378,8 @@ def main():
     ctx = HydroContext(w.nspec)
     lev = DeviceLevel.from_workload(ctx, w)
     lev.set_precision(args.precision == "mixed")
+    if args.branches:
+        lev.set_branches(True)
     lev.init_dt()
     torch.cuda.synchronize()
     cells = w.valid_cells()
@@ -416,7 +428,7 @@ def main():
     dom = max(kt, key=lambda k: kt[k])
     dom_ms = kt[dom] / launches.get(dom, 1)   # average launch duration
     achieved = kbytes[dom] / (dom_ms * 1e-3)
-    summary = PROFILE_SUMMARY_MOL if args.mol else PROFILE_SUMMARY
+    summary = PROFILE_SUMMARIES[(bool(args.mol), args.precision == "mixed")]
     traffic = None
     if (args.workload == "premixed" and n == 128 and args.box == 32
             and not args.autotune and not args.laxf):
@@ -432,13 +444,19 @@ def main():
     nvar = w.nvar
     step_bytes = cells * 2 * nvar * 8
     hbm_frac_step = step_bytes / (ms_step * 1e-3) / HBM_PEAK
-    fp64 = None
+    fp64 = fp32 = None
     if traffic is not None:
         fl = profiled_step_flops(summary)
         if fl:
             fp64 = {"flop_per_step": fl, "achieved_tflops": fl / (ms_step * 1e-3) / 1e12,
                     "peak_tflops": FP64_PEAK / 1e12,
                     "frac": fl / (ms_step * 1e-3) / FP64_PEAK}
+        fl32 = profiled_step_flops(summary, "FP32") if args.precision == "mixed" else None
+        if fl32:
+            fp32 = {"flop_per_step": fl32,
+                    "achieved_tflops": fl32 / (ms_step * 1e-3) / 1e12,
+                    "peak_tflops": FP32_PEAK / 1e12,
+                    "frac": fl32 / (ms_step * 1e-3) / FP32_PEAK}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -464,10 +482,12 @@ def main():
                    "boxes": len(w.boxes), "nspec": w.nspec,
                    "parallelism": "single GPU, all boxes per launch",
                    "autotuned": bool(args.autotune), "graph_steps": args.graph,
+                   "branch_streams": bool(args.branches),
                    "plans": plans},
         "roofline": roof,
         "hbm_frac_step": hbm_frac_step,
         "fp64_step": fp64,
+        "fp32_step": fp32,
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
         "cpu_baseline": cpu,
     }

@@ -309,6 +309,17 @@ int pc_level_set_precision_v1(pc_level_t* lev, int mixed);
  * steps into a hipGraph once (re-captured after autotuning) and replays it;
  * leftover steps and timed steps launch directly.  0 turns graphs off. */
 int pc_level_set_graph_v1(pc_level_t* lev, int steps_per_graph);
+/* on = 1: the stages of the three directions that do not depend on each
+ * other (trace + first fluxes, the two transverse fluxes, the final fluxes
+ * of a direction; the MOL fluxes) run on three streams of the level, forked
+ * from and joined back to the stream a step is issued on, so kernels of
+ * different directions overlap on the device.  Results are unchanged (each
+ * kernel writes only its own direction's arrays).  Steps that time their
+ * kernels (pc_level_set_timing_v1) and steps captured into a hipGraph
+ * (pc_level_set_graph_v1) stay on one stream.  No reference
+ * counterpart (the reference issues its kernels on one stream).
+ * Synchronous. */
+int pc_level_set_branches_v1(pc_level_t* lev, int on);
 /* okToContinue (Source/PeleC.cpp:1310-1340): synchronous; returns
  * PC_ERR_NAN if any step produced NaN/Inf, PC_ERR_DT if dt < dt_cutoff. */
 int pc_level_status_v1(pc_level_t* lev, int* bad, int* small_dt);
@@ -397,6 +408,10 @@ int pc_batch_add_v1(pc_batch_t* batch, pc_box_t bx, const int* bclo,
                     int prim_mode, void* stream, int* slot);
 /* One step of each listed request (distinct live slots); asynchronous. */
 int pc_batch_step_v1(pc_batch_t* batch, const int* slots, int n, void* stream);
+/* on = 1: the batch's steps run the three directions' independent stages on
+ * three streams, as pc_level_set_branches_v1; results are unchanged.  No
+ * reference counterpart.  Synchronous. */
+int pc_batch_set_branches_v1(pc_batch_t* batch, int on);
 /* The request's whole state FAB (grow(bx,4) x NVAR, Array4 layout) into buf
  * (host or device memory); synchronous. */
 int pc_batch_read_v1(pc_batch_t* batch, int slot, double* buf);

@@ -764,6 +764,47 @@ int pc_mol_src_box_v1(pc_ctx_t* c, pc_box_t bx, pc_array4_t sarr,
 }  // extern "C"
 
 // ==================================================================== level
+// Three streams for the three directions of a step, forked from and joined
+// back to the stream the step is issued on.  Event bev[e][b] marks the end of
+// branch b's phase e (bev[0][0]: the fork point on the step's stream).
+struct Branches {
+  bool on = false;
+  bool made = false;
+  hipStream_t br[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t bev[4][3];
+  hipError_t make() {
+    if (made) return hipSuccess;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 3 && e == hipSuccess; ++b)
+      e = hipStreamCreateWithFlags(&br[b], hipStreamNonBlocking);
+    for (int x = 0; x < 4 && e == hipSuccess; ++x)
+      for (int b = 0; b < 3 && e == hipSuccess; ++b)
+        e = hipEventCreateWithFlags(&bev[x][b], hipEventDisableTiming);
+    made = e == hipSuccess;
+    return e;
+  }
+  void release() {
+    if (!made) return;
+    for (int b = 0; b < 3; ++b) (void)hipStreamDestroy(br[b]);
+    for (int x = 0; x < 4; ++x)
+      for (int b = 0; b < 3; ++b) (void)hipEventDestroy(bev[x][b]);
+    made = false;
+  }
+  void fork(hipStream_t st) {  // the branches wait for the step's stream
+    (void)hipEventRecord(bev[0][0], st);
+    for (int b = 0; b < 3; ++b) (void)hipStreamWaitEvent(br[b], bev[0][0], 0);
+  }
+  void done(int e, int b) { (void)hipEventRecord(bev[e][b], br[b]); }
+  void cross(int e) {  // each branch waits for the other branches' phase e
+    for (int b = 0; b < 3; ++b)
+      for (int o = 0; o < 3; ++o)
+        if (o != b) (void)hipStreamWaitEvent(br[b], bev[e][o], 0);
+  }
+  void join(int e, hipStream_t st) {  // the step's stream waits for all
+    for (int b = 0; b < 3; ++b) (void)hipStreamWaitEvent(st, bev[e][b], 0);
+  }
+};
+
 struct pc_level {
   pc_ctx* ctx = nullptr;
   pc_params_t prm;
@@ -797,6 +838,7 @@ struct pc_level {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipStream_t cap = nullptr;
+  Branches brs;  // per-direction streams (pc_level_set_branches_v1)
 };
 
 static void plan_free(StagePlan& P) {
@@ -863,6 +905,7 @@ static void level_free(pc_level* L) {
   for (int s = 0; s < ST_COUNT; ++s) plan_free(L->plans[s]);
   if (L->ev_made)
     for (int e = 0; e <= PC_NKERN; ++e) (void)hipEventDestroy(L->ev[e]);
+  L->brs.release();
   delete L;
 }
 
@@ -1247,8 +1290,49 @@ int pc_level_enable_timing_v1(pc_level_t* L, int on) {
 // primitives, three flux launches and an update; 13 launches).  With timing
 // on, an event follows every launch and the step ends with a host wait, so
 // per-kernel device times accumulate in ktime[] under the kernel's id.
+// The directional stages of a CTU step (trace and first fluxes, transverse
+// fluxes, final fluxes) over the three branch streams.  A phase waits for the
+// other branches' previous phase where it reads their output: the transverse
+// fluxes of direction d read the other directions' first fluxes, the final
+// fluxes of d the transverse fluxes computed on the other two branches.
+template <class EOS, class H>
+static void branch_ctu_fluxes(H* L, const Sched* S, const StagePlan* P,
+                              hipStream_t st, Branches& B) {
+  B.fork(st);
+  for (int d = 0; d < 3; ++d) {
+    launch_stage<EOS>(L, ST_TRACE, S[ST_TRACE], P[ST_TRACE], B.br[d], d);
+    launch_stage<EOS>(L, ST_FLUX1_X + d, S[ST_FLUX1_X + d], P[ST_FLUX1_X + d], B.br[d]);
+    B.done(1, d);
+  }
+  B.cross(1);
+  for (int d = 0; d < 3; ++d) {
+    for (int x = ST_TRANS_XY + 2 * d; x < ST_TRANS_XY + 2 * d + 2; ++x)
+      launch_stage<EOS>(L, x, S[x], P[x], B.br[d]);
+    B.done(2, d);
+  }
+  B.cross(2);
+  for (int d = 0; d < 3; ++d) {
+    launch_stage<EOS>(L, ST_FINAL_X + d, S[ST_FINAL_X + d], P[ST_FINAL_X + d], B.br[d]);
+    B.done(3, d);
+  }
+  B.join(3, st);
+}
+
+// The three MOL flux directions of one Heun stage over the branch streams.
+template <class EOS, class H>
+static void branch_mol_fluxes(H* L, const Sched* S, const StagePlan* P,
+                              hipStream_t st, Branches& B) {
+  B.fork(st);
+  for (int d = 0; d < 3; ++d) {
+    launch_stage<EOS>(L, ST_MOL_X + d, S[ST_MOL_X + d], P[ST_MOL_X + d], B.br[d]);
+    B.done(1, d);
+  }
+  B.join(1, st);
+}
+
 template <class EOS>
-static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
+static int level_step_eos(pc_level* L, int nsteps, hipStream_t st,
+                          bool capturing = false) {
   const Sched* S = L->scs;
   const StagePlan* P = L->plans;
   for (int s = 0; s < nsteps; ++s) {
@@ -1266,13 +1350,29 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st) {
       mark(id);
     };
     mark(-1);
+    // Branch mode (not while timing kernels one by one): the stages of the
+    // three directions that do not depend on each other run on three
+    // streams.  Graph captures stay on one stream: a capture forked across
+    // the branch streams crashed the process (host segmentation fault in the
+    // step call; cause not found), so that path is off.
+    const bool brm = L->brs.on && !tm && !capturing;
     if (L->mol) {
       for (int h = 0; h < 2; ++h) {
         run(ST_GHOST, PC_K_GHOST);
         run(ST_MOL_PRIM, PC_K_CTOPRIM);
-        for (int d = 0; d < 3; ++d) run(ST_MOL_X + d, PC_K_MOL_X + d);
+        if (brm) {
+          branch_mol_fluxes<EOS>(L, S, P, st, L->brs);
+        } else {
+          for (int d = 0; d < 3; ++d) run(ST_MOL_X + d, PC_K_MOL_X + d);
+        }
         run(ST_MOL_UPD, h == 0 ? PC_K_MOL_UPD1 : PC_K_MOL_UPD2, h);
       }
+    } else if (brm) {
+      run(ST_GHOST, PC_K_GHOST);
+      run(ST_CTOPRIM, PC_K_CTOPRIM);
+      run(ST_DIVU, PC_K_DIVU);
+      branch_ctu_fluxes<EOS>(L, S, P, st, L->brs);
+      run(ST_UPDATE, PC_K_UPDATE);
     } else {
       run(ST_GHOST, PC_K_GHOST);
       run(ST_CTOPRIM, PC_K_CTOPRIM);
@@ -1413,7 +1513,7 @@ static int level_step_graph(pc_level* L, int nsteps, hipStream_t st) {
     if (!L->cap) HIPCHK(c, hipStreamCreateWithFlags(&L->cap, hipStreamNonBlocking));
     HIPCHK(c, hipStreamSynchronize(st));
     HIPCHK(c, hipStreamBeginCapture(L->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = level_step_eos<EOS>(L, L->gsteps, L->cap);
+    const int rc = level_step_eos<EOS>(L, L->gsteps, L->cap, true);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(L->cap, &g);
     if (rc != PC_OK) {
@@ -1496,6 +1596,14 @@ int pc_level_set_graph_v1(pc_level_t* L, int steps_per_graph) {
   if (!L || steps_per_graph < 0) return PC_ERR_ARG;
   graph_drop(L);
   L->gsteps = steps_per_graph;
+  return PC_OK;
+}
+
+int pc_level_set_branches_v1(pc_level_t* L, int on) {
+  if (!L || (on != 0 && on != 1)) return PC_ERR_ARG;
+  if (on) HIPCHK(L->ctx, L->brs.make());
+  graph_drop(L);
+  L->brs.on = on != 0;
   return PC_OK;
 }
 
@@ -2089,6 +2197,7 @@ struct pc_batch {
   std::vector<int> cur;   // the stepped subset the tables describe
   StagePlan plans[ST_COUNT];
   Sched scs[ST_COUNT];
+  Branches brs;  // per-direction streams (pc_batch_set_branches_v1)
 };
 
 static size_t batch_slot_bytes(const pc_batch* B, const IBox& bx) {
@@ -2108,6 +2217,7 @@ static void batch_free(pc_batch* B) {
   if (B->dstart) (void)hipFree(B->dstart);
   if (B->dids) (void)hipFree(B->dids);
   if (B->dinit) (void)hipFree(B->dinit);
+  B->brs.release();
   delete B;
 }
 
@@ -2147,8 +2257,14 @@ static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
   const StagePlan* P = B->plans;
   if (B->mol) {
     for (int h = 0; h < 2; ++h) {
-      for (int x : {ST_GHOST, ST_MOL_PRIM, ST_MOL_X, ST_MOL_Y, ST_MOL_Z})
+      for (int x : {ST_GHOST, ST_MOL_PRIM})
         launch_stage<EOS>(B, x, S[x], P[x], st);
+      if (B->brs.on) {
+        branch_mol_fluxes<EOS>(B, S, P, st, B->brs);
+      } else {
+        for (int x : {ST_MOL_X, ST_MOL_Y, ST_MOL_Z})
+          launch_stage<EOS>(B, x, S[x], P[x], st);
+      }
       launch_stage<EOS>(B, ST_MOL_UPD, S[ST_MOL_UPD], P[ST_MOL_UPD], st, h);
     }
     hipLaunchKernelGGL(k_dt_finalize_list, dim3((n + 63) / 64), dim3(64), 0, st,
@@ -2158,9 +2274,13 @@ static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
   launch_stage<EOS>(B, ST_GHOST, S[ST_GHOST], P[ST_GHOST], st);
   launch_stage<EOS>(B, ST_CTOPRIM, S[ST_CTOPRIM], P[ST_CTOPRIM], st);
   launch_stage<EOS>(B, ST_DIVU, S[ST_DIVU], P[ST_DIVU], st);
-  launch_stage<EOS>(B, ST_TRACE, S[ST_TRACE], P[ST_TRACE], st);
-  for (int x = ST_FLUX1_X; x <= ST_FINAL_Z; ++x)
-    launch_stage<EOS>(B, x, S[x], P[x], st);
+  if (B->brs.on) {
+    branch_ctu_fluxes<EOS>(B, S, P, st, B->brs);
+  } else {
+    launch_stage<EOS>(B, ST_TRACE, S[ST_TRACE], P[ST_TRACE], st);
+    for (int x = ST_FLUX1_X; x <= ST_FINAL_Z; ++x)
+      launch_stage<EOS>(B, x, S[x], P[x], st);
+  }
   launch_stage<EOS>(B, ST_UPDATE, S[ST_UPDATE], P[ST_UPDATE], st);
   hipLaunchKernelGGL(k_dt_finalize_list, dim3((n + 63) / 64), dim3(64), 0, st,
                      B->ds, (const int*)B->dids, n, B->la, 0);
@@ -2332,6 +2452,13 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
   HIPCHK(c, hipStreamSynchronize(st));
   S.live = true;
   *slot_out = id;
+  return PC_OK;
+}
+
+int pc_batch_set_branches_v1(pc_batch_t* B, int on) {
+  if (!B || (on != 0 && on != 1)) return PC_ERR_ARG;
+  if (on) HIPCHK(B->ctx, B->brs.make());
+  B->brs.on = on != 0;
   return PC_OK;
 }
 

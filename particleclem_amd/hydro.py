@@ -267,6 +267,11 @@ class DeviceLevel:
         """Replay steps from a hipGraph of `steps_per_graph` steps (0: off)."""
         self._chk(self.L.pc_level_set_graph_v1(self.h, int(steps_per_graph)))
 
+    def set_branches(self, on=True):
+        """Run the three directions' independent stages on three streams
+        (pc_level_set_branches_v1); results are unchanged."""
+        self._chk(self.L.pc_level_set_branches_v1(self.h, 1 if on else 0))
+
     def write_plotfile(self, path, variables=None, time=0.0, step=0,
                        prob_lo=(0.0, 0.0, 0.0), stream=None):
         """AMReX plotfile directory `path` of the level (pc_level_write_plotfile_v1):

@@ -65,6 +65,10 @@ class Batch:
             int(prim_mode), _stream_ptr(stream), C.byref(slot)))
         return slot.value
 
+    def set_branches(self, on=True):
+        """Three streams for the three directions' stages (results unchanged)."""
+        self._chk(self.L.pc_batch_set_branches_v1(self.h, 1 if on else 0))
+
     def step(self, slots, stream=None):
         if not slots:
             return
@@ -155,10 +159,12 @@ def request_fields(req):
 class Server:
     """The request loop.  `budget_frac` of the device memory is one arena
     shared by the tube and slab batches; requests are admitted first come
-    first served while their state and workspace fit."""
+    first served while their state and workspace fit.  overlap: each batch
+    steps on a stream of its own (the tube and slab batches' kernels
+    overlap) and runs the three directions' stages on three streams."""
 
     def __init__(self, mixed=True, budget_frac=0.8, budget_bytes=None,
-                 max_slots=512, keep_results=False, mol=False):
+                 max_slots=512, keep_results=False, mol=False, overlap=True):
         import torch
         self.ctx = {TUBE: HydroContext(1), SLAB: HydroContext(9)}
         if budget_bytes is None:
@@ -173,6 +179,11 @@ class Server:
         self.batch[TUBE] = Batch(self.ctx[TUBE], params, mixed,
                                  share=self.batch[SLAB], max_slots=max_slots)
         self.keep_results = keep_results
+        self.streams = {k: None for k in self.batch}
+        if overlap:
+            for k, b in self.batch.items():
+                b.set_branches(True)
+                self.streams[k] = torch.cuda.Stream()
 
     def close(self):
         for b in self.batch.values():
@@ -211,7 +222,7 @@ class Server:
                     time.sleep(max(0.0, pending[0].t_arrive - (time.perf_counter() - t0)))
                 continue
             for kind, b in self.batch.items():
-                b.step([r.slot for r in active if r.kind == kind])
+                b.step([r.slot for r in active if r.kind == kind], self.streams[kind])
             ticks += 1
             for r in active:
                 r.done += 1
