@@ -315,7 +315,7 @@ int pc_level_set_graph_v1(pc_level_t* lev, int steps_per_graph);
  * from and joined back to the stream a step is issued on, so kernels of
  * different directions overlap on the device.  Results are unchanged (each
  * kernel writes only its own direction's arrays).  Steps that time their
- * kernels (pc_level_set_timing_v1) and steps captured into a hipGraph
+ * kernels (pc_level_enable_timing_v1) and steps captured into a hipGraph
  * (pc_level_set_graph_v1) stay on one stream.  No reference
  * counterpart (the reference issues its kernels on one stream).
  * Synchronous. */
