@@ -765,43 +765,44 @@ int pc_mol_src_box_v1(pc_ctx_t* c, pc_box_t bx, pc_array4_t sarr,
 
 // ==================================================================== level
 // Three streams for the three directions of a step, forked from and joined
-// back to the stream the step is issued on.  Event bev[e][b] marks the end of
-// branch b's phase e (bev[0][0]: the fork point on the step's stream).
+// back to the stream the step is issued on.  Event ev[k] marks the end of a
+// launch (or of the fork point on the step's stream); a launch on one branch
+// waits only for the launches of other branches whose output it reads.
 struct Branches {
+  static constexpr int NEV = 16;
   bool on = false;
   bool made = false;
   hipStream_t br[3] = {nullptr, nullptr, nullptr};
-  hipEvent_t bev[4][3];
+  hipEvent_t ev[NEV];
   hipError_t make() {
     if (made) return hipSuccess;
     hipError_t e = hipSuccess;
     for (int b = 0; b < 3 && e == hipSuccess; ++b)
       e = hipStreamCreateWithFlags(&br[b], hipStreamNonBlocking);
-    for (int x = 0; x < 4 && e == hipSuccess; ++x)
-      for (int b = 0; b < 3 && e == hipSuccess; ++b)
-        e = hipEventCreateWithFlags(&bev[x][b], hipEventDisableTiming);
+    for (int x = 0; x < NEV && e == hipSuccess; ++x)
+      e = hipEventCreateWithFlags(&ev[x], hipEventDisableTiming);
     made = e == hipSuccess;
     return e;
   }
   void release() {
     if (!made) return;
     for (int b = 0; b < 3; ++b) (void)hipStreamDestroy(br[b]);
-    for (int x = 0; x < 4; ++x)
-      for (int b = 0; b < 3; ++b) (void)hipEventDestroy(bev[x][b]);
+    for (int x = 0; x < NEV; ++x) (void)hipEventDestroy(ev[x]);
     made = false;
   }
+  // event slots: 0 fork point, 1-3 first fluxes of d, 4-12 transverse flux
+  // (d|t) at 4 + 3d + t, 13-15 last launch of branch d
+  static int F1(int d) { return 1 + d; }
+  static int TR(int d, int t) { return 4 + 3 * d + t; }
+  static int END(int d) { return 13 + d; }
   void fork(hipStream_t st) {  // the branches wait for the step's stream
-    (void)hipEventRecord(bev[0][0], st);
-    for (int b = 0; b < 3; ++b) (void)hipStreamWaitEvent(br[b], bev[0][0], 0);
+    (void)hipEventRecord(ev[0], st);
+    for (int b = 0; b < 3; ++b) (void)hipStreamWaitEvent(br[b], ev[0], 0);
   }
-  void done(int e, int b) { (void)hipEventRecord(bev[e][b], br[b]); }
-  void cross(int e) {  // each branch waits for the other branches' phase e
-    for (int b = 0; b < 3; ++b)
-      for (int o = 0; o < 3; ++o)
-        if (o != b) (void)hipStreamWaitEvent(br[b], bev[e][o], 0);
-  }
-  void join(int e, hipStream_t st) {  // the step's stream waits for all
-    for (int b = 0; b < 3; ++b) (void)hipStreamWaitEvent(st, bev[e][b], 0);
+  void done(int k, int b) { (void)hipEventRecord(ev[k], br[b]); }
+  void wait(int b, int k) { (void)hipStreamWaitEvent(br[b], ev[k], 0); }
+  void join(hipStream_t st) {  // the step's stream waits for every branch
+    for (int b = 0; b < 3; ++b) (void)hipStreamWaitEvent(st, ev[END(b)], 0);
   }
 };
 
@@ -1290,11 +1291,12 @@ int pc_level_enable_timing_v1(pc_level_t* L, int on) {
 // primitives, three flux launches and an update; 13 launches).  With timing
 // on, an event follows every launch and the step ends with a host wait, so
 // per-kernel device times accumulate in ktime[] under the kernel's id.
-// The directional stages of a CTU step (trace and first fluxes, transverse
-// fluxes, final fluxes) over the three branch streams.  A phase waits for the
-// other branches' previous phase where it reads their output: the transverse
-// fluxes of direction d read the other directions' first fluxes, the final
-// fluxes of d the transverse fluxes computed on the other two branches.
+// The directional stages of a CTU step over the three branch streams:
+// branch d runs trace_d and flux1_d, then the transverse fluxes (d|t) (each
+// after the first fluxes of t, computed on branch t), then final_d (after the
+// transverse fluxes (t0|t1) and (t1|t0) of the other two directions, which
+// the other two branches compute).  Every read of another branch's output
+// waits for the event recorded after the launch that writes it.
 template <class EOS, class H>
 static void branch_ctu_fluxes(H* L, const Sched* S, const StagePlan* P,
                               hipStream_t st, Branches& B) {
@@ -1302,20 +1304,24 @@ static void branch_ctu_fluxes(H* L, const Sched* S, const StagePlan* P,
   for (int d = 0; d < 3; ++d) {
     launch_stage<EOS>(L, ST_TRACE, S[ST_TRACE], P[ST_TRACE], B.br[d], d);
     launch_stage<EOS>(L, ST_FLUX1_X + d, S[ST_FLUX1_X + d], P[ST_FLUX1_X + d], B.br[d]);
-    B.done(1, d);
+    B.done(Branches::F1(d), d);
   }
-  B.cross(1);
-  for (int d = 0; d < 3; ++d) {
-    for (int x = ST_TRANS_XY + 2 * d; x < ST_TRANS_XY + 2 * d + 2; ++x)
+  for (int d = 0; d < 3; ++d)
+    for (int t = 0; t < 3; ++t) {
+      if (t == d) continue;
+      const int x = ST_TRANS_XY + 2 * d + (t > d ? t - 1 : t);  // (d|t)
+      B.wait(d, Branches::F1(t));
       launch_stage<EOS>(L, x, S[x], P[x], B.br[d]);
-    B.done(2, d);
-  }
-  B.cross(2);
+      B.done(Branches::TR(d, t), d);
+    }
   for (int d = 0; d < 3; ++d) {
+    const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
+    B.wait(d, Branches::TR(t0, t1));
+    B.wait(d, Branches::TR(t1, t0));
     launch_stage<EOS>(L, ST_FINAL_X + d, S[ST_FINAL_X + d], P[ST_FINAL_X + d], B.br[d]);
-    B.done(3, d);
+    B.done(Branches::END(d), d);
   }
-  B.join(3, st);
+  B.join(st);
 }
 
 // The three MOL flux directions of one Heun stage over the branch streams.
@@ -1325,9 +1331,9 @@ static void branch_mol_fluxes(H* L, const Sched* S, const StagePlan* P,
   B.fork(st);
   for (int d = 0; d < 3; ++d) {
     launch_stage<EOS>(L, ST_MOL_X + d, S[ST_MOL_X + d], P[ST_MOL_X + d], B.br[d]);
-    B.done(1, d);
+    B.done(Branches::END(d), d);
   }
-  B.join(1, st);
+  B.join(st);
 }
 
 template <class EOS>
