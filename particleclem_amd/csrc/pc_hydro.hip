@@ -319,6 +319,9 @@ struct StagePlan {
 
 // logical blocks of one job for a stage and block shape
 static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz) {
+  // the ghost fill enumerates its cells itself (launch shapes apply only
+  // through persistence)
+  if (st == ST_GHOST) return (ghost_shell_count(J.lo, J.hi) + BLOCK - 1) / BLOCK;
   int lo[3], n[3];
   stage_region(st, J.lo, J.hi, lo, n);
   if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return 0;
@@ -1462,6 +1465,9 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
     int bestc = -1;
     StagePlan bestP;
     for (int k = 0; k < kNTune && rc == PC_OK; ++k) {
+      // the ghost fill enumerates its cells itself: only the persistence of
+      // the linear shapes differs
+      if (s == ST_GHOST && kTuneCand[k][0] >= 0) continue;
       StagePlan P;
       const int cap = kTuneCand[k][3] * ncu;
       if (!plan_make(L, s, kTuneCand[k][0], kTuneCand[k][1], kTuneCand[k][2],

@@ -37,12 +37,8 @@ PCD void b_ghost(const Sched& sc, const KArgs& ka, int lb_, LevelArgs la) {
   constexpr int NVAR = 7 + NS;
   int local;
   const Job J = load_job(sc, lb_, local);
-  int lo[3], n[3], i, j, k;
-  stage_region(ST_GHOST, J.lo, J.hi, lo, n);
-  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  if (i >= J.lo[0] && i <= J.hi[0] && j >= J.lo[1] && j <= J.hi[1] &&
-      k >= J.lo[2] && k <= J.hi[2])
-    return;
+  int i, j, k;
+  if (!ghost_shell_cell(J.lo, J.hi, local, i, j, k)) return;
   // periodic_self: 0 = boxes tile the level domain, 1 = every box is its own
   // periodic domain, 2 = every box is its own domain with its own BCs
   int D0[3], D1[3], bl[3], bh[3];
