@@ -245,6 +245,94 @@ def serve(args):
     print(json.dumps(out))
 
 
+def dropin(args):
+    """The drop-in form: PeleC's construct_hydro_source calling the library
+    once per box (pc_hydro_source_box_v1: primitives, trace, Riemann solves,
+    transverse corrections, flux adjustment and dU/dt of one box, with its
+    ghost cells already filled, caller-owned FABs for the state, hydro
+    source, fluxes, areas and volume), over the 64 boxes of 32^3 of the
+    premixed 128^3 level per evaluation, the boxes spread over --streams
+    streams in rotation as AMReX's MFIter does on GPUs (4 by default).
+    value = valid cells x evaluations / wall time; no ghost fill, update or
+    time step (the caller's part)."""
+    import ctypes as C
+    import torch
+    from particleclem_amd import _lib, workloads
+    from particleclem_amd.ctypes_types import Array4, make_box, make_params, null_a4
+    from particleclem_amd.hydro import (DeviceLevel, HydroContext, _d3, _i3,
+                                        _stream_ptr, a4_from_tensor)
+    n = args.n or 128
+    w = workloads.premixed(n=n, box=args.box, nsteps=1)
+    ctx = HydroContext(w.nspec)
+    lev = DeviceLevel.from_workload(ctx, w)
+    lev.init_dt()
+    lev.fill_ghosts()
+    dt = lev.get_dt()
+    nv = w.nvar
+    dx = w.dx
+    prm = make_params(**w.params)
+    streams = [torch.cuda.Stream() for _ in range(max(1, args.streams))]
+    L = _lib.lib()
+    calls, keep = [], []
+    for b, (lo, hi) in enumerate(w.boxes):
+        U = torch.empty(lev.fab_shape(b), dtype=torch.float64, device="cuda")
+        _lib.check(L.pc_level_read_box_v1(lev.h, b, C.c_void_p(U.data_ptr())))
+        shp = tuple(hi[d] - lo[d] + 1 for d in (2, 1, 0))
+        hs = torch.zeros((nv,) + shp, dtype=torch.float64, device="cuda")
+        fl, ar = [], []
+        for d in range(3):
+            fs = list(shp)
+            fs[2 - d] += 1
+            fl.append(torch.zeros((nv,) + tuple(fs), dtype=torch.float64, device="cuda"))
+            ar.append(torch.full((1,) + tuple(fs), [dx[1] * dx[2], dx[2] * dx[0],
+                                                    dx[0] * dx[1]][d],
+                                 dtype=torch.float64, device="cuda"))
+        vol = torch.full((1,) + shp, dx[0] * dx[1] * dx[2], dtype=torch.float64,
+                         device="cuda")
+        g4 = [x - 4 for x in lo]
+        keep += [U, hs, vol] + fl + ar
+        calls.append((ctx.h, make_box(lo, hi), _i3(w.domlo), _i3(w.domhi),
+                      _i3(w.bclo), _i3(w.bchi), a4_from_tensor(U, g4), null_a4(),
+                      a4_from_tensor(hs, lo), _d3(dx), float(dt), C.byref(prm),
+                      (Array4 * 3)(*[a4_from_tensor(f, lo) for f in fl]),
+                      (Array4 * 3)(*[a4_from_tensor(a, lo) for a in ar]),
+                      a4_from_tensor(vol, lo), _stream_ptr(streams[b % len(streams)])))
+    lev.close()
+    fn = L.pc_hydro_source_box_v1
+
+    def evaluation():
+        for c in calls:
+            rc = fn(*c)
+            if rc != 0:
+                _lib.check(rc, ctx.h)
+
+    for _ in range(args.warmup):
+        evaluation()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        evaluation()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    cells = w.valid_cells()
+    out = {
+        "metric": METRIC, "value": cells * args.steps / sec / 1e6,
+        "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"drop-in: pc_hydro_source_box_v1 once per box, "
+                               f"{len(w.boxes)} boxes of {args.box}^3 (premixed "
+                               f"H2/air {n}^3, 9 species, PPM) per evaluation; "
+                               f"hydro source and fluxes only (no ghost fill, "
+                               f"update or dt), boxes over {len(streams)} streams",
+                   "valid_cells": cells, "boxes": len(w.boxes), "calls_per_step": len(calls)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    ctx.close()
+    print(json.dumps(out))
+
+
 def host_cpu():
     model = "unknown"
     try:
@@ -282,7 +370,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="premixed",
-                    choices=["premixed", "sod", "ragged", "serving"])
+                    choices=["premixed", "sod", "ragged", "serving", "dropin"])
     ap.add_argument("--rate", type=float, default=100.0,
                     help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=300,
@@ -296,6 +384,8 @@ def main():
                     help="MOL with the Lax-Friedrichs flux (use_laxf_flux = 1)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick every stage's launch shape on the level first")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="dropin: streams the per-box calls rotate over")
     ap.add_argument("--branches", type=int, default=1, choices=[0, 1],
                     help="1: the three directions' stages on three streams in "
                          "the timed steps (kernel timing steps run on one)")
@@ -332,6 +422,8 @@ def main():
     torch.cuda.set_device(0)
     if args.workload == "serving":
         return serve(args)
+    if args.workload == "dropin":
+        return dropin(args)
 
     if args.workload == "premixed":
         n = args.n or 128

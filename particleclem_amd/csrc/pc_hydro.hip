@@ -27,10 +27,18 @@ struct pc_ctx {
   int ns = 0;
   int device = 0;
   std::string err;
-  // drop-in scratch (stream-ordered reuse)
-  double* ws = nullptr;
-  size_t ws_bytes = 0;
-  hipStream_t ws_stream = nullptr;
+  // drop-in scratch: one workspace per stream the box drivers are called on
+  // (reused in stream order).  A caller may spread its boxes over several
+  // streams, as AMReX's MFIter does on GPUs; calls on different streams then
+  // run concurrently and must not share a workspace.
+  struct Scratch {
+    hipStream_t st = nullptr;
+    double* p = nullptr;
+    size_t bytes = 0;
+  };
+  static constexpr int MAXWS = 16;
+  std::vector<Scratch> wsl;
+  int ws_evict = 0;
   // job descriptors of the box drivers: a ring of device slots filled from
   // pinned host slots; a slot is reused once the event of its last use fired
   static constexpr int RING = 64;
@@ -278,26 +286,41 @@ static int ctx_job_done(pc_ctx* c, int slot, hipStream_t st) {
   return PC_OK;
 }
 
-static int ctx_scratch(pc_ctx* c, size_t bytes, hipStream_t st) {
-  if (bytes <= c->ws_bytes) {
-    c->ws_stream = st;
-    return PC_OK;
+// The workspace of stream st, at least `bytes` large, in *base.
+static int ctx_scratch(pc_ctx* c, size_t bytes, hipStream_t st, double** base) {
+  int k = -1;
+  for (int i = 0; i < (int)c->wsl.size(); ++i)
+    if (c->wsl[i].st == st) k = i;
+  if (k < 0) {
+    if ((int)c->wsl.size() < pc_ctx::MAXWS) {
+      c->wsl.push_back(pc_ctx::Scratch{});
+      k = (int)c->wsl.size() - 1;
+    } else {
+      // more streams than workspaces: take one over once the device is idle
+      // (its stream may no longer exist)
+      k = c->ws_evict;
+      c->ws_evict = (c->ws_evict + 1) % pc_ctx::MAXWS;
+      HIPCHK(c, hipDeviceSynchronize());
+    }
   }
-  if (c->ws) {
-    // the previous user may still run on its stream
-    if (c->ws_stream) HIPCHK(c, hipStreamSynchronize(c->ws_stream));
-    HIPCHK(c, hipDeviceSynchronize());
-    HIPCHK(c, hipFree(c->ws));
-    c->ws = nullptr;
-    c->ws_bytes = 0;
+  pc_ctx::Scratch& s = c->wsl[k];
+  s.st = st;
+  if (bytes > s.bytes) {
+    if (s.p) {
+      // the previous calls on this stream may still run
+      HIPCHK(c, hipDeviceSynchronize());
+      HIPCHK(c, hipFree(s.p));
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    const size_t nb = std::max(bytes, (size_t)64 << 20);
+    if (hipMalloc(&s.p, nb) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, PC_ERR_NOMEM, "scratch allocation of %zu bytes failed", nb);
+    }
+    s.bytes = nb;
   }
-  size_t nb = std::max(bytes, (size_t)64 << 20);
-  if (hipMalloc(&c->ws, nb) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(c, PC_ERR_NOMEM, "scratch allocation of %zu bytes failed", nb);
-  }
-  c->ws_bytes = nb;
-  c->ws_stream = st;
+  *base = s.p;
   return PC_OK;
 }
 
@@ -478,8 +501,9 @@ int pc_ctx_create(int eos_kind, pc_ctx_t** out) {
 
 int pc_ctx_destroy(pc_ctx_t* c) {
   if (!c) return PC_ERR_ARG;
-  if (c->ws || c->djobs) (void)hipDeviceSynchronize();
-  if (c->ws) (void)hipFree(c->ws);
+  if (!c->wsl.empty() || c->djobs) (void)hipDeviceSynchronize();
+  for (auto& s : c->wsl)
+    if (s.p) (void)hipFree(s.p);
   if (c->djobs) {
     (void)hipFree(c->djobs);
     (void)hipHostFree(c->hjobs);
@@ -619,8 +643,9 @@ int pc_umdrv_v1(pc_ctx_t* c, double /*time*/, pc_box_t bx, const int* domlo,
   }
   hipStream_t st = (hipStream_t)stream;
   const WsLayout L = ws_layout(b, c->ns, false, false, false);
-  if ((rc = ctx_scratch(c, L.total * sizeof(double), st))) return rc;
-  if ((rc = finish_umdrv_job(c, J, b, c->ws, flx, a, vol, uout))) return rc;
+  double* ws;
+  if ((rc = ctx_scratch(c, L.total * sizeof(double), st, &ws))) return rc;
+  if ((rc = finish_umdrv_job(c, J, b, ws, flx, a, vol, uout))) return rc;
   set_geom(J, dx);
   J.dt_val = dt;
   const KArgs ka = umdrv_args(dx, dt, plm_iorder, use_flattening, difmag);
@@ -665,12 +690,13 @@ int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
   const WsLayout Lq = ws_layout(b, c->ns, true, src.p != nullptr, false);
   const WsLayout L = ws_layout(b, c->ns, false, false, false);
   const size_t qpart = Lq.off_tr[0];
-  if ((rc = ctx_scratch(c, (qpart + L.total) * sizeof(double), st))) return rc;
-  J.q = fab_view(c->ws + Lq.off_q, Lq.q);
-  J.qa = fab_view(c->ws + Lq.off_qa, Lq.q);
-  J.srcq = src.p ? fab_view(c->ws + Lq.off_srcq, Lq.q) : null_view();
+  double* ws;
+  if ((rc = ctx_scratch(c, (qpart + L.total) * sizeof(double), st, &ws))) return rc;
+  J.q = fab_view(ws + Lq.off_q, Lq.q);
+  J.qa = fab_view(ws + Lq.off_qa, Lq.q);
+  J.srcq = src.p ? fab_view(ws + Lq.off_srcq, Lq.q) : null_view();
   for (int d = 0; d < 3; ++d) J.qe[d] = Lq.q.hi[d] + 1;
-  if ((rc = finish_umdrv_job(c, J, b, c->ws + qpart, flx, a, vol, hyd_src)))
+  if ((rc = finish_umdrv_job(c, J, b, ws + qpart, flx, a, vol, hyd_src)))
     return rc;
   set_geom(J, dx);
   J.dt_val = dt;
@@ -743,12 +769,13 @@ int pc_mol_src_box_v1(pc_ctx_t* c, pc_box_t bx, pc_array4_t sarr,
   }
   hipStream_t st = (hipStream_t)stream;
   const WsLayout W = ws_layout(b, c->ns, true, false, flx == nullptr, true);
-  if ((rc = ctx_scratch(c, W.total * sizeof(double), st))) return rc;
-  J.q = fab_view(c->ws + W.off_q, W.q);
-  J.qa = fab_view(c->ws + W.off_qa, W.q);
+  double* ws;
+  if ((rc = ctx_scratch(c, W.total * sizeof(double), st, &ws))) return rc;
+  J.q = fab_view(ws + W.off_q, W.q);
+  J.qa = fab_view(ws + W.off_qa, W.q);
   for (int d = 0; d < 3; ++d) {
     J.qe[d] = W.q.hi[d] + 1;
-    if (!flx) J.flx[d] = fab_view(c->ws + W.off_fl[d], W.fl[d]);
+    if (!flx) J.flx[d] = fab_view(ws + W.off_fl[d], W.fl[d]);
   }
   KArgs ka;
   memset(&ka, 0, sizeof(ka));
