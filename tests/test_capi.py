@@ -115,3 +115,39 @@ def test_level_rejects_non_tiling_boxes():
     p = make_params(ppm_type=1)
     assert L.pc_level_create_v1(h, C.byref(d), C.byref(p), C.byref(lev)) == -1
     L.pc_ctx_destroy(h)
+
+
+def test_empty_inputs_are_rejected_with_codes():
+    """Empty boxes and empty box tables are argument errors (PC_ERR_ARG),
+    caught before any device work; a batch step over no requests and the
+    branch-stream switches reject null handles and bad values."""
+    from particleclem_amd.ctypes_types import Box, LevelDesc
+    L = _lib.lib()
+    h = _ctx(1)
+    i3 = lambda v: (C.c_int * 3)(*v)
+    lo, hi = [0, 0, 0], [-1, 7, 7]          # empty in x
+    flx = (Array4 * 3)(null_a4(), null_a4(), null_a4())
+    rc = L.pc_hydro_source_box_v1(h, make_box(lo, hi), i3([0, 0, 0]), i3([7, 7, 7]),
+                                  i3([0, 0, 0]), i3([0, 0, 0]), null_a4(), null_a4(),
+                                  null_a4(), (C.c_double * 3)(0.1, 0.1, 0.1), 1e-3,
+                                  C.byref(make_params(ppm_type=1)), flx, flx, null_a4(),
+                                  None)
+    assert rc == -1
+    assert "empty box" in L.pc_last_error(h).decode()
+    d = LevelDesc()
+    d.nboxes = 0
+    d.domlo = i3([0, 0, 0])
+    d.domhi = i3([7, 7, 7])
+    d.dx = (C.c_double * 3)(0.1, 0.1, 0.1)
+    lev = C.c_void_p()
+    p = make_params(ppm_type=1)
+    assert L.pc_level_create_v1(h, C.byref(d), C.byref(p), C.byref(lev)) == -1
+    boxes = (Box * 2)(make_box([0, 0, 0], [7, 7, 7]), make_box([8, 0, 0], [7, 7, 7]))
+    d.nboxes = 2
+    d.boxes = C.cast(boxes, C.POINTER(Box))
+    assert L.pc_level_create_v1(h, C.byref(d), C.byref(p), C.byref(lev)) == -1
+    assert "empty box 1" in L.pc_last_error(h).decode()
+    assert L.pc_batch_step_v1(None, None, 0, None) == -1
+    assert L.pc_level_set_branches_v1(None, 1) == -1
+    assert L.pc_batch_set_branches_v1(None, 1) == -1
+    L.pc_ctx_destroy(h)
