@@ -807,12 +807,20 @@ struct Branches {
   hipError_t make() {
     if (made) return hipSuccess;
     hipError_t e = hipSuccess;
-    for (int b = 0; b < 3 && e == hipSuccess; ++b)
-      e = hipStreamCreateWithFlags(&br[b], hipStreamNonBlocking);
-    for (int x = 0; x < NEV && e == hipSuccess; ++x)
-      e = hipEventCreateWithFlags(&ev[x], hipEventDisableTiming);
-    made = e == hipSuccess;
-    return e;
+    int nb = 0, ne = 0;  // streams and events created so far
+    while (nb < 3 &&
+           (e = hipStreamCreateWithFlags(&br[nb], hipStreamNonBlocking)) == hipSuccess)
+      ++nb;
+    while (e == hipSuccess && ne < NEV &&
+           (e = hipEventCreateWithFlags(&ev[ne], hipEventDisableTiming)) == hipSuccess)
+      ++ne;
+    if (e != hipSuccess) {  // undo the part that was created
+      for (int b = 0; b < nb; ++b) (void)hipStreamDestroy(br[b]);
+      for (int x = 0; x < ne; ++x) (void)hipEventDestroy(ev[x]);
+      return e;
+    }
+    made = true;
+    return hipSuccess;
   }
   void release() {
     if (!made) return;
