@@ -68,6 +68,52 @@ PCD VS<T> split(const VT<T>& v) {
   return VS<T>{v};
 }
 
+// A read-only view whose loads are buffer loads: the array's base in a
+// buffer resource (4 SGPRs for the whole array), the component offset in the
+// instruction's scalar offset and the cell offset a 32-bit VGPR byte offset
+// (never negative: every access lies in the view's box), so a load needs no
+// per-component VGPR address arithmetic and no per-component base register.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+#else
+struct BufRsrc {  // the host pass parses device code but never runs it
+  int w[4];
+};
+#endif
+template <class T>
+struct VB {
+  BufRsrc r;
+  int js, ks, ns, b0, b1, b2;
+  PCD T operator()(int i, int j, int k, int n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned off =
+        (unsigned)((i - b0) + (j - b1) * js + (k - b2) * ks) * (unsigned)sizeof(T);
+    const int soff = n * ns * (int)sizeof(T);
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+#else
+    return T(0);
+#endif
+  }
+};
+template <class T>
+PCD VB<T> bufview(const VT<T>& v) {
+  VB<T> b;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // raw buffer (stride 0), 2 GiB range, the data format word of gfx9 parts
+  b.r = __builtin_amdgcn_make_buffer_rsrc((void*)v.p, (short)0, 0x7fffffff, 0x00020000);
+#endif
+  b.js = v.js;
+  b.ks = v.ks;
+  b.ns = v.ns;
+  b.b0 = v.b0;
+  b.b1 = v.b1;
+  b.b2 = v.b2;
+  return b;
+}
+
 // Compact layouts of the stage intermediates.
 //   traces: comps [0, QT) = minus state (left of face), [QT, 2QT) = plus
 //           state; per state rho, u, v, w, p, rhoe, Y[NS]  (QT = 6 + NS)
@@ -571,7 +617,7 @@ PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
   int lo[3], n[3], i, j, k;
   stage_region(ST_TRACE, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const auto& q = J.q;
+  const VB<FP> q = bufview(J.q);
   const FP dtdx = get_dt(J, ka) / J.dx[D];
   constexpr int QUN = QU + D;
   constexpr int QUT = (D == 0) ? QV : ((D == 1) ? QW : QU);
@@ -1134,9 +1180,9 @@ PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
        (q3(i, j, k + 1, 0) + q3(i, j, k, 0)) *
            (q3(i, j, k + 1, 1) * A(2, i, j, k + 1) - q3(i, j, k, 1) * A(2, i, j, k))) /
       v;
-  const V& f0 = J.flx[0];
-  const V& f1 = J.flx[1];
-  const V& f2 = J.flx[2];
+  const VB<double> f0 = bufview(J.flx[0]);
+  const VB<double> f1 = bufview(J.flx[1]);
+  const VB<double> f2 = bufview(J.flx[2]);
 #pragma unroll
   for (int m = 0; m < NVAR; ++m)
     h[m] += (f0(i, j, k, m) - f0(i + 1, j, k, m) + f1(i, j, k, m) -

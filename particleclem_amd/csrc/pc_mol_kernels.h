@@ -29,8 +29,8 @@ PCD FP mol_lim(FP l, FP r) {
 // state of Source/MOL.cpp:61-104.  LEFT: qtempl of face c+1/2 (+ half a
 // slope); otherwise qtempr of face c-1/2 (- half a slope).  Outputs in the
 // normal frame: rho, un, ut (first transverse), utt (second), p, Y.
-template <int NS, int D, bool LEFT, class FP>
-PCD void mol_face_state(const VT<FP>& q, const VT<FP>& qa, bool slopes, int i,
+template <int NS, int D, bool LEFT, class FP, class VW>
+PCD void mol_face_state(const VW& q, const VW& qa, bool slopes, int i,
                         int j, int k, FP& rho, FP& un, FP& ut, FP& utt, FP& p,
                         FP* Y) {
   constexpr int IU = QU + D;
@@ -120,10 +120,11 @@ PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
   step<D>(im, jm, km, -1);
   FP rl, ul, vl, v2l, pl, rr, ur, vr, v2r, pr;
   FP Yl[NS], Yr[NS];
-  mol_face_state<NS, D, true, FP>(J.q, J.qa, slopes, im, jm, km, rl, ul, vl,
-                                  v2l, pl, Yl);
-  mol_face_state<NS, D, false, FP>(J.q, J.qa, slopes, i, j, k, rr, ur, vr,
-                                   v2r, pr, Yr);
+  const VB<FP> q = bufview(J.q), qa = bufview(J.qa);
+  mol_face_state<NS, D, true, FP>(q, qa, slopes, im, jm, km, rl, ul, vl, v2l,
+                                  pl, Yl);
+  mol_face_state<NS, D, false, FP>(q, qa, slopes, i, j, k, rr, ur, vr, v2r,
+                                   pr, Yr);
   RiemannOut<EOS, FP> o;
   if constexpr (LAXF) {
     o = laxf<EOS, FP>(rl, ul, vl, v2l, pl, Yl, rr, ur, vr, v2r, pr, Yr);
