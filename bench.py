@@ -212,7 +212,8 @@ def serve(args):
     p50/p99 latency (arrival to result ready, queueing included)."""
     from particleclem_amd import serving as sv
     reqs = sv.request_stream(args.requests, args.rate)
-    srv = sv.Server(mixed=(args.precision == "mixed"), mol=args.mol)
+    srv = sv.Server(mixed=(args.precision == "mixed"), mol=args.mol,
+                    bypass_depth=args.bypass_depth)
     warm = sv.request_stream(4, 0, seed=1, n_lo=16, n_hi=16, n_med=16, steps=(3, 3))
     srv.run(warm, realtime=False)        # kernel loading, allocator warm-up
     served = srv.run(reqs, realtime=True)
@@ -235,7 +236,8 @@ def serve(args):
                                f"sides lognormal in [32, 128] (median 64), 10-200 "
                                f"steps, {'MOL' if args.mol else 'PPM'}, CFL 0.3, "
                                f"continuous batching under an 80% HBM budget "
-                               f"({budget / 2**30:.0f} GiB)",
+                               f"({budget / 2**30:.0f} GiB), admission bypass "
+                               f"depth {args.bypass_depth}",
                    "requests": len(served), "cell_updates": cells,
                    "failed": bad},
         "latency_ms": summ["latency_ms"], "queue_ms": summ["queue_ms"],
@@ -384,6 +386,9 @@ def main():
                     help="MOL with the Lax-Friedrichs flux (use_laxf_flux = 1)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick every stage's launch shape on the level first")
+    ap.add_argument("--bypass-depth", type=int, default=0,
+                    help="serving: requests that may be admitted past a "
+                         "head request that does not fit")
     ap.add_argument("--streams", type=int, default=4,
                     help="dropin: streams the per-box calls rotate over")
     ap.add_argument("--branches", type=int, default=1, choices=[0, 1],

@@ -95,6 +95,7 @@ class Request:
         self.rid, self.kind, self.n, self.nsteps = rid, kind, n, nsteps
         self.ratio, self.phi, self.t_arrive, self.seed = ratio, phi, t_arrive, seed
         self.slot = None
+        self.blocked_since = None  # tick at which it first waited at the head
         self.done = 0
         self.t_admit = self.t_done = None
         self.result = None
@@ -164,7 +165,8 @@ class Server:
     overlap) and runs the three directions' stages on three streams."""
 
     def __init__(self, mixed=True, budget_frac=0.8, budget_bytes=None,
-                 max_slots=512, keep_results=False, mol=False, overlap=True):
+                 max_slots=512, keep_results=False, mol=False, overlap=True,
+                 bypass_depth=0, bypass_ticks=50):
         import torch
         self.ctx = {TUBE: HydroContext(1), SLAB: HydroContext(9)}
         if budget_bytes is None:
@@ -179,6 +181,11 @@ class Server:
         self.batch[TUBE] = Batch(self.ctx[TUBE], params, mixed,
                                  share=self.batch[SLAB], max_slots=max_slots)
         self.keep_results = keep_results
+        # admission past a request that does not fit: up to bypass_depth
+        # later requests that fit are admitted, until the head has waited
+        # bypass_ticks ticks (then the queue holds until the head fits)
+        self.bypass_depth = bypass_depth
+        self.bypass_ticks = bypass_ticks
         self.streams = {k: None for k in self.batch}
         if overlap:
             for k, b in self.batch.items():
@@ -206,16 +213,24 @@ class Server:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].t_arrive <= now):
                 waiting.append(pending.popleft())
-            while waiting:
-                r = waiting[0]
+            i = 0
+            while i < len(waiting):
+                r = waiting[i]
                 b = self.batch[r.kind]
                 if b.slot_bytes(r.n) > b.memory()[2]:
-                    break
+                    if i == 0 and r.blocked_since is None:
+                        r.blocked_since = ticks
+                    head = waiting[0]
+                    if (i + 1 > self.bypass_depth or
+                            ticks - head.blocked_since >= self.bypass_ticks):
+                        break
+                    i += 1
+                    continue
                 P, bclo, bchi, mode = request_fields(r)
                 r.slot = b.add(r.n, bclo, bchi, P, mode)
                 del P
                 r.t_admit = time.perf_counter() - t0
-                waiting.popleft()
+                del waiting[i]
                 active.append(r)
             if not active:
                 if pending and realtime:
