@@ -3,8 +3,8 @@
 
 Default workload: premixed H2/air, 9 species (LiDryer ideal-gas mixture),
 hydro only, 128^3 cells as 64 boxes of 32^3 with 4 ghost cells, PPM, the
-reference's approximate Riemann solver, CFL 0.3 (the BASELINE's FP64
-premixed configuration).  One "step" = ghost fill + the full hydro step
+reference's approximate Riemann solver, CFL 0.3 (the FP64 premixed H2/air
+workload the throughput metric is quoted on).  One "step" = ghost fill + the full hydro step
 (ctoprim -> CTU fluxes -> flux divergence -> S_new -> internal-energy reset
 and e->T) + the next dt, for every box of the level, on the GPU.
 

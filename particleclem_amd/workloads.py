@@ -144,7 +144,7 @@ def premixed(n=128, box=32, seed=DEFAULT_SEED, nsteps=50, phi=0.8,
     Y = (1-s) Y_unburnt(phi) + s Y_burnt(phi); p = 1 atm; velocity
     (0, 0, w0) cm/s plus seeded Gaussian noise (sigma cm/s, all three
     components).  Domain [0,1]^3 cm as (n/box)^3 boxes of box^3; periodic in
-    x and y, outflow in z.  PPM (the BASELINE's stated method)."""
+    x and y, outflow in z.  PPM."""
     dx = [1.0 / n] * 3
     nb = n // box
     boxes = []
