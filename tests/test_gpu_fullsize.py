@@ -1,9 +1,9 @@
 """The GPU path at the workloads' full sizes.
 
 * The Sod tube of 64^3 cells as one box, 10 PPM steps, and the premixed
-  H2/air level of 128^3 cells as 64 boxes of 32^3, its first step: per cell
-  against the oracle (1e-12 relative to each component's scale; bitwise in
-  practice).
+  H2/air level of 128^3 cells as 64 boxes of 32^3, its first step (Godunov
+  and method of lines): per cell against the oracle (1e-12 relative to each
+  component's scale; bitwise in practice).
 * The ragged level of 300 boxes (sides 8-64), each box its own periodic
   domain: the first step against the oracle on one box of each side (the
   boxes are independent domains; the oracle steps them with the level's dt,
@@ -34,6 +34,13 @@ def test_premixed_128cubed_first_step(cuda):
     w = wl.premixed(n=128, box=32, nsteps=1)
     assert len(w.boxes) == 64
     assert_parity(w, 1, label="premixed 128^3 / 64 boxes of 32^3")
+
+
+def test_premixed_128cubed_mol_first_step(cuda):
+    """The method-of-lines step (do_mol_advance, both Heun stages) on the
+    same level."""
+    w = wl.premixed(n=128, box=32, nsteps=1)
+    assert_parity(w, 1, dict(do_mol=1), label="premixed 128^3 MOL")
 
 
 def _box_totals(lev, nboxes):
