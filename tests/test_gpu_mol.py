@@ -49,6 +49,18 @@ def test_ragged_mol_matches_oracle(cuda):
     assert_parity(w, 2, MOL, label="ragged MOL")
 
 
+@pytest.mark.parametrize("over", [MOL, LAXF], ids=["riemann", "laxf"])
+def test_mol_odd_boxes_every_boundary_kind(cuda, over):
+    """Boxes of odd, unequal sides with inflow, no-slip, slip, symmetry and
+    outflow faces (the ghost fill's first-order extrapolation and
+    reflections in both Heun stages)."""
+    w = wl.sod(n=21, ny=13, nz=7, max_grid=11, nsteps=2, xbc="slipwall")
+    w.bclo = [1, 5, 3]
+    w.bchi = [2, 4, 3]
+    assert len(w.boxes) > 2
+    assert_parity(w, 3, over, label=f"sod 21x13x7 MOL, all BC kinds {over}")
+
+
 def _box_inputs(kind):
     w = wl.sod(n=16, nsteps=0) if kind == 1 else wl.premixed(n=16, box=16, nsteps=0)
     H = ob.HostLevel(w.nspec, w.boxes, w.domlo, w.domhi, w.bclo, w.bchi, w.dx)
