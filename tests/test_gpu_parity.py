@@ -46,6 +46,16 @@ def test_option_variants_premixed(cuda, over):
     assert_parity(w, 2, over, label=f"premixed 16^3 {over}")
 
 
+@pytest.mark.parametrize("ppm_type", [1, 0], ids=["ppm", "plm"])
+def test_premixed_every_boundary_kind(cuda, ppm_type):
+    """The 9-species level with an inflow face (the low-order boundary
+    fluxes), no-slip, slip and symmetry walls and an outflow face."""
+    w = wl.premixed(n=16, box=8, nsteps=2)
+    w.bclo = [1, 5, 3]
+    w.bchi = [2, 4, 2]
+    assert_parity(w, 2, dict(ppm_type=ppm_type), label=f"premixed 16^3, all BC kinds, ppm={ppm_type}")
+
+
 def test_odd_boxes_walls_both_ends(cuda):
     """Boxes of odd, unequal sides on a domain with slip and no-slip walls
     (reflecting ghost fill and the wall Riemann faces)."""
