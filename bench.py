@@ -560,12 +560,13 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
         v, s = cpu_baseline(w, fabs0, dt_cpu, args.cpu_steps, threads)
         model, ncpu = host_cpu()
-        cpu = {"value": v, "unit": "Mcell-updates/s", "cores": threads,
+        used = min(threads, len(w.boxes))  # the oracle runs one box per thread
+        cpu = {"value": v, "unit": "Mcell-updates/s", "cores": used,
                "kind": "port",
                "sample": f"{args.cpu_steps} full step(s) of the same level "
                          f"({cells} cells) from the GPU run's state and dt, "
                          f"median of 3 timings ({s:.2f} s), OpenMP over "
-                         f"boxes, {threads} threads",
+                         f"{len(w.boxes)} boxes, {threads} threads ({used} busy)",
                "host_cpu": model, "host_logical_cpus": ncpu}
 
     out = {
