@@ -560,13 +560,22 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
         v, s = cpu_baseline(w, fabs0, dt_cpu, args.cpu_steps, threads)
         model, ncpu = host_cpu()
-        used = min(threads, len(w.boxes))  # the oracle runs one box per thread
+        # the oracle's Godunov step spreads tiles of full x by 8 by 8 cells
+        # over the threads (the reference CPU path's MFIter tiling), its MOL
+        # step whole boxes
+        if args.mol:
+            units, what = len(w.boxes), "boxes"
+        else:
+            units = sum(-(-(hi[1] - lo[1] + 1) // 8) * -(-(hi[2] - lo[2] + 1) // 8)
+                        for lo, hi in w.boxes)
+            what = f"tiles (full x by 8 by 8) of {len(w.boxes)} boxes"
+        used = min(threads, units)
         cpu = {"value": v, "unit": "Mcell-updates/s", "cores": used,
                "kind": "port",
                "sample": f"{args.cpu_steps} full step(s) of the same level "
                          f"({cells} cells) from the GPU run's state and dt, "
                          f"median of 3 timings ({s:.2f} s), OpenMP over "
-                         f"{len(w.boxes)} boxes, {threads} threads ({used} busy)",
+                         f"{units} {what}, {threads} threads ({used} busy)",
                "host_cpu": model, "host_logical_cpus": ncpu}
 
     out = {

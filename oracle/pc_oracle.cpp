@@ -276,23 +276,50 @@ int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
   }
   double dt = *dtp;
   if (prm.do_mol) return mol_level_step_t<EOS>(L, prm, arena, nsteps, dtp);
+  // The reference's CPU path works on tiles (MFIter with TilingIfNotGPU,
+  // Source/Hydro.cpp:60-73; AMReX's default CPU tile is the whole x extent
+  // by 8 by 8): the hydro source of every tile is formed from the old state
+  // (each tile's primitives on grow(tile,4) of its box's FAB), then every
+  // box is updated.  Each cell's source depends only on its stencil, so the
+  // tiling changes no value; it spreads a box over threads.
+  const int nb = (int)L.boxes.size();
+  std::vector<std::pair<int, Box>> tiles;
+  for (int b = 0; b < nb; ++b) {
+    const Box& bx = L.boxes[b];
+    for (int k0 = bx.lo[2]; k0 <= bx.hi[2]; k0 += 8)
+      for (int j0 = bx.lo[1]; j0 <= bx.hi[1]; j0 += 8) {
+        Box t = bx;
+        t.lo[1] = j0;
+        t.hi[1] = std::min(j0 + 7, bx.hi[1]);
+        t.lo[2] = k0;
+        t.hi[2] = std::min(k0 + 7, bx.hi[2]);
+        tiles.push_back({b, t});
+      }
+  }
+  std::vector<Fab> hsrcs;
+  hsrcs.reserve(nb);
+  for (int b = 0; b < nb; ++b) hsrcs.emplace_back(L.boxes[b], H::NVAR);
   for (int s = 0; s < nsteps; ++s) {
     fill_ghosts(L, arena);
     int err = 0;
-    const int nb = (int)L.boxes.size();
+    const int nt = (int)tiles.size();
 #pragma omp parallel for schedule(dynamic) reduction(min : err)
-    for (int b = 0; b < nb; ++b) {
-      const Box bx = L.boxes[b];
+    for (int t = 0; t < nt; ++t) {
+      const int b = tiles[t].first;
+      const Box bx = tiles[t].second;
+      const Box& box = L.boxes[b];
       int sdlo[3], sdhi[3], sblo[3], sbhi[3];
       for (int d = 0; d < 3; ++d) {
-        sdlo[d] = L.self ? bx.lo[d] : dlo[d];
-        sdhi[d] = L.self ? bx.hi[d] : dhi[d];
+        sdlo[d] = L.self ? box.lo[d] : dlo[d];
+        sdhi[d] = L.self ? box.hi[d] : dhi[d];
         sblo[d] = L.self ? BC_INTERIOR : blo[d];
         sbhi[d] = L.self ? BC_INTERIOR : bhi[d];
       }
       const A4 S = L.fab(arena, b);
-      Fab hsrc(bx, H::NVAR);
-      std::fill(hsrc.v.begin(), hsrc.v.end(), 0.0);  // hydro_source.setVal(0)
+      const A4& hs = hsrcs[b].a;
+      ParallelFor(bx, [&](int i, int j, int k) {  // hydro_source.setVal(0)
+        for (int n = 0; n < H::NVAR; ++n) hs(i, j, k, n) = 0.0;
+      });
       Fab f0(surroundingNodes(bx, 0), H::NVAR), f1(surroundingNodes(bx, 1), H::NVAR),
           f2(surroundingNodes(bx, 2), H::NVAR);
       const A4 flx[3] = {f0.a, f1.a, f2.a};
@@ -304,22 +331,28 @@ int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
       std::fill(vol.v.begin(), vol.v.end(), vol_v);
       const A4 a[3] = {a0.a, a1.a, a2.a};
       int rc = hydro_source_box_t<EOS>(bx, sdlo, sdhi, sblo, sbhi, S, nullptr,
-                                       hsrc.a, dx, dt, prm, flx, a, vol.a);
-      if (rc) { err = rc; continue; }
-      // construct_Snew (Source/Advance.cpp:397-416) + computeTemp twice
-      // (Source/Advance.cpp:352,388) + post_timestep computeTemp
-      // (Source/PeleC.cpp:1108-1110).  The second construct_Snew rebuilds
-      // the same S_new from S_old, so the sequence reduces to one update,
-      // reset + e->T from the old T, then reset + e->T from that T.
-      ParallelFor(bx, [&](int i, int j, int k) {
-        for (int n = 0; n < H::NVAR; ++n) S(i, j, k, n) += dt * hsrc.a(i, j, k, n);
+                                       hs, dx, dt, prm, flx, a, vol.a);
+      if (rc) err = rc;
+    }
+    if (err) return err;
+    // construct_Snew (Source/Advance.cpp:397-416) + computeTemp twice
+    // (Source/Advance.cpp:352,388) + post_timestep computeTemp
+    // (Source/PeleC.cpp:1108-1110).  The second construct_Snew rebuilds
+    // the same S_new from S_old, so the sequence reduces to one update,
+    // reset + e->T from the old T, then reset + e->T from that T.
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < nt; ++t) {
+      const int b = tiles[t].first;
+      const A4 S = L.fab(arena, b);
+      const A4& hs = hsrcs[b].a;
+      ParallelFor(tiles[t].second, [&](int i, int j, int k) {
+        for (int n = 0; n < H::NVAR; ++n) S(i, j, k, n) += dt * hs(i, j, k, n);
         H::rst_int_e_default(i, j, k, S);
         H::cmpTemp(i, j, k, S);
         H::rst_int_e_default(i, j, k, S);
         H::cmpTemp(i, j, k, S);
       });
     }
-    if (err) return err;
     // computeNewDt (Source/PeleC.cpp:964-1034), single level, no stop_time
     double dtn = estdt_level<EOS>(L, prm, arena, dx);
     if (prm.fixed_dt <= 0.0) dtn = mn(dtn, prm.change_max * dt);
