@@ -213,13 +213,14 @@ def serve(args):
     from particleclem_amd import serving as sv
     reqs = sv.request_stream(args.requests, args.rate)
     srv = sv.Server(mixed=(args.precision == "mixed"), mol=args.mol,
-                    bypass_depth=args.bypass_depth)
+                    bypass_depth=args.bypass_depth, admission=args.admission)
     warm = sv.request_stream(4, 0, seed=1, n_lo=16, n_hi=16, n_med=16, steps=(3, 3))
     srv.run(warm, realtime=False)        # kernel loading, allocator warm-up
     served = srv.run(reqs, realtime=True)
     summ = sv.summarize(served, srv.wall)
     bad = sum(1 for r in served if r.status != 0)
     budget = srv.budget
+    srv_admission, plan_s = srv.admission, srv.plan_s
     srv.close()
     cells = sum(r.cell_updates for r in served)
     out = {
@@ -236,12 +237,13 @@ def serve(args):
                                f"sides lognormal in [32, 128] (median 64), 10-200 "
                                f"steps, {'MOL' if args.mol else 'PPM'}, CFL 0.3, "
                                f"continuous batching under an 80% HBM budget "
-                               f"({budget / 2**30:.0f} GiB), admission bypass "
-                               f"depth {args.bypass_depth}",
+                               f"({budget / 2**30:.0f} GiB), admission "
+                               f"{srv_admission} (depth {args.bypass_depth})",
                    "requests": len(served), "cell_updates": cells,
                    "failed": bad},
         "latency_ms": summ["latency_ms"], "queue_ms": summ["queue_ms"],
         "wall_s": summ["wall_s"], "ticks": srv.ticks,
+        "admission_plan_s": plan_s,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -386,9 +388,15 @@ def main():
                     help="MOL with the Lax-Friedrichs flux (use_laxf_flux = 1)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick every stage's launch shape on the level first")
-    ap.add_argument("--bypass-depth", type=int, default=0,
-                    help="serving: requests that may be admitted past a "
+    ap.add_argument("--bypass-depth", type=int, default=64,
+                    help="serving: requests examined for admission past a "
                          "head request that does not fit")
+    ap.add_argument("--admission", default="backfill",
+                    choices=["fcfs", "bypass", "backfill"],
+                    help="serving: admission past a head that does not fit; "
+                         "backfill admits only requests that do not delay "
+                         "the head, bypass any that fit (for a bounded wait "
+                         "of the head), fcfs none")
     ap.add_argument("--streams", type=int, default=4,
                     help="dropin: streams the per-box calls rotate over")
     ap.add_argument("--branches", type=int, default=1, choices=[0, 1],

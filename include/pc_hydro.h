@@ -398,6 +398,19 @@ int pc_batch_destroy_v1(pc_batch_t* batch);
 long long pc_batch_slot_bytes_v1(const pc_batch_t* batch, pc_box_t bx);
 int pc_batch_memory_v1(const pc_batch_t* batch, long long* total,
                        long long* used, long long* largest_free);
+/* The arena's placement rule, for callers that plan admissions ahead: a
+ * request takes one block of its bytes rounded up to a multiple of 256, at
+ * the start of the free range of lowest offset that holds it (first fit);
+ * a removed request's block merges with free neighbours.  No reference
+ * counterpart (admission planning of an ensemble server). */
+/* The free ranges in ascending offset order: the first min(cap, *count)
+ * into offsets[] / sizes[] (bytes from the arena's base); *count = how many
+ * there are. */
+int pc_batch_free_ranges_v1(const pc_batch_t* batch, long long* offsets,
+                            long long* sizes, int cap, int* count);
+/* The block of a live request: offset and bytes (rounded as above). */
+int pc_batch_slot_extent_v1(const pc_batch_t* batch, int slot,
+                            long long* offset, long long* bytes);
 /* Admit a request: box bx (its own domain), PC_BC_* codes per direction,
  * cell size dx[3], primitive fields over bx as pc_level_prim_to_cons_v1
  * (device memory, prim_mode 0 or 1).  Builds the conservative state and the

@@ -31,7 +31,8 @@ EXPORTS = [
     "pc_level_autotune_v1", "pc_level_plan_v1", "pc_level_set_plan_v1",
     "pc_level_set_graph_v1", "pc_level_set_branches_v1", "pc_level_set_precision_v1",
     "pc_batch_create_v1", "pc_batch_destroy_v1", "pc_batch_slot_bytes_v1",
-    "pc_batch_memory_v1", "pc_batch_add_v1", "pc_batch_step_v1", "pc_batch_set_branches_v1",
+    "pc_batch_memory_v1", "pc_batch_free_ranges_v1", "pc_batch_slot_extent_v1",
+    "pc_batch_add_v1", "pc_batch_step_v1", "pc_batch_set_branches_v1",
     "pc_batch_read_v1", "pc_batch_status_v1", "pc_batch_remove_v1",
 ]
 
@@ -119,6 +120,8 @@ def lib():
     L.pc_batch_slot_bytes_v1.argtypes = [vp, Box]
     L.pc_batch_slot_bytes_v1.restype = ll
     L.pc_batch_memory_v1.argtypes = [vp, P(ll), P(ll), P(ll)]
+    L.pc_batch_free_ranges_v1.argtypes = [vp, P(ll), P(ll), C.c_int, P(C.c_int)]
+    L.pc_batch_slot_extent_v1.argtypes = [vp, C.c_int, P(ll), P(ll)]
     L.pc_batch_add_v1.argtypes = [vp, Box, i3, i3, d3, Array4, C.c_int, vp, P(C.c_int)]
     L.pc_batch_step_v1.argtypes = [vp, P(C.c_int), C.c_int, vp]
     L.pc_batch_set_branches_v1.argtypes = [vp, C.c_int]

@@ -2425,6 +2425,32 @@ int pc_batch_memory_v1(const pc_batch_t* B, long long* total, long long* used,
   return PC_OK;
 }
 
+int pc_batch_free_ranges_v1(const pc_batch_t* B, long long* offsets,
+                            long long* sizes, int cap, int* count) {
+  if (!B || !count || cap < 0 || (cap > 0 && (!offsets || !sizes)))
+    return PC_ERR_ARG;
+  int n = 0;
+  for (const auto& f : B->arena->fr) {
+    if (n < cap) {
+      offsets[n] = (long long)f.first;
+      sizes[n] = (long long)f.second;
+    }
+    ++n;
+  }
+  *count = n;
+  return PC_OK;
+}
+
+int pc_batch_slot_extent_v1(const pc_batch_t* B, int slot, long long* offset,
+                            long long* bytes) {
+  if (!B || slot < 0 || slot >= B->max_slots || !B->slots[slot].live ||
+      !offset || !bytes)
+    return PC_ERR_ARG;
+  *offset = (long long)B->slots[slot].off;
+  *bytes = (long long)Arena::round(B->slots[slot].bytes);
+  return PC_OK;
+}
+
 int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
                     const int* bchi, const double* dx, pc_array4_t prim,
                     int prim_mode, void* stream, int* slot_out) {

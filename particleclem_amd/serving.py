@@ -9,6 +9,7 @@ Request fields are generated on the device from the request's own seed
 (torch's counter-based CUDA generator), so a request starts from the same
 bits whether it runs alone or in a mix.
 """
+import bisect
 import ctypes as C
 import math
 import time
@@ -56,6 +57,20 @@ class Batch:
         self._chk(self.L.pc_batch_memory_v1(self.h, C.byref(t), C.byref(u), C.byref(f)))
         return t.value, u.value, f.value
 
+    def free_ranges(self):
+        """[(offset, bytes)] of the (possibly shared) arena, ascending."""
+        n = C.c_int()
+        self._chk(self.L.pc_batch_free_ranges_v1(self.h, None, None, 0, C.byref(n)))
+        cap = n.value
+        off, size = (C.c_longlong * max(cap, 1))(), (C.c_longlong * max(cap, 1))()
+        self._chk(self.L.pc_batch_free_ranges_v1(self.h, off, size, cap, C.byref(n)))
+        return [(off[i], size[i]) for i in range(min(cap, n.value))]
+
+    def extent(self, slot):
+        o, b = C.c_longlong(), C.c_longlong()
+        self._chk(self.L.pc_batch_slot_extent_v1(self.h, slot, C.byref(o), C.byref(b)))
+        return o.value, b.value
+
     def add(self, n, bclo, bchi, prim, prim_mode, stream=None):
         slot = C.c_int()
         dx = (C.c_double * 3)(*([1.0 / n] * 3))
@@ -95,6 +110,8 @@ class Request:
         self.rid, self.kind, self.n, self.nsteps = rid, kind, n, nsteps
         self.ratio, self.phi, self.t_arrive, self.seed = ratio, phi, t_arrive, seed
         self.slot = None
+        self.bytes = None          # arena bytes of its slot
+        self.off = None            # arena offset while it runs
         self.blocked_since = None  # tick at which it first waited at the head
         self.done = 0
         self.t_admit = self.t_done = None
@@ -103,6 +120,110 @@ class Request:
     @property
     def cell_updates(self):
         return self.n ** 3 * self.nsteps
+
+
+class ArenaModel:
+    """Host mirror of the batch arena's placement rule: a request takes one
+    block of its bytes rounded up to 256 at the start of the free range of
+    lowest offset that holds it; a freed block merges with free neighbours.
+    Built from the arena's free ranges, it predicts where admissions go and
+    when a request fits as running ones finish."""
+
+    def __init__(self, ranges):
+        self.fr = sorted((int(o), int(s)) for o, s in ranges if s > 0)
+
+    @staticmethod
+    def round(b):
+        return (int(b) + 255) & ~255
+
+    def copy(self):
+        m = ArenaModel(())
+        m.fr = list(self.fr)
+        return m
+
+    def fits(self, b):
+        b = self.round(b)
+        return any(s >= b for _, s in self.fr)
+
+    def alloc(self, b):
+        b = self.round(b)
+        for i, (o, s) in enumerate(self.fr):
+            if s >= b:
+                if s > b:
+                    self.fr[i] = (o + b, s - b)
+                else:
+                    del self.fr[i]
+                return o
+        return -1
+
+    def release(self, off, b):
+        b = self.round(b)
+        i = bisect.bisect_left(self.fr, (off, 0))
+        if i < len(self.fr) and off + b == self.fr[i][0]:
+            b += self.fr[i][1]
+            del self.fr[i]
+        if i > 0 and self.fr[i - 1][0] + self.fr[i - 1][1] == off:
+            self.fr[i - 1] = (self.fr[i - 1][0], self.fr[i - 1][1] + b)
+        else:
+            self.fr.insert(i, (off, b))
+
+
+def ticks_until_fit(model, held, need):
+    """Ticks until a block of `need` bytes fits, releasing the running
+    requests' blocks held = [(ticks left, offset, bytes)] as they finish;
+    None if it does not fit even with all of them released.  (Releasing
+    only ever widens free ranges, so the first fit is the answer.)"""
+    if model.fits(need):
+        return 0
+    m = model.copy()
+    for left, off, b in sorted(held):
+        m.release(off, b)
+        if m.fits(need):
+            return left
+    return None
+
+
+def backfill_plan(waiting, held, model, depth):
+    """Admissions of one tick under EASY backfilling (Lifka, 1995) over the
+    first-fit arena.  waiting: queued requests in arrival order (.bytes,
+    .nsteps); held: [(ticks left, offset, bytes)] of the running requests;
+    model: the arena now.  Requests are admitted from the head while they
+    fit.  A head that does not fit is given a reservation: S, the ticks until
+    it fits as running requests finish.  Of the next `depth` requests, one
+    that fits now is admitted when the head still fits after S ticks with it
+    admitted too (it finishes within S ticks, or its block leaves the head
+    room), so no admission delays the head.  model and held are updated;
+    returns [(index into waiting, predicted offset)]."""
+    out = []
+    i = 0
+    while i < len(waiting) and model.fits(waiting[i].bytes):
+        r = waiting[i]
+        off = model.alloc(r.bytes)
+        held.append((r.nsteps, off, ArenaModel.round(r.bytes)))
+        out.append((i, off))
+        i += 1
+    if i == len(waiting) or depth <= 0:
+        return out
+    need = waiting[i].bytes
+    S = ticks_until_fit(model, held, need)
+    if S is None:
+        return out
+    for j in range(i + 1, min(len(waiting), i + 1 + depth)):
+        c = waiting[j]
+        if not model.fits(c.bytes):
+            continue
+        m2 = model.copy()
+        off = m2.alloc(c.bytes)
+        h = (c.nsteps, off, ArenaModel.round(c.bytes))
+        # one that finishes by tick S leaves the arena at S as it was
+        if c.nsteps > S:
+            s2 = ticks_until_fit(m2, held + [h], need)
+            if s2 is None or s2 > S:
+                continue
+        model.fr = m2.fr
+        held.append(h)
+        out.append((j, off))
+    return out
 
 
 def request_stream(count, rate, seed=20261203, n_lo=32, n_hi=128, n_med=64,
@@ -159,14 +280,16 @@ def request_fields(req):
 
 class Server:
     """The request loop.  `budget_frac` of the device memory is one arena
-    shared by the tube and slab batches; requests are admitted first come
-    first served while their state and workspace fit.  overlap: each batch
+    shared by the tube and slab batches; requests are admitted in arrival
+    order while their state and workspace fit, and past a head that does
+    not fit as `admission` says (default: backfilling that never delays the
+    head).  overlap: each batch
     steps on a stream of its own (the tube and slab batches' kernels
     overlap) and runs the three directions' stages on three streams."""
 
     def __init__(self, mixed=True, budget_frac=0.8, budget_bytes=None,
                  max_slots=512, keep_results=False, mol=False, overlap=True,
-                 bypass_depth=0, bypass_ticks=50):
+                 bypass_depth=64, bypass_ticks=50, admission="backfill"):
         import torch
         self.ctx = {TUBE: HydroContext(1), SLAB: HydroContext(9)}
         if budget_bytes is None:
@@ -181,11 +304,18 @@ class Server:
         self.batch[TUBE] = Batch(self.ctx[TUBE], params, mixed,
                                  share=self.batch[SLAB], max_slots=max_slots)
         self.keep_results = keep_results
-        # admission past a request that does not fit: up to bypass_depth
-        # later requests that fit are admitted, until the head has waited
-        # bypass_ticks ticks (then the queue holds until the head fits)
-        self.bypass_depth = bypass_depth
+        # admission past a request that does not fit.  "backfill": of the
+        # next bypass_depth requests, those that do not delay the head's
+        # reservation (backfill_plan).  "bypass": up to bypass_depth later
+        # requests that fit are admitted, until the head has waited
+        # bypass_ticks ticks (then the queue holds until the head fits).
+        # "fcfs": none.
+        if admission not in ("fcfs", "bypass", "backfill"):
+            raise ValueError(f"unknown admission policy {admission!r}")
+        self.admission = admission
+        self.bypass_depth = bypass_depth if admission != "fcfs" else 0
         self.bypass_ticks = bypass_ticks
+        self.plan_s = 0.0          # host seconds spent planning admissions
         self.streams = {k: None for k in self.batch}
         if overlap:
             for k, b in self.batch.items():
@@ -197,6 +327,33 @@ class Server:
             b.close()
         for c in self.ctx.values():
             c.close()
+
+    def _admit(self, r, t0):
+        b = self.batch[r.kind]
+        P, bclo, bchi, mode = request_fields(r)
+        r.slot = b.add(r.n, bclo, bchi, P, mode)
+        del P
+        r.off, r.bytes = b.extent(r.slot)
+        r.t_admit = time.perf_counter() - t0
+
+    def _admit_backfill(self, waiting, active, t0):
+        for r in waiting:
+            if r.bytes is None:
+                r.bytes = ArenaModel.round(self.batch[r.kind].slot_bytes(r.n))
+        t = time.perf_counter()
+        model = ArenaModel(self.batch[SLAB].free_ranges())
+        held = [(r.nsteps - r.done, r.off, r.bytes) for r in active]
+        plan = backfill_plan(waiting, held, model, self.bypass_depth)
+        self.plan_s += time.perf_counter() - t
+        for j, off in plan:
+            r = waiting[j]
+            self._admit(r, t0)
+            if r.off != off:
+                raise RuntimeError(f"arena placed request {r.rid} at {r.off}, "
+                                   f"the admission plan at {off}")
+            active.append(r)
+        for j, _ in sorted(plan, reverse=True):
+            del waiting[j]
 
     def run(self, requests, realtime=True, on_tick=None):
         """Serve `requests` (sorted by arrival).  realtime: arrivals follow
@@ -213,8 +370,10 @@ class Server:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].t_arrive <= now):
                 waiting.append(pending.popleft())
+            if self.admission == "backfill":
+                self._admit_backfill(waiting, active, t0)
             i = 0
-            while i < len(waiting):
+            while self.admission != "backfill" and i < len(waiting):
                 r = waiting[i]
                 b = self.batch[r.kind]
                 if b.slot_bytes(r.n) > b.memory()[2]:
@@ -226,10 +385,7 @@ class Server:
                         break
                     i += 1
                     continue
-                P, bclo, bchi, mode = request_fields(r)
-                r.slot = b.add(r.n, bclo, bchi, P, mode)
-                del P
-                r.t_admit = time.perf_counter() - t0
+                self._admit(r, t0)
                 del waiting[i]
                 active.append(r)
             if not active:

@@ -36,3 +36,160 @@ def test_summary_percentiles():
     assert s["requests"] == 100
     cells = sum(r.n ** 3 * r.nsteps for r in reqs)
     assert abs(s["mcell_updates_per_s"] - cells / 10.0 / 1e6) < 1e-9
+
+
+class _Units:
+    """Brute-force arena: a map of 256-byte units; a request takes the
+    lowest run of free units long enough for it."""
+
+    def __init__(self, nunits):
+        self.free = np.ones(nunits, dtype=bool)
+
+    def _runs(self):
+        d = np.diff(np.concatenate(([0], self.free.astype(np.int8), [0])))
+        return np.flatnonzero(d == 1), np.flatnonzero(d == -1)
+
+    def alloc(self, nbytes):
+        need = -(-nbytes // 256)
+        s, e = self._runs()
+        ok = np.flatnonzero(e - s >= need)
+        if not len(ok):
+            return -1
+        u = int(s[ok[0]])
+        assert self.free[u:u + need].all()
+        self.free[u:u + need] = False
+        return u * 256
+
+    def release(self, off, nbytes):
+        u, need = off // 256, -(-nbytes // 256)
+        assert not self.free[u:u + need].any()
+        self.free[u:u + need] = True
+
+    def fits(self, nbytes):
+        s, e = self._runs()
+        return bool(np.any(e - s >= -(-nbytes // 256)))
+
+    def ranges(self):
+        s, e = self._runs()
+        return [(int(a) * 256, int(b - a) * 256) for a, b in zip(s, e)]
+
+
+def test_arena_model_places_as_first_fit_over_units():
+    rng = np.random.default_rng(11)
+    ref = _Units(4096)
+    m = sv.ArenaModel([(0, 4096 * 256)])
+    live = []
+    for _ in range(3000):
+        if live and rng.random() < 0.45:
+            off, b = live.pop(int(rng.integers(len(live))))
+            ref.release(off, b)
+            m.release(off, b)
+        else:
+            b = int(rng.integers(1, 300 * 256))
+            assert m.fits(b) == ref.fits(b)
+            o = ref.alloc(b)
+            assert m.alloc(b) == o
+            if o >= 0:
+                live.append((o, b))
+        assert m.fr == ref.ranges()
+
+
+def test_ticks_until_fit_against_stepping_the_clock():
+    rng = np.random.default_rng(12)
+    for _ in range(200):
+        ref = _Units(1024)
+        held = []
+        for _ in range(int(rng.integers(1, 40))):
+            b = int(rng.integers(1, 80 * 256))
+            o = ref.alloc(b)
+            if o >= 0:
+                held.append((int(rng.integers(1, 30)), o, b))
+        m = sv.ArenaModel(ref.ranges())
+        need = int(rng.integers(1, 1100 * 256))
+        got = sv.ticks_until_fit(m, held, need)
+        want = None
+        for t in range(0, 31):
+            for left, o, b in held:
+                if left == t:
+                    ref.release(o, b)
+            if ref.fits(need):
+                want = t
+                break
+        assert got == want
+
+
+class _Req:
+    def __init__(self, rid, nbytes, nsteps, arrive):
+        self.rid, self.bytes, self.nsteps, self.arrive = rid, nbytes, nsteps, arrive
+        self.done = 0
+        self.off = self.admit = None
+
+
+def _simulate(reqs, arena, depth):
+    """Tick loop of the serving queue on the brute-force arena; admissions
+    from backfill_plan (depth 0: first come first served).  Returns the
+    admission ticks and, per head that had to wait, (first tick blocked,
+    reserved tick)."""
+    ref = _Units(arena // 256)
+    pending = sorted(reqs, key=lambda r: r.arrive)
+    waiting, active, promised = [], [], {}
+    tick = 0
+    while pending or waiting or active:
+        while pending and pending[0].arrive <= tick:
+            waiting.append(pending.pop(0))
+        held = [(r.nsteps - r.done, r.off, r.bytes) for r in active]
+        # the head left after the first-come admissions, and its reservation
+        probe, ph = sv.ArenaModel(ref.ranges()), list(held)
+        k = 0
+        while k < len(waiting) and probe.fits(waiting[k].bytes):
+            o = probe.alloc(waiting[k].bytes)
+            ph.append((waiting[k].nsteps, o, waiting[k].bytes))
+            k += 1
+        if k < len(waiting) and waiting[k].rid not in promised:
+            s = sv.ticks_until_fit(probe, ph, waiting[k].bytes)
+            assert s is not None
+            promised[waiting[k].rid] = (tick, tick + s)
+        plan = sv.backfill_plan(waiting, held, sv.ArenaModel(ref.ranges()), depth)
+        for j, off in plan:
+            r = waiting[j]
+            r.off = ref.alloc(r.bytes)
+            assert r.off == off
+            r.admit = tick
+            active.append(r)
+        for j, _ in sorted(plan, reverse=True):
+            del waiting[j]
+        for r in active:
+            r.done += 1
+        for r in [r for r in active if r.done == r.nsteps]:
+            ref.release(r.off, r.bytes)
+        active = [r for r in active if r.done < r.nsteps]
+        tick += 1
+    return {r.rid: r.admit for r in reqs}, promised
+
+
+def _stream(seed, count=300):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(0.5, size=count)).astype(int)
+    sides = np.clip(np.round(64 * np.exp(0.5 * rng.standard_normal(count))), 32, 128)
+    return [_Req(i, int((sides[i] / 16) ** 3 * 256) + int(rng.integers(0, 256)),
+                 int(rng.integers(10, 201)), int(t[i])) for i in range(count)]
+
+
+def test_backfill_never_delays_the_head_and_cuts_waits():
+    arena = 8000 * 256   # about 16 of the largest requests
+    for seed in (1, 2, 3):
+        fc, _ = _simulate(_stream(seed), arena, 0)
+        reqs = _stream(seed)
+        bf, promised = _simulate(reqs, arena, 64)
+        assert all(v is not None for v in bf.values())
+        assert promised
+        for rid, (blocked, reserved) in promised.items():
+            assert blocked <= bf[rid] <= reserved, (seed, rid, blocked, bf[rid], reserved)
+        arrive = {r.rid: r.arrive for r in reqs}
+        wf = np.array([fc[i] - arrive[i] for i in arrive])
+        wb = np.array([bf[i] - arrive[i] for i in arrive])
+        print(f"seed {seed}: wait ticks fcfs p50 {np.median(wf):.0f} "
+              f"p99 {np.percentile(wf, 99):.0f} mean {wf.mean():.1f}; backfill "
+              f"p50 {np.median(wb):.0f} p99 {np.percentile(wb, 99):.0f} "
+              f"mean {wb.mean():.1f}")
+        assert wb.mean() <= wf.mean()
