@@ -168,60 +168,79 @@ class ArenaModel:
             self.fr.insert(i, (off, b))
 
 
-def ticks_until_fit(model, held, need):
+def ticks_until_fit(model, held, need, kind=None, slots=None):
     """Ticks until a block of `need` bytes fits, releasing the running
-    requests' blocks held = [(ticks left, offset, bytes)] as they finish;
-    None if it does not fit even with all of them released.  (Releasing
-    only ever widens free ranges, so the first fit is the answer.)"""
-    if model.fits(need):
+    requests' blocks held = [(ticks left, offset, bytes, kind)] as they
+    finish; with slots ({kind: free request slots}) the request also needs
+    a free slot of its kind.  None if that never happens with all of them
+    released.  (Releasing only ever widens free ranges and frees slots, so
+    the first fit is the answer.)"""
+    free = slots[kind] if slots is not None else 1
+    if free > 0 and model.fits(need):
         return 0
     m = model.copy()
-    for left, off, b in sorted(held):
+    for left, off, b, k in sorted(held):
         m.release(off, b)
-        if m.fits(need):
+        free += k == kind
+        if free > 0 and m.fits(need):
             return left
     return None
 
 
-def backfill_plan(waiting, held, model, depth):
+def backfill_plan(waiting, held, model, depth, slots=None):
     """Admissions of one tick under EASY backfilling (Lifka, 1995) over the
     first-fit arena.  waiting: queued requests in arrival order (.bytes,
-    .nsteps); held: [(ticks left, offset, bytes)] of the running requests;
-    model: the arena now.  Requests are admitted from the head while they
+    .nsteps, .kind); held: [(ticks left, offset, bytes, kind)] of the
+    running requests; model: the arena now; slots: free request slots per
+    kind (None: unlimited).  A request fits when its block fits and a slot
+    of its kind is free.  Requests are admitted from the head while they
     fit.  A head that does not fit is given a reservation: S, the ticks until
     it fits as running requests finish.  Of the next `depth` requests, one
     that fits now is admitted when the head still fits after S ticks with it
     admitted too (it finishes within S ticks, or its block leaves the head
     room), so no admission delays the head.  model and held are updated;
     returns [(index into waiting, predicted offset)]."""
+    free = dict(slots) if slots is not None else None
+    fits = lambda r: (free is None or free[r.kind] > 0) and model.fits(r.bytes)
+
+    def take(r, off):
+        held.append((r.nsteps, off, ArenaModel.round(r.bytes), r.kind))
+        if free is not None:
+            free[r.kind] -= 1
+
     out = []
     i = 0
-    while i < len(waiting) and model.fits(waiting[i].bytes):
+    while i < len(waiting) and fits(waiting[i]):
         r = waiting[i]
         off = model.alloc(r.bytes)
-        held.append((r.nsteps, off, ArenaModel.round(r.bytes)))
+        take(r, off)
         out.append((i, off))
         i += 1
     if i == len(waiting) or depth <= 0:
         return out
-    need = waiting[i].bytes
-    S = ticks_until_fit(model, held, need)
+    head = waiting[i]
+    S = ticks_until_fit(model, held, head.bytes, head.kind, free)
     if S is None:
         return out
     for j in range(i + 1, min(len(waiting), i + 1 + depth)):
         c = waiting[j]
-        if not model.fits(c.bytes):
+        if not fits(c):
             continue
         m2 = model.copy()
         off = m2.alloc(c.bytes)
-        h = (c.nsteps, off, ArenaModel.round(c.bytes))
-        # one that finishes by tick S leaves the arena at S as it was
+        h = (c.nsteps, off, ArenaModel.round(c.bytes), c.kind)
+        # one that finishes by tick S leaves the arena and the slots at S
+        # as they were
         if c.nsteps > S:
-            s2 = ticks_until_fit(m2, held + [h], need)
+            f2 = None
+            if free is not None:
+                f2 = dict(free)
+                f2[c.kind] -= 1
+            s2 = ticks_until_fit(m2, held + [h], head.bytes, head.kind, f2)
             if s2 is None or s2 > S:
                 continue
         model.fr = m2.fr
-        held.append(h)
+        take(c, off)
         out.append((j, off))
     return out
 
@@ -304,6 +323,7 @@ class Server:
         self.batch[TUBE] = Batch(self.ctx[TUBE], params, mixed,
                                  share=self.batch[SLAB], max_slots=max_slots)
         self.keep_results = keep_results
+        self.max_slots = max_slots
         # admission past a request that does not fit.  "backfill": of the
         # next bypass_depth requests, those that do not delay the head's
         # reservation (backfill_plan).  "bypass": up to bypass_depth later
@@ -336,14 +356,21 @@ class Server:
         r.off, r.bytes = b.extent(r.slot)
         r.t_admit = time.perf_counter() - t0
 
+    def _free_slots(self, active):
+        free = {k: self.max_slots for k in self.batch}
+        for r in active:
+            free[r.kind] -= 1
+        return free
+
     def _admit_backfill(self, waiting, active, t0):
         for r in waiting:
             if r.bytes is None:
                 r.bytes = ArenaModel.round(self.batch[r.kind].slot_bytes(r.n))
         t = time.perf_counter()
         model = ArenaModel(self.batch[SLAB].free_ranges())
-        held = [(r.nsteps - r.done, r.off, r.bytes) for r in active]
-        plan = backfill_plan(waiting, held, model, self.bypass_depth)
+        held = [(r.nsteps - r.done, r.off, r.bytes, r.kind) for r in active]
+        plan = backfill_plan(waiting, held, model, self.bypass_depth,
+                             self._free_slots(active))
         self.plan_s += time.perf_counter() - t
         for j, off in plan:
             r = waiting[j]
@@ -373,10 +400,11 @@ class Server:
             if self.admission == "backfill":
                 self._admit_backfill(waiting, active, t0)
             i = 0
+            free = self._free_slots(active)
             while self.admission != "backfill" and i < len(waiting):
                 r = waiting[i]
                 b = self.batch[r.kind]
-                if b.slot_bytes(r.n) > b.memory()[2]:
+                if b.slot_bytes(r.n) > b.memory()[2] or free[r.kind] == 0:
                     if i == 0 and r.blocked_since is None:
                         r.blocked_since = ticks
                     head = waiting[0]
@@ -386,6 +414,7 @@ class Server:
                     i += 1
                     continue
                 self._admit(r, t0)
+                free[r.kind] -= 1
                 del waiting[i]
                 active.append(r)
             if not active:

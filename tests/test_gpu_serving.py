@@ -118,12 +118,15 @@ def test_admission_under_a_tight_budget(cuda):
           f"{len(done)} requests served")
 
 
-def test_backfill_admission_matches_first_come_results(cuda):
-    """Backfilling under a tight budget: every placement the admission plan
-    predicts is where the arena puts the request (the server raises
-    otherwise), the arena's free ranges and the live blocks tile it, and
-    each request's result is bitwise the one it gets under first come first
-    served (its arithmetic does not depend on when it was admitted)."""
+@pytest.mark.parametrize("max_slots", [512, 3])
+def test_backfill_admission_matches_first_come_results(cuda, max_slots):
+    """Backfilling under a tight budget (and, with 3 slots per batch, a
+    request-slot limit that binds before memory does): every placement the
+    admission plan predicts is where the arena puts the request (the server
+    raises otherwise), the arena's free ranges and the live blocks tile it,
+    no batch holds more than max_slots requests, and each request's result
+    is bitwise the one it gets under first come first served (its
+    arithmetic does not depend on when it was admitted)."""
     reqs = small_stream(16, seed=5)
     probe = sv.Server(mixed=True, budget_bytes=64 << 20)
     per = max(probe.batch[r.kind].slot_bytes(r.n) for r in reqs)
@@ -132,13 +135,15 @@ def test_backfill_admission_matches_first_come_results(cuda):
     out, ticks = {}, {}
     for adm in ("fcfs", "backfill"):
         srv = sv.Server(mixed=True, budget_bytes=budget, keep_results=True,
-                        admission=adm, bypass_depth=8)
+                        admission=adm, bypass_depth=8, max_slots=max_slots)
         total = srv.batch[sv.SLAB].memory()[0]
 
         def tick(_, active, srv=srv, total=total):
             free = sum(b for _, b in srv.batch[sv.SLAB].free_ranges())
             held = sum(r.bytes for r in active)
             assert free + held == total
+            for kind in srv.batch:
+                assert sum(r.kind == kind for r in active) <= max_slots
 
         done = srv.run(small_stream(16, seed=5), realtime=False, on_tick=tick)
         ticks[adm] = srv.ticks
@@ -148,6 +153,6 @@ def test_backfill_admission_matches_first_come_results(cuda):
     for rid, r in out["fcfs"].items():
         assert np.array_equal(out["backfill"][rid].result, r.result), rid
     order = lambda a: [r.rid for r in sorted(out[a].values(), key=lambda r: r.t_admit)]
-    print(f"budget {budget >> 20} MiB: ticks fcfs {ticks['fcfs']}, backfill "
+    print(f"budget {budget >> 20} MiB, {max_slots} slots: ticks fcfs {ticks['fcfs']}, backfill "
           f"{ticks['backfill']}; admission order fcfs {order('fcfs')}, backfill "
           f"{order('backfill')}")

@@ -96,40 +96,44 @@ def test_arena_model_places_as_first_fit_over_units():
 
 def test_ticks_until_fit_against_stepping_the_clock():
     rng = np.random.default_rng(12)
-    for _ in range(200):
+    for trial in range(400):
         ref = _Units(1024)
         held = []
         for _ in range(int(rng.integers(1, 40))):
             b = int(rng.integers(1, 80 * 256))
             o = ref.alloc(b)
             if o >= 0:
-                held.append((int(rng.integers(1, 30)), o, b))
+                held.append((int(rng.integers(1, 30)), o, b, int(rng.integers(2))))
         m = sv.ArenaModel(ref.ranges())
         need = int(rng.integers(1, 1100 * 256))
-        got = sv.ticks_until_fit(m, held, need)
-        want = None
+        kind = int(rng.integers(2))
+        slots = None if trial % 2 else {0: int(rng.integers(-3, 2)), 1: int(rng.integers(-3, 2))}
+        got = sv.ticks_until_fit(m, held, need, kind, slots)
+        want, free = None, (slots[kind] if slots else 1)
         for t in range(0, 31):
-            for left, o, b in held:
+            for left, o, b, k in held:
                 if left == t:
                     ref.release(o, b)
-            if ref.fits(need):
+                    free += k == kind
+            if free > 0 and ref.fits(need):
                 want = t
                 break
         assert got == want
 
 
 class _Req:
-    def __init__(self, rid, nbytes, nsteps, arrive):
+    def __init__(self, rid, nbytes, nsteps, arrive, kind):
         self.rid, self.bytes, self.nsteps, self.arrive = rid, nbytes, nsteps, arrive
+        self.kind = kind
         self.done = 0
         self.off = self.admit = None
 
 
-def _simulate(reqs, arena, depth):
-    """Tick loop of the serving queue on the brute-force arena; admissions
-    from backfill_plan (depth 0: first come first served).  Returns the
-    admission ticks and, per head that had to wait, (first tick blocked,
-    reserved tick)."""
+def _simulate(reqs, arena, depth, max_slots):
+    """Tick loop of the serving queue on the brute-force arena with
+    max_slots request slots per kind; admissions from backfill_plan (depth
+    0: first come first served).  Returns the admission ticks and, per head
+    that had to wait, (first tick blocked, reserved tick)."""
     ref = _Units(arena // 256)
     pending = sorted(reqs, key=lambda r: r.arrive)
     waiting, active, promised = [], [], {}
@@ -137,19 +141,23 @@ def _simulate(reqs, arena, depth):
     while pending or waiting or active:
         while pending and pending[0].arrive <= tick:
             waiting.append(pending.pop(0))
-        held = [(r.nsteps - r.done, r.off, r.bytes) for r in active]
+        held = [(r.nsteps - r.done, r.off, r.bytes, r.kind) for r in active]
+        slots = {q: max_slots - sum(r.kind == q for r in active) for q in (0, 1)}
+        assert min(slots.values()) >= 0
         # the head left after the first-come admissions, and its reservation
-        probe, ph = sv.ArenaModel(ref.ranges()), list(held)
+        probe, ph, pf = sv.ArenaModel(ref.ranges()), list(held), dict(slots)
         k = 0
-        while k < len(waiting) and probe.fits(waiting[k].bytes):
+        while (k < len(waiting) and pf[waiting[k].kind] > 0
+               and probe.fits(waiting[k].bytes)):
             o = probe.alloc(waiting[k].bytes)
-            ph.append((waiting[k].nsteps, o, waiting[k].bytes))
+            ph.append((waiting[k].nsteps, o, waiting[k].bytes, waiting[k].kind))
+            pf[waiting[k].kind] -= 1
             k += 1
         if k < len(waiting) and waiting[k].rid not in promised:
-            s = sv.ticks_until_fit(probe, ph, waiting[k].bytes)
+            s = sv.ticks_until_fit(probe, ph, waiting[k].bytes, waiting[k].kind, pf)
             assert s is not None
             promised[waiting[k].rid] = (tick, tick + s)
-        plan = sv.backfill_plan(waiting, held, sv.ArenaModel(ref.ranges()), depth)
+        plan = sv.backfill_plan(waiting, held, sv.ArenaModel(ref.ranges()), depth, slots)
         for j, off in plan:
             r = waiting[j]
             r.off = ref.alloc(r.bytes)
@@ -172,15 +180,17 @@ def _stream(seed, count=300):
     t = np.cumsum(rng.exponential(0.5, size=count)).astype(int)
     sides = np.clip(np.round(64 * np.exp(0.5 * rng.standard_normal(count))), 32, 128)
     return [_Req(i, int((sides[i] / 16) ** 3 * 256) + int(rng.integers(0, 256)),
-                 int(rng.integers(10, 201)), int(t[i])) for i in range(count)]
+                 int(rng.integers(10, 201)), int(t[i]), int(rng.integers(2)))
+            for i in range(count)]
 
 
 def test_backfill_never_delays_the_head_and_cuts_waits():
     arena = 8000 * 256   # about 16 of the largest requests
-    for seed in (1, 2, 3):
-        fc, _ = _simulate(_stream(seed), arena, 0)
+    # memory binds alone (1000 slots), slots bind alone (9), both (24, 28)
+    for seed, max_slots in ((1, 1000), (2, 1000), (5, 9), (3, 24), (6, 28)):
+        fc, _ = _simulate(_stream(seed), arena, 0, max_slots)
         reqs = _stream(seed)
-        bf, promised = _simulate(reqs, arena, 64)
+        bf, promised = _simulate(reqs, arena, 64, max_slots)
         assert all(v is not None for v in bf.values())
         assert promised
         for rid, (blocked, reserved) in promised.items():
@@ -188,7 +198,7 @@ def test_backfill_never_delays_the_head_and_cuts_waits():
         arrive = {r.rid: r.arrive for r in reqs}
         wf = np.array([fc[i] - arrive[i] for i in arrive])
         wb = np.array([bf[i] - arrive[i] for i in arrive])
-        print(f"seed {seed}: wait ticks fcfs p50 {np.median(wf):.0f} "
+        print(f"seed {seed}, {max_slots} slots per kind: wait ticks fcfs p50 {np.median(wf):.0f} "
               f"p99 {np.percentile(wf, 99):.0f} mean {wf.mean():.1f}; backfill "
               f"p50 {np.median(wb):.0f} p99 {np.percentile(wb, 99):.0f} "
               f"mean {wb.mean():.1f}")
