@@ -131,7 +131,7 @@ def profiled_step_flops(summary, kind="FP64"):
 
 
 def oracle_norms(w, gpu_fabs, host_fabs):
-    """L1/Linf relative errors in rho and T of GPU box states against the
+    """L1/Linf relative errors in rho and T of device box states against the
     oracle's (valid cells)."""
     import numpy as np
     acc = {q: [0.0, 0.0, 0.0, 0.0] for q in ("rho", "T")}
