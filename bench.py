@@ -397,8 +397,9 @@ def main():
                          "backfill admits only requests that do not delay "
                          "the head, bypass any that fit (for a bounded wait "
                          "of the head), fcfs none")
-    ap.add_argument("--streams", type=int, default=4,
-                    help="dropin: streams the per-box calls rotate over")
+    ap.add_argument("--streams", type=int, default=16,
+                    help="dropin: streams the per-box calls rotate over "
+                         "(AMReX: amrex.max_gpu_streams, default 4)")
     ap.add_argument("--branches", type=int, default=1, choices=[0, 1],
                     help="1: the three directions' stages on three streams in "
                          "the timed steps (kernel timing steps run on one)")
