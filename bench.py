@@ -256,9 +256,10 @@ def dropin(args):
     ghost cells already filled, caller-owned FABs for the state, hydro
     source, fluxes, areas and volume), over the 64 boxes of 32^3 of the
     premixed 128^3 level per evaluation, the boxes spread over --streams
-    streams in rotation as AMReX's MFIter does on GPUs (4 by default).
+    streams in rotation as AMReX's MFIter does on GPUs (16 by default).
     value = valid cells x evaluations / wall time; no ghost fill, update or
-    time step (the caller's part)."""
+    time step (the caller's part).  host_issue_ms_per_step: host time spent
+    in the calls (they return once their launches are queued)."""
     import ctypes as C
     import torch
     from particleclem_amd import _lib, workloads
@@ -314,8 +315,11 @@ def dropin(args):
         evaluation()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    issue = 0.0
     for _ in range(args.steps):
+        ti = time.perf_counter()
         evaluation()
+        issue += time.perf_counter() - ti
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     cells = w.valid_cells()
@@ -331,6 +335,7 @@ def dropin(args):
                                f"hydro source and fluxes only (no ghost fill, "
                                f"update or dt), boxes over {len(streams)} streams",
                    "valid_cells": cells, "boxes": len(w.boxes), "calls_per_step": len(calls)},
+        "host_issue_ms_per_step": issue / args.steps * 1e3,
         "roofline": None, "cpu_baseline": None,
     }
     ctx.close()
