@@ -33,6 +33,17 @@ def test_premixed_9species(cuda):
     assert_parity(w, 2, label="premixed 32^3 / 16^3 boxes")
 
 
+@pytest.mark.parametrize("over,nsteps", [({}, 10), (dict(do_mol=1), 5)],
+                         ids=["ctu-10-steps", "mol-5-steps"])
+def test_premixed_32cubed_boxes_many_steps(cuda, over, nsteps):
+    """The premixed level in 32^3 boxes (8 of them, 64^3 cells) over several
+    steps: every cell of the state and every dt of the chain (each step's
+    dt comes from the previous step's state on the device)."""
+    w = wl.premixed(n=64, box=32, nsteps=nsteps)
+    assert len(w.boxes) == 8
+    assert_parity(w, nsteps, over, label=f"premixed 64^3 / 32^3 boxes {over}, {nsteps} steps")
+
+
 @pytest.mark.parametrize("over", [dict(ppm_type=0, plm_iorder=1),
                                   dict(ppm_type=0, plm_iorder=2),
                                   dict(ppm_type=1, use_flattening=0),
