@@ -105,6 +105,11 @@ typedef struct pc_params {
   int do_mol;          /*                                        (0)         */
   int mol_iorder;      /* 1 or 2                                 (2)         */
   int use_laxf_flux;   /* 1: Lax-Friedrichs flux in MOL          (0)         */
+  /* computeNewDt / computeInitialDt's stop_time limit (Source/PeleC.cpp:
+   * 1018-1027, 1061-1068; amr.stop_time): a step that would end past
+   * stop_time - 0.001 dt is shortened to end on stop_time.  < 0: no limit.
+   * Steps issued once the level time has reached stop_time take dt = 0. */
+  double stop_time;    /*                                        (-1)        */
 } pc_params_t;
 
 void pc_params_default(pc_params_t* p);
@@ -192,6 +197,14 @@ int pc_hydro_source_box_v1(pc_ctx_t* ctx, pc_box_t bx, const int* domlo,
  * memory so consecutive steps need no host synchronisation. */
 typedef struct pc_level pc_level_t;
 
+/* Ghost cells beyond Inflow and UserBC domain faces: the reference fills
+ * them by ext_dir, i.e. the problem's bcnormal (Source/Setup.cpp:44-60),
+ * which is problem code outside this path.  The level and batch paths fill
+ * them by first-order extrapolation instead (the Outflow rule); the face
+ * fluxes there still come from pc_low_order_boundary's first-order states
+ * (Source/Godunov.cpp:7-55), as in the reference.  A caller that needs its
+ * own inflow state drives the box entry points (pc_hydro_source_box_v1,
+ * pc_umdrv_v1) with ghost cells it has filled. */
 typedef struct pc_level_desc {
   int nboxes;
   const pc_box_t* boxes; /* host array of nboxes valid boxes            */
@@ -218,7 +231,14 @@ int pc_level_read_box_v1(pc_level_t* lev, int b, double* buf);
 /* Set / get the device-resident dt (synchronous). */
 int pc_level_set_dt_v1(pc_level_t* lev, double dt);
 int pc_level_get_dt_v1(pc_level_t* lev, double* dt);
-/* dt = init_shrink * estTimeStep(state) (or fixed_dt), on the device. */
+/* Set / get the level time (state[State_Type].curTime(), 0 at creation),
+ * which every step advances by its dt on the device and against which the
+ * stop_time limit applies (synchronous). */
+int pc_level_set_time_v1(pc_level_t* lev, double time);
+int pc_level_get_time_v1(pc_level_t* lev, double* time);
+/* dt = init_shrink * estTimeStep(state) (or fixed_dt), limited by stop_time
+ * at the level time (computeInitialDt, Source/PeleC.cpp:1041-1075), on the
+ * device. */
 int pc_level_init_dt_v1(pc_level_t* lev, void* stream);
 /* Advance nsteps steps (asynchronous on `stream`): Godunov steps, or with
  * prm->do_mol = 1 at creation the two-stage MOL steps of do_mol_advance

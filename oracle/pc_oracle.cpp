@@ -181,6 +181,27 @@ double estdt_level(const HostLevel& L, const pc_params_t& prm,
   return estdt;
 }
 
+// The stop_time limit of computeNewDt / computeInitialDt (Source/PeleC.cpp:
+// 1018-1027, 1061-1068): a step that would end within 0.001 dt of stop_time
+// (or past it) is shortened to land on it.  stop_time < 0: no limit.
+inline double clamp_stop_time(double dt0, double cur_time, double stop_time) {
+  const double dt_eps = 0.001 * dt0;
+  if (stop_time >= 0.0) {
+    if ((cur_time + dt0) > (stop_time - dt_eps)) dt0 = stop_time - cur_time;
+  }
+  return dt0;
+}
+
+// computeNewDt (Source/PeleC.cpp:964-1034) for a single level after a step
+// of dt that ended at cur_time.
+template <class EOS>
+double new_dt(const HostLevel& L, const pc_params_t& prm, const double* arena,
+              const double* dx, double dt, double cur_time) {
+  double dtn = estdt_level<EOS>(L, prm, arena, dx);
+  if (prm.fixed_dt <= 0.0) dtn = mn(dtn, prm.change_max * dt);
+  return clamp_stop_time(dtn, cur_time, prm.stop_time);
+}
+
 // Uniform-geometry face areas and cell volumes of a box (Cartesian), the
 // areas over the faces of grow(bx,3) that the MOL flux loop visits (the
 // reference's area MultiFab carries ghost cells).
@@ -204,7 +225,7 @@ struct BoxGeom {
 // post_timestep's computeTemp (Source/PeleC.cpp:1108-1110) and computeNewDt.
 template <class EOS>
 int mol_level_step_t(HostLevel& L, const pc_params_t& prm, double* arena,
-                     int nsteps, double* dtp) {
+                     int nsteps, double* dtp, double* timep) {
   using H = Hydro<EOS>;
   const int nb = (int)L.boxes.size();
   const int iorder = prm.plm_iorder;  // Source/Diffusion.cpp:371-373
@@ -248,9 +269,8 @@ int mol_level_step_t(HostLevel& L, const pc_params_t& prm, double* arena,
         }
       }
     }
-    double dtn = estdt_level<EOS>(L, prm, arena, L.dx);
-    if (prm.fixed_dt <= 0.0) dtn = mn(dtn, prm.change_max * dt);
-    dt = dtn;
+    *timep += dt;
+    dt = new_dt<EOS>(L, prm, arena, L.dx, dt, *timep);
   }
   *dtp = dt;
   return 0;
@@ -258,7 +278,8 @@ int mol_level_step_t(HostLevel& L, const pc_params_t& prm, double* arena,
 
 template <class EOS>
 int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
-                 double* arena, int nsteps, double* dtp, int nthreads) {
+                 double* arena, int nsteps, double* dtp, double* timep,
+                 int nthreads) {
   using H = Hydro<EOS>;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -275,7 +296,7 @@ int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
     blo[d] = desc->bclo[d]; bhi[d] = desc->bchi[d];
   }
   double dt = *dtp;
-  if (prm.do_mol) return mol_level_step_t<EOS>(L, prm, arena, nsteps, dtp);
+  if (prm.do_mol) return mol_level_step_t<EOS>(L, prm, arena, nsteps, dtp, timep);
   // The reference's CPU path works on tiles (MFIter with TilingIfNotGPU,
   // Source/Hydro.cpp:60-73; AMReX's default CPU tile is the whole x extent
   // by 8 by 8): the hydro source of every tile is formed from the old state
@@ -353,10 +374,10 @@ int level_step_t(const pc_level_desc_t* desc, const pc_params_t& prm,
         H::cmpTemp(i, j, k, S);
       });
     }
-    // computeNewDt (Source/PeleC.cpp:964-1034), single level, no stop_time
-    double dtn = estdt_level<EOS>(L, prm, arena, dx);
-    if (prm.fixed_dt <= 0.0) dtn = mn(dtn, prm.change_max * dt);
-    dt = dtn;
+    // computeNewDt (Source/PeleC.cpp:964-1034), single level, at the time
+    // the step ended
+    *timep += dt;
+    dt = new_dt<EOS>(L, prm, arena, dx, dt, *timep);
   }
   *dtp = dt;
   return 0;
@@ -437,6 +458,86 @@ int riemann_point(const double* L5, const double* spl, const double* R5,
   Hydro<EOS>::riemann(L5[0], L5[1], L5[2], L5[3], L5[4], spl, R5[0], R5[1],
                       R5[2], R5[3], R5[4], spr, 1, cav, o[0], o[1], fy, o[2],
                       o[3], o[4], o[5], o[6], o[7], o[8], o[9], o[10], o[11]);
+  return 0;
+}
+
+// pc_transdo (Source/Godunov.H:455-633) on the cell (0,0,0) of a 2^3 patch:
+// qplus = qnormp(cell), qminus = qnormm(cell + e_dir); the transverse-t
+// fluxes and Godunov states on the cell's low (fl_lo, qi_lo) and high t-faces.
+template <class EOS>
+int transdo_point(int dir, int t, const double* qplus, const double* qminus,
+                  const double* fl_lo, const double* fl_hi, const double* qi_lo,
+                  const double* qi_hi, double gamc, double cdtdx,
+                  double* out_p, double* out_m) {
+  using H = Hydro<EOS>;
+  const Box b2{{0, 0, 0}, {1, 1, 1}};
+  Fab qm(b2, H::QVAR), qp(b2, H::QVAR), qnm(b2, H::QVAR), qnp(b2, H::QVAR),
+      fl(b2, H::NVAR), qa(b2, NQAUX), qi(b2, NGDNV);
+  const IV c = ivec(0, 0, 0), cn = shift(c, dir, 1), ct = shift(c, t, 1);
+  for (int n = 0; n < H::QVAR; ++n) {
+    at(qnp.a, c, n) = qplus[n];
+    at(qnm.a, cn, n) = qminus[n];
+  }
+  for (int n = 0; n < H::NVAR; ++n) {
+    at(fl.a, c, n) = fl_lo[n];
+    at(fl.a, ct, n) = fl_hi[n];
+  }
+  for (int n = 0; n < NGDNV; ++n) {
+    at(qi.a, c, n) = qi_lo[n];
+    at(qi.a, ct, n) = qi_hi[n];
+  }
+  at(qa.a, c, QGAMC) = gamc;
+  H::transdo(0, 0, 0, dir, t, qm.a, qp.a, qnm.a, qnp.a, fl.a, qa.a, qi.a, cdtdx);
+  for (int n = 0; n < H::QVAR; ++n) {
+    out_p[n] = at(qp.a, c, n);
+    out_m[n] = at(qm.a, cn, n);
+  }
+  return 0;
+}
+
+// pc_transdd (Source/Godunov.H:639-845) on the cell (0,0,0): fl0/qi0 on the
+// low and high faces of the first transverse direction (y for dir x, x
+// otherwise), fl1/qi1 of the second (z for dir x or y, y for dir z).
+template <class EOS>
+int transdd_point(int dir, const double* qplus, const double* qminus,
+                  const double* fl0_lo, const double* fl0_hi,
+                  const double* fl1_lo, const double* fl1_hi,
+                  const double* qi0_lo, const double* qi0_hi,
+                  const double* qi1_lo, const double* qi1_hi, double gamc,
+                  const double* srcq, double hdt, double c0, double c1,
+                  double* out_p, double* out_m) {
+  using H = Hydro<EOS>;
+  const Box b2{{0, 0, 0}, {1, 1, 1}};
+  Fab qm(b2, H::QVAR), qp(b2, H::QVAR), qnm(b2, H::QVAR), qnp(b2, H::QVAR),
+      f0(b2, H::NVAR), f1(b2, H::NVAR), qa(b2, NQAUX), q0(b2, NGDNV),
+      q1(b2, NGDNV), sq(b2, H::QVAR);
+  const int l1 = (dir == 0) ? 1 : 0, l2 = (dir == 2) ? 1 : 2;
+  const IV c = ivec(0, 0, 0), cn = shift(c, dir, 1), c0p = shift(c, l1, 1),
+           c1p = shift(c, l2, 1);
+  for (int n = 0; n < H::QVAR; ++n) {
+    at(qnp.a, c, n) = qplus[n];
+    at(qnm.a, cn, n) = qminus[n];
+    at(sq.a, c, n) = srcq ? srcq[n] : 0.0;
+  }
+  for (int n = 0; n < H::NVAR; ++n) {
+    at(f0.a, c, n) = fl0_lo[n];
+    at(f0.a, c0p, n) = fl0_hi[n];
+    at(f1.a, c, n) = fl1_lo[n];
+    at(f1.a, c1p, n) = fl1_hi[n];
+  }
+  for (int n = 0; n < NGDNV; ++n) {
+    at(q0.a, c, n) = qi0_lo[n];
+    at(q0.a, c0p, n) = qi0_hi[n];
+    at(q1.a, c, n) = qi1_lo[n];
+    at(q1.a, c1p, n) = qi1_hi[n];
+  }
+  at(qa.a, c, QGAMC) = gamc;
+  H::transdd(0, 0, 0, dir, qm.a, qp.a, qnm.a, qnp.a, f0.a, f1.a, q0.a, q1.a,
+             qa.a, sq.a, hdt, c0, c1);
+  for (int n = 0; n < H::QVAR; ++n) {
+    out_p[n] = at(qp.a, c, n);
+    out_m[n] = at(qm.a, cn, n);
+  }
   return 0;
 }
 
@@ -565,6 +666,7 @@ void pc_ref_params_default(pc_params_t* p) {
   p->max_dt = 1.e200;
   p->fixed_dt = -1.0;
   p->dt_cutoff = 0.0;
+  p->stop_time = -1.0;
   p->allow_small_energy = 1;
   p->allow_negative_energy = 1;
   p->dual_energy_update_E_from_e = 1;
@@ -661,9 +763,23 @@ int pc_ref_level_estdt(int eos, const pc_level_desc_t* desc,
   return 0;
 }
 
+int pc_ref_level_init_dt(int eos, const pc_level_desc_t* desc,
+                         const pc_params_t* prm, const double* arena,
+                         double time, double* dt_out) {
+  // computeInitialDt (Source/PeleC.cpp:1041-1075) -> initialTimeStep
+  // (:811-826) for one level: init_shrink * estTimeStep, stop_time limit
+  HostLevel L = make_level(7 + eos, desc);
+  double est = 0.0;
+  DISPATCH(eos, est = estdt_level<E>(L, *prm, arena, desc->dx));
+  *dt_out = clamp_stop_time(prm->init_shrink * est, time, prm->stop_time);
+  return 0;
+}
+
 int pc_ref_level_step(int eos, const pc_level_desc_t* desc,
                       const pc_params_t* prm, double* arena, int nsteps,
-                      double* dt, int nthreads) {
+                      double* dt, double* time, int nthreads) {
+  double t_local = 0.0;
+  if (!time) time = &t_local;
   if (prm->use_hybrid_weno || !prm->allow_small_energy ||
       !prm->allow_negative_energy)
     return PC_ERR_UNSUPPORTED;
@@ -671,7 +787,7 @@ int pc_ref_level_step(int eos, const pc_level_desc_t* desc,
   if (prm->do_mol && prm->mol_iorder != 1 && prm->mol_iorder != 2)
     return PC_ERR_MOL_IORDER;
   int rc = 0;
-  DISPATCH(eos, rc = level_step_t<E>(desc, *prm, arena, nsteps, dt, nthreads));
+  DISPATCH(eos, rc = level_step_t<E>(desc, *prm, arena, nsteps, dt, time, nthreads));
   return rc;
 }
 
@@ -741,6 +857,59 @@ int pc_ref_derive(int eos, pc_box_t bx, pc_array4_t u, const int* codes,
   int rc = 0;
   DISPATCH(eos, rc = derive_t<E>(b, to_a4(u), codes, comps, nplot, to_a4(out), dx));
   return rc;
+}
+
+int pc_ref_transdo_point(int eos, int dir, int t, const double* qplus,
+                         const double* qminus, const double* fl_lo,
+                         const double* fl_hi, const double* qi_lo,
+                         const double* qi_hi, double gamc, double cdtdx,
+                         double* out_p, double* out_m) {
+  if (dir < 0 || dir > 2 || t < 0 || t > 2 || t == dir) return PC_ERR_ARG;
+  int rc = 0;
+  DISPATCH(eos, rc = transdo_point<E>(dir, t, qplus, qminus, fl_lo, fl_hi, qi_lo,
+                                      qi_hi, gamc, cdtdx, out_p, out_m));
+  return rc;
+}
+
+int pc_ref_transdd_point(int eos, int dir, const double* qplus,
+                         const double* qminus, const double* fl0_lo,
+                         const double* fl0_hi, const double* fl1_lo,
+                         const double* fl1_hi, const double* qi0_lo,
+                         const double* qi0_hi, const double* qi1_lo,
+                         const double* qi1_hi, double gamc, const double* srcq,
+                         double hdt, double c0, double c1, double* out_p,
+                         double* out_m) {
+  if (dir < 0 || dir > 2) return PC_ERR_ARG;
+  int rc = 0;
+  DISPATCH(eos, rc = transdd_point<E>(dir, qplus, qminus, fl0_lo, fl0_hi, fl1_lo,
+                                      fl1_hi, qi0_lo, qi0_hi, qi1_lo, qi1_hi,
+                                      gamc, srcq, hdt, c0, c1, out_p, out_m));
+  return rc;
+}
+
+double pc_ref_pdivu_point(const double* pg, const double* un,
+                          const double* area, double vol) {
+  // pc_pdivu (Source/Godunov.H:1033-1060) of the cell (0,0,0); per direction
+  // d the low face holds pg[2d], un[2d], area[2d] and the high face the
+  // entries 2d+1.
+  const Box b2{{0, 0, 0}, {1, 1, 1}};
+  Fab qe[3] = {Fab(b2, NGDNV), Fab(b2, NGDNV), Fab(b2, NGDNV)};
+  Fab ar[3] = {Fab(b2, 1), Fab(b2, 1), Fab(b2, 1)};
+  Fab v(b2, 1), out(b2, 1);
+  const int gdv[3] = {GDU, GDV, GDW};
+  for (int d = 0; d < 3; ++d) {
+    const IV c = ivec(0, 0, 0), ch = shift(c, d, 1);
+    at(qe[d].a, c, GDPRES) = pg[2 * d];
+    at(qe[d].a, ch, GDPRES) = pg[2 * d + 1];
+    at(qe[d].a, c, gdv[d]) = un[2 * d];
+    at(qe[d].a, ch, gdv[d]) = un[2 * d + 1];
+    at(ar[d].a, c, 0) = area[2 * d];
+    at(ar[d].a, ch, 0) = area[2 * d + 1];
+  }
+  v.a(0, 0, 0) = vol;
+  Hydro<GammaLaw>::pdivu_cell(0, 0, 0, out.a, qe[0].a, qe[1].a, qe[2].a,
+                              ar[0].a, ar[1].a, ar[2].a, v.a);
+  return out.a(0, 0, 0);
 }
 
 int pc_ref_laxf(int eos, const double* left5, const double* spl,

@@ -44,11 +44,17 @@ int pc_ref_level_estdt(int eos, const pc_level_desc_t* desc,
                        const pc_params_t* prm, const double* arena,
                        double* dt_out);
 /* nsteps Godunov steps; *dt is the dt of the first step on entry and the dt
- * of the next step on return (computeNewDt).  nthreads > 0 sets the OpenMP
- * thread count (boxes are the parallel unit). */
+ * of the next step on return (computeNewDt, with the stop_time limit at the
+ * time each step ended).  *time (may be NULL: 0) is the level time, advanced
+ * by each step.  nthreads > 0 sets the OpenMP thread count. */
 int pc_ref_level_step(int eos, const pc_level_desc_t* desc,
                       const pc_params_t* prm, double* arena, int nsteps,
-                      double* dt, int nthreads);
+                      double* dt, double* time, int nthreads);
+/* computeInitialDt for one level at `time`: init_shrink * estTimeStep (or
+ * fixed_dt), limited by stop_time. */
+int pc_ref_level_init_dt(int eos, const pc_level_desc_t* desc,
+                         const pc_params_t* prm, const double* arena,
+                         double time, double* dt_out);
 
 /* Conservative state from primitive fields (pc_initdata pattern).
  * mode 0: prim = (rho, p, u, v, w, Y[NS]); mode 1: prim = (p, T, u, v, w, Y). */
@@ -69,6 +75,23 @@ int pc_ref_eos(int eos, int which, double a, double b, double c,
 int pc_ref_riemann(int eos, const double* left5, const double* spl,
                    const double* right5, const double* spr, double cav,
                    double* out12);
+/* pc_transdo / pc_transdd / pc_pdivu on one cell, for known-answer tests
+ * (argument meaning in pc_oracle.cpp: transdo_point, transdd_point). */
+int pc_ref_transdo_point(int eos, int dir, int t, const double* qplus,
+                         const double* qminus, const double* fl_lo,
+                         const double* fl_hi, const double* qi_lo,
+                         const double* qi_hi, double gamc, double cdtdx,
+                         double* out_p, double* out_m);
+int pc_ref_transdd_point(int eos, int dir, const double* qplus,
+                         const double* qminus, const double* fl0_lo,
+                         const double* fl0_hi, const double* fl1_lo,
+                         const double* fl1_hi, const double* qi0_lo,
+                         const double* qi0_hi, const double* qi1_lo,
+                         const double* qi1_hi, double gamc, const double* srcq,
+                         double hdt, double c0, double c1, double* out_p,
+                         double* out_m);
+double pc_ref_pdivu_point(const double* pg, const double* un,
+                          const double* area, double vol);
 int pc_ref_ppm_reconstruct(const double* s5, double flatn, double* smsp);
 int pc_ref_ppm_int_profile(double sm, double sp, double sc, double u, double c,
                            double dtdx, double* Ip3, double* Im3);

@@ -20,7 +20,8 @@ EXPORTS = [
     "pc_level_destroy_v1",
     "pc_level_box_offset_v1", "pc_level_box_size_v1", "pc_level_state_ptr_v1",
     "pc_level_state_size_v1", "pc_level_write_box_v1", "pc_level_read_box_v1",
-    "pc_level_set_dt_v1", "pc_level_get_dt_v1", "pc_level_init_dt_v1",
+    "pc_level_set_dt_v1", "pc_level_get_dt_v1", "pc_level_set_time_v1",
+    "pc_level_get_time_v1", "pc_level_init_dt_v1",
     "pc_level_step_v1", "pc_level_fill_ghosts_v1", "pc_level_estdt_v1",
     "pc_level_write_plotfile_v1", "pc_write_plotfile_v1", "pc_fluxreg_create_v1",
     "pc_fluxreg_destroy_v1", "pc_fluxreg_nfaces_v1", "pc_fluxreg_reset_v1",
@@ -143,6 +144,8 @@ def lib():
     L.pc_level_read_box_v1.argtypes = [vp, C.c_int, vp]
     L.pc_level_set_dt_v1.argtypes = [vp, C.c_double]
     L.pc_level_get_dt_v1.argtypes = [vp, d3]
+    L.pc_level_set_time_v1.argtypes = [vp, C.c_double]
+    L.pc_level_get_time_v1.argtypes = [vp, d3]
     L.pc_level_init_dt_v1.argtypes = [vp, vp]
     L.pc_level_step_v1.argtypes = [vp, C.c_int, vp]
     L.pc_level_fill_ghosts_v1.argtypes = [vp, vp]

@@ -107,8 +107,15 @@ static V null_view() {
   return v;
 }
 
-// Array4 -> V; fails when an offset would not fit 32 bits or the view does
-// not cover `need` with ncomp >= nc.
+// Largest FAB the kernels address: views index elements with 32-bit ints,
+// and the buffer-load views (VB) give the hardware the whole array as one
+// buffer whose byte offsets (component offset + cell offset) must stay below
+// 2^31.  Every array a kernel reads or writes is held to this many bytes.
+constexpr long long MAX_FAB_BYTES = 2147483647LL;
+
+// Array4 -> V; fails when the FAB spans more than MAX_FAB_BYTES (its byte
+// offsets would not fit the kernels' 32-bit offsets) or the view does not
+// cover `need` with ncomp >= nc.
 static int a4_view(pc_ctx* c, const pc_array4_t& a, const IBox& need, int nc,
                    const char* name, V& v) {
   if (!a.p) return fail(c, PC_ERR_ARG, "%s: null pointer", name);
@@ -126,10 +133,11 @@ static int a4_view(pc_ctx* c, const pc_array4_t& a, const IBox& need, int nc,
                         a.jstride * (a.end[1] - a.begin[1]) +
                         a.kstride * (a.end[2] - a.begin[2]) +
                         a.nstride * (long long)(a.ncomp > 0 ? a.ncomp : 1);
-  if (ext > 2147483647LL || a.jstride > 2147483647LL ||
-      a.kstride > 2147483647LL || a.nstride > 2147483647LL)
-    return fail(c, PC_ERR_UNSUPPORTED, "%s: FAB too large for 32-bit offsets",
-                name);
+  if (a.jstride < 0 || a.kstride < 0 || a.nstride < 0 ||
+      ext > MAX_FAB_BYTES / (long long)sizeof(double))
+    return fail(c, PC_ERR_UNSUPPORTED,
+                "%s: FAB of %lld doubles exceeds the %lld bytes the kernels "
+                "address with 32-bit offsets", name, ext, MAX_FAB_BYTES);
   v.p = a.p;
   v.js = (int)a.jstride;
   v.ks = (int)a.kstride;
@@ -220,6 +228,35 @@ static WsLayout ws_layout(const IBox& bx, int ns, bool with_q, bool with_src,
   L.off_flat = take(L.g1.npts());
   L.total = o;
   return L;
+}
+
+// The largest array of a box's state and workspace in bytes: the state FAB
+// (grow(bx,4) x NVAR) and every stage array of the layout (q x QVAR, traces x
+// 2 QT, fluxes, ...), each of which one view addresses.
+static long long box_max_array_bytes(const IBox& bx, int ns, const WsLayout& W) {
+  const int QVAR = 8 + ns, NVAR = 7 + ns, QT = 6 + ns, FT = 7 + ns;
+  long long m = growb(bx, 4).npts() * (long long)NVAR;
+  auto mx_ = [&](long long v) { m = v > m ? v : m; };
+  mx_(W.q.npts() * (long long)QVAR);
+  for (int d = 0; d < 3; ++d) {
+    mx_(W.fl[d].npts() * (long long)NVAR);
+    if (!W.mol) {
+      mx_(W.tr[d].npts() * 2LL * QT);
+      mx_(W.f1[d].npts() * (long long)FT);
+    }
+  }
+  if (!W.mol)
+    for (int t = 0; t < 6; ++t) mx_(W.ft[t].npts() * (long long)FT);
+  return m * (long long)sizeof(double);
+}
+
+static int ws_guard(pc_ctx* c, const IBox& bx, const WsLayout& W) {
+  const long long big = box_max_array_bytes(bx, c->ns, W);
+  if (big > MAX_FAB_BYTES)
+    return fail(c, PC_ERR_UNSUPPORTED,
+                "box: a stage array of %lld bytes exceeds the %lld the kernels "
+                "address with 32-bit offsets", big, MAX_FAB_BYTES);
+  return PC_OK;
 }
 
 static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
@@ -479,6 +516,7 @@ void pc_params_default(pc_params_t* p) {
   p->do_mol = 0;
   p->mol_iorder = 2;
   p->use_laxf_flux = 0;
+  p->stop_time = -1.0;
 }
 
 int pc_ctx_create(int eos_kind, pc_ctx_t** out) {
@@ -643,6 +681,7 @@ int pc_umdrv_v1(pc_ctx_t* c, double /*time*/, pc_box_t bx, const int* domlo,
   }
   hipStream_t st = (hipStream_t)stream;
   const WsLayout L = ws_layout(b, c->ns, false, false, false);
+  if ((rc = ws_guard(c, b, L))) return rc;
   double* ws;
   if ((rc = ctx_scratch(c, L.total * sizeof(double), st, &ws))) return rc;
   if ((rc = finish_umdrv_job(c, J, b, ws, flx, a, vol, uout))) return rc;
@@ -690,6 +729,7 @@ int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
   const WsLayout Lq = ws_layout(b, c->ns, true, src.p != nullptr, false);
   const WsLayout L = ws_layout(b, c->ns, false, false, false);
   const size_t qpart = Lq.off_tr[0];
+  if ((rc = ws_guard(c, b, L))) return rc;
   double* ws;
   if ((rc = ctx_scratch(c, (qpart + L.total) * sizeof(double), st, &ws))) return rc;
   J.q = fab_view(ws + Lq.off_q, Lq.q);
@@ -769,6 +809,7 @@ int pc_mol_src_box_v1(pc_ctx_t* c, pc_box_t bx, pc_array4_t sarr,
   }
   hipStream_t st = (hipStream_t)stream;
   const WsLayout W = ws_layout(b, c->ns, true, false, flx == nullptr, true);
+  if ((rc = ws_guard(c, b, W))) return rc;
   double* ws;
   if ((rc = ctx_scratch(c, W.total * sizeof(double), st, &ws))) return rc;
   J.q = fab_view(ws + W.off_q, W.q);
@@ -1072,14 +1113,17 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
           level_free(L);
           return fail(c, PC_ERR_ARG, "box %d outside the domain", b);
         }
-    if (growb(bx, 4).npts() * NVAR > 2147483647LL) {
+    const WsLayout W = ws_layout(bx, ns, true, false, true, L->mol);
+    const long long big = box_max_array_bytes(bx, ns, W);
+    if (big > MAX_FAB_BYTES) {
       level_free(L);
-      return fail(c, PC_ERR_UNSUPPORTED, "box %d too large", b);
+      return fail(c, PC_ERR_UNSUPPORTED,
+                  "box %d: an array of %lld bytes exceeds the %lld the kernels "
+                  "address with 32-bit offsets", b, big, MAX_FAB_BYTES);
     }
     L->boxes.push_back(bx);
     L->off.push_back(o);
     o += growb(bx, 4).npts() * NVAR;
-    const WsLayout W = ws_layout(bx, ns, true, false, true, L->mol);
     wsoff.push_back(wso);
     wso += W.total;
   }
@@ -1186,6 +1230,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   L->la.change_max = prm->change_max;
   L->la.init_shrink = prm->init_shrink;
   L->la.dt_cutoff = prm->dt_cutoff;
+  L->la.stop_time = prm->stop_time;
   memset(&L->ka, 0, sizeof(L->ka));
   L->ka.difmag = prm->difmag;
   L->ka.plm_iorder = prm->plm_iorder;
@@ -1195,6 +1240,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   z.minbits = 0x7FEFFFFFFFFFFFFFull;
   z.bad = 0;
   z.small = 0;
+  z.time = 0.0;
   (void)hipMemcpy(L->ds, &z, sizeof(z), hipMemcpyHostToDevice);
   if (hipDeviceSynchronize() != hipSuccess) {
     level_free(L);
@@ -1253,6 +1299,18 @@ int pc_level_get_dt_v1(pc_level_t* L, double* dt) {
   HIPCHK(L->ctx, hipMemcpy(dt, &L->ds->dt, sizeof(double), hipMemcpyDeviceToHost));
   return PC_OK;
 }
+int pc_level_set_time_v1(pc_level_t* L, double time) {
+  if (!L) return PC_ERR_ARG;
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  HIPCHK(L->ctx, hipMemcpy(&L->ds->time, &time, sizeof(double), hipMemcpyHostToDevice));
+  return PC_OK;
+}
+int pc_level_get_time_v1(pc_level_t* L, double* time) {
+  if (!L || !time) return PC_ERR_ARG;
+  HIPCHK(L->ctx, hipDeviceSynchronize());
+  HIPCHK(L->ctx, hipMemcpy(time, &L->ds->time, sizeof(double), hipMemcpyDeviceToHost));
+  return PC_OK;
+}
 
 static int level_fill(pc_level* L, hipStream_t st) {
   if (L->ctx->ns == 1)
@@ -1286,7 +1344,7 @@ int pc_level_init_dt_v1(pc_level_t* L, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (L->ctx->ns == 1) level_estdt_launch<EosGamma>(L, st);
   else level_estdt_launch<EosLiDryer>(L, st);
-  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 1);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, DT_INIT);
   HIPCHK(L->ctx, hipGetLastError());
   return PC_OK;
 }
@@ -1299,9 +1357,7 @@ int pc_level_estdt_v1(pc_level_t* L, double* dt_out, void* stream) {
   HIPCHK(L->ctx, hipMemcpy(&save, L->ds, sizeof(save), hipMemcpyDeviceToHost));
   if (L->ctx->ns == 1) level_estdt_launch<EosGamma>(L, st);
   else level_estdt_launch<EosLiDryer>(L, st);
-  LevelArgs la = L->la;
-  la.init_shrink = 1.0;
-  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, la, 1);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, DT_EST);
   HIPCHK(L->ctx, hipStreamSynchronize(st));
   DtState now;
   HIPCHK(L->ctx, hipMemcpy(&now, L->ds, sizeof(now), hipMemcpyDeviceToHost));
@@ -1430,7 +1486,7 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st,
         run(x, PC_K_FINAL_X + (x - ST_FINAL_X));
       run(ST_UPDATE, PC_K_UPDATE);
     }
-    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, 0);
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, DT_NEW);
     mark(PC_K_DT);
     if (tm) {
       (void)hipEventSynchronize(L->ev[ne - 1]);
@@ -2315,7 +2371,7 @@ static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
       launch_stage<EOS>(B, ST_MOL_UPD, S[ST_MOL_UPD], P[ST_MOL_UPD], st, h);
     }
     hipLaunchKernelGGL(k_dt_finalize_list, dim3((n + 63) / 64), dim3(64), 0, st,
-                       B->ds, (const int*)B->dids, n, B->la, 0);
+                       B->ds, (const int*)B->dids, n, B->la, DT_NEW);
     return;
   }
   launch_stage<EOS>(B, ST_GHOST, S[ST_GHOST], P[ST_GHOST], st);
@@ -2330,7 +2386,7 @@ static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
   }
   launch_stage<EOS>(B, ST_UPDATE, S[ST_UPDATE], P[ST_UPDATE], st);
   hipLaunchKernelGGL(k_dt_finalize_list, dim3((n + 63) / 64), dim3(64), 0, st,
-                     B->ds, (const int*)B->dids, n, B->la, 0);
+                     B->ds, (const int*)B->dids, n, B->la, DT_NEW);
 }
 
 extern "C" {
@@ -2371,6 +2427,7 @@ int pc_batch_create_v1(pc_ctx_t* c, const pc_params_t* prm, int mixed,
   B->la.change_max = prm->change_max;
   B->la.init_shrink = prm->init_shrink;
   B->la.dt_cutoff = prm->dt_cutoff;
+  B->la.stop_time = prm->stop_time;
   auto dalloc = [&](void** p, size_t bytes) {
     if (hipMalloc(p, bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -2467,6 +2524,14 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
   int rc;
   V pv;
   if ((rc = a4_view(c, prim, box, 5 + c->ns, "prim", pv))) return rc;
+  {
+    const WsLayout W = ws_layout(box, c->ns, true, false, true, B->mol);
+    const long long big = box_max_array_bytes(box, c->ns, W);
+    if (big > MAX_FAB_BYTES)
+      return fail(c, PC_ERR_UNSUPPORTED,
+                  "request box: an array of %lld bytes exceeds the %lld the "
+                  "kernels address with 32-bit offsets", big, MAX_FAB_BYTES);
+  }
   const size_t bytes = batch_slot_bytes(B, box);
   const long long off = B->arena->alloc(bytes);
   if (off < 0)
@@ -2508,6 +2573,7 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
   z.minbits = 0x7FEFFFFFFFFFFFFFull;
   z.bad = 0;
   z.small = 0;
+  z.time = 0.0;
   HIPCHK(c, hipStreamSynchronize(st));
   HIPCHK(c, hipMemcpy(B->dinit, &J, sizeof(Job), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(B->ds + id, &z, sizeof(z), hipMemcpyHostToDevice));
@@ -2520,7 +2586,7 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
     LAUNCH(k_prim_to_cons<EosLiDryer>, g, st, s0, B->ka, pv, prim_mode);
     LAUNCH((k_estdt<EosLiDryer, false>), g, st, s0, B->ka, B->ds + id);
   }
-  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, B->ds + id, B->la, 1);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, B->ds + id, B->la, DT_INIT);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(st));
   S.live = true;

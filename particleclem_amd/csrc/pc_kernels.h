@@ -126,12 +126,13 @@ PCD VB<T> bufview(const VT<T>& v) {
 // either.
 // Device-resident time-step state of a level or of one request: dt of the
 // next step, the running minimum of dx/(c+|u|) (bits of a positive double),
-// and the NaN and dt_cutoff flags.
+// the NaN and dt_cutoff flags and the time the state has reached.
 struct DtState {
   double dt;
   unsigned long long minbits;
   int bad;
   int small;
+  double time;
 };
 
 template <class FP>

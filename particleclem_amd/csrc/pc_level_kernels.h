@@ -13,6 +13,7 @@ struct LevelArgs {
   int dlo[3], dhi[3], bclo[3], bchi[3];
   const int* cellmap;  // domain cell -> box (domain mode)
   double cfl, max_dt, fixed_dt, change_max, init_shrink, dt_cutoff;
+  double stop_time;  // < 0: no stop_time limit
 };
 
 // Source index of ghost coordinate x (Source/Setup.cpp:42-121 BC map:
@@ -238,10 +239,18 @@ __global__ void __launch_bounds__(BLOCK)
   block_min_to(m, &ds->minbits);
 }
 
+// What dt_finalize computes from the partial minimum.
+enum DtMode {
+  DT_NEW = 0,   // computeNewDt after a step of ds->dt: time += dt first
+  DT_INIT = 1,  // computeInitialDt at ds->time
+  DT_EST = 2    // estTimeStep alone (no init_shrink, no stop_time)
+};
+
 // estTimeStep (Source/PeleC.cpp:829-961) from the partial minimum, then
-// computeNewDt (:964-1034, single level, no stop_time) or, with init = 1,
-// initialTimeStep (:811-826).
-PCD void dt_finalize(DtState* ds, const LevelArgs& la, int init) {
+// computeNewDt (:964-1034, single level) or computeInitialDt (:1041-1075 ->
+// initialTimeStep :811-826), both limited by stop_time (:1018-1027,
+// 1061-1068) at the time the state has reached.
+PCD void dt_finalize(DtState* ds, const LevelArgs& la, int mode) {
   double est;
   if (la.fixed_dt > 0.0) {
     est = la.fixed_dt;
@@ -254,27 +263,36 @@ PCD void dt_finalize(DtState* ds, const LevelArgs& la, int init) {
     if (eh < est) est = eh;
   }
   double dtn;
-  if (init) {
-    dtn = la.init_shrink * est;
-  } else {
+  if (mode == DT_EST) {
     dtn = est;
-    if (la.fixed_dt <= 0.0) dtn = mn(dtn, la.change_max * ds->dt);
+  } else {
+    if (mode == DT_INIT) {
+      dtn = la.init_shrink * est;
+    } else {
+      ds->time += ds->dt;
+      dtn = est;
+      if (la.fixed_dt <= 0.0) dtn = mn(dtn, la.change_max * ds->dt);
+    }
+    const double dt_eps = 0.001 * dtn;
+    if (la.stop_time >= 0.0) {
+      if ((ds->time + dtn) > (la.stop_time - dt_eps)) dtn = la.stop_time - ds->time;
+    }
   }
   ds->dt = dtn;
   if (dtn < la.dt_cutoff) ds->small = 1;
   ds->minbits = 0x7FEFFFFFFFFFFFFFull;
 }
 
-__global__ void k_dt_finalize(DtState* ds, LevelArgs la, int init) {
+__global__ void k_dt_finalize(DtState* ds, LevelArgs la, int mode) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  dt_finalize(ds, la, init);
+  dt_finalize(ds, la, mode);
 }
 
 // the same for the requests ids[0..n) of a batch, one thread each
 __global__ void k_dt_finalize_list(DtState* base, const int* ids, int n,
-                                   LevelArgs la, int init) {
+                                   LevelArgs la, int mode) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dt_finalize(base + ids[t], la, init);
+  if (t < n) dt_finalize(base + ids[t], la, mode);
 }
 
 // primitive -> conservative (pc_initdata pattern; see workloads.py)

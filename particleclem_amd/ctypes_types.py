@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("do_mol", C.c_int),
         ("mol_iorder", C.c_int),
         ("use_laxf_flux", C.c_int),
+        ("stop_time", C.c_double),
     ]
 
 
@@ -70,6 +71,7 @@ DEFAULTS = dict(
     max_dt=1.0e200, fixed_dt=-1.0, dt_cutoff=0.0, allow_small_energy=1,
     allow_negative_energy=1, dual_energy_update_E_from_e=1,
     dual_energy_eta2=1.0e-4, do_mol=0, mol_iorder=2, use_laxf_flux=0,
+    stop_time=-1.0,
 )
 
 

@@ -207,6 +207,14 @@ class DeviceLevel:
         self._chk(self.L.pc_level_get_dt_v1(self.h, C.byref(v)))
         return v.value
 
+    def set_time(self, t):
+        self._chk(self.L.pc_level_set_time_v1(self.h, float(t)))
+
+    def get_time(self):
+        v = C.c_double()
+        self._chk(self.L.pc_level_get_time_v1(self.h, C.byref(v)))
+        return v.value
+
     def init_dt(self, stream=None):
         self._chk(self.L.pc_level_init_dt_v1(self.h, _stream_ptr(stream)))
 

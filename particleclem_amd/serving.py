@@ -22,6 +22,8 @@ from .ctypes_types import Box, make_box, make_params
 from .hydro import HydroContext, _stream_ptr, a4_from_tensor
 from .workloads import BC, burnt_h2_air, unburnt_h2_air
 
+PC_ERR_NOMEM = -8
+
 check = _lib.check
 TUBE, SLAB = 0, 1          # request kinds: gamma-law shock tube, premixed slab
 NSPEC = {TUBE: 1, SLAB: 9}
@@ -116,6 +118,8 @@ class Request:
         self.done = 0
         self.t_admit = self.t_done = None
         self.result = None
+        self.status = None
+        self.rejected = False      # larger than the whole arena: never runs
 
     @property
     def cell_updates(self):
@@ -396,7 +400,17 @@ class Server:
         while pending or waiting or active:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].t_arrive <= now):
-                waiting.append(pending.popleft())
+                r = pending.popleft()
+                b = self.batch[r.kind]
+                if ArenaModel.round(b.slot_bytes(r.n)) > b.memory()[0]:
+                    # it would not fit even with every other request gone:
+                    # answer it at once instead of queueing it forever
+                    r.rejected = True
+                    r.status = PC_ERR_NOMEM
+                    r.t_admit = r.t_done = now
+                    served.append(r)
+                    continue
+                waiting.append(r)
             if self.admission == "backfill":
                 self._admit_backfill(waiting, active, t0)
             i = 0
@@ -447,9 +461,11 @@ class Server:
 
 
 def summarize(served, wall):
+    rejected = sum(1 for r in served if r.rejected)
+    served = [r for r in served if not r.rejected]
     lat = np.array([r.t_done - r.t_arrive for r in served])
     cells = sum(r.cell_updates for r in served)
-    return {"requests": len(served), "wall_s": wall,
+    return {"requests": len(served), "rejected": rejected, "wall_s": wall,
             "mcell_updates_per_s": cells / wall / 1e6,
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                            "p99": float(np.percentile(lat, 99) * 1e3),

@@ -12,7 +12,9 @@ from particleclem_amd.ctypes_types import (Array4, Box, LevelDesc, Params,
                                            a4_from_numpy, make_box)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# PC_ORACLE_SO names another build of the oracle (the planted-mistake check
+# in tests/test_oracle_mutants.py loads deliberately broken copies this way)
+ORACLE_SO = os.environ.get("PC_ORACLE_SO") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 _lib = None
 
@@ -43,7 +45,15 @@ def lib():
     L.pc_ref_level_fill_ghosts.argtypes = [C.c_int, P(LevelDesc), d3]
     L.pc_ref_level_estdt.argtypes = [C.c_int, P(LevelDesc), P(Params), d3, d3]
     L.pc_ref_level_step.argtypes = [C.c_int, P(LevelDesc), P(Params), d3,
-                                    C.c_int, d3, C.c_int]
+                                    C.c_int, d3, d3, C.c_int]
+    L.pc_ref_level_init_dt.argtypes = [C.c_int, P(LevelDesc), P(Params), d3,
+                                       C.c_double, d3]
+    L.pc_ref_transdo_point.argtypes = [C.c_int, C.c_int, C.c_int] + [d3] * 6 + [
+        C.c_double, C.c_double, d3, d3]
+    L.pc_ref_transdd_point.argtypes = [C.c_int, C.c_int] + [d3] * 10 + [
+        C.c_double, d3, C.c_double, C.c_double, C.c_double, d3, d3]
+    L.pc_ref_pdivu_point.argtypes = [d3, d3, d3, C.c_double]
+    L.pc_ref_pdivu_point.restype = C.c_double
     L.pc_ref_prim_to_cons.argtypes = [C.c_int, Box, Array4, C.c_int, Array4]
     L.pc_ref_eos.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                              C.c_double, d3, d3]
@@ -211,10 +221,56 @@ class HostLevel:
                                         C.byref(out)) == 0
         return out.value
 
+    time = 0.0
+
+    def init_dt(self, prm):
+        """computeInitialDt at the level's time (init_shrink, stop_time)."""
+        out = C.c_double(0.0)
+        assert lib().pc_ref_level_init_dt(self.kind, C.byref(self.desc),
+                                          C.byref(prm), dptr(self.arena),
+                                          self.time, C.byref(out)) == 0
+        return out.value
+
     def step(self, prm, nsteps, dt, nthreads=0):
         d = C.c_double(dt)
+        t = C.c_double(self.time)
         rc = lib().pc_ref_level_step(self.kind, C.byref(self.desc), C.byref(prm),
                                      dptr(self.arena), nsteps, C.byref(d),
-                                     nthreads)
+                                     C.byref(t), nthreads)
         assert rc == 0, rc
+        self.time = t.value
         return d.value
+
+
+def transdo(kind, dir, t, qplus, qminus, fl_lo, fl_hi, qi_lo, qi_hi, gamc,
+            cdtdx):
+    """pc_transdo on one cell: returns the corrected (plus, minus) states."""
+    qv = 8 + kind
+    op, om = np.zeros(qv), np.zeros(qv)
+    a = [np.ascontiguousarray(x, dtype=np.float64)
+         for x in (qplus, qminus, fl_lo, fl_hi, qi_lo, qi_hi)]
+    assert lib().pc_ref_transdo_point(kind, dir, t, *[dptr(x) for x in a], gamc,
+                                      cdtdx, dptr(op), dptr(om)) == 0
+    return op, om
+
+
+def transdd(kind, dir, qplus, qminus, f0, f1, qi0, qi1, gamc, srcq, hdt, c0,
+            c1):
+    """pc_transdd on one cell; f0/f1/qi0/qi1 are (lo, hi) pairs."""
+    qv = 8 + kind
+    op, om = np.zeros(qv), np.zeros(qv)
+    a = [np.ascontiguousarray(x, dtype=np.float64)
+         for x in (qplus, qminus, f0[0], f0[1], f1[0], f1[1], qi0[0], qi0[1],
+                   qi1[0], qi1[1])]
+    s = np.ascontiguousarray(srcq if srcq is not None else np.zeros(qv),
+                             dtype=np.float64)
+    assert lib().pc_ref_transdd_point(kind, dir, *[dptr(x) for x in a], gamc,
+                                      dptr(s), hdt, c0, c1, dptr(op),
+                                      dptr(om)) == 0
+    return op, om
+
+
+def pdivu(pg, un, area, vol):
+    """pc_pdivu of one cell from its six faces' (p_gd, u_gd, area)."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (pg, un, area)]
+    return lib().pc_ref_pdivu_point(*[dptr(x) for x in a], vol)

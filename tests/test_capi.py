@@ -151,3 +151,51 @@ def test_empty_inputs_are_rejected_with_codes():
     assert L.pc_level_set_branches_v1(None, 1) == -1
     assert L.pc_batch_set_branches_v1(None, 1) == -1
     L.pc_ctx_destroy(h)
+
+
+def test_boxes_beyond_32bit_offsets_are_rejected():
+    """Kernels address a FAB with 32-bit offsets (buffer loads take the
+    byte offset of component and cell as one 32-bit value): a level box
+    whose stage arrays would exceed 2 GiB (248^3 with 9 species: q alone is
+    256^3 x 17 doubles = 2.28 GB) and a pc_umdrv_v1 call with a q FAB over
+    2 GiB fail with PC_ERR_UNSUPPORTED before any device work."""
+    from particleclem_amd.ctypes_types import Box, LevelDesc
+    L = _lib.lib()
+    h = _ctx(9)
+    i3 = lambda v: (C.c_int * 3)(*v)
+    d = LevelDesc()
+    d.nboxes = 1
+    for n, want in ((248, -7), (200, None)):
+        boxes = (Box * 1)(make_box([0, 0, 0], [n - 1] * 3))
+        d.boxes = C.cast(boxes, C.POINTER(Box))
+        d.domlo = i3([0, 0, 0])
+        d.domhi = i3([n - 1] * 3)
+        d.dx = (C.c_double * 3)(0.1, 0.1, 0.1)
+        lev = C.c_void_p()
+        p = make_params(ppm_type=1)
+        if want is None:
+            continue  # 200^3 passes the size check (allocation needs a GPU)
+        assert L.pc_level_create_v1(h, C.byref(d), C.byref(p), C.byref(lev)) == want
+        assert "32-bit offsets" in L.pc_last_error(h).decode()
+    # drop-in: a 16^3 box whose caller q FAB spans 17 x 2^27 doubles
+    lo, hi = [0, 0, 0], [15, 15, 15]
+    g = [-4] * 3, [19] * 3
+    big_q = _a4(*g, 17)
+    big_q.nstride = 1 << 27
+    flx = (Array4 * 3)(*[_a4(lo, [hi[0] + (k == 0), hi[1] + (k == 1), hi[2] + (k == 2)], 16)
+                         for k in range(3)])
+    ar = (Array4 * 3)(*[_a4(lo, [hi[0] + (k == 0), hi[1] + (k == 1), hi[2] + (k == 2)], 1)
+                        for k in range(3)])
+    rc = L.pc_umdrv_v1(h, 0.0, make_box(lo, hi), i3([0, 0, 0]), i3([15] * 3),
+                       i3([0, 0, 0]), i3([0, 0, 0]), _a4(*g, 16), _a4(lo, hi, 16),
+                       big_q, _a4(*g, 6), null_a4(), (C.c_double * 3)(0.1, 0.1, 0.1),
+                       1e-3, 1, 4, 1, 0, 1, 0.1, flx, ar, _a4(lo, hi, 1), None, None)
+    assert rc == -7, rc
+    assert "32-bit offsets" in L.pc_last_error(h).decode()
+    L.pc_ctx_destroy(h)
+
+
+def test_params_carry_stop_time():
+    p = Params()
+    _lib.lib().pc_params_default(C.byref(p))
+    assert p.stop_time == -1.0

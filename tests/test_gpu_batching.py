@@ -109,6 +109,7 @@ def test_graph_on_multibox_sod_matches_oracle(cuda):
     nbits = sum(compare(g[VALID], r[VALID])[1] for g, r in zip(got, ref))
     print(f"sod32 graph: max rel err {worst:.3e}, {nbits} values not bitwise")
     assert worst <= 1e-12
+    assert nbits == 0
     lev.close()
     ctx.close()
 

@@ -92,6 +92,8 @@ def run_box(kind, w, bclo, bchi, src_scale=0.0, ncalls=1, ppm=1):
           f"rel err flx x/y/z, dU/dt: {[f'{e:.2e}' for e, _, _ in errs]}; "
           f"{sum(n for _, n, _ in errs)}/{sum(t for _, _, t in errs)} not bitwise")
     ctx.close()
+    # FP64: every flux and rate value bit for bit the oracle's
+    assert sum(n for _, n, _ in errs) == 0, [n for _, n, _ in errs]
     return max(e for e, _, _ in errs)
 
 
@@ -189,13 +191,15 @@ def test_umdrv_with_caller_primitives(cuda, with_src):
               (dsrcq, g5[0]) if with_src else None, dx, dt, 1, 4, 1, 0.1,
               [(f, lo) for f in dflx], [(T(a), lo) for a in area], (T(vol), lo))
     torch.cuda.synchronize()
-    e = [compare(dq.cpu().numpy(), q_r)[0], compare(dqa.cpu().numpy(), qa_r)[0],
-         compare(duout.cpu().numpy(), uout_r, scale=rate_scale(U, dt))[0]]
+    c = [compare(dq.cpu().numpy(), q_r), compare(dqa.cpu().numpy(), qa_r),
+         compare(duout.cpu().numpy(), uout_r, scale=rate_scale(U, dt))]
     if with_src:
-        e.append(compare(dsrcq.cpu().numpy(), srcq_r)[0])
-    e += [compare(dflx[d].cpu().numpy(), flx_r[d])[0] for d in range(3)]
-    print("umdrv drop-in errors", e)
+        c.append(compare(dsrcq.cpu().numpy(), srcq_r))
+    c += [compare(dflx[d].cpu().numpy(), flx_r[d]) for d in range(3)]
+    e = [x[0] for x in c]
+    print("umdrv drop-in errors", e, "not bitwise", [x[1] for x in c])
     assert max(e) <= 1e-12, e
+    assert all(x[1] == 0 for x in c), [x[1] for x in c]
     ctx.close()
 
 

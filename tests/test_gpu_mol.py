@@ -107,6 +107,7 @@ def test_mol_src_box_matches_oracle(cuda, kind, laxf):
           f"bitwise), flx {[f'{e:.2e}' for e, _, _ in e_flx]}")
     assert e_src[0] <= 1e-12
     assert all(e <= 1e-12 for e, _, _ in e_flx)
+    assert e_src[1] == 0 and all(n == 0 for _, n, _ in e_flx)
     assert np.array_equal(out2.cpu().numpy(), out.cpu().numpy())
     ctx.close()
 

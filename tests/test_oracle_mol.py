@@ -254,5 +254,5 @@ def test_mol_iorder_is_validated():
     H, prm = oracle_level(w, dict(do_mol=1, mol_iorder=3))
     d = C.c_double(1e-3)
     rc = ob.lib().pc_ref_level_step(1, C.byref(H.desc), C.byref(prm),
-                                    ob.dptr(H.arena), 1, C.byref(d), 0)
+                                    ob.dptr(H.arena), 1, C.byref(d), None, 0)
     assert rc == -10

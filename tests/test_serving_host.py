@@ -1,6 +1,7 @@
 """Host logic of the serving loop (CPU): the seeded request stream and the
 latency summary."""
 import numpy as np
+import pytest
 
 from particleclem_amd import serving as sv
 
@@ -203,3 +204,41 @@ def test_backfill_never_delays_the_head_and_cuts_waits():
               f"p50 {np.median(wb):.0f} p99 {np.percentile(wb, 99):.0f} "
               f"mean {wb.mean():.1f}")
         assert wb.mean() <= wf.mean()
+
+
+class _FakeBatch:
+    """Host stand-in for a Batch: an arena of `total` bytes, no device."""
+
+    def __init__(self, total, per_cell):
+        self.total, self.per_cell = total, per_cell
+
+    def slot_bytes(self, n):
+        return n ** 3 * self.per_cell
+
+    def memory(self):
+        return self.total, 0, self.total
+
+    def free_ranges(self):
+        return [(0, self.total)]
+
+
+@pytest.mark.timeout(30)
+@pytest.mark.parametrize("admission", ["backfill", "bypass", "fcfs"])
+def test_request_larger_than_the_arena_is_rejected_not_queued(admission):
+    """A request whose slot exceeds the whole arena can never be admitted:
+    the loop answers it with PC_ERR_NOMEM at once (it used to wait for it
+    forever with nothing running)."""
+    srv = sv.Server.__new__(sv.Server)
+    srv.batch = {sv.TUBE: _FakeBatch(1 << 20, 1000), sv.SLAB: _FakeBatch(1 << 20, 1000)}
+    srv.max_slots = 4
+    srv.admission = admission
+    srv.bypass_depth = 0 if admission == "fcfs" else 8
+    srv.bypass_ticks = 50
+    srv.plan_s = 0.0
+    srv.keep_results = False
+    srv.streams = {k: None for k in srv.batch}
+    reqs = [sv.Request(0, sv.SLAB, 128, 10, 2.0, 1.0, 0.0, 1),
+            sv.Request(1, sv.TUBE, 64, 10, 2.0, 1.0, 0.0, 2)]
+    out = srv.run(reqs, realtime=False)
+    assert sorted(r.rid for r in out) == [0, 1]
+    assert all(r.rejected and r.status == sv.PC_ERR_NOMEM for r in out)
