@@ -185,6 +185,27 @@ int pc_hydro_source_box_v1(pc_ctx_t* ctx, pc_box_t bx, const int* domlo,
                            const pc_params_t* prm, const pc_array4_t* flx,
                            const pc_array4_t* a, pc_array4_t vol, void* stream);
 
+/* pc_hydro_source_box_v1 without any device allocation, for callers that
+ * cannot allocate (code under hipGraph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments
+ * and results; the workspace (q, qaux, srcq, the CTU intermediates and the
+ * job descriptor) lives in one static device array of the library, 96 MiB
+ * per GPU, held until the process ends: any box of up to 32^3 cells fits
+ * (9 species, with sources); a larger one returns PC_ERR_UNSUPPORTED before
+ * any device work.  Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (different streams, or another thread) race;
+ * calls in order on one stream do not.  Nothing in the array needs clearing
+ * (each call writes every element it reads first). */
+int pc_hydro_source_box_noalloc_v1(pc_ctx_t* ctx, pc_box_t bx,
+                                   const int* domlo, const int* domhi,
+                                   const int* bclo, const int* bchi,
+                                   pc_array4_t sarr, pc_array4_t src,
+                                   pc_array4_t hyd_src, const double* dx,
+                                   double dt, const pc_params_t* prm,
+                                   const pc_array4_t* flx,
+                                   const pc_array4_t* a, pc_array4_t vol,
+                                   void* stream);
+
 /* -------------------------------------------------------- level (boxes) --- */
 /* A single AMR level as a table of boxes (a BoxArray on one GPU), each with
  * its own FAB of NVAR comps and 4 ghost cells in one device arena, uniform

@@ -16,7 +16,8 @@ EXPORTS = [
     "pc_params_default", "pc_ctx_create", "pc_ctx_destroy",
     "pc_ctx_num_species", "pc_ctx_nvar", "pc_ctx_qvar", "pc_last_error",
     "pc_ctoprim_v1", "pc_srctoprim_v1", "pc_umdrv_v1",
-    "pc_hydro_source_box_v1", "pc_mol_src_box_v1", "pc_level_create_v1",
+    "pc_hydro_source_box_v1", "pc_hydro_source_box_noalloc_v1",
+    "pc_mol_src_box_v1", "pc_level_create_v1",
     "pc_level_destroy_v1",
     "pc_level_box_offset_v1", "pc_level_box_size_v1", "pc_level_state_ptr_v1",
     "pc_level_state_size_v1", "pc_level_write_box_v1", "pc_level_read_box_v1",
@@ -94,6 +95,7 @@ def lib():
                                          Array4, Array4, d3, C.c_double,
                                          P(Params), P(Array4), P(Array4),
                                          Array4, vp]
+    L.pc_hydro_source_box_noalloc_v1.argtypes = L.pc_hydro_source_box_v1.argtypes
     L.pc_mol_src_box_v1.argtypes = [vp, Box, Array4, Array4, P(Params),
                                     P(Array4), P(Array4), Array4, vp]
     L.pc_level_write_plotfile_v1.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double,

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -699,13 +700,24 @@ int pc_umdrv_v1(pc_ctx_t* c, double /*time*/, pc_box_t bx, const int* domlo,
   return ctx_job_done(c, slot, st);
 }
 
-int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
-                           const int* domhi, const int* bclo, const int* bchi,
-                           pc_array4_t sarr, pc_array4_t src,
-                           pc_array4_t hyd_src, const double* dx, double dt,
-                           const pc_params_t* prm, const pc_array4_t* flx,
-                           const pc_array4_t* a, pc_array4_t vol,
-                           void* stream) {
+}  // extern "C"
+
+// One construct_hydro_source box (Source/Hydro.cpp:73-252): argument checks
+// and the views of the caller's arrays; the workspace comes after.
+struct HsbCall {
+  IBox b;
+  Job J;
+  V vsrc;
+  KArgs ka;
+  WsLayout Lq, L;  // [q, qaux, srcq] part and CTU part of the workspace
+  size_t qpart = 0, total = 0;  // doubles
+};
+
+static int hsb_setup(pc_ctx* c, pc_box_t bx, const int* domlo,
+                     const int* domhi, const int* bclo, const int* bchi,
+                     const pc_array4_t& sarr, const pc_array4_t& src,
+                     const double* dx, double dt, const pc_params_t* prm,
+                     const pc_array4_t* flx, const pc_array4_t* a, HsbCall& H) {
   if (!c || !prm || !box_ok(bx) || !domlo || !domhi || !bclo || !bchi || !dx ||
       !flx || !a)
     return fail(c, PC_ERR_ARG, "null argument or empty box");
@@ -715,51 +727,155 @@ int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
   if (!prm->allow_small_energy || !prm->allow_negative_energy)
     return fail(c, PC_ERR_UNSUPPORTED,
                 "allow_small_energy = allow_negative_energy = 1 required");
-  const IBox b = mkbox(bx.lo, bx.hi);
-  const IBox g4 = growb(b, 4);
-  Job J;
+  H.b = mkbox(bx.lo, bx.hi);
+  const IBox g4 = growb(H.b, 4);
+  Job& J = H.J;
   memset(&J, 0, sizeof(J));
-  for (int d = 0; d < 3; ++d) { J.lo[d] = b.lo[d]; J.hi[d] = b.hi[d]; }
+  for (int d = 0; d < 3; ++d) { J.lo[d] = H.b.lo[d]; J.hi[d] = H.b.hi[d]; }
   set_domain(J, domlo, domhi, bclo, bchi);
   if ((rc = a4_view(c, sarr, g4, 7 + c->ns, "sarr", J.U))) return rc;
-  V vsrc = null_view();
-  if (src.p && (rc = a4_view(c, src, g4, 7 + c->ns, "src", vsrc))) return rc;
-  hipStream_t st = (hipStream_t)stream;
-  // scratch: [q, qaux, srcq] then the CTU intermediates
-  const WsLayout Lq = ws_layout(b, c->ns, true, src.p != nullptr, false);
-  const WsLayout L = ws_layout(b, c->ns, false, false, false);
-  const size_t qpart = Lq.off_tr[0];
-  if ((rc = ws_guard(c, b, L))) return rc;
-  double* ws;
-  if ((rc = ctx_scratch(c, (qpart + L.total) * sizeof(double), st, &ws))) return rc;
-  J.q = fab_view(ws + Lq.off_q, Lq.q);
-  J.qa = fab_view(ws + Lq.off_qa, Lq.q);
-  J.srcq = src.p ? fab_view(ws + Lq.off_srcq, Lq.q) : null_view();
-  for (int d = 0; d < 3; ++d) J.qe[d] = Lq.q.hi[d] + 1;
-  if ((rc = finish_umdrv_job(c, J, b, ws + qpart, flx, a, vol, hyd_src)))
-    return rc;
+  H.vsrc = null_view();
+  if (src.p && (rc = a4_view(c, src, g4, 7 + c->ns, "src", H.vsrc))) return rc;
+  H.Lq = ws_layout(H.b, c->ns, true, src.p != nullptr, false);
+  H.L = ws_layout(H.b, c->ns, false, false, false);
+  H.qpart = H.Lq.off_tr[0];
+  H.total = H.qpart + H.L.total;
+  if ((rc = ws_guard(c, H.b, H.L))) return rc;
   set_geom(J, dx);
   J.dt_val = dt;
-  KArgs ka = umdrv_args(dx, dt, prm->plm_iorder, prm->use_flattening,
-                        prm->difmag);
+  H.ka = umdrv_args(dx, dt, prm->plm_iorder, prm->use_flattening, prm->difmag);
   // srctoprim reads qaux's dp/de and dp/drho, which ctoprim only writes
   // when asked for the full primitive set
-  if (src.p) ka.full_prims = 1;
-  const int gq = grid1(ST_CTOPRIM, J);
-  Sched sc;
-  int slot;
-  if ((rc = ctx_job(c, J, st, &sc, &slot))) return rc;
+  if (src.p) H.ka.full_prims = 1;
+  return PC_OK;
+}
+
+// Workspace views (ws: H.total doubles) and the caller's output views.
+static int hsb_views(pc_ctx* c, HsbCall& H, double* ws, const pc_array4_t* flx,
+                     const pc_array4_t* a, const pc_array4_t& vol,
+                     const pc_array4_t& hyd_src) {
+  Job& J = H.J;
+  J.q = fab_view(ws + H.Lq.off_q, H.Lq.q);
+  J.qa = fab_view(ws + H.Lq.off_qa, H.Lq.q);
+  J.srcq = H.vsrc.p ? fab_view(ws + H.Lq.off_srcq, H.Lq.q) : null_view();
+  for (int d = 0; d < 3; ++d) J.qe[d] = H.Lq.q.hi[d] + 1;
+  return finish_umdrv_job(c, J, H.b, ws + H.qpart, flx, a, vol, hyd_src);
+}
+
+// K1, K2 and pc_umdrv's launches for the job the kernels read through sc.
+static int hsb_launch(pc_ctx* c, const HsbCall& H, const Sched& sc, int ppm_type,
+                      hipStream_t st) {
+  const int gq = grid1(ST_CTOPRIM, H.J);
+  const bool src = H.vsrc.p != nullptr;
   if (c->eos == PC_EOS_GAMMALAW) {
-    LAUNCH(k_ctoprim<EosGamma>, gq, st, sc, ka);
-    if (src.p) LAUNCH(k_srctoprim<1>, gq, st, sc, ka, vsrc);
-    launch_ctu_consup<EosGamma>(J, sc, ka, prm->ppm_type, st);
+    LAUNCH(k_ctoprim<EosGamma>, gq, st, sc, H.ka);
+    if (src) LAUNCH(k_srctoprim<1>, gq, st, sc, H.ka, H.vsrc);
+    launch_ctu_consup<EosGamma>(H.J, sc, H.ka, ppm_type, st);
   } else {
-    LAUNCH(k_ctoprim<EosLiDryer>, gq, st, sc, ka);
-    if (src.p) LAUNCH(k_srctoprim<9>, gq, st, sc, ka, vsrc);
-    launch_ctu_consup<EosLiDryer>(J, sc, ka, prm->ppm_type, st);
+    LAUNCH(k_ctoprim<EosLiDryer>, gq, st, sc, H.ka);
+    if (src) LAUNCH(k_srctoprim<9>, gq, st, sc, H.ka, H.vsrc);
+    launch_ctu_consup<EosLiDryer>(H.J, sc, H.ka, ppm_type, st);
   }
   HIPCHK(c, hipGetLastError());
+  return PC_OK;
+}
+
+// ------------------------------------------- allocation-free box driver --
+// The workspace of pc_hydro_source_box_noalloc_v1: a job descriptor, then
+// q/qaux/srcq and the CTU intermediates of one box.  Sized for any box of
+// up to 32^3 cells with 9 species and sources (89.7 MiB); every element a
+// call reads it has written earlier in the same call, so nothing needs
+// clearing.  One copy per GPU, shared by every call on it.
+constexpr size_t NOALLOC_JOB_DOUBLES = (sizeof(Job) + 255) / 256 * 32;
+constexpr size_t NOALLOC_WS_DOUBLES = (size_t)96 << 17;  // 96 MiB of doubles
+static __device__ __align__(256) double scratch[NOALLOC_JOB_DOUBLES + NOALLOC_WS_DOUBLES];
+
+// Device address of `scratch` per device, looked up on the first call there
+// (the array's name is a host-side stand-in, never a device address).
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64];
+
+static int noalloc_scratch(pc_ctx* c, double** out) {
+  int dev = 0;
+  HIPCHK(c, hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(c, PC_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(c, PC_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                  hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PC_OK;
+}
+
+// The job descriptor travels as a kernel argument into the scratch array.
+__global__ void k_put_job(Job j, Job* dst) {
+  const int* s = (const int*)&j;
+  int* d = (int*)dst;
+  for (int w = threadIdx.x; w < (int)(sizeof(Job) / sizeof(int)); w += blockDim.x)
+    d[w] = s[w];
+}
+static_assert(sizeof(Job) % sizeof(int) == 0, "job descriptor in whole words");
+
+extern "C" {
+
+int pc_hydro_source_box_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
+                           const int* domhi, const int* bclo, const int* bchi,
+                           pc_array4_t sarr, pc_array4_t src,
+                           pc_array4_t hyd_src, const double* dx, double dt,
+                           const pc_params_t* prm, const pc_array4_t* flx,
+                           const pc_array4_t* a, pc_array4_t vol,
+                           void* stream) {
+  HsbCall H;
+  int rc;
+  if ((rc = hsb_setup(c, bx, domlo, domhi, bclo, bchi, sarr, src, dx, dt, prm,
+                      flx, a, H)))
+    return rc;
+  hipStream_t st = (hipStream_t)stream;
+  double* ws;
+  if ((rc = ctx_scratch(c, H.total * sizeof(double), st, &ws))) return rc;
+  if ((rc = hsb_views(c, H, ws, flx, a, vol, hyd_src))) return rc;
+  Sched sc;
+  int slot;
+  if ((rc = ctx_job(c, H.J, st, &sc, &slot))) return rc;
+  if ((rc = hsb_launch(c, H, sc, prm->ppm_type, st))) return rc;
   return ctx_job_done(c, slot, st);
+}
+
+int pc_hydro_source_box_noalloc_v1(pc_ctx_t* c, pc_box_t bx, const int* domlo,
+                                   const int* domhi, const int* bclo,
+                                   const int* bchi, pc_array4_t sarr,
+                                   pc_array4_t src, pc_array4_t hyd_src,
+                                   const double* dx, double dt,
+                                   const pc_params_t* prm,
+                                   const pc_array4_t* flx, const pc_array4_t* a,
+                                   pc_array4_t vol, void* stream) {
+  HsbCall H;
+  int rc;
+  if ((rc = hsb_setup(c, bx, domlo, domhi, bclo, bchi, sarr, src, dx, dt, prm,
+                      flx, a, H)))
+    return rc;
+  if (H.total > NOALLOC_WS_DOUBLES)
+    return fail(c, PC_ERR_UNSUPPORTED,
+                "box needs %zu bytes of workspace, more than the %zu of the "
+                "allocation-free scratch (boxes up to 32^3 cells fit)",
+                H.total * sizeof(double), NOALLOC_WS_DOUBLES * sizeof(double));
+  double* base;
+  if ((rc = noalloc_scratch(c, &base))) return rc;
+  Job* djob = (Job*)base;
+  if ((rc = hsb_views(c, H, base + NOALLOC_JOB_DOUBLES, flx, a, vol, hyd_src)))
+    return rc;
+  H.J.dtp = &djob->dt_val;  // the call's dt travels in the job descriptor
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_put_job, dim3(1), dim3(64), 0, st, H.J, djob);
+  HIPCHK(c, hipGetLastError());
+  return hsb_launch(c, H, one_job_sched(djob), prm->ppm_type, st);
 }
 
 }  // extern "C"
