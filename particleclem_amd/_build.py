@@ -14,7 +14,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpc_hydro.so")
 SOURCES = ["pc_hydro.hip", "pc_plotfile.cpp"]
 HEADERS = ["pc_phys.h", "pc_kernels.h", "pc_level_kernels.h", "pc_mol_kernels.h",
-           "pc_derive_kernels.h", "pc_plotfile.h", "pc_fluxreg.h", "pc_shell.h"]
+           "pc_derive_kernels.h", "pc_plotfile.h", "pc_fluxreg.h", "pc_shell.h",
+           "pc_trace_lds.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -19,6 +19,7 @@
 #include "pc_fluxreg.h"
 #include "pc_mol_kernels.h"
 #include "pc_plotfile.h"
+#include "pc_trace_lds.h"
 
 using namespace pcd;
 
@@ -379,7 +380,13 @@ struct StagePlan {
 };
 
 // logical blocks of one job for a stage and block shape
+// lx = -2: the LDS tiles of the PPM trace (pc_trace_lds.h)
 static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz) {
+  if (lx == -2) {
+    int lo[3], n[3];
+    stage_region(st, J.lo, J.hi, lo, n);
+    return trace_lds_blocks(n);
+  }
   // the ghost fill enumerates its cells itself (launch shapes apply only
   // through persistence)
   if (st == ST_GHOST) return (ghost_shell_count(J.lo, J.hi) + BLOCK - 1) / BLOCK;
@@ -401,6 +408,22 @@ static int grid1(int st, const Job& J) {
     if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0,     \
                                        stream, __VA_ARGS__);                 \
   } while (0)
+#define LAUNCH_SH(kern, grid, shmem, stream, ...)                            \
+  do {                                                                       \
+    if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), shmem, \
+                                       stream, __VA_ARGS__);                 \
+  } while (0)
+
+// Largest LDS trace window (cells per component) over a set of jobs.
+static int trace_cells_of(const Job* jobs, int n) {
+  int m = 0;
+  for (int b = 0; b < n; ++b) {
+    int lo[3], e[3];
+    stage_region(ST_TRACE, jobs[b].lo, jobs[b].hi, lo, e);
+    m = std::max(m, trace_lds_cells(e));
+  }
+  return m;
+}
 
 // The CTU stages from primitives to final fluxes for one EOS.  With a job
 // table (sc.jobs) the grids come from `plans`; otherwise from job j0.
@@ -411,9 +434,11 @@ static void run_ctu(const Job& j0, const Sched& s1, const KArgs& ka,
   auto S = [&](int) { return s1; };
   LAUNCH(k_divu<>, G(ST_DIVU), st, S(ST_DIVU), ka);
   if (ppm_type == 1) {
-    LAUNCH((k_trace_ppm<EOS, 0>), G(ST_TRACE), st, S(ST_TRACE), ka);
-    LAUNCH((k_trace_ppm<EOS, 1>), G(ST_TRACE), st, S(ST_TRACE), ka);
-    LAUNCH((k_trace_ppm<EOS, 2>), G(ST_TRACE), st, S(ST_TRACE), ka);
+    const int gt = (int)stage_blocks(ST_TRACE, j0, -2, 0, 0);
+    const size_t sh = (size_t)trace_cells_of(&j0, 1) * (5 + EOS::NS) * sizeof(double);
+    LAUNCH_SH((k_trace_ppm_lds<EOS, 0>), gt, sh, st, S(ST_TRACE), ka);
+    LAUNCH_SH((k_trace_ppm_lds<EOS, 1>), gt, sh, st, S(ST_TRACE), ka);
+    LAUNCH_SH((k_trace_ppm_lds<EOS, 2>), gt, sh, st, S(ST_TRACE), ka);
   } else {
     LAUNCH((k_trace_plm<EOS, 0>), G(ST_TRACE), st, S(ST_TRACE), ka);
     LAUNCH((k_trace_plm<EOS, 1>), G(ST_TRACE), st, S(ST_TRACE), ka);
@@ -1026,6 +1051,7 @@ struct pc_level {
   std::vector<Job> hjobs;      // host copy of the job table (plans)
   int mixed = 0;               // 1: FP32 reconstruction/Riemann/intermediates
   bool mol = false;            // method-of-lines step (do_mol)
+  int trace_cells = 0;         // largest LDS trace window of the boxes
   bool laxf = false;           // MOL with the Lax-Friedrichs flux
   double tuned_ms[ST_COUNT] = {};
   // hipGraph of gsteps consecutive steps, replayed by pc_level_step_v1
@@ -1047,6 +1073,8 @@ static void plan_free(StagePlan& P) {
 // and at most `cap` workgroups (0: one per logical block).
 static bool plan_make(pc_level* L, int s, int lx, int ly, int lz, int cap,
                       StagePlan& P) {
+  // the PPM trace always runs on its LDS tiles (only persistence varies)
+  if (s == ST_TRACE && L->prm.ppm_type == 1 && !L->mol) lx = -2;
   const int nb = (int)L->hjobs.size();
   std::vector<int> start(nb), map;
   for (int b = 0; b < nb; ++b) {
@@ -1119,9 +1147,10 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
     case ST_DIVU: LAUNCH((k_divu<FP, P>), grid, st, sc, ka); break;
     case ST_TRACE:
       if (L->prm.ppm_type == 1) {
-        if (tr(0)) LAUNCH((k_trace_ppm<EOS, 0, FP, P>), grid, st, sc, ka);
-        if (tr(1)) LAUNCH((k_trace_ppm<EOS, 1, FP, P>), grid, st, sc, ka);
-        if (tr(2)) LAUNCH((k_trace_ppm<EOS, 2, FP, P>), grid, st, sc, ka);
+        const size_t sh = (size_t)L->trace_cells * (5 + EOS::NS) * sizeof(FP);
+        if (tr(0)) LAUNCH_SH((k_trace_ppm_lds<EOS, 0, FP, P>), grid, sh, st, sc, ka);
+        if (tr(1)) LAUNCH_SH((k_trace_ppm_lds<EOS, 1, FP, P>), grid, sh, st, sc, ka);
+        if (tr(2)) LAUNCH_SH((k_trace_ppm_lds<EOS, 2, FP, P>), grid, sh, st, sc, ka);
       } else {
         if (tr(0)) LAUNCH((k_trace_plm<EOS, 0, FP, P>), grid, st, sc, ka);
         if (tr(1)) LAUNCH((k_trace_plm<EOS, 1, FP, P>), grid, st, sc, ka);
@@ -1294,6 +1323,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   }
   // per-stage launch plans: runs of BLOCK cells, one workgroup each
   L->hjobs = jobs;
+  L->trace_cells = trace_cells_of(jobs.data(), (int)jobs.size());
   for (int s = 0; s < ST_COUNT; ++s) {
     if (!plan_make(L, s, -1, 0, 0, 0, L->plans[s])) {
       level_free(L);
@@ -2401,6 +2431,7 @@ struct pc_batch {
   pc_params_t prm;
   int mixed = 0;
   bool mol = false;       // method-of-lines steps (do_mol)
+  int trace_cells = 0;    // largest LDS trace window of the stepped set
   bool laxf = false;      // MOL with the Lax-Friedrichs flux
   KArgs ka;
   LevelArgs la;
@@ -2449,11 +2480,13 @@ static int batch_tables(pc_batch* B, const std::vector<int>& ids,
   std::vector<Job> jobs(n);
   std::vector<int> start((size_t)ST_COUNT * B->max_slots, 0);
   for (int k = 0; k < n; ++k) jobs[k] = B->slots[ids[k]].job;
+  B->trace_cells = trace_cells_of(jobs.data(), n);
   for (int s = 0; s < ST_COUNT; ++s) {
     int tot = 0;
+    const int lx = (s == ST_TRACE && B->prm.ppm_type == 1 && !B->mol) ? -2 : -1;
     for (int k = 0; k < n; ++k) {
       start[(size_t)s * B->max_slots + k] = tot;
-      tot += (int)stage_blocks(s, jobs[k], -1, 0, 0);
+      tot += (int)stage_blocks(s, jobs[k], lx, 0, 0);
     }
     B->plans[s].nblocks = B->plans[s].grid = tot;
     B->plans[s].lx = -1;
