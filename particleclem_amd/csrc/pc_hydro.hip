@@ -380,12 +380,13 @@ struct StagePlan {
 };
 
 // logical blocks of one job for a stage and block shape
-// lx = -2: the LDS tiles of the PPM trace (pc_trace_lds.h)
+// lx = -2: the LDS tiles of the PPM trace, -3: the pencil tiles of the fused
+// PPM trace + first fluxes (pc_trace_lds.h)
 static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz) {
-  if (lx == -2) {
+  if (lx == -2 || lx == -3) {
     int lo[3], n[3];
     stage_region(st, J.lo, J.hi, lo, n);
-    return trace_lds_blocks(n);
+    return lx == -2 ? trace_lds_blocks(n) : pencil_blocks(n);
   }
   // the ghost fill enumerates its cells itself (launch shapes apply only
   // through persistence)
@@ -414,15 +415,34 @@ static int grid1(int st, const Job& J) {
                                        stream, __VA_ARGS__);                 \
   } while (0)
 
-// Largest LDS trace window (cells per component) over a set of jobs.
-static int trace_cells_of(const Job* jobs, int n) {
+// Largest LDS trace window (cells per component) over a set of jobs, for
+// the trace tiles or (pencil) the fused trace + first-flux tiles.
+static int trace_cells_of(const Job* jobs, int n, bool pencil = false) {
   int m = 0;
   for (int b = 0; b < n; ++b) {
     int lo[3], e[3];
     stage_region(ST_TRACE, jobs[b].lo, jobs[b].hi, lo, e);
-    m = std::max(m, trace_lds_cells(e));
+    m = std::max(m, pencil ? pencil_cells(e) : trace_lds_cells(e));
   }
   return m;
+}
+
+// Whether every job's trace region fits the pencils of the fused PPM trace +
+// first-flux kernel (else the trace and the first fluxes run separately).
+static bool pencils_fit(const Job* jobs, int n) {
+  for (int b = 0; b < n; ++b) {
+    int lo[3], e[3];
+    stage_region(ST_TRACE, jobs[b].lo, jobs[b].hi, lo, e);
+    if (!pencil_fits(e)) return false;
+  }
+  return true;
+}
+
+// Dynamic LDS bytes of the fused kernel: the staging window, then the minus
+// states of a tile's cells.
+template <int NS, class FP>
+static size_t pencil_lds_bytes(int cells) {
+  return (size_t)std::max(cells * (5 + NS), BLOCK * (6 + NS)) * sizeof(FP);
 }
 
 // The CTU stages from primitives to final fluxes for one EOS.  With a job
@@ -433,7 +453,14 @@ static void run_ctu(const Job& j0, const Sched& s1, const KArgs& ka,
   auto G = [&](int stage) { return grid1(stage, j0); };
   auto S = [&](int) { return s1; };
   LAUNCH(k_divu<>, G(ST_DIVU), st, S(ST_DIVU), ka);
-  if (ppm_type == 1) {
+  const bool fused = ppm_type == 1 && pencils_fit(&j0, 1);
+  if (fused) {
+    const int gt = (int)stage_blocks(ST_TRACE, j0, -3, 0, 0);
+    const size_t sh = pencil_lds_bytes<EOS::NS, double>(trace_cells_of(&j0, 1, true));
+    LAUNCH_SH((k_trace_flux1_lds<EOS, 0>), gt, sh, st, S(ST_TRACE), ka);
+    LAUNCH_SH((k_trace_flux1_lds<EOS, 1>), gt, sh, st, S(ST_TRACE), ka);
+    LAUNCH_SH((k_trace_flux1_lds<EOS, 2>), gt, sh, st, S(ST_TRACE), ka);
+  } else if (ppm_type == 1) {
     const int gt = (int)stage_blocks(ST_TRACE, j0, -2, 0, 0);
     const size_t sh = (size_t)trace_cells_of(&j0, 1) * (5 + EOS::NS) * sizeof(double);
     LAUNCH_SH((k_trace_ppm_lds<EOS, 0>), gt, sh, st, S(ST_TRACE), ka);
@@ -444,9 +471,11 @@ static void run_ctu(const Job& j0, const Sched& s1, const KArgs& ka,
     LAUNCH((k_trace_plm<EOS, 1>), G(ST_TRACE), st, S(ST_TRACE), ka);
     LAUNCH((k_trace_plm<EOS, 2>), G(ST_TRACE), st, S(ST_TRACE), ka);
   }
-  LAUNCH((k_flux1<EOS, 0>), G(ST_FLUX1_X), st, S(ST_FLUX1_X), ka);
-  LAUNCH((k_flux1<EOS, 1>), G(ST_FLUX1_Y), st, S(ST_FLUX1_Y), ka);
-  LAUNCH((k_flux1<EOS, 2>), G(ST_FLUX1_Z), st, S(ST_FLUX1_Z), ka);
+  if (!fused) {
+    LAUNCH((k_flux1<EOS, 0>), G(ST_FLUX1_X), st, S(ST_FLUX1_X), ka);
+    LAUNCH((k_flux1<EOS, 1>), G(ST_FLUX1_Y), st, S(ST_FLUX1_Y), ka);
+    LAUNCH((k_flux1<EOS, 2>), G(ST_FLUX1_Z), st, S(ST_FLUX1_Z), ka);
+  }
   LAUNCH((k_trans<EOS, 0, 1>), G(ST_TRANS_XY), st, S(ST_TRANS_XY), ka);
   LAUNCH((k_trans<EOS, 0, 2>), G(ST_TRANS_XZ), st, S(ST_TRANS_XZ), ka);
   LAUNCH((k_trans<EOS, 1, 0>), G(ST_TRANS_YX), st, S(ST_TRANS_YX), ka);
@@ -1052,6 +1081,7 @@ struct pc_level {
   int mixed = 0;               // 1: FP32 reconstruction/Riemann/intermediates
   bool mol = false;            // method-of-lines step (do_mol)
   int trace_cells = 0;         // largest LDS trace window of the boxes
+  bool fuse = false;           // PPM trace and first fluxes in one kernel
   bool laxf = false;           // MOL with the Lax-Friedrichs flux
   double tuned_ms[ST_COUNT] = {};
   // hipGraph of gsteps consecutive steps, replayed by pc_level_step_v1
@@ -1074,7 +1104,7 @@ static void plan_free(StagePlan& P) {
 static bool plan_make(pc_level* L, int s, int lx, int ly, int lz, int cap,
                       StagePlan& P) {
   // the PPM trace always runs on its LDS tiles (only persistence varies)
-  if (s == ST_TRACE && L->prm.ppm_type == 1 && !L->mol) lx = -2;
+  if (s == ST_TRACE && L->prm.ppm_type == 1 && !L->mol) lx = L->fuse ? -3 : -2;
   const int nb = (int)L->hjobs.size();
   std::vector<int> start(nb), map;
   for (int b = 0; b < nb; ++b) {
@@ -1146,7 +1176,12 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
     case ST_CTOPRIM: LAUNCH((k_ctoprim<EOS, FP, P>), grid, st, sc, ka); break;
     case ST_DIVU: LAUNCH((k_divu<FP, P>), grid, st, sc, ka); break;
     case ST_TRACE:
-      if (L->prm.ppm_type == 1) {
+      if (L->prm.ppm_type == 1 && L->fuse) {
+        const size_t sh = pencil_lds_bytes<EOS::NS, FP>(L->trace_cells);
+        if (tr(0)) LAUNCH_SH((k_trace_flux1_lds<EOS, 0, FP, P>), grid, sh, st, sc, ka);
+        if (tr(1)) LAUNCH_SH((k_trace_flux1_lds<EOS, 1, FP, P>), grid, sh, st, sc, ka);
+        if (tr(2)) LAUNCH_SH((k_trace_flux1_lds<EOS, 2, FP, P>), grid, sh, st, sc, ka);
+      } else if (L->prm.ppm_type == 1) {
         const size_t sh = (size_t)L->trace_cells * (5 + EOS::NS) * sizeof(FP);
         if (tr(0)) LAUNCH_SH((k_trace_ppm_lds<EOS, 0, FP, P>), grid, sh, st, sc, ka);
         if (tr(1)) LAUNCH_SH((k_trace_ppm_lds<EOS, 1, FP, P>), grid, sh, st, sc, ka);
@@ -1157,9 +1192,10 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
         if (tr(2)) LAUNCH((k_trace_plm<EOS, 2, FP, P>), grid, st, sc, ka);
       }
       break;
-    case ST_FLUX1_X: LAUNCH((k_flux1<EOS, 0, FP, P>), grid, st, sc, ka); break;
-    case ST_FLUX1_Y: LAUNCH((k_flux1<EOS, 1, FP, P>), grid, st, sc, ka); break;
-    case ST_FLUX1_Z: LAUNCH((k_flux1<EOS, 2, FP, P>), grid, st, sc, ka); break;
+    // with the fused PPM trace the first fluxes come from the trace stage
+    case ST_FLUX1_X: if (!L->fuse) LAUNCH((k_flux1<EOS, 0, FP, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_Y: if (!L->fuse) LAUNCH((k_flux1<EOS, 1, FP, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_Z: if (!L->fuse) LAUNCH((k_flux1<EOS, 2, FP, P>), grid, st, sc, ka); break;
     case ST_TRANS_XY: LAUNCH((k_trans<EOS, 0, 1, FP, P>), grid, st, sc, ka); break;
     case ST_TRANS_XZ: LAUNCH((k_trans<EOS, 0, 2, FP, P>), grid, st, sc, ka); break;
     case ST_TRANS_YX: LAUNCH((k_trans<EOS, 1, 0, FP, P>), grid, st, sc, ka); break;
@@ -1323,7 +1359,8 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   }
   // per-stage launch plans: runs of BLOCK cells, one workgroup each
   L->hjobs = jobs;
-  L->trace_cells = trace_cells_of(jobs.data(), (int)jobs.size());
+  L->fuse = !L->mol && prm->ppm_type == 1 && pencils_fit(jobs.data(), (int)jobs.size());
+  L->trace_cells = trace_cells_of(jobs.data(), (int)jobs.size(), L->fuse);
   for (int s = 0; s < ST_COUNT; ++s) {
     if (!plan_make(L, s, -1, 0, 0, 0, L->plans[s])) {
       level_free(L);
@@ -2432,6 +2469,7 @@ struct pc_batch {
   int mixed = 0;
   bool mol = false;       // method-of-lines steps (do_mol)
   int trace_cells = 0;    // largest LDS trace window of the stepped set
+  bool fuse = false;      // PPM trace and first fluxes in one kernel
   bool laxf = false;      // MOL with the Lax-Friedrichs flux
   KArgs ka;
   LevelArgs la;
@@ -2480,10 +2518,12 @@ static int batch_tables(pc_batch* B, const std::vector<int>& ids,
   std::vector<Job> jobs(n);
   std::vector<int> start((size_t)ST_COUNT * B->max_slots, 0);
   for (int k = 0; k < n; ++k) jobs[k] = B->slots[ids[k]].job;
-  B->trace_cells = trace_cells_of(jobs.data(), n);
+  B->fuse = !B->mol && B->prm.ppm_type == 1 && pencils_fit(jobs.data(), n);
+  B->trace_cells = trace_cells_of(jobs.data(), n, B->fuse);
   for (int s = 0; s < ST_COUNT; ++s) {
     int tot = 0;
-    const int lx = (s == ST_TRACE && B->prm.ppm_type == 1 && !B->mol) ? -2 : -1;
+    const int lx = (s == ST_TRACE && B->prm.ppm_type == 1 && !B->mol)
+                       ? (B->fuse ? -3 : -2) : -1;
     for (int k = 0; k < n; ++k) {
       start[(size_t)s * B->max_slots + k] = tot;
       tot += (int)stage_blocks(s, jobs[k], lx, 0, 0);

@@ -66,72 +66,24 @@ __host__ __device__ inline long long trace_lds_blocks(const int n[3]) {
   return m;
 }
 
+// PPM states of one cell from a staged window (component m of window cell w
+// at lds[m * nwin + w]; rho, u_n, p, u_t, u_tt, then the species): the plus
+// state P on the cell's low D-face and the minus state M on its high D-face
+// (Source/PPM.cpp:82-423; the reference also traces rho e, a profile it
+// never uses).  wc: the cell's window index, wstep: one cell along D.
 template <class EOS, int D, class FP>
-PCD void b_trace_ppm_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
+PCD void ppm_cell_states(const FP* lds, int nwin, int wc, int wstep, FP flatv,
+                         FP dtdx, FaceState<EOS::NS, FP>& P,
+                         FaceState<EOS::NS, FP>& M) {
   constexpr int NS = EOS::NS;
-  int local;
-  const JobT<FP> J = load_job<FP>(sc, lb_, local);
-  int lo[3], n[3];
-  stage_region(ST_TRACE, J.lo, J.hi, lo, n);
-  const TraceTile T = trace_tile(D, n);
-  constexpr int AB = (D == 2) ? 2 : 1;  // axis b
-  constexpr int AC = 3 - AB;            // axis c
-  const int tile = local / BLOCK, t = local % BLOCK;
-  if (tile >= T.nta * T.ntb * T.nc) return;  // whole workgroup: no barrier crossed
-  const int ia = tile % T.nta, r = tile / T.nta;
-  const int ib = r % T.ntb, ic = r / T.ntb;
-  // cell of this thread
-  const int ta_ = t % T.ta, tb_ = t / T.ta;
-  int ijk[3];
-  ijk[0] = lo[0] + ia * T.ta + ta_;
-  ijk[AB] = lo[AB] + ib * T.tb + tb_;
-  ijk[AC] = lo[AC] + ic;
-  const bool active = tb_ < T.tb && ijk[0] < lo[0] + n[0] && ijk[AB] < lo[AB] + n[AB];
-  const int i = ijk[0], j = ijk[1], k = ijk[2];
-  // staging window: the tile plus 2 cells each side along D
-  constexpr int HA = (D == 0) ? 2 : 0, HB = (D == 0) ? 0 : 2;
-  const int wa = T.ta + 2 * HA, wb = T.tb + 2 * HB;
-  const int nwin = wa * wb;
-  const int a0 = lo[0] + ia * T.ta - HA, b0 = lo[AB] + ib * T.tb - HB;
-  const VB<FP> q = bufview(J.q);
-  const int qlo[3] = {J.q.b0, J.q.b1, J.q.b2};
-  // every component the stencils read, staged once: rho, u_n, p, u_t, u_tt,
-  // then the species (the reference also traces rho e, but that profile is
-  // never used); component m at lds[m * nwin + w]
-  constexpr int QUN = QU + D;
   constexpr int QUT = (D == 0) ? QV : ((D == 1) ? QW : QU);
   constexpr int QUTT = (D == 0) ? QW : ((D == 1) ? QU : QV);
-  for (int w = threadIdx.x; w < nwin; w += BLOCK) {
-    const int wa_ = w % wa, wb_ = w / wa;
-    int g[3];
-    g[0] = a0 + wa_;
-    g[AB] = b0 + wb_;
-    g[AC] = lo[AC] + ic;
-    const bool in = g[0] >= qlo[0] && g[0] < J.qe[0] && g[1] >= qlo[1] &&
-                    g[1] < J.qe[1] && g[2] >= qlo[2] && g[2] < J.qe[2];
-    if (in) {
-      lds[0 * nwin + w] = q(g[0], g[1], g[2], QRHO);
-      lds[1 * nwin + w] = q(g[0], g[1], g[2], QUN);
-      lds[2 * nwin + w] = q(g[0], g[1], g[2], QPRES);
-      lds[3 * nwin + w] = q(g[0], g[1], g[2], QUT);
-      lds[4 * nwin + w] = q(g[0], g[1], g[2], QUTT);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) lds[(5 + s) * nwin + w] = q(g[0], g[1], g[2], QFS + s);
-    }
-  }
-  __syncthreads();
-  if (!active) return;
-  // window index of this thread's cell; a stencil point m cells along D
-  const int wc = (ta_ + HA) + wa * (tb_ + HB);
-  const int wstep = (D == 0) ? 1 : wa;
   FP sm, sp, s6;
-  const FP dtdx = get_dt(J, ka) / J.dx[D];
   FP Yc[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) Yc[s] = lds[(5 + s) * nwin + wc];
   const FP cc = EOS::RPY2Cs(lds[wc], lds[2 * nwin + wc], Yc);
   const FP un = lds[1 * nwin + wc];
-  const FP flatv = ka.use_flattening ? FP(J.flat(i, j, k, 0)) : FP(1.0);
   const FP spd0 = un - cc, spd1 = un, spd2 = un + cc;
   auto rc = [&](int slot) {
     FP s[5];
@@ -163,7 +115,6 @@ PCD void b_trace_ppm_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
   rc(4);
   const FP Im_tt1 = ppm_im(sm, sp, s6, spd1, dtdx);
   const FP Ip_tt1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  FaceState<NS, FP> P, M;
   FP YrefP[NS], YrefM[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -246,6 +197,76 @@ PCD void b_trace_ppm_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
     const FP eint = EOS::RYP2E(M.rho, M.Y, M.p);
     M.rhoe = M.rho * eint;
   }
+}
+
+// Stage the 5 + NS components the PPM stencils read for the window cells
+// [0, nwin): window cell w is (a0 + w % wa, b0 + w / wa) on axes (x, AB) at
+// index c on the third axis; cells outside the primitive box are skipped.
+template <int NS, int D, int AB, class FP>
+PCD void stage_ppm_window(const JobT<FP>& J, FP* lds, int nwin, int wa, int a0,
+                          int b0, int c) {
+  constexpr int AC = 3 - AB;
+  constexpr int QUN = QU + D;
+  constexpr int QUT = (D == 0) ? QV : ((D == 1) ? QW : QU);
+  constexpr int QUTT = (D == 0) ? QW : ((D == 1) ? QU : QV);
+  const VB<FP> q = bufview(J.q);
+  const int qlo[3] = {J.q.b0, J.q.b1, J.q.b2};
+  for (int w = threadIdx.x; w < nwin; w += BLOCK) {
+    const int wa_ = w % wa, wb_ = w / wa;
+    int g[3];
+    g[0] = a0 + wa_;
+    g[AB] = b0 + wb_;
+    g[AC] = c;
+    const bool in = g[0] >= qlo[0] && g[0] < J.qe[0] && g[1] >= qlo[1] &&
+                    g[1] < J.qe[1] && g[2] >= qlo[2] && g[2] < J.qe[2];
+    if (in) {
+      lds[0 * nwin + w] = q(g[0], g[1], g[2], QRHO);
+      lds[1 * nwin + w] = q(g[0], g[1], g[2], QUN);
+      lds[2 * nwin + w] = q(g[0], g[1], g[2], QPRES);
+      lds[3 * nwin + w] = q(g[0], g[1], g[2], QUT);
+      lds[4 * nwin + w] = q(g[0], g[1], g[2], QUTT);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) lds[(5 + s) * nwin + w] = q(g[0], g[1], g[2], QFS + s);
+    }
+  }
+}
+
+template <class EOS, int D, class FP>
+PCD void b_trace_ppm_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
+  constexpr int NS = EOS::NS;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3];
+  stage_region(ST_TRACE, J.lo, J.hi, lo, n);
+  const TraceTile T = trace_tile(D, n);
+  constexpr int AB = (D == 2) ? 2 : 1;  // axis b
+  constexpr int AC = 3 - AB;            // axis c
+  const int tile = local / BLOCK, t = local % BLOCK;
+  if (tile >= T.nta * T.ntb * T.nc) return;  // whole workgroup: no barrier crossed
+  const int ia = tile % T.nta, r = tile / T.nta;
+  const int ib = r % T.ntb, ic = r / T.ntb;
+  // cell of this thread
+  const int ta_ = t % T.ta, tb_ = t / T.ta;
+  int ijk[3];
+  ijk[0] = lo[0] + ia * T.ta + ta_;
+  ijk[AB] = lo[AB] + ib * T.tb + tb_;
+  ijk[AC] = lo[AC] + ic;
+  const bool active = tb_ < T.tb && ijk[0] < lo[0] + n[0] && ijk[AB] < lo[AB] + n[AB];
+  const int i = ijk[0], j = ijk[1], k = ijk[2];
+  // staging window: the tile plus 2 cells each side along D
+  constexpr int HA = (D == 0) ? 2 : 0, HB = (D == 0) ? 0 : 2;
+  const int wa = T.ta + 2 * HA, wb = T.tb + 2 * HB;
+  const int nwin = wa * wb;
+  const int a0 = lo[0] + ia * T.ta - HA, b0 = lo[AB] + ib * T.tb - HB;
+  stage_ppm_window<NS, D, AB, FP>(J, lds, nwin, wa, a0, b0, lo[AC] + ic);
+  __syncthreads();
+  if (!active) return;
+  const int wc = (ta_ + HA) + wa * (tb_ + HB);
+  const int wstep = (D == 0) ? 1 : wa;
+  const FP dtdx = get_dt(J, ka) / J.dx[D];
+  const FP flatv = ka.use_flattening ? FP(J.flat(i, j, k, 0)) : FP(1.0);
+  FaceState<NS, FP> P, M;
+  ppm_cell_states<EOS, D, FP>(lds, nwin, wc, wstep, flatv, dtdx, P, M);
   const auto& tr = J.tr[D];
   store_trace<NS, FP>(tr, i, j, k, 1, P);
   int i1 = i, j1 = j, k1 = k;
@@ -267,6 +288,161 @@ __global__ void __launch_bounds__(BLOCK)
     }
   } else {
     b_trace_ppm_lds<EOS, D, FP>(sc, ka, first_block(), lds);
+  }
+}
+
+// ------------------------------------------ fused trace + first fluxes --
+// Pencil tiles: the whole trace-region extent along D in one workgroup (the
+// extent must be at most BLOCK), with as many neighbouring pencils as fit
+// (ta along x by tb along axis b, as trace_tile; along D the tile is the
+// full extent).  Every D-face of the first-flux region then has both of its
+// cells in the workgroup.
+__host__ __device__ inline TraceTile pencil_tile(int D, const int n[3]) {
+  TraceTile t;
+  const int b = trace_axis_b(D), c = 3 - b;
+  if (D == 0) {
+    t.ta = n[0];
+    t.tb = BLOCK / t.ta;
+    const int fit = TRACE_LDS_CELLS / (t.ta + 4);
+    if (t.tb > fit) t.tb = fit;
+    if (t.tb > n[b]) t.tb = n[b];
+  } else {
+    t.tb = n[b];
+    t.ta = BLOCK / t.tb;
+    const int fit = TRACE_LDS_CELLS / (t.tb + 4);
+    if (t.ta > fit) t.ta = fit;
+    if (t.ta > n[0]) t.ta = n[0];
+  }
+  t.nta = (n[0] + t.ta - 1) / t.ta;
+  t.ntb = (n[b] + t.tb - 1) / t.tb;
+  t.nc = n[c];
+  return t;
+}
+
+// The fused kernel applies when every extent of the trace region fits one
+// workgroup's pencil.
+__host__ __device__ inline bool pencil_fits(const int n[3]) {
+  return n[0] <= BLOCK && n[1] <= BLOCK && n[2] <= BLOCK;
+}
+
+__host__ __device__ inline long long pencil_blocks(const int n[3]) {
+  long long m = 0;
+  for (int D = 0; D < 3; ++D) {
+    const TraceTile t = pencil_tile(D, n);
+    const long long c = (long long)t.nta * t.ntb * t.nc;
+    if (c > m) m = c;
+  }
+  return m;
+}
+
+__host__ __device__ inline int pencil_cells(const int n[3]) {
+  int m = 0;
+  for (int D = 0; D < 3; ++D) {
+    const TraceTile t = pencil_tile(D, n);
+    const int c = (D == 0) ? (t.ta + 4) * t.tb : t.ta * (t.tb + 4);
+    if (c > m) m = c;
+  }
+  return m;
+}
+
+// PPM trace of direction D and, from the traced states still in the
+// workgroup, the first fluxes on the D-faces (Source/Godunov.cpp:182-242
+// after the trace of :138-204): the plus state P of a cell stays in its
+// thread, the minus state M goes through LDS to the thread of the next cell
+// along D, whose thread solves the face between them.  Traces are stored for
+// the transverse and final stages as by b_trace_ppm_lds; fluxes as b_flux1.
+// LDS: the staging window, then reused for M (QT components per tile cell).
+template <class EOS, int D, class FP>
+PCD void b_trace_flux1_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
+  constexpr int NS = EOS::NS;
+  constexpr int QT = 6 + NS;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3];
+  stage_region(ST_TRACE, J.lo, J.hi, lo, n);
+  const TraceTile T = pencil_tile(D, n);
+  constexpr int AB = (D == 2) ? 2 : 1;  // axis b
+  constexpr int AC = 3 - AB;            // axis c
+  const int tile = local / BLOCK, t = local % BLOCK;
+  if (tile >= T.nta * T.ntb * T.nc) return;  // whole workgroup: no barrier crossed
+  const int ia = tile % T.nta, r = tile / T.nta;
+  const int ib = r % T.ntb, ic = r / T.ntb;
+  const int ta_ = t % T.ta, tb_ = t / T.ta;
+  int ijk[3];
+  ijk[0] = lo[0] + ia * T.ta + ta_;
+  ijk[AB] = lo[AB] + ib * T.tb + tb_;
+  ijk[AC] = lo[AC] + ic;
+  const bool active = tb_ < T.tb && ijk[0] < lo[0] + n[0] && ijk[AB] < lo[AB] + n[AB];
+  const int i = ijk[0], j = ijk[1], k = ijk[2];
+  constexpr int HA = (D == 0) ? 2 : 0, HB = (D == 0) ? 0 : 2;
+  const int wa = T.ta + 2 * HA, wb = T.tb + 2 * HB;
+  const int nwin = wa * wb;
+  stage_ppm_window<NS, D, AB, FP>(J, lds, nwin, wa, lo[0] + ia * T.ta - HA,
+                                  lo[AB] + ib * T.tb - HB, lo[AC] + ic);
+  __syncthreads();
+  FaceState<NS, FP> P, M;
+  if (active) {
+    const int wc = (ta_ + HA) + wa * (tb_ + HB);
+    const int wstep = (D == 0) ? 1 : wa;
+    const FP dtdx = get_dt(J, ka) / J.dx[D];
+    const FP flatv = ka.use_flattening ? FP(J.flat(i, j, k, 0)) : FP(1.0);
+    ppm_cell_states<EOS, D, FP>(lds, nwin, wc, wstep, flatv, dtdx, P, M);
+    const auto& tr = J.tr[D];
+    store_trace<NS, FP>(tr, i, j, k, 1, P);
+    int i1 = i, j1 = j, k1 = k;
+    step<D>(i1, j1, k1, 1);
+    store_trace<NS, FP>(tr, i1, j1, k1, 0, M);
+  }
+  __syncthreads();  // the window is read: reuse the buffer for M
+  const int ncell = T.ta * T.tb;
+  const int me = ta_ + T.ta * tb_;
+  if (active) {
+    lds[0 * ncell + me] = M.rho;
+    lds[1 * ncell + me] = M.u;
+    lds[2 * ncell + me] = M.v;
+    lds[3 * ncell + me] = M.w;
+    lds[4 * ncell + me] = M.p;
+    lds[5 * ncell + me] = M.rhoe;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) lds[(6 + s) * ncell + me] = M.Y[s];
+  }
+  __syncthreads();
+  // this cell's low D-face; the region's first cell along D has none
+  const int pd = (D == 0) ? ta_ : tb_;
+  if (!active || pd == 0) return;
+  const int prev = (D == 0) ? me - 1 : me - T.ta;
+  FaceState<NS, FP> Lf;
+  Lf.rho = lds[0 * ncell + prev];
+  Lf.u = lds[1 * ncell + prev];
+  Lf.v = lds[2 * ncell + prev];
+  Lf.w = lds[3 * ncell + prev];
+  Lf.p = lds[4 * ncell + prev];
+  Lf.rhoe = lds[5 * ncell + prev];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Lf.Y[s] = lds[(6 + s) * ncell + prev];
+  int im = i, jm = j, km = k;
+  step<D>(im, jm, km, -1);
+  const FP cav = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  Flux<NS, FP> F;
+  cmpflx<EOS, D>(Lf, P, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
+  store_flux<NS, FP>(J.f1[D], i, j, k, F);
+  (void)QT;
+}
+
+// Dynamic LDS: max((5 + NS) * pencil_cells, (6 + NS) * BLOCK) elements of FP.
+template <class EOS, int D, class FP = double, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_trace_flux1_lds(Sched sc, KArgs ka) {
+  extern __shared__ __align__(16) unsigned char pc_trace_dyn[];
+  FP* lds = (FP*)pc_trace_dyn;
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x) {
+      b_trace_flux1_lds<EOS, D, FP>(sc, ka, lb_, lds);
+      __syncthreads();  // the next tile restages the buffer
+    }
+  } else {
+    b_trace_flux1_lds<EOS, D, FP>(sc, ka, first_block(), lds);
   }
 }
 
