@@ -49,10 +49,11 @@ FP32_PEAK = 157.3e12
 MOL_LAUNCHES = {"ghost": 2, "ctoprim": 2, "mol_x": 2, "mol_y": 2, "mol_z": 2}
 
 
-def stage_bytes(nspec, boxes, mol=False):
+def stage_bytes(nspec, boxes, mol=False, fused=True):
     """Compulsory bytes of one launch of each kernel over the level: every
     input element read once and every output element written once on the
-    kernel's region (stencil re-reads are not counted)."""
+    kernel's region (stencil re-reads are not counted).  fused: the PPM
+    trace launch also forms the first fluxes (k_trace_flux1_lds)."""
     Q, N, QT, FT = 8 + nspec, 7 + nspec, 6 + nspec, 7 + nspec
     tot = {}
 
@@ -89,7 +90,10 @@ def stage_bytes(nspec, boxes, mol=False):
             add("trace_" + a, g1, QT + 2 * QT)
             f1 = list(t)
             f1[d] = n[d] + 1
-            add("flux1_" + a, f1[0] * f1[1] * f1[2], 2 * QT + 1 + FT)
+            if fused:  # the faces' states come from the trace, not from HBM
+                add("trace_" + a, f1[0] * f1[1] * f1[2], 1 + FT)
+            else:
+                add("flux1_" + a, f1[0] * f1[1] * f1[2], 2 * QT + 1 + FT)
             fin = list(n)
             fin[d] = n[d] + 1
             add("final_" + a, fin[0] * fin[1] * fin[2], 2 * QT + 2 * FT + 2 * N + 5)
@@ -110,10 +114,25 @@ def profiled_traffic(summary, stage):
     path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
+    col = None
     for line in open(path):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) > 6 and cols[1] == stage:
-            return float(cols[6]) * 1e6
+        if "HBM MB" in cols and "stage" in cols:
+            col = cols.index("HBM MB")
+        elif col is not None and len(cols) > col and cols[1] == stage:
+            return float(cols[col]) * 1e6
+    return None
+
+
+def profiled_step_traffic(summary):
+    """Counted HBM bytes of one level step from the same summary, or None."""
+    path = os.path.join(ROOT, "profiles", summary)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("Sum over one step") and "HBM traffic" in line:
+            tok = line.split("HBM traffic ")[1].split(" GB")[0]
+            return float(tok) * 1e9
     return None
 
 
@@ -342,6 +361,84 @@ def dropin(args):
     print(json.dumps(out))
 
 
+def forms(args, ctx, w):
+    """The forms beside the plain FP64 level step, measured in the same run:
+    mixed precision on the same level (lockstep against FP64, error norms),
+    the ragged 300-box level (FP64, branch streams) and a serving run of the
+    ensemble generator (sides 32-128, 10-200 steps, mixed precision)."""
+    import torch
+    from particleclem_amd import workloads
+    from particleclem_amd.hydro import DeviceLevel, HydroContext
+    out = {}
+    t0 = time.perf_counter()
+    pc = precision_check(ctx, w, [1, 10, 50], not args.no_cpu_baseline)
+    cells = w.valid_cells()
+    out["mixed"] = {
+        "workload": "the same premixed 128^3 level, FP32 reconstruction, "
+                    "limiters, Riemann solves and intermediates, FP64 EOS "
+                    "inversion, internal energy and update; FP64 and mixed "
+                    "levels stepped in lockstep from the initial state, each "
+                    "with its own CFL dt, synchronised at steps 1, 10, 50",
+        "fp64_ms_per_step": pc["fp64_ms_per_step"],
+        "mixed_ms_per_step": pc["mixed_ms_per_step"],
+        "mixed_value": cells / (pc["mixed_ms_per_step"] * 1e-3) / 1e6,
+        "speedup": pc["mixed_speedup"],
+        "errors_vs_fp64": pc["errors"], "wall_s": time.perf_counter() - t0}
+    # ragged level (c2)
+    t0 = time.perf_counter()
+    wr = workloads.ragged(nboxes=300, nsteps=args.forms_steps)
+    cr = HydroContext(wr.nspec)
+    lr = DeviceLevel.from_workload(cr, wr)
+    lr.set_branches(True)
+    lr.init_dt()
+    lr.step(2)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    lr.step(args.forms_steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - ts
+    rc, bad, small = lr.status()
+    rcells = wr.valid_cells()
+    lr.close()
+    cr.close()
+    out["ragged"] = {
+        "workload": "one AMR level as a ragged table of 300 boxes, sides from "
+                    "{8,16,24,32,48,64} with probabilities (0.1,0.2,0.2,0.25,"
+                    "0.15,0.1), each its own periodic domain, premixed 9-species "
+                    "generator, FP64, PPM, one launch per stage over all boxes, "
+                    "branch streams",
+        "valid_cells": rcells, "steps": args.forms_steps,
+        "ms_per_step": sec / args.forms_steps * 1e3,
+        "value": rcells * args.forms_steps / sec / 1e6,
+        "hbm_frac_step": rcells * 2 * wr.nvar * 8 / (sec / args.forms_steps) / HBM_PEAK,
+        "status": rc, "wall_s": time.perf_counter() - t0}
+    # serving (c4)
+    from particleclem_amd import serving as sv
+    t0 = time.perf_counter()
+    reqs = sv.request_stream(args.serving_requests, args.serving_rate)
+    srv = sv.Server(mixed=True)
+    warm = sv.request_stream(4, 0, seed=1, n_lo=16, n_hi=16, n_med=16, steps=(3, 3))
+    srv.run(warm, realtime=False)
+    served = srv.run(reqs, realtime=True)
+    summ = sv.summarize(served, srv.wall)
+    budget = srv.budget
+    srv.close()
+    out["serving"] = {
+        "workload": f"{args.serving_requests} requests of the ensemble generator "
+                    f"(half 3D shock tubes, gamma-law, pressure ratio 2-20; half "
+                    f"premixed H2/air slabs, 9 species, phi 0.5-1.2; sides "
+                    f"lognormal in [32, 128], median 64; 10-200 steps), Poisson "
+                    f"arrivals at {args.serving_rate}/s, mixed precision, PPM, "
+                    f"continuous batching under an 80% HBM budget "
+                    f"({budget / 2**30:.0f} GiB), backfilling admission",
+        "requests": summ["requests"], "rejected": summ["rejected"],
+        "failed": sum(1 for r in served if r.status not in (0, None) and not r.rejected),
+        "value": summ["mcell_updates_per_s"], "unit": "Mcell-updates/s",
+        "latency_ms": summ["latency_ms"], "queue_ms": summ["queue_ms"],
+        "wall_s": time.perf_counter() - t0}
+    return out
+
+
 def host_cpu():
     model = "unknown"
     try:
@@ -422,6 +519,17 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="premixed default workload: skip the three forms "
+                         "measured after the main line (mixed precision, the "
+                         "ragged 300-box level, a serving run)")
+    ap.add_argument("--forms-steps", type=int, default=10,
+                    help="timed steps of the ragged-level form")
+    ap.add_argument("--serving-requests", type=int, default=200,
+                    help="requests of the serving form (the ensemble "
+                         "generator: sides 32-128, 10-200 steps)")
+    ap.add_argument("--serving-rate", type=float, default=50.0,
+                    help="Poisson arrival rate of the serving form (req/s)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (the "
                          "reference kernel for scaling HBM counters)")
@@ -534,7 +642,8 @@ def main():
     torch.cuda.synchronize()
     kt = lev.kernel_times()
     lev.enable_timing(False)
-    kbytes = stage_bytes(w.nspec, w.boxes, mol=args.mol)
+    fused = (not args.mol) and w.params.get("ppm_type", 1) == 1 and kt.get("flux1_x", 1.0) < 0.02
+    kbytes = stage_bytes(w.nspec, w.boxes, mol=args.mol, fused=fused)
     launches = MOL_LAUNCHES if args.mol else {}
     dom = max(kt, key=lambda k: kt[k])
     dom_ms = kt[dom] / launches.get(dom, 1)   # average launch duration
@@ -545,16 +654,31 @@ def main():
             and not args.autotune and not args.laxf):
         traffic = profiled_traffic(summary, dom)
     kname = ("k_mol_update" if dom.startswith("mol_update") else
-             "k_mol_flux" if dom.startswith("mol_") else "k_" + dom.split("_")[0])
-    roof = {"bound": "hbm", "kernel": f"{kname} ({dom})",
-            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "traffic_source": (f"profiles/{summary}" if traffic else None),
-            "bytes_per_launch": kbytes[dom], "ms_per_launch": dom_ms,
-            "launches_per_step": launches.get(dom, 1)}
+             "k_mol_flux" if dom.startswith("mol_") else
+             "k_trace_flux1_lds" if dom.startswith("trace_") and kt.get("flux1_x", 1) < 0.02
+             else "k_" + dom.split("_")[0])
+    # the dominant kernel against HBM on its own stage bytes (every input
+    # element of its region read once, every output written once)
+    roof_kernel = {"bound": "hbm", "kernel": f"{kname} ({dom})",
+                   "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                   "frac": achieved / HBM_PEAK, "traffic": traffic,
+                   "traffic_source": (f"profiles/{summary}" if traffic else None),
+                   "bytes_per_launch": kbytes[dom], "ms_per_launch": dom_ms,
+                   "launches_per_step": launches.get(dom, 1)}
     nvar = w.nvar
+    # the step against HBM on the algorithmic bytes of the workload, B_alg =
+    # 2 NVAR 8 bytes per cell-update (read U^n, write U^n+1 once)
     step_bytes = cells * 2 * nvar * 8
     hbm_frac_step = step_bytes / (ms_step * 1e-3) / HBM_PEAK
+    step_traffic = None
+    if traffic is not None:
+        step_traffic = profiled_step_traffic(summary)
+    roof = {"bound": "hbm", "kernel": "one level step (every launch of the step)",
+            "achieved": step_bytes / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": hbm_frac_step, "traffic": step_traffic,
+            "traffic_source": (f"profiles/{summary}" if step_traffic else None),
+            "bytes_per_launch": step_bytes, "ms_per_launch": ms_step,
+            "alg_bytes_per_cell_update": 2 * nvar * 8}
     fp64 = fp32 = None
     if traffic is not None:
         fl = profiled_step_flops(summary)
@@ -606,6 +730,7 @@ def main():
                    "branch_streams": bool(args.branches),
                    "plans": plans},
         "roofline": roof,
+        "roofline_dominant_kernel": roof_kernel,
         "hbm_frac_step": hbm_frac_step,
         "fp64_step": fp64,
         "fp32_step": fp32,
@@ -617,6 +742,10 @@ def main():
         out["precision_check"] = precision_check(
             ctx, w, sorted(int(x) for x in args.error_steps.split(",")),
             not args.no_cpu_baseline)
+    default_line = (args.workload == "premixed" and n == 128 and args.box == 32
+                    and not args.mol and args.precision == "fp64")
+    if default_line and not args.no_forms:
+        out["forms"] = forms(args, ctx, w)
     print(json.dumps(out))
     ctx.close()
 
