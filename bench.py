@@ -451,6 +451,47 @@ def host_cpu():
     return model, os.cpu_count()
 
 
+_NATIVE_CHECK = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[2])
+sys.path.insert(0, sys.argv[2] + "/tests")
+from particleclem_amd import workloads as wl
+from parity_util import run_oracle
+w = wl.premixed(n=16, box=8, nsteps=2)
+r, dts = run_oracle(w, 2)
+np.save(sys.argv[1], np.concatenate([x.ravel() for x in r] + [np.array(dts)]))
+"""
+
+
+def native_oracle():
+    """Build the oracle for this host (-O3 -march=native, oracle/Makefile
+    `native`) and check it bitwise against the portable build on a small
+    premixed level (two steps); returns (path or None, note)."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    odir = os.path.join(ROOT, "oracle")
+    so = os.path.join(odir, "liboracle_native.so")
+    r = subprocess.run(["make", "-s", "-C", odir, "native"], capture_output=True, text=True)
+    if r.returncode != 0 or not os.path.exists(so):
+        return None, "native build failed: " + r.stderr[-200:]
+    with tempfile.TemporaryDirectory() as td:
+        script = os.path.join(td, "check.py")
+        open(script, "w").write(_NATIVE_CHECK)
+        outs = []
+        for lib in (os.path.join(odir, "liboracle.so"), so):
+            out = os.path.join(td, "o%d" % len(outs))
+            env = dict(os.environ, PC_ORACLE_SO=lib)
+            if subprocess.run([sys.executable, script, out, ROOT], env=env,
+                              capture_output=True).returncode != 0:
+                return None, "native check failed to run"
+            outs.append(np.load(out + ".npy"))
+    if not np.array_equal(outs[0], outs[1]):
+        return None, "native build differs from the portable one: not used"
+    return so, "-O3 -march=native, bitwise equal to the portable build on a 2-step premixed check"
+
+
 def cpu_baseline(w, gpu_fabs, dt, steps, threads, reps=3):
     """The oracle stepping the same level from the same state and dt:
     median over `reps` timings of `steps` steps each."""
@@ -696,6 +737,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+        nso, build = native_oracle()
+        if nso and "oracle_bind" not in sys.modules:
+            os.environ["PC_ORACLE_SO"] = nso
+        else:
+            build = "portable -O3 -mfma (" + build + ")"
         v, s = cpu_baseline(w, fabs0, dt_cpu, args.cpu_steps, threads)
         model, ncpu = host_cpu()
         # the oracle's Godunov step spreads tiles of full x by 8 by 8 cells
@@ -714,7 +760,7 @@ def main():
                          f"({cells} cells) from the GPU run's state and dt, "
                          f"median of 3 timings ({s:.2f} s), OpenMP over "
                          f"{units} {what}, {threads} threads ({used} busy)",
-               "host_cpu": model, "host_logical_cpus": ncpu}
+               "host_cpu": model, "host_logical_cpus": ncpu, "build": build}
 
     out = {
         "metric": METRIC, "value": value, "unit": "Mcell-updates/s",

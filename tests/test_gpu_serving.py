@@ -156,3 +156,46 @@ def test_backfill_admission_matches_first_come_results(cuda, max_slots):
     print(f"budget {budget >> 20} MiB, {max_slots} slots: ticks fcfs {ticks['fcfs']}, backfill "
           f"{ticks['backfill']}; admission order fcfs {order('fcfs')}, backfill "
           f"{order('backfill')}")
+
+
+def test_ensemble_generator_shape_alone_equals_mix_under_budget(cuda):
+    """Requests of the ensemble workload's own shape: 16 draws of the seeded
+    generator (half gamma-law shock tubes with pressure ratio 2-20, half
+    9-species premixed slabs with phi 0.5-1.2, sides lognormal in [32, 128]
+    with median 64), mixed precision, step counts capped at 12 (the
+    generator draws 10-200).  Served together under 80% of an arena that
+    holds about five of the largest requests (so admission queues and
+    backfills), no tick uses more than that budget, and every request's
+    result is bitwise the one it gets served alone."""
+    reqs = sv.request_stream(16, rate=0, seed=20261203)
+    for r in reqs:
+        r.nsteps = min(r.nsteps, 12)
+    sides = sorted(r.n for r in reqs)
+    assert 32 <= sides[0] and sides[-1] <= 128 and sides[0] < sides[-1]
+    assert {r.kind for r in reqs} == {sv.TUBE, sv.SLAB}
+    probe = sv.Server(mixed=True, budget_bytes=64 << 20)
+    per = max(probe.batch[r.kind].slot_bytes(r.n) for r in reqs)
+    probe.close()
+    budget = int(0.8 * 5 * per)
+    srv = sv.Server(mixed=True, budget_bytes=budget, keep_results=True)
+    peak = [0]
+
+    def tick(_, active):
+        used = srv.batch[sv.SLAB].memory()[1]
+        peak[0] = max(peak[0], used)
+        assert used <= budget
+
+    mix = {r.rid: r for r in srv.run(reqs, realtime=False, on_tick=tick)}
+    ticks = srv.ticks
+    srv.close()
+    assert len(mix) == len(reqs) and all(r.status == 0 for r in mix.values())
+    for r in reqs:
+        one = sv.Request(r.rid, r.kind, r.n, r.nsteps, r.ratio, r.phi, 0.0, r.seed)
+        s1 = sv.Server(mixed=True, budget_bytes=per + (64 << 20), keep_results=True)
+        got = s1.run([one], realtime=False)[0]
+        s1.close()
+        assert got.status == 0
+        assert np.array_equal(got.result, mix[r.rid].result), (r.rid, r.kind, r.n)
+    print(f"16 generator requests (sides {sides[0]}-{sides[-1]}), budget "
+          f"{budget >> 20} MiB, peak {peak[0] >> 20} MiB, {ticks} ticks: each "
+          f"bitwise equal alone and in the mix")
