@@ -356,11 +356,16 @@ int pc_level_set_graph_v1(pc_level_t* lev, int steps_per_graph);
  * from and joined back to the stream a step is issued on, so kernels of
  * different directions overlap on the device.  Results are unchanged (each
  * kernel writes only its own direction's arrays).  Steps that time their
- * kernels (pc_level_enable_timing_v1) and steps captured into a hipGraph
- * (pc_level_set_graph_v1) stay on one stream.  No reference
- * counterpart (the reference issues its kernels on one stream).
- * Synchronous. */
+ * kernels (pc_level_enable_timing_v1) stay on one stream.  A hipGraph of
+ * pc_level_set_graph_v1 is captured on one stream and its chain of kernel
+ * nodes then given the same dependencies, so replays run the directions
+ * concurrently too.  No reference counterpart (the reference issues its
+ * kernels on one stream).  Synchronous. */
 int pc_level_set_branches_v1(pc_level_t* lev, int on);
+/* The level's current hipGraph: *nodes = its kernel nodes (0 before the
+ * first capture), *branched = 1 when it carries the branch-stream
+ * dependencies.  No reference counterpart. */
+int pc_level_graph_info_v1(const pc_level_t* lev, int* nodes, int* branched);
 /* okToContinue (Source/PeleC.cpp:1310-1340): synchronous; returns
  * PC_ERR_NAN if any step produced NaN/Inf, PC_ERR_DT if dt < dt_cutoff. */
 int pc_level_status_v1(pc_level_t* lev, int* bad, int* small_dt);

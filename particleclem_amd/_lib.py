@@ -31,7 +31,8 @@ EXPORTS = [
     "pc_level_enable_timing_v1", "pc_level_kernel_times_v1",
     "pc_level_status_v1", "pc_level_prim_to_cons_v1",
     "pc_level_autotune_v1", "pc_level_plan_v1", "pc_level_set_plan_v1",
-    "pc_level_set_graph_v1", "pc_level_set_branches_v1", "pc_level_set_precision_v1",
+    "pc_level_set_graph_v1", "pc_level_set_branches_v1", "pc_level_graph_info_v1",
+    "pc_level_set_precision_v1",
     "pc_batch_create_v1", "pc_batch_destroy_v1", "pc_batch_slot_bytes_v1",
     "pc_batch_memory_v1", "pc_batch_free_ranges_v1", "pc_batch_slot_extent_v1",
     "pc_batch_add_v1", "pc_batch_step_v1", "pc_batch_set_branches_v1",
@@ -116,6 +117,7 @@ def lib():
     L.pc_level_plan_v1.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_double)]
     L.pc_level_set_graph_v1.argtypes = [vp, C.c_int]
     L.pc_level_set_branches_v1.argtypes = [vp, C.c_int]
+    L.pc_level_graph_info_v1.argtypes = [vp, P(C.c_int), P(C.c_int)]
     L.pc_level_set_precision_v1.argtypes = [vp, C.c_int]
     ll = C.c_longlong
     L.pc_batch_create_v1.argtypes = [vp, P(Params), C.c_int, ll, vp, C.c_int, P(vp)]

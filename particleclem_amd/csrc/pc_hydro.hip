@@ -404,15 +404,31 @@ static int grid1(int st, const Job& J) {
 }
 
 
+// While a level step is captured into a hipGraph, every launch appends its
+// stage tag (stage * 8 + direction, 7 for none) to t_tags, so the captured
+// chain of kernel nodes can be rewired into the branch-stream dependencies.
+static thread_local std::vector<int>* t_tags = nullptr;
+static thread_local int t_tag = -1;
+constexpr int TAG_DT = ST_COUNT * 8 + 7;
+static inline void tag_launch() {
+  if (t_tags) t_tags->push_back(t_tag);
+}
+
 #define LAUNCH(kern, grid, stream, ...)                                      \
   do {                                                                       \
-    if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0,     \
-                                       stream, __VA_ARGS__);                 \
+    if ((grid) > 0) {                                                        \
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, stream,           \
+                         __VA_ARGS__);                                       \
+      tag_launch();                                                          \
+    }                                                                        \
   } while (0)
 #define LAUNCH_SH(kern, grid, shmem, stream, ...)                            \
   do {                                                                       \
-    if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), shmem, \
-                                       stream, __VA_ARGS__);                 \
+    if ((grid) > 0) {                                                        \
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), shmem, stream,       \
+                         __VA_ARGS__);                                       \
+      tag_launch();                                                          \
+    }                                                                        \
   } while (0)
 
 // Largest LDS trace window (cells per component) over a set of jobs, for
@@ -1082,6 +1098,8 @@ struct pc_level {
   bool mol = false;            // method-of-lines step (do_mol)
   int trace_cells = 0;         // largest LDS trace window of the boxes
   bool fuse = false;           // PPM trace and first fluxes in one kernel
+  bool gbranched = false;      // the captured graph carries the branch DAG
+  int gnodes = 0;              // kernel nodes of the captured graph
   bool laxf = false;           // MOL with the Lax-Friedrichs flux
   double tuned_ms[ST_COUNT] = {};
   // hipGraph of gsteps consecutive steps, replayed by pc_level_step_v1
@@ -1629,15 +1647,15 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st,
       }
     };
     auto run = [&](int stage, int id, int sub = -1) {
+      t_tag = stage * 8 + (sub < 0 ? 7 : sub);
       launch_stage<EOS>(L, stage, S[stage], P[stage], st, sub);
       mark(id);
     };
     mark(-1);
     // Branch mode (not while timing kernels one by one): the stages of the
     // three directions that do not depend on each other run on three
-    // streams.  Graph captures stay on one stream: a capture forked across
-    // the branch streams crashed the process (host segmentation fault in the
-    // step call; cause not found), so that path is off.
+    // streams.  A graph capture stays on one stream; the captured chain is
+    // rewired into the same dependencies afterwards (rewire_branches).
     const bool brm = L->brs.on && !tm && !capturing;
     if (L->mol) {
       for (int h = 0; h < 2; ++h) {
@@ -1670,6 +1688,8 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st,
       run(ST_UPDATE, PC_K_UPDATE);
     }
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, L->ds, L->la, DT_NEW);
+    t_tag = TAG_DT;
+    tag_launch();
     mark(PC_K_DT);
     if (tm) {
       (void)hipEventSynchronize(L->ev[ne - 1]);
@@ -1792,6 +1812,89 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
   return rc;
 }
 
+// Rewire a graph captured on one stream (a chain of kernel nodes, tags[i] the
+// stage tag of node i) into the dependencies of the branch streams
+// (branch_ctu_fluxes, branch_mol_fluxes): per direction d, trace_d (and the
+// first fluxes) after divu; the transverse fluxes (d|t) after the first
+// fluxes of d and t; final_d after the transverse fluxes (t0|t1) and (t1|t0)
+// and the first fluxes of d; the update after the three finals; the MOL
+// fluxes of the three directions after the primitives, the update after all
+// three; every other node after its predecessor.  Returns false, with the
+// graph unchanged, when the chain does not have that shape.
+static bool rewire_branches(hipGraph_t g, const std::vector<int>& tags) {
+  const size_t n = tags.size();
+  std::vector<hipGraphNode_t> chain;
+  size_t nr = 0;
+  if (hipGraphGetRootNodes(g, nullptr, &nr) != hipSuccess || nr != 1) return false;
+  hipGraphNode_t cur;
+  if (hipGraphGetRootNodes(g, &cur, &nr) != hipSuccess) return false;
+  while (true) {
+    chain.push_back(cur);
+    size_t nd = 0;
+    if (hipGraphNodeGetDependentNodes(cur, nullptr, &nd) != hipSuccess || nd > 1)
+      return false;
+    if (nd == 0) break;
+    if (hipGraphNodeGetDependentNodes(cur, &cur, &nd) != hipSuccess) return false;
+    if (chain.size() > n) return false;
+  }
+  if (chain.size() != n) return false;
+  std::vector<std::vector<int>> dep(n);
+  int F1[3] = {-1, -1, -1}, FIN[3] = {-1, -1, -1}, MOL[3] = {-1, -1, -1};
+  int TR[3][3];
+  for (auto& r : TR) r[0] = r[1] = r[2] = -1;
+  int join = -1;
+  for (size_t i = 0; i < n; ++i) {
+    const int st = tags[i] / 8, sub = tags[i] % 8;
+    const int k = (int)i;
+    if (st == ST_TRACE && sub < 3) {
+      dep[i] = {join};
+      F1[sub] = k;
+    } else if (st >= ST_FLUX1_X && st <= ST_FLUX1_Z) {
+      const int d = st - ST_FLUX1_X;
+      dep[i] = {F1[d]};
+      F1[d] = k;
+    } else if (st >= ST_TRANS_XY && st <= ST_TRANS_ZY) {
+      const int x = st - ST_TRANS_XY, d = x / 2;
+      const int t = (x % 2 == 0) ? (d == 0 ? 1 : 0) : (d == 2 ? 1 : 2);
+      dep[i] = {F1[d], F1[t]};
+      TR[d][t] = k;
+    } else if (st >= ST_FINAL_X && st <= ST_FINAL_Z) {
+      const int d = st - ST_FINAL_X;
+      const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
+      if (FIN[d] < 0 || (i > 0 && tags[i - 1] != tags[i]))
+        dep[i] = {TR[t0][t1], TR[t1][t0], F1[d]};
+      else
+        dep[i] = {FIN[d]};  // the low-order faces of the same stage
+      FIN[d] = k;
+    } else if (st == ST_UPDATE) {
+      dep[i] = {FIN[0], FIN[1], FIN[2]};
+      join = k;
+      for (int d = 0; d < 3; ++d) F1[d] = FIN[d] = -1;
+      for (auto& r : TR) r[0] = r[1] = r[2] = -1;
+    } else if (st >= ST_MOL_X && st <= ST_MOL_Z) {
+      dep[i] = {join};
+      MOL[st - ST_MOL_X] = k;
+    } else if (st == ST_MOL_UPD) {
+      dep[i] = {MOL[0], MOL[1], MOL[2]};
+      join = k;
+      MOL[0] = MOL[1] = MOL[2] = -1;
+    } else {  // ghost, primitives, divu, dt: after the previous node
+      if (i > 0) dep[i] = {k - 1};
+      join = k;
+    }
+    for (int x : dep[i])
+      if (x < 0) return false;
+  }
+  for (size_t i = 0; i + 1 < n; ++i)
+    if (hipGraphRemoveDependencies(g, &chain[i], &chain[i + 1], 1) != hipSuccess)
+      return false;
+  for (size_t i = 0; i < n; ++i)
+    for (int x : dep[i])
+      if (hipGraphAddDependencies(g, &chain[x], &chain[i], 1) != hipSuccess)
+        return false;
+  return true;
+}
+
 template <class EOS>
 static int level_step_graph(pc_level* L, int nsteps, hipStream_t st) {
   pc_ctx* c = L->ctx;
@@ -1799,7 +1902,10 @@ static int level_step_graph(pc_level* L, int nsteps, hipStream_t st) {
     if (!L->cap) HIPCHK(c, hipStreamCreateWithFlags(&L->cap, hipStreamNonBlocking));
     HIPCHK(c, hipStreamSynchronize(st));
     HIPCHK(c, hipStreamBeginCapture(L->cap, hipStreamCaptureModeThreadLocal));
+    std::vector<int> tags;
+    t_tags = &tags;
     const int rc = level_step_eos<EOS>(L, L->gsteps, L->cap, true);
+    t_tags = nullptr;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(L->cap, &g);
     if (rc != PC_OK) {
@@ -1808,6 +1914,10 @@ static int level_step_graph(pc_level* L, int nsteps, hipStream_t st) {
     }
     if (e != hipSuccess)
       return fail(c, PC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    // with branch streams on, the replayed graph runs the directions'
+    // stages concurrently, as the direct launches do
+    L->gbranched = L->brs.on && rewire_branches(g, tags);
+    L->gnodes = (int)tags.size();
     L->graph = g;
     HIPCHK(c, hipGraphInstantiate(&L->gexec, L->graph, nullptr, nullptr, 0));
     L->gvalid = true;
@@ -1882,6 +1992,13 @@ int pc_level_set_graph_v1(pc_level_t* L, int steps_per_graph) {
   if (!L || steps_per_graph < 0) return PC_ERR_ARG;
   graph_drop(L);
   L->gsteps = steps_per_graph;
+  return PC_OK;
+}
+
+int pc_level_graph_info_v1(const pc_level_t* L, int* nodes, int* branched) {
+  if (!L || !nodes || !branched) return PC_ERR_ARG;
+  *nodes = L->gvalid ? L->gnodes : 0;
+  *branched = L->gvalid && L->gbranched ? 1 : 0;
   return PC_OK;
 }
 

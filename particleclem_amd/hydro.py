@@ -275,6 +275,12 @@ class DeviceLevel:
         """Replay steps from a hipGraph of `steps_per_graph` steps (0: off)."""
         self._chk(self.L.pc_level_set_graph_v1(self.h, int(steps_per_graph)))
 
+    def graph_info(self):
+        """(kernel nodes, whether the graph carries the branch DAG)."""
+        n, b = C.c_int(), C.c_int()
+        self._chk(self.L.pc_level_graph_info_v1(self.h, C.byref(n), C.byref(b)))
+        return n.value, bool(b.value)
+
     def set_branches(self, on=True):
         """Run the three directions' independent stages on three streams
         (pc_level_set_branches_v1); results are unchanged."""

@@ -115,15 +115,12 @@ def test_graph_on_multibox_sod_matches_oracle(cuda):
 
 
 @pytest.mark.parametrize("mode", [
-    "ctu-ppm", "ctu-plm-mixed", "mol",
-    pytest.param("ctu-graph", marks=pytest.mark.skip(
-        reason="a graph capture forked across the branch streams crashed the host "
-               "process; captures now stay on one stream and this case is not run")),
-    "walls-inflow"])
+    "ctu-ppm", "ctu-plm-mixed", "mol", "ctu-graph", "mol-graph", "walls-inflow"])
 def test_branch_streams_are_bitwise(cuda, mode):
     """The three directions' stages on three streams (direct launches, MOL,
-    mixed precision, a low-order inflow face) give the single-stream result,
-    bitwise, after each call."""
+    mixed precision, a low-order inflow face, hipGraph replays rewired into
+    the branch dependencies) give the single-stream result, bitwise, after
+    each call."""
     over = {}
     if mode == "walls-inflow":
         w = wl.sod(n=24, ny=16, nz=8, max_grid=8, nsteps=0, xbc="slipwall")
@@ -133,7 +130,7 @@ def test_branch_streams_are_bitwise(cuda, mode):
         w = wl.premixed(n=32, box=16, nsteps=0)
     if mode == "ctu-plm-mixed":
         over = dict(ppm_type=0)
-    if mode == "mol":
+    if mode in ("mol", "mol-graph"):
         over = dict(do_mol=1)
     nb = len(w.boxes)
     ctx0, ref = level(w, **over)
@@ -142,7 +139,7 @@ def test_branch_streams_are_bitwise(cuda, mode):
         ref.set_precision(True)
         lev.set_precision(True)
     lev.set_branches(True)
-    if mode == "ctu-graph":
+    if mode.endswith("graph"):
         lev.set_graph(2)
     for n in (1, 3, 2):
         ref.step(n)
@@ -152,6 +149,11 @@ def test_branch_streams_are_bitwise(cuda, mode):
               f"{sum(int(np.count_nonzero(x != y)) for x, y in zip(a, b))} differ")
         assert same(a, b)
         assert lev.get_dt() == ref.get_dt()
+    if mode.endswith("graph"):
+        # captured on one stream, replayed with the branch dependencies
+        nodes, branched = lev.graph_info()
+        print(f"{mode}: graph of {nodes} kernel nodes, branched {branched}")
+        assert nodes > 0 and branched
     assert lev.status()[0] == 0
     for x in (lev, ref):
         x.close()
