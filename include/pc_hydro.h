@@ -189,7 +189,7 @@ int pc_hydro_source_box_v1(pc_ctx_t* ctx, pc_box_t bx, const int* domlo,
  * cannot allocate (code under hipGraph capture, tight serving loops, hosts
  * that forbid hipMalloc).  No counterpart in the reference.  Same arguments
  * and results; the workspace (q, qaux, srcq, the CTU intermediates and the
- * job descriptor) lives in one static device array of the library, 96 MiB
+ * job descriptor) lives in one static device array of the library, 112 MiB
  * per GPU, held until the process ends: any box of up to 32^3 cells fits
  * (9 species, with sources); a larger one returns PC_ERR_UNSUPPORTED before
  * any device work.  Every call on a GPU shares that GPU's array, so calls

@@ -154,8 +154,8 @@ static int a4_view(pc_ctx* c, const pc_array4_t& a, const IBox& need, int nc,
 // Sizes (doubles) of one box's intermediates; `full` adds q/qaux/srcq (the
 // box driver computes them) and the final fluxes (level path).
 struct WsLayout {
-  IBox q, tr[3], f1[3], ft[6], fl[3], node, g1;
-  size_t off_q = 0, off_qa = 0, off_srcq = 0, off_tr[3], off_f1[3], off_ft[6],
+  IBox q, tr[3], f1[3], dl[3], ft[6], fl[3], node, g1;
+  size_t off_q = 0, off_qa = 0, off_srcq = 0, off_tr[3], off_f1[3], off_dl[3], off_ft[6],
          off_fl[3], off_qec[3], off_divu = 0, off_sold = 0, off_flat = 0,
          total = 0;
   bool mol = false;
@@ -205,6 +205,13 @@ static WsLayout ws_layout(const IBox& bx, int ns, bool with_q, bool with_src,
     L.f1[d] = b;
     L.off_f1[d] = take(b.npts() * FT);
   }
+  for (int d = 0; d < 3; ++d) {  // per-cell corrections: cells [L, H] along d
+    IBox b = growb(bx, 1);
+    b.lo[d] = bx.lo[d];
+    b.hi[d] = bx.hi[d];
+    L.dl[d] = b;
+    L.off_dl[d] = take(b.npts() * (6 + ns));
+  }
   for (int s = 0; s < 6; ++s) {
     const int D = s / 2;
     const int T = (s % 2 == 0) ? (D == 0 ? 1 : 0) : (D == 2 ? 1 : 2);
@@ -245,6 +252,7 @@ static long long box_max_array_bytes(const IBox& bx, int ns, const WsLayout& W) 
     if (!W.mol) {
       mx_(W.tr[d].npts() * 2LL * QT);
       mx_(W.f1[d].npts() * (long long)FT);
+      mx_(W.dl[d].npts() * (long long)(6 + ns));
     }
   }
   if (!W.mol)
@@ -271,7 +279,7 @@ static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
   if (L.mol) {
     for (int d = 0; d < 3; ++d) {
       if (with_flx) J.flx[d] = fab_view(base + L.off_fl[d], L.fl[d]);
-      J.tr[d] = J.f1[d] = J.qec[d] = null_view();
+      J.tr[d] = J.f1[d] = J.dl[d] = J.qec[d] = null_view();
     }
     for (int s = 0; s < 6; ++s) J.ft[s] = null_view();
     J.divu = J.flat = null_view();
@@ -283,6 +291,7 @@ static void ws_fill_job(Job& J, double* base, const WsLayout& L, bool with_q,
   for (int d = 0; d < 3; ++d) {
     J.tr[d] = fab_view(base + L.off_tr[d], L.tr[d]);
     J.f1[d] = fab_view(base + L.off_f1[d], L.f1[d]);
+    J.dl[d] = fab_view(base + L.off_dl[d], L.dl[d]);
     if (with_flx) J.flx[d] = fab_view(base + L.off_fl[d], L.fl[d]);
     J.qec[d] = fab_view(base + L.off_qec[d], L.fl[d]);
   }
@@ -455,10 +464,10 @@ static bool pencils_fit(const Job* jobs, int n) {
 }
 
 // Dynamic LDS bytes of the fused kernel: the staging window, then the minus
-// states of a tile's cells.
+// states and the face fluxes of a tile's cells.
 template <int NS, class FP>
 static size_t pencil_lds_bytes(int cells) {
-  return (size_t)std::max(cells * (5 + NS), BLOCK * (6 + NS)) * sizeof(FP);
+  return (size_t)std::max(cells * (5 + NS), BLOCK * (7 + NS)) * sizeof(FP);
 }
 
 // The CTU stages from primitives to final fluxes for one EOS.  With a job
@@ -491,6 +500,9 @@ static void run_ctu(const Job& j0, const Sched& s1, const KArgs& ka,
     LAUNCH((k_flux1<EOS, 0>), G(ST_FLUX1_X), st, S(ST_FLUX1_X), ka);
     LAUNCH((k_flux1<EOS, 1>), G(ST_FLUX1_Y), st, S(ST_FLUX1_Y), ka);
     LAUNCH((k_flux1<EOS, 2>), G(ST_FLUX1_Z), st, S(ST_FLUX1_Z), ka);
+    LAUNCH((k_tcorr<0, EOS::NS>), G(ST_FLUX1_X), st, S(ST_FLUX1_X), ka);
+    LAUNCH((k_tcorr<1, EOS::NS>), G(ST_FLUX1_Y), st, S(ST_FLUX1_Y), ka);
+    LAUNCH((k_tcorr<2, EOS::NS>), G(ST_FLUX1_Z), st, S(ST_FLUX1_Z), ka);
   }
   LAUNCH((k_trans<EOS, 0, 1>), G(ST_TRANS_XY), st, S(ST_TRANS_XY), ka);
   LAUNCH((k_trans<EOS, 0, 2>), G(ST_TRANS_XZ), st, S(ST_TRANS_XZ), ka);
@@ -853,11 +865,11 @@ static int hsb_launch(pc_ctx* c, const HsbCall& H, const Sched& sc, int ppm_type
 // ------------------------------------------- allocation-free box driver --
 // The workspace of pc_hydro_source_box_noalloc_v1: a job descriptor, then
 // q/qaux/srcq and the CTU intermediates of one box.  Sized for any box of
-// up to 32^3 cells with 9 species and sources (89.7 MiB); every element a
+// up to 32^3 cells with 9 species and sources (102.4 MiB); every element a
 // call reads it has written earlier in the same call, so nothing needs
 // clearing.  One copy per GPU, shared by every call on it.
 constexpr size_t NOALLOC_JOB_DOUBLES = (sizeof(Job) + 255) / 256 * 32;
-constexpr size_t NOALLOC_WS_DOUBLES = (size_t)96 << 17;  // 96 MiB of doubles
+constexpr size_t NOALLOC_WS_DOUBLES = (size_t)112 << 17;  // 112 MiB of doubles
 static __device__ __align__(256) double scratch[NOALLOC_JOB_DOUBLES + NOALLOC_WS_DOUBLES];
 
 // Device address of `scratch` per device, looked up on the first call there
@@ -1211,9 +1223,24 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
       }
       break;
     // with the fused PPM trace the first fluxes come from the trace stage
-    case ST_FLUX1_X: if (!L->fuse) LAUNCH((k_flux1<EOS, 0, FP, P>), grid, st, sc, ka); break;
-    case ST_FLUX1_Y: if (!L->fuse) LAUNCH((k_flux1<EOS, 1, FP, P>), grid, st, sc, ka); break;
-    case ST_FLUX1_Z: if (!L->fuse) LAUNCH((k_flux1<EOS, 2, FP, P>), grid, st, sc, ka); break;
+    case ST_FLUX1_X:
+      if (!L->fuse) {
+        LAUNCH((k_flux1<EOS, 0, FP, P>), grid, st, sc, ka);
+        LAUNCH((k_tcorr<0, EOS::NS, FP, P>), grid, st, sc, ka);
+      }
+      break;
+    case ST_FLUX1_Y:
+      if (!L->fuse) {
+        LAUNCH((k_flux1<EOS, 1, FP, P>), grid, st, sc, ka);
+        LAUNCH((k_tcorr<1, EOS::NS, FP, P>), grid, st, sc, ka);
+      }
+      break;
+    case ST_FLUX1_Z:
+      if (!L->fuse) {
+        LAUNCH((k_flux1<EOS, 2, FP, P>), grid, st, sc, ka);
+        LAUNCH((k_tcorr<2, EOS::NS, FP, P>), grid, st, sc, ka);
+      }
+      break;
     case ST_TRANS_XY: LAUNCH((k_trans<EOS, 0, 1, FP, P>), grid, st, sc, ka); break;
     case ST_TRANS_XZ: LAUNCH((k_trans<EOS, 0, 2, FP, P>), grid, st, sc, ka); break;
     case ST_TRANS_YX: LAUNCH((k_trans<EOS, 1, 0, FP, P>), grid, st, sc, ka); break;

@@ -149,6 +149,7 @@ struct JobT {
   VT<FP> q, qa, srcq;               // primitives on grow(bx,4); srcq.p may be 0
   VT<FP> tr[3];                     // traces per direction
   VT<FP> f1[3];                     // first fluxes
+  VT<FP> dl[3];                     // their per-cell transverse corrections
   VT<FP> ft[6];                     // transverse fluxes, [2*d + slot(d,t)]
   VT<double> flx[3], qec[3];        // final fluxes (NVAR) and p_gd, un_gd
   VT<double> area[3], vol;          // p == 0: uniform geometry
@@ -825,6 +826,85 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+// The transverse correction a cell's first fluxes in direction T make to the
+// traces of the other two directions (pc_transdo's flux differences and its
+// Godunov pressure term, Source/Godunov.H:455-633): cdtdx (F_hi - F_lo) of
+// rho, mx, my, mz, rhoE and rho Y, and cdtdx (dAup + pav dAu (Gamma1 - 1)),
+// in the order pc_transdo forms them.  Stored per cell in dl[T] with the
+// components d[0..4], ds[0..NS), pt.
+template <int NS, class FP>
+PCD void make_tcorr(const Flux<NS, FP>& lo, const Flux<NS, FP>& hi, FP gamc,
+                    FP c, TCorr<NS, FP>& D) {
+#pragma unroll
+  for (int m = 0; m < 5; ++m) D.d[m] = c * (hi.f[m] - lo.f[m]);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) D.ds[s] = c * (hi.f[6 + s] - lo.f[6 + s]);
+  const FP dAup = hi.pgd * hi.ugd - lo.pgd * lo.ugd;
+  const FP pav = FP(0.5) * (hi.pgd + lo.pgd);
+  const FP dAu = hi.ugd - lo.ugd;
+  D.pt = c * (dAup + pav * dAu * (gamc - FP(1.0)));
+}
+template <int NS, class FP, class VW>
+PCD void store_tcorr(const VW& v, int i, int j, int k, const TCorr<NS, FP>& D) {
+#pragma unroll
+  for (int m = 0; m < 5; ++m) v(i, j, k, m) = D.d[m];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) v(i, j, k, 5 + s) = D.ds[s];
+  v(i, j, k, 5 + NS) = D.pt;
+}
+template <int NS, class FP, class VW>
+PCD void load_tcorr(const VW& v, int i, int j, int k, TCorr<NS, FP>& D) {
+#pragma unroll
+  for (int m = 0; m < 5; ++m) D.d[m] = v(i, j, k, m);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) D.ds[s] = v(i, j, k, 5 + s);
+  D.pt = v(i, j, k, 5 + NS);
+}
+// a face's flux from a compact flux array (the components make_tcorr reads)
+template <int NS, class FP, class VW>
+PCD void load_flux(const VW& f, int i, int j, int k, Flux<NS, FP>& F) {
+#pragma unroll
+  for (int m = 0; m < 5; ++m) F.f[m] = f(i, j, k, m);
+  F.pgd = f(i, j, k, 5);
+  F.ugd = f(i, j, k, 6);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) F.f[6 + s] = f(i, j, k, 7 + s);
+}
+
+// ------------------------------------------ K8' per-cell corrections ----
+// The first fluxes' transverse corrections of the cells [L, H] along D
+// (x [L-1, H+1] across) from the face fluxes f1[D] (the unfused path; the
+// fused trace kernel forms them from the faces in its workgroup).
+template <int D, class FP, int NS>
+PCD void b_tcorr(const Sched& sc, const KArgs& ka, int lb_) {
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_FLUX1_X + D, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  if (idx > J.hi[D]) return;  // the last face row has no cell above it
+  int ip = i, jp = j, kp = k;
+  step<D>(ip, jp, kp, 1);
+  Flux<NS, FP> Flo, Fhi;
+  load_flux<NS, FP>(J.f1[D], i, j, k, Flo);
+  load_flux<NS, FP>(J.f1[D], ip, jp, kp, Fhi);
+  const FP cdtdx = FP(1.0) / FP(3.0) * get_dt(J, ka) / J.dx[D];
+  TCorr<NS, FP> C;
+  make_tcorr<NS, FP>(Flo, Fhi, FP(J.qa(i, j, k, QGAMC)), cdtdx, C);
+  store_tcorr<NS, FP>(J.dl[D], i, j, k, C);
+}
+template <int D, int NS, class FP = double, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_tcorr(Sched sc, KArgs ka) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_tcorr<D, FP, NS>(sc, ka, lb_);
+  } else {
+    b_tcorr<D, FP, NS>(sc, ka, first_block());
+  }
+}
+
 // flux differences across cell c in direction T from a compact flux array
 template <int NS, int T, class FP, class VW>
 PCD void dflux(const VW& f, int i, int j, int k, FP c, DFlux<NS, FP>& D) {
@@ -856,16 +936,17 @@ PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   FaceState<NS, FP> s, L, R;
-  DFlux<NS, FP> df;
-  const VS<FP> tr = split(J.tr[D]), f1 = split(J.f1[T]), qa = split(J.qa);
+  TCorr<NS, FP> df;
+  const VS<FP> tr = split(J.tr[D]), dl = split(J.dl[T]), qa = split(J.qa);
+  (void)cdtdx;
   // minus side: cell f - e_D, state on its high face
   load_trace<NS, FP>(tr, i, j, k, 0, s);
-  dflux<NS, T, FP>(f1, im, jm, km, cdtdx, df);
-  transdo_side<NS>(s, df, qa(im, jm, km, QGAMC), cdtdx, L);
+  load_tcorr<NS, FP>(dl, im, jm, km, df);
+  transdo_side<NS>(s, df, L);
   // plus side: cell f, state on its low face
   load_trace<NS, FP>(tr, i, j, k, 1, s);
-  dflux<NS, T, FP>(f1, i, j, k, cdtdx, df);
-  transdo_side<NS>(s, df, qa(i, j, k, QGAMC), cdtdx, R);
+  load_tcorr<NS, FP>(dl, i, j, k, df);
+  transdo_side<NS>(s, df, R);
   const FP cav = FP(0.5) * (qa(i, j, k, QC) + qa(im, jm, km, QC));
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
   Flux<NS, FP> F;

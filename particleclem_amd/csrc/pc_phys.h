@@ -905,6 +905,51 @@ PCD void transdo_side(const FaceState<NS, FP>& s, const DFlux<NS, FP>& D, FP gam
   o.p = mx(pnewr, vsmall<FP>());
 }
 
+// The same correction with the Godunov pressure term already formed per
+// cell: pt = cdtdx (dAup + pav dAu (Gamma1 - 1)) of that cell's two faces.
+template <int NS, class FP = double>
+struct TCorr {
+  FP d[5];
+  FP ds[NS];
+  FP pt;
+};
+
+template <int NS, class FP>
+PCD void transdo_side(const FaceState<NS, FP>& s, const TCorr<NS, FP>& D,
+                      FaceState<NS, FP>& o) {
+  const FP flxrho = D.d[0], flxu = D.d[1], flxv = D.d[2], flxw = D.d[3],
+               flxe = D.d[4];
+  const FP rrnew = s.rho - flxrho;
+  const FP rrinv = FP(1.0) / rrnew;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const FP rr = s.rho;
+    const FP compo = rr * s.Y[n] - D.ds[n];
+    o.Y[n] = div_by(compo, rrnew, rrinv);
+  }
+  const FP rrr = s.rho;
+  FP rur = s.u, rvr = s.v, rwr = s.w;
+  const FP ekinr = FP(0.5) * rrr * (rur * rur + rvr * rvr + rwr * rwr);
+  rur *= rrr;
+  rvr *= rrr;
+  rwr *= rrr;
+  const FP rer = s.rhoe + ekinr;
+  const FP rrnewr = rrr - flxrho;
+  const FP runewr = rur - flxu;
+  const FP rvnewr = rvr - flxv;
+  const FP rwnewr = rwr - flxw;
+  const FP renewr = rer - flxe;
+  o.rho = rrnewr;
+  o.u = div_by(runewr, rrnewr, rrinv);
+  o.v = div_by(rvnewr, rrnewr, rrinv);
+  o.w = div_by(rwnewr, rrnewr, rrinv);
+  const FP rhoekinr = div_by(
+      FP(0.5) * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr), rrnewr, rrinv);
+  o.rhoe = renewr - rhoekinr;
+  const FP pnewr = s.p - D.pt;
+  o.p = mx(pnewr, vsmall<FP>());
+}
+
 // pc_transdd, one side (Source/Godunov.H:639-845, passive :78-132): the
 // dir-state corrected by both transverse flux differences (D0 scaled by
 // cdtdx0, D1 by cdtdx1 when they were formed) plus hdt * srcQ.

@@ -409,28 +409,56 @@ PCD void b_trace_flux1_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
   __syncthreads();
   // this cell's low D-face; the region's first cell along D has none
   const int pd = (D == 0) ? ta_ : tb_;
-  if (!active || pd == 0) return;
-  const int prev = (D == 0) ? me - 1 : me - T.ta;
-  FaceState<NS, FP> Lf;
-  Lf.rho = lds[0 * ncell + prev];
-  Lf.u = lds[1 * ncell + prev];
-  Lf.v = lds[2 * ncell + prev];
-  Lf.w = lds[3 * ncell + prev];
-  Lf.p = lds[4 * ncell + prev];
-  Lf.rhoe = lds[5 * ncell + prev];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) Lf.Y[s] = lds[(6 + s) * ncell + prev];
-  int im = i, jm = j, km = k;
-  step<D>(im, jm, km, -1);
-  const FP cav = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
-  const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  const int nd = (D == 0) ? n[0] : n[AB];
+  const bool face = active && pd > 0;
   Flux<NS, FP> F;
-  cmpflx<EOS, D>(Lf, P, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
-  store_flux<NS, FP>(J.f1[D], i, j, k, F);
+  if (face) {
+    const int prev = (D == 0) ? me - 1 : me - T.ta;
+    FaceState<NS, FP> Lf;
+    Lf.rho = lds[0 * ncell + prev];
+    Lf.u = lds[1 * ncell + prev];
+    Lf.v = lds[2 * ncell + prev];
+    Lf.w = lds[3 * ncell + prev];
+    Lf.p = lds[4 * ncell + prev];
+    Lf.rhoe = lds[5 * ncell + prev];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) Lf.Y[s] = lds[(6 + s) * ncell + prev];
+    int im = i, jm = j, km = k;
+    step<D>(im, jm, km, -1);
+    const FP cav = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+    const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+    cmpflx<EOS, D>(Lf, P, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
+  }
+  // the flux of the face above each cell comes from the next thread along D:
+  // the cells [L, H] along D get the transverse correction their two faces
+  // make (the transverse stage reads these per-cell values)
+  __syncthreads();  // every M read: the buffer now takes the face fluxes
+  if (face) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) lds[m * ncell + me] = F.f[m];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) lds[(5 + s) * ncell + me] = F.f[6 + s];
+    lds[(5 + NS) * ncell + me] = F.pgd;
+    lds[(6 + NS) * ncell + me] = F.ugd;
+  }
+  __syncthreads();
+  if (!face || pd + 1 >= nd) return;
+  const int next = (D == 0) ? me + 1 : me + T.ta;
+  Flux<NS, FP> Fh;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) Fh.f[m] = lds[m * ncell + next];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Fh.f[6 + s] = lds[(5 + s) * ncell + next];
+  Fh.pgd = lds[(5 + NS) * ncell + next];
+  Fh.ugd = lds[(6 + NS) * ncell + next];
+  const FP cdtdx = FP(1.0) / FP(3.0) * get_dt(J, ka) / J.dx[D];
+  TCorr<NS, FP> C;
+  make_tcorr<NS, FP>(F, Fh, FP(J.qa(i, j, k, QGAMC)), cdtdx, C);
+  store_tcorr<NS, FP>(J.dl[D], i, j, k, C);
   (void)QT;
 }
 
-// Dynamic LDS: max((5 + NS) * pencil_cells, (6 + NS) * BLOCK) elements of FP.
+// Dynamic LDS: max((5 + NS) * pencil_cells, (7 + NS) * BLOCK) elements of FP.
 template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_trace_flux1_lds(Sched sc, KArgs ka) {
