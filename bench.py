@@ -53,7 +53,9 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
     """Compulsory bytes of one launch of each kernel over the level: every
     input element read once and every output element written once on the
     kernel's region (stencil re-reads are not counted).  fused: the PPM
-    trace launch also forms the first fluxes (k_trace_flux1_lds)."""
+    trace launch also forms the first fluxes and each cell's transverse
+    correction (k_trace_flux1_lds); otherwise k_flux1 writes the face fluxes
+    and k_tcorr turns them into the corrections."""
     Q, N, QT, FT = 8 + nspec, 7 + nspec, 6 + nspec, 7 + nspec
     tot = {}
 
@@ -90,10 +92,17 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
             add("trace_" + a, g1, QT + 2 * QT)
             f1 = list(t)
             f1[d] = n[d] + 1
+            # cells [L, H] along d (transverse [L-1, H+1]) get a correction
+            # of QT values (rho, m, E, rho Y and the pressure term) from
+            # Gamma1 and their two faces' fluxes
+            c1 = list(t)
+            c1[d] = n[d]
             if fused:  # the faces' states come from the trace, not from HBM
-                add("trace_" + a, f1[0] * f1[1] * f1[2], 1 + FT)
+                add("trace_" + a, f1[0] * f1[1] * f1[2], 1)
+                add("trace_" + a, c1[0] * c1[1] * c1[2], 1 + QT)
             else:
                 add("flux1_" + a, f1[0] * f1[1] * f1[2], 2 * QT + 1 + FT)
+                add("flux1_" + a, c1[0] * c1[1] * c1[2], FT + 1 + QT)
             fin = list(n)
             fin[d] = n[d] + 1
             add("final_" + a, fin[0] * fin[1] * fin[2], 2 * QT + 2 * FT + 2 * N + 5)
@@ -102,7 +111,9 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
             r = list(n)
             r[D] = n[D] + 1
             r[3 - D - T] = n[3 - D - T] + 2
-            add("trans_" + name, r[0] * r[1] * r[2], 2 * QT + 2 * FT + 2)
+            # both traces of the face, the t-correction of one cell per
+            # face, c of the cell, the flux written
+            add("trans_" + name, r[0] * r[1] * r[2], 2 * QT + QT + 1 + FT)
         add("update", v, 6 * N + 6)
         add("dt", 0, 0)
     return tot
