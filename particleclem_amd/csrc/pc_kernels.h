@@ -568,173 +568,8 @@ PCD FP flat_cell(const JobT<FP>& J, int i, int j, int k) {
 }
 
 // ------------------------------------------------------ K3-5 trace (PPM) --
-// Source/PPM.cpp:82-423 for one cell and one direction (no WENO).
-template <class EOS, int D, class FP>
-PCD void b_trace_ppm(const Sched& sc, const KArgs& ka, int lb_) {
-  constexpr int NS = EOS::NS;
-  int local;
-  const JobT<FP> J = load_job<FP>(sc, lb_, local);
-  int lo[3], n[3], i, j, k;
-  stage_region(ST_TRACE, J.lo, J.hi, lo, n);
-  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const VB<FP> q = bufview(J.q);
-  const FP dtdx = get_dt(J, ka) / J.dx[D];
-  constexpr int QUN = QU + D;
-  constexpr int QUT = (D == 0) ? QV : ((D == 1) ? QW : QU);
-  constexpr int QUTT = (D == 0) ? QW : ((D == 1) ? QU : QV);
-  auto at = [&](int m, int comp) {
-    int ii = i, jj = j, kk = k;
-    step<D>(ii, jj, kk, m);
-    return q(ii, jj, kk, comp);
-  };
-  FP Yc[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) Yc[s] = q(i, j, k, QFS + s);
-  const FP cc = EOS::RPY2Cs(q(i, j, k, QRHO), q(i, j, k, QPRES), Yc);
-  const FP un = q(i, j, k, QUN);
-  const FP flat = ka.use_flattening ? FP(J.flat(i, j, k, 0)) : FP(1.0);
-  const FP spd0 = un - cc, spd1 = un, spd2 = un + cc;
-  FP sm, sp, s6;
-  auto recon = [&](int comp) {
-    FP s[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) s[m] = at(m - 2, comp);
-    ppm_reconstruct(s, flat, sm, sp);
-    s6 = ppm_s6(sm, sp, s[2]);
-  };
-  // integrals needed by the characteristic projection
-  recon(QRHO);
-  const FP Im_r0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const FP Im_r1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const FP Ip_r2 = ppm_ip(sm, sp, s6, spd2, dtdx);
-  const FP Ip_r1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  recon(QUN);
-  const FP Im_u0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const FP Im_u2 = ppm_im(sm, sp, s6, spd2, dtdx);
-  const FP Ip_u2 = ppm_ip(sm, sp, s6, spd2, dtdx);
-  const FP Ip_u0 = ppm_ip(sm, sp, s6, spd0, dtdx);
-  recon(QPRES);
-  const FP Im_p0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const FP Im_p1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const FP Im_p2 = ppm_im(sm, sp, s6, spd2, dtdx);
-  const FP Ip_p0 = ppm_ip(sm, sp, s6, spd0, dtdx);
-  const FP Ip_p1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  const FP Ip_p2 = ppm_ip(sm, sp, s6, spd2, dtdx);
-  recon(QREINT);
-  const FP Im_e0 = ppm_im(sm, sp, s6, spd0, dtdx);
-  const FP Im_e1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const FP Ip_e2 = ppm_ip(sm, sp, s6, spd2, dtdx);
-  const FP Ip_e1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  recon(QUT);
-  const FP Im_t1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const FP Ip_t1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  recon(QUTT);
-  const FP Im_tt1 = ppm_im(sm, sp, s6, spd1, dtdx);
-  const FP Ip_tt1 = ppm_ip(sm, sp, s6, spd1, dtdx);
-  FaceState<NS, FP> P, M;
-  FP YrefP[NS], YrefM[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    recon(QFS + s);
-    YrefP[s] = ppm_im(sm, sp, s6, spd0, dtdx);
-    P.Y[s] = ppm_im(sm, sp, s6, spd1, dtdx);
-    YrefM[s] = ppm_ip(sm, sp, s6, spd2, dtdx);
-    M.Y[s] = ppm_ip(sm, sp, s6, spd1, dtdx);
-  }
-  FP vel[3];
-  // ---- plus state on face i (reference state: fastest wave to the left)
-  {
-    FP rho_ref = Im_r0;
-    const FP un_ref = Im_u0;
-    FP p_ref = Im_p0;
-    const FP rhoe_g_ref = Im_e0;
-    rho_ref = mx(rho_ref, rmin<FP>());
-    const FP rho_ref_inv = FP(1.0) / rho_ref;
-    p_ref = mx(p_ref, rmin<FP>());
-    const FP cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefP);
-    const FP csq_ref = cc_ref * cc_ref;
-    const FP cc_ref_inv = FP(1.0) / cc_ref;
-    (void)rhoe_g_ref;
-    const FP dum = un_ref - Im_u0;
-    const FP dptotm = p_ref - Im_p0;
-    const FP drho = rho_ref - Im_r1;
-    const FP dptot = p_ref - Im_p1;
-    const FP dup = un_ref - Im_u2;
-    const FP dptotp = p_ref - Im_p2;
-    FP alpham =
-        FP(0.5) * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
-    FP alphap =
-        FP(0.5) * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
-    FP alpha0r = drho - dptot / csq_ref;
-    alpham = un - cc > FP(0.0) ? FP(0.0) : -alpham;
-    alphap = un + cc > FP(0.0) ? FP(0.0) : -alphap;
-    alpha0r = un > FP(0.0) ? FP(0.0) : -alpha0r;
-    P.rho = mx(rmin<FP>(), rho_ref + alphap + alpham + alpha0r);
-    vel[D] = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
-    P.p = mx(rmin<FP>(), p_ref + (alphap + alpham) * csq_ref);
-    vel[QUT - QU] = Im_t1;
-    vel[QUTT - QU] = Im_tt1;
-    P.u = vel[0];
-    P.v = vel[1];
-    P.w = vel[2];
-    const FP eint = EOS::RYP2E(P.rho, P.Y, P.p);
-    P.rhoe = P.rho * eint;
-  }
-  // ---- minus state on face i+1 (fastest wave to the right)
-  {
-    FP rho_ref = Ip_r2;
-    const FP un_ref = Ip_u2;
-    FP p_ref = Ip_p2;
-    rho_ref = mx(rho_ref, rmin<FP>());
-    const FP rho_ref_inv = FP(1.0) / rho_ref;
-    p_ref = mx(p_ref, rmin<FP>());
-    const FP cc_ref = EOS::RPY2Cs(rho_ref, p_ref, YrefM);
-    const FP csq_ref = cc_ref * cc_ref;
-    const FP cc_ref_inv = FP(1.0) / cc_ref;
-    const FP dum = un_ref - Ip_u0;
-    const FP dptotm = p_ref - Ip_p0;
-    const FP drho = rho_ref - Ip_r1;
-    const FP dptot = p_ref - Ip_p1;
-    const FP dup = un_ref - Ip_u2;
-    const FP dptotp = p_ref - Ip_p2;
-    FP alpham =
-        FP(0.5) * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
-    FP alphap =
-        FP(0.5) * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
-    FP alpha0r = drho - dptot / csq_ref;
-    alpham = un - cc > FP(0.0) ? -alpham : FP(0.0);
-    alphap = un + cc > FP(0.0) ? -alphap : FP(0.0);
-    alpha0r = un > FP(0.0) ? -alpha0r : FP(0.0);
-    M.rho = mx(rmin<FP>(), rho_ref + alphap + alpham + alpha0r);
-    vel[D] = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
-    M.p = mx(rmin<FP>(), p_ref + (alphap + alpham) * csq_ref);
-    vel[QUT - QU] = Ip_t1;
-    vel[QUTT - QU] = Ip_tt1;
-    M.u = vel[0];
-    M.v = vel[1];
-    M.w = vel[2];
-    const FP eint = EOS::RYP2E(M.rho, M.Y, M.p);
-    M.rhoe = M.rho * eint;
-  }
-  (void)Ip_e2;
-  (void)Ip_e1;
-  (void)Im_e1;
-  const auto& tr = J.tr[D];
-  store_trace<NS, FP>(tr, i, j, k, 1, P);
-  int i1 = i, j1 = j, k1 = k;
-  step<D>(i1, j1, k1, 1);
-  store_trace<NS, FP>(tr, i1, j1, k1, 0, M);
-}
-template <class EOS, int D, class FP = double, bool PERSIST = false>
-__global__ void __launch_bounds__(BLOCK)
-    k_trace_ppm(Sched sc, KArgs ka) {
-  if constexpr (PERSIST) {
-    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_trace_ppm<EOS, D, FP>(sc, ka, lb_);
-  } else {
-    b_trace_ppm<EOS, D, FP>(sc, ka, first_block());
-  }
-}
+// The PPM trace (Source/PPM.cpp:82-423) runs with its stencils staged in
+// LDS: ppm_cell_states and the trace kernels of pc_trace_lds.h.
 
 // --------------------------------------------------- K3' trace (PLM) ----
 // Source/Godunov.cpp:139-180 (one direction per launch)

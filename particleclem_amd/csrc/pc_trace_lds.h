@@ -6,8 +6,9 @@
 // one index along the third.  The tile's primitives plus the 2-cell PPM
 // halo along the traced direction are loaded from HBM once, coalesced along
 // x, into LDS (the 5 + NS components the stencils read, one barrier); every
-// thread then reads its cell and its 5-point stencils from LDS.  Arithmetic,
-// operation order and outputs are those of b_trace_ppm in pc_kernels.h.
+// thread then reads its cell and its 5-point stencils from LDS.  Arithmetic
+// and operation order are the reference's (ppm_cell_states); outputs are the
+// traces of the Compact layouts in pc_kernels.h.
 #pragma once
 #include "pc_kernels.h"
 

@@ -87,3 +87,14 @@ def test_stop_time_last_step(cuda, over):
     dt0 = H.init_dt(prm)
     over = dict(over, stop_time=2.6 * dt0)
     assert_parity(w, 5, over, label=f"sod16 stop_time {over}")
+
+
+@pytest.mark.parametrize("idir", [0, 1, 2], ids=["x", "y", "z"])
+def test_long_box_unfused_ppm_trace(cuda, idir):
+    """A box longer than one workgroup's pencil (300 cells along the tube):
+    the PPM trace runs on its LDS tiles with the first fluxes and the cell
+    corrections in kernels of their own instead of the fused pencil
+    kernel."""
+    w = wl.sod(n=300, ny=8, nz=8, idir=idir, nsteps=2)
+    assert len(w.boxes) == 1
+    assert_parity(w, 2, label=f"sod 300x8x8 along {'xyz'[idir]}")
