@@ -106,14 +106,19 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
             fin = list(n)
             fin[d] = n[d] + 1
             add("final_" + a, fin[0] * fin[1] * fin[2], 2 * QT + 2 * FT + 2 * N + 5)
-        for D, T, name in [(0, 1, "xy"), (0, 2, "xz"), (1, 0, "yx"),
-                           (1, 2, "yz"), (2, 0, "zx"), (2, 1, "zy")]:
-            r = list(n)
+        for D, name in [(0, "xy"), (1, "yx"), (2, "zx")]:
+            # one launch per direction for both (d|t): on the d-faces with
+            # both transverse indices in [L-1, H+1] the face's traces and c
+            # are read once; each (d|t) reads the t-correction of one cell
+            # per face and writes its flux where t lies in [L, H]
+            T0, T1 = [t for t in range(3) if t != D]
+            r = [x + 2 for x in n]
             r[D] = n[D] + 1
-            r[3 - D - T] = n[3 - D - T] + 2
-            # both traces of the face, the t-correction of one cell per
-            # face, c of the cell, the flux written
-            add("trans_" + name, r[0] * r[1] * r[2], 2 * QT + QT + 1 + FT)
+            add("trans_" + name, r[0] * r[1] * r[2], 2 * QT + 1)
+            for T in (T0, T1):
+                e = list(r)
+                e[T] = n[T]
+                add("trans_" + name, e[0] * e[1] * e[2], QT + FT)
         add("update", v, 6 * N + 6)
         add("dt", 0, 0)
     return tot

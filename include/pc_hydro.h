@@ -301,7 +301,10 @@ int pc_write_plotfile_v1(const char* dir, int nvars, const char* const* names,
  * launch order: */
 enum {
   PC_K_GHOST = 0, PC_K_CTOPRIM, PC_K_DIVU, PC_K_TRACE_X, PC_K_TRACE_Y,
-  PC_K_TRACE_Z, PC_K_FLUX1_X, PC_K_FLUX1_Y, PC_K_FLUX1_Z, PC_K_TRANS_XY,
+  PC_K_TRACE_Z, PC_K_FLUX1_X, PC_K_FLUX1_Y, PC_K_FLUX1_Z,
+  /* the transverse fluxes of d: both (d|t) in one launch, timed under the
+   * first t (PC_K_TRANS_XY, _YX, _ZX); _XZ, _YZ and _ZY read 0 */
+  PC_K_TRANS_XY,
   PC_K_TRANS_XZ, PC_K_TRANS_YX, PC_K_TRANS_YZ, PC_K_TRANS_ZX, PC_K_TRANS_ZY,
   PC_K_FINAL_X, PC_K_FINAL_Y, PC_K_FINAL_Z, PC_K_UPDATE, PC_K_DT,
   /* method-of-lines step (do_mol = 1): ghost fill and primitives count

@@ -504,12 +504,9 @@ static void run_ctu(const Job& j0, const Sched& s1, const KArgs& ka,
     LAUNCH((k_tcorr<1, EOS::NS>), G(ST_FLUX1_Y), st, S(ST_FLUX1_Y), ka);
     LAUNCH((k_tcorr<2, EOS::NS>), G(ST_FLUX1_Z), st, S(ST_FLUX1_Z), ka);
   }
-  LAUNCH((k_trans<EOS, 0, 1>), G(ST_TRANS_XY), st, S(ST_TRANS_XY), ka);
-  LAUNCH((k_trans<EOS, 0, 2>), G(ST_TRANS_XZ), st, S(ST_TRANS_XZ), ka);
-  LAUNCH((k_trans<EOS, 1, 0>), G(ST_TRANS_YX), st, S(ST_TRANS_YX), ka);
-  LAUNCH((k_trans<EOS, 1, 2>), G(ST_TRANS_YZ), st, S(ST_TRANS_YZ), ka);
-  LAUNCH((k_trans<EOS, 2, 0>), G(ST_TRANS_ZX), st, S(ST_TRANS_ZX), ka);
-  LAUNCH((k_trans<EOS, 2, 1>), G(ST_TRANS_ZY), st, S(ST_TRANS_ZY), ka);
+  LAUNCH((k_trans<EOS, 0>), G(ST_TRANS_XY), st, S(ST_TRANS_XY), ka);
+  LAUNCH((k_trans<EOS, 1>), G(ST_TRANS_YX), st, S(ST_TRANS_YX), ka);
+  LAUNCH((k_trans<EOS, 2>), G(ST_TRANS_ZX), st, S(ST_TRANS_ZX), ka);
   if (j0.srcq.p) {
     LAUNCH((k_final<EOS, 0, true>), G(ST_FINAL_X), st, S(ST_FINAL_X), ka);
     LAUNCH((k_final<EOS, 1, true>), G(ST_FINAL_Y), st, S(ST_FINAL_Y), ka);
@@ -1241,12 +1238,12 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
         LAUNCH((k_tcorr<2, EOS::NS, FP, P>), grid, st, sc, ka);
       }
       break;
-    case ST_TRANS_XY: LAUNCH((k_trans<EOS, 0, 1, FP, P>), grid, st, sc, ka); break;
-    case ST_TRANS_XZ: LAUNCH((k_trans<EOS, 0, 2, FP, P>), grid, st, sc, ka); break;
-    case ST_TRANS_YX: LAUNCH((k_trans<EOS, 1, 0, FP, P>), grid, st, sc, ka); break;
-    case ST_TRANS_YZ: LAUNCH((k_trans<EOS, 1, 2, FP, P>), grid, st, sc, ka); break;
-    case ST_TRANS_ZX: LAUNCH((k_trans<EOS, 2, 0, FP, P>), grid, st, sc, ka); break;
-    case ST_TRANS_ZY: LAUNCH((k_trans<EOS, 2, 1, FP, P>), grid, st, sc, ka); break;
+    // both transverse fluxes of a direction under its first stage; the
+    // second stage's region is empty
+    case ST_TRANS_XY: LAUNCH((k_trans<EOS, 0, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_YX: LAUNCH((k_trans<EOS, 1, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_ZX: LAUNCH((k_trans<EOS, 2, FP, P>), grid, st, sc, ka); break;
+    case ST_TRANS_XZ: case ST_TRANS_YZ: case ST_TRANS_ZY: break;
     case ST_FINAL_X:
       LAUNCH((k_final<EOS, 0, false, FP, P>), grid, st, sc, ka);
       if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0, FP, P>), grid, st, sc, ka);
@@ -1609,15 +1606,15 @@ int pc_level_enable_timing_v1(pc_level_t* L, int on) {
 }  // extern "C"
 
 // One step of the whole level, each launch covering every box: the
-// Godunov step (21 launches) or the MOL step (two Heun stages of ghost fill,
+// Godunov step (14 to 23 launches) or the MOL step (two Heun stages of ghost fill,
 // primitives, three flux launches and an update; 13 launches).  With timing
 // on, an event follows every launch and the step ends with a host wait, so
 // per-kernel device times accumulate in ktime[] under the kernel's id.
 // The directional stages of a CTU step over the three branch streams:
-// branch d runs trace_d and flux1_d, then the transverse fluxes (d|t) (each
-// after the first fluxes of t, computed on branch t), then final_d (after the
-// transverse fluxes (t0|t1) and (t1|t0) of the other two directions, which
-// the other two branches compute).  Every read of another branch's output
+// branch d runs trace_d and flux1_d, then the two transverse fluxes (d|t)
+// in one launch (after the first fluxes of both t, computed on branches t),
+// then final_d (after the transverse fluxes (t0|t1) and (t1|t0) of the other
+// two directions, which the other two branches compute).  Every read of another branch's output
 // waits for the event recorded after the launch that writes it.
 template <class EOS, class H>
 static void branch_ctu_fluxes(H* L, const Sched* S, const StagePlan* P,
@@ -1628,14 +1625,14 @@ static void branch_ctu_fluxes(H* L, const Sched* S, const StagePlan* P,
     launch_stage<EOS>(L, ST_FLUX1_X + d, S[ST_FLUX1_X + d], P[ST_FLUX1_X + d], B.br[d]);
     B.done(Branches::F1(d), d);
   }
-  for (int d = 0; d < 3; ++d)
-    for (int t = 0; t < 3; ++t) {
-      if (t == d) continue;
-      const int x = ST_TRANS_XY + 2 * d + (t > d ? t - 1 : t);  // (d|t)
-      B.wait(d, Branches::F1(t));
-      launch_stage<EOS>(L, x, S[x], P[x], B.br[d]);
-      B.done(Branches::TR(d, t), d);
-    }
+  for (int d = 0; d < 3; ++d) {  // (d|t0) and (d|t1) in one launch
+    const int x = ST_TRANS_XY + 2 * d;
+    for (int t = 0; t < 3; ++t)
+      if (t != d) B.wait(d, Branches::F1(t));
+    launch_stage<EOS>(L, x, S[x], P[x], B.br[d]);
+    for (int t = 0; t < 3; ++t)
+      if (t != d) B.done(Branches::TR(d, t), d);
+  }
   for (int d = 0; d < 3; ++d) {
     const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
     B.wait(d, Branches::TR(t0, t1));
@@ -1842,8 +1839,8 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
 // Rewire a graph captured on one stream (a chain of kernel nodes, tags[i] the
 // stage tag of node i) into the dependencies of the branch streams
 // (branch_ctu_fluxes, branch_mol_fluxes): per direction d, trace_d (and the
-// first fluxes) after divu; the transverse fluxes (d|t) after the first
-// fluxes of d and t; final_d after the transverse fluxes (t0|t1) and (t1|t0)
+// first fluxes) after divu; the transverse fluxes of d (one node for both t)
+// after the first fluxes of all three directions; final_d after the transverse fluxes (t0|t1) and (t1|t0)
 // and the first fluxes of d; the update after the three finals; the MOL
 // fluxes of the three directions after the primitives, the update after all
 // three; every other node after its predecessor.  Returns false, with the
@@ -1881,10 +1878,11 @@ static bool rewire_branches(hipGraph_t g, const std::vector<int>& tags) {
       dep[i] = {F1[d]};
       F1[d] = k;
     } else if (st >= ST_TRANS_XY && st <= ST_TRANS_ZY) {
-      const int x = st - ST_TRANS_XY, d = x / 2;
-      const int t = (x % 2 == 0) ? (d == 0 ? 1 : 0) : (d == 2 ? 1 : 2);
-      dep[i] = {F1[d], F1[t]};
-      TR[d][t] = k;
+      // both transverse fluxes of d (the second stage launches nothing)
+      const int d = (st - ST_TRANS_XY) / 2;
+      const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
+      dep[i] = {F1[d], F1[t0], F1[t1]};
+      TR[d][t0] = TR[d][t1] = k;
     } else if (st >= ST_FINAL_X && st <= ST_FINAL_Z) {
       const int d = st - ST_FINAL_X;
       const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;

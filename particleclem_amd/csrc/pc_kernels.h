@@ -13,7 +13,9 @@
 //                coefficient on the cells grow(bx,1)
 //   trace_d      cells grow(bx,1); writes qm on face c+1 and qp on face c
 //   flux1_d      d-faces [L, H+1] x transverse [L-1, H+1]
-//   trans_(d|t)  d-faces [L, H+1] x t in [L, H] x t' in [L-1, H+1]
+//   trans_d      d-faces [L, H+1] x [L-1, H+1]^2: (d|t) where t is in
+//                [L, H], for both transverse t (under the stage of the
+//                first t; the stage of the second is empty)
 //   final_d      d-faces [L, H+1] x transverse [L, H]
 //   update       cells [L, H]
 // and of the method-of-lines step:
@@ -302,17 +304,18 @@ __host__ __device__ inline void stage_region(int st, const int L[3],
       set(D, L[D], H[D] + 1);
       break;
     }
-    case ST_TRANS_XY: case ST_TRANS_XZ: case ST_TRANS_YX:
-    case ST_TRANS_YZ: case ST_TRANS_ZX: case ST_TRANS_ZY: {
-      const int idx = st - ST_TRANS_XY;
-      const int D = idx / 2;
-      const int T = (idx % 2 == 0) ? (D == 0 ? 1 : 0) : (D == 2 ? 1 : 2);
-      const int Tp = 3 - D - T;
+    // both transverse fluxes of direction D in one launch under the first
+    // slot, (D|t0) where t0 is in [L, H] and (D|t1) where t1 is; the second
+    // slot is empty
+    case ST_TRANS_XY: case ST_TRANS_YX: case ST_TRANS_ZX: {
+      const int D = (st - ST_TRANS_XY) / 2;
+      for (int d = 0; d < 3; ++d) set(d, L[d] - 1, H[d] + 1);
       set(D, L[D], H[D] + 1);
-      set(T, L[T], H[T]);
-      set(Tp, L[Tp] - 1, H[Tp] + 1);
       break;
     }
+    case ST_TRANS_XZ: case ST_TRANS_YZ: case ST_TRANS_ZY:
+      for (int d = 0; d < 3; ++d) set(d, L[d], L[d] - 1);
+      break;
     case ST_FINAL_X: case ST_FINAL_Y: case ST_FINAL_Z: {
       const int D = st - ST_FINAL_X;
       set(D, L[D], H[D] + 1);
@@ -757,23 +760,17 @@ PCD void dflux(const VW& f, int i, int j, int k, FP c, DFlux<NS, FP>& D) {
 }
 
 // ------------------------------------- K9-16 transverse (d|t) fluxes ----
-// pc_transdo on the two cells of a d-face, then pc_cmpflx
+// pc_transdo on the two cells of d-face (i,j,k), then pc_cmpflx
 // (Source/Godunov.cpp:246-384).
 template <class EOS, int D, int T, class FP>
-PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
+PCD void trans_face(const JobT<FP>& J, int i, int j, int k,
+                    Flux<EOS::NS, FP>& F) {
   constexpr int NS = EOS::NS;
-  int local;
-  const JobT<FP> J = load_job<FP>(sc, lb_, local);
-  int lo[3], n[3], i, j, k;
-  stage_region(ST_TRANS_XY + 2 * D + tslot(D, T), J.lo, J.hi, lo, n);
-  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const FP cdtdx = FP(1.0) / FP(3.0) * get_dt(J, ka) / J.dx[T];
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   FaceState<NS, FP> s, L, R;
   TCorr<NS, FP> df;
   const VS<FP> tr = split(J.tr[D]), dl = split(J.dl[T]), qa = split(J.qa);
-  (void)cdtdx;
   // minus side: cell f - e_D, state on its high face
   load_trace<NS, FP>(tr, i, j, k, 0, s);
   load_tcorr<NS, FP>(dl, im, jm, km, df);
@@ -784,18 +781,41 @@ PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   transdo_side<NS>(s, df, R);
   const FP cav = FP(0.5) * (qa(i, j, k, QC) + qa(im, jm, km, QC));
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-  Flux<NS, FP> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
-  store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T)]), i, j, k, F);
 }
-template <class EOS, int D, int T, class FP = double, bool PERSIST = false>
+// Both transverse fluxes of a d-face in one thread: the second reads the
+// face's traces again right after the first (from L2, not HBM).
+template <class EOS, int D, class FP>
+PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
+  constexpr int NS = EOS::NS;
+  constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_TRANS_XY + 2 * D, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  (void)ka;
+  const int c[3] = {i, j, k};
+  // (D|t) is read on the faces whose t index lies in [L, H]
+  if (c[T0] >= J.lo[T0] && c[T0] <= J.hi[T0]) {
+    Flux<NS, FP> F;
+    trans_face<EOS, D, T0, FP>(J, i, j, k, F);
+    store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T0)]), i, j, k, F);
+  }
+  if (c[T1] >= J.lo[T1] && c[T1] <= J.hi[T1]) {
+    Flux<NS, FP> F;
+    trans_face<EOS, D, T1, FP>(J, i, j, k, F);
+    store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T1)]), i, j, k, F);
+  }
+}
+template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_trans(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_trans<EOS, D, T, FP>(sc, ka, lb_);
+      b_trans<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_trans<EOS, D, T, FP>(sc, ka, first_block());
+    b_trans<EOS, D, FP>(sc, ka, first_block());
   }
 }
 
