@@ -468,7 +468,10 @@ int pc_batch_slot_extent_v1(const pc_batch_t* batch, int slot,
 int pc_batch_add_v1(pc_batch_t* batch, pc_box_t bx, const int* bclo,
                     const int* bchi, const double* dx, pc_array4_t prim,
                     int prim_mode, void* stream, int* slot);
-/* One step of each listed request (distinct live slots); asynchronous. */
+/* One step of each listed request (distinct live slots); asynchronous.  When
+ * the set of listed slots changes, its job tables go to the device on
+ * `stream` through the next of four rotating table sets; the host waits only
+ * if that set's last step is still queued. */
 int pc_batch_step_v1(pc_batch_t* batch, const int* slots, int n, void* stream);
 /* on = 1: the batch's steps run the three directions' independent stages on
  * three streams, as pc_level_set_branches_v1; results are unchanged.  No
@@ -480,7 +483,17 @@ int pc_batch_read_v1(pc_batch_t* batch, int slot, double* buf);
 /* dt of its next step and the NaN / dt_cutoff flags; synchronous. */
 int pc_batch_status_v1(pc_batch_t* batch, int slot, double* dt, int* bad,
                        int* small_dt);
-/* Retire a request and return its memory (waits for `stream`). */
+/* The same without waiting for the device (no reference counterpart; for a
+ * request loop that keeps steps queued ahead of the host): snapshot queues a
+ * copy of the request's dt state to pinned host memory on `stream` (after
+ * its last step); snapshot_status returns what the copy brought once the
+ * caller has seen the stream pass that point (an event). */
+int pc_batch_snapshot_v1(pc_batch_t* batch, int slot, void* stream);
+int pc_batch_snapshot_status_v1(const pc_batch_t* batch, int slot, double* dt,
+                                int* bad, int* small_dt);
+/* Retire a request and return its memory.  stream: waited for first (the
+ * stream of the request's last step or read); NULL: no wait, the caller has
+ * seen the request's work complete. */
 int pc_batch_remove_v1(pc_batch_t* batch, int slot, void* stream);
 
 #ifdef __cplusplus

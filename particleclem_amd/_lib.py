@@ -37,6 +37,7 @@ EXPORTS = [
     "pc_batch_memory_v1", "pc_batch_free_ranges_v1", "pc_batch_slot_extent_v1",
     "pc_batch_add_v1", "pc_batch_step_v1", "pc_batch_set_branches_v1",
     "pc_batch_read_v1", "pc_batch_status_v1", "pc_batch_remove_v1",
+    "pc_batch_snapshot_v1", "pc_batch_snapshot_status_v1",
 ]
 
 PC_OK = 0
@@ -133,6 +134,9 @@ def lib():
     L.pc_batch_read_v1.argtypes = [vp, C.c_int, vp]
     L.pc_batch_status_v1.argtypes = [vp, C.c_int, P(C.c_double), P(C.c_int), P(C.c_int)]
     L.pc_batch_remove_v1.argtypes = [vp, C.c_int, vp]
+    L.pc_batch_snapshot_v1.argtypes = [vp, C.c_int, vp]
+    L.pc_batch_snapshot_status_v1.argtypes = [vp, C.c_int, P(C.c_double), P(C.c_int),
+                                              P(C.c_int)]
     L.pc_level_set_plan_v1.argtypes = [vp] + [C.c_int] * 5
     L.pc_level_create_v1.argtypes = [vp, P(LevelDesc), P(Params), P(vp)]
     L.pc_level_destroy_v1.argtypes = [vp]

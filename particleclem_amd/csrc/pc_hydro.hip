@@ -2620,10 +2620,27 @@ struct pc_batch {
   Arena* arena = nullptr;  // possibly shared with other batches
   std::vector<BatchSlot> slots;
   int max_slots = 0;
-  Job* djobs = nullptr;   // device job table of the stepped subset
+  // The tables of the stepped subset in NTAB rotating sets, so a new set is
+  // written (pinned host -> device, on the step's stream) while steps that
+  // read an older one may still be queued: set k is rewritten only after
+  // tev[k], recorded behind the last step that read it, has completed.
+  static constexpr int NTAB = 4;
+  Job* djobs = nullptr;   // device job table of the stepped subset (set tab)
   int* dstart = nullptr;  // device [ST_COUNT][max_slots] first blocks
   int* dids = nullptr;    // device slot ids of the stepped subset
-  Job* dinit = nullptr;   // device job of a request being admitted
+  Job* tjobs[NTAB] = {};  // device sets
+  int* tstart[NTAB] = {};
+  int* tids[NTAB] = {};
+  Job* hjobs[NTAB] = {};  // pinned host staging of each set
+  int* hstart[NTAB] = {};
+  int* hids[NTAB] = {};
+  hipEvent_t tev[NTAB] = {};
+  bool tev_made = false;
+  int tab = 0;
+  Job* dslot = nullptr;     // device job of each slot (admission kernels)
+  Job* hslot = nullptr;     // pinned staging of dslot
+  DtState* hds0 = nullptr;  // pinned initial dt state of each slot
+  DtState* hsnap = nullptr; // pinned snapshots of each slot's dt state
   std::vector<int> cur;   // the stepped subset the tables describe
   StagePlan plans[ST_COUNT];
   Sched scs[ST_COUNT];
@@ -2643,25 +2660,43 @@ static void batch_free(pc_batch* B) {
     delete B->arena;
   }
   if (B->ds) (void)hipFree(B->ds);
-  if (B->djobs) (void)hipFree(B->djobs);
-  if (B->dstart) (void)hipFree(B->dstart);
-  if (B->dids) (void)hipFree(B->dids);
-  if (B->dinit) (void)hipFree(B->dinit);
+  for (int k = 0; k < pc_batch::NTAB; ++k) {
+    if (B->tjobs[k]) (void)hipFree(B->tjobs[k]);
+    if (B->tstart[k]) (void)hipFree(B->tstart[k]);
+    if (B->tids[k]) (void)hipFree(B->tids[k]);
+    if (B->hjobs[k]) (void)hipHostFree(B->hjobs[k]);
+    if (B->hstart[k]) (void)hipHostFree(B->hstart[k]);
+    if (B->hids[k]) (void)hipHostFree(B->hids[k]);
+    if (B->tev_made) (void)hipEventDestroy(B->tev[k]);
+  }
+  if (B->dslot) (void)hipFree(B->dslot);
+  if (B->hslot) (void)hipHostFree(B->hslot);
+  if (B->hds0) (void)hipHostFree(B->hds0);
+  if (B->hsnap) (void)hipHostFree(B->hsnap);
   B->brs.release();
   delete B;
 }
 
-// Job table, per-stage first blocks and slot ids of `ids` on the device.
+// Job table, per-stage first blocks and slot ids of `ids` on the device, in
+// the next of the rotating table sets (copied on stream st, no host wait
+// unless that set's last reader is still queued).
 static int batch_tables(pc_batch* B, const std::vector<int>& ids,
                         hipStream_t st) {
   if (ids == B->cur) return PC_OK;
   pc_ctx* c = B->ctx;
   const int n = (int)ids.size();
-  std::vector<Job> jobs(n);
-  std::vector<int> start((size_t)ST_COUNT * B->max_slots, 0);
+  const int t = (B->tab + 1) % pc_batch::NTAB;
+  HIPCHK(c, hipEventSynchronize(B->tev[t]));
+  Job* jobs = B->hjobs[t];
+  int* start = B->hstart[t];
+  memset(start, 0, sizeof(int) * ST_COUNT * B->max_slots);
   for (int k = 0; k < n; ++k) jobs[k] = B->slots[ids[k]].job;
-  B->fuse = !B->mol && B->prm.ppm_type == 1 && pencils_fit(jobs.data(), n);
-  B->trace_cells = trace_cells_of(jobs.data(), n, B->fuse);
+  B->fuse = !B->mol && B->prm.ppm_type == 1 && pencils_fit(jobs, n);
+  B->trace_cells = trace_cells_of(jobs, n, B->fuse);
+  B->tab = t;
+  B->djobs = B->tjobs[t];
+  B->dstart = B->tstart[t];
+  B->dids = B->tids[t];
   for (int s = 0; s < ST_COUNT; ++s) {
     int tot = 0;
     const int lx = (s == ST_TRACE && B->prm.ppm_type == 1 && !B->mol)
@@ -2675,12 +2710,11 @@ static int batch_tables(pc_batch* B, const std::vector<int>& ids,
     B->scs[s] = Sched{B->djobs, nullptr, B->dstart + (size_t)s * B->max_slots,
                       tot, -1, 0, 0, n};
   }
-  // the previous step may still read the old tables
-  HIPCHK(c, hipStreamSynchronize(st));
-  HIPCHK(c, hipMemcpy(B->djobs, jobs.data(), sizeof(Job) * n, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(B->dstart, start.data(), sizeof(int) * start.size(),
-                      hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(B->dids, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  memcpy(B->hids[t], ids.data(), sizeof(int) * n);
+  HIPCHK(c, hipMemcpyAsync(B->djobs, jobs, sizeof(Job) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B->dstart, start, sizeof(int) * ST_COUNT * B->max_slots,
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B->dids, B->hids[t], sizeof(int) * n, hipMemcpyHostToDevice, st));
   B->cur = ids;
   return PC_OK;
 }
@@ -2780,14 +2814,47 @@ int pc_batch_create_v1(pc_ctx_t* c, const pc_params_t* prm, int mixed,
     }
     B->arena->fr[0] = B->arena->size;
   }
-  if (!dalloc((void**)&B->ds, sizeof(DtState) * max_slots) ||
-      !dalloc((void**)&B->djobs, sizeof(Job) * max_slots) ||
-      !dalloc((void**)&B->dstart, sizeof(int) * ST_COUNT * max_slots) ||
-      !dalloc((void**)&B->dids, sizeof(int) * max_slots) ||
-      !dalloc((void**)&B->dinit, sizeof(Job))) {
+  auto halloc = [&](void** p, size_t bytes) {
+    if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return false;
+    }
+    return true;
+  };
+  bool ok = dalloc((void**)&B->ds, sizeof(DtState) * max_slots) &&
+            dalloc((void**)&B->dslot, sizeof(Job) * max_slots) &&
+            halloc((void**)&B->hslot, sizeof(Job) * max_slots) &&
+            halloc((void**)&B->hds0, sizeof(DtState) * max_slots) &&
+            halloc((void**)&B->hsnap, sizeof(DtState) * max_slots);
+  for (int k = 0; ok && k < pc_batch::NTAB; ++k)
+    ok = dalloc((void**)&B->tjobs[k], sizeof(Job) * max_slots) &&
+         dalloc((void**)&B->tstart[k], sizeof(int) * ST_COUNT * max_slots) &&
+         dalloc((void**)&B->tids[k], sizeof(int) * max_slots) &&
+         halloc((void**)&B->hjobs[k], sizeof(Job) * max_slots) &&
+         halloc((void**)&B->hstart[k], sizeof(int) * ST_COUNT * max_slots) &&
+         halloc((void**)&B->hids[k], sizeof(int) * max_slots);
+  if (ok) {
+    int made = 0;
+    while (made < pc_batch::NTAB &&
+           hipEventCreateWithFlags(&B->tev[made], hipEventDisableTiming) == hipSuccess)
+      ++made;
+    if (made < pc_batch::NTAB) {
+      for (int k = 0; k < made; ++k) (void)hipEventDestroy(B->tev[k]);
+      (void)hipGetLastError();
+      ok = false;
+    } else {
+      B->tev_made = true;
+    }
+  }
+  if (!ok) {
     batch_free(B);
     return fail(c, PC_ERR_NOMEM, "batch tables allocation failed");
   }
+  B->tab = 0;
+  B->djobs = B->tjobs[0];
+  B->dstart = B->tstart[0];
+  B->dids = B->tids[0];
   *out = B;
   return PC_OK;
 }
@@ -2905,10 +2972,14 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
   z.bad = 0;
   z.small = 0;
   z.time = 0.0;
-  HIPCHK(c, hipStreamSynchronize(st));
-  HIPCHK(c, hipMemcpy(B->dinit, &J, sizeof(Job), hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(B->ds + id, &z, sizeof(z), hipMemcpyHostToDevice));
-  const Sched s0 = one_job_sched(B->dinit);
+  // the slot's staging is free: a slot is reused only after its previous
+  // request's work has completed (pc_batch_remove_v1)
+  B->hslot[id] = J;
+  B->hds0[id] = z;
+  HIPCHK(c, hipMemcpyAsync(B->dslot + id, B->hslot + id, sizeof(Job),
+                           hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B->ds + id, B->hds0 + id, sizeof(z), hipMemcpyHostToDevice, st));
+  const Sched s0 = one_job_sched(B->dslot + id);
   const int g = grid1(ST_UPDATE, J);
   if (c->ns == 1) {
     LAUNCH(k_prim_to_cons<EosGamma>, g, st, s0, B->ka, pv, prim_mode);
@@ -2952,7 +3023,28 @@ int pc_batch_step_v1(pc_batch_t* B, const int* slots, int n, void* stream) {
   else
     batch_step_eos<EosLiDryer>(B, n, st);
   HIPCHK(c, hipGetLastError());
+  // the current table set is in use until this step has run
+  HIPCHK(c, hipEventRecord(B->tev[B->tab], st));
   return PC_OK;
+}
+
+int pc_batch_snapshot_v1(pc_batch_t* B, int slot, void* stream) {
+  if (!B || slot < 0 || slot >= B->max_slots || !B->slots[slot].live)
+    return PC_ERR_ARG;
+  HIPCHK(B->ctx, hipMemcpyAsync(B->hsnap + slot, B->ds + slot, sizeof(DtState),
+                                hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return PC_OK;
+}
+
+int pc_batch_snapshot_status_v1(const pc_batch_t* B, int slot, double* dt,
+                                int* bad, int* small_dt) {
+  if (!B || slot < 0 || slot >= B->max_slots || !B->slots[slot].live)
+    return PC_ERR_ARG;
+  const DtState s = B->hsnap[slot];
+  if (dt) *dt = s.dt;
+  if (bad) *bad = s.bad;
+  if (small_dt) *small_dt = s.small;
+  return s.bad ? PC_ERR_NAN : (s.small ? PC_ERR_DT : PC_OK);
 }
 
 int pc_batch_read_v1(pc_batch_t* B, int slot, double* buf) {
@@ -2981,7 +3073,7 @@ int pc_batch_status_v1(pc_batch_t* B, int slot, double* dt, int* bad,
 int pc_batch_remove_v1(pc_batch_t* B, int slot, void* stream) {
   if (!B || slot < 0 || slot >= B->max_slots || !B->slots[slot].live)
     return PC_ERR_ARG;
-  HIPCHK(B->ctx, hipStreamSynchronize((hipStream_t)stream));
+  if (stream) HIPCHK(B->ctx, hipStreamSynchronize((hipStream_t)stream));
   BatchSlot& S = B->slots[slot];
   B->arena->release(S.off, S.bytes);
   S.live = false;

@@ -4,6 +4,9 @@ the device memory budget allows, and advance together: every tick steps
 each admitted request once, with one launch per stage over all the requests
 of one mechanism (continuous batching: a request that arrives while others
 run joins at the next tick, a finished one leaves and frees its memory).
+The host queues ticks ahead of the device and never waits for it as a
+whole: completions are seen through events and the finished requests' dt
+states through asynchronous snapshots.
 
 Request fields are generated on the device from the request's own seed
 (torch's counter-based CUDA generator), so a request starts from the same
@@ -103,8 +106,23 @@ class Batch:
                                        C.byref(small))
         return rc, dt.value, bad.value, small.value
 
-    def remove(self, slot, stream=None):
-        self._chk(self.L.pc_batch_remove_v1(self.h, slot, _stream_ptr(stream)))
+    def snapshot(self, slot, stream=None):
+        """Queue a copy of the request's dt state to the host on `stream`."""
+        self._chk(self.L.pc_batch_snapshot_v1(self.h, slot, _stream_ptr(stream)))
+
+    def snapshot_status(self, slot):
+        """(status, dt, bad, small) from the last snapshot, once it has
+        landed (no device wait)."""
+        dt, bad, small = C.c_double(), C.c_int(), C.c_int()
+        rc = self.L.pc_batch_snapshot_status_v1(self.h, slot, C.byref(dt), C.byref(bad),
+                                                C.byref(small))
+        return rc, dt.value, bad.value, small.value
+
+    def remove(self, slot, stream=None, wait=True):
+        """Return the request's memory after waiting for `stream` (the
+        current stream by default); wait=False: its work is known done."""
+        ptr = _stream_ptr(stream) if wait else C.c_void_p(None)
+        self._chk(self.L.pc_batch_remove_v1(self.h, slot, ptr))
 
 
 class Request:
@@ -120,6 +138,7 @@ class Request:
         self.result = None
         self.status = None
         self.rejected = False      # larger than the whole arena: never runs
+        self.ev = None             # event after its last step, while retiring
 
     @property
     def cell_updates(self):
@@ -360,21 +379,24 @@ class Server:
         r.off, r.bytes = b.extent(r.slot)
         r.t_admit = time.perf_counter() - t0
 
-    def _free_slots(self, active):
+    def _free_slots(self, holding):
         free = {k: self.max_slots for k in self.batch}
-        for r in active:
+        for r in holding:
             free[r.kind] -= 1
         return free
 
-    def _admit_backfill(self, waiting, active, t0):
+    def _admit_backfill(self, waiting, active, retiring, t0):
         for r in waiting:
             if r.bytes is None:
                 r.bytes = ArenaModel.round(self.batch[r.kind].slot_bytes(r.n))
         t = time.perf_counter()
         model = ArenaModel(self.batch[SLAB].free_ranges())
-        held = [(r.nsteps - r.done, r.off, r.bytes, r.kind) for r in active]
+        # a retiring request's block returns once its last step has run (by
+        # the next tick, as far as the plan is concerned)
+        held = ([(r.nsteps - r.done, r.off, r.bytes, r.kind) for r in active] +
+                [(1, r.off, r.bytes, r.kind) for r in retiring])
         plan = backfill_plan(waiting, held, model, self.bypass_depth,
-                             self._free_slots(active))
+                             self._free_slots(active + retiring))
         self.plan_s += time.perf_counter() - t
         for j, off in plan:
             r = waiting[j]
@@ -386,18 +408,49 @@ class Server:
         for j, _ in sorted(plan, reverse=True):
             del waiting[j]
 
-    def run(self, requests, realtime=True, on_tick=None):
+    def run(self, requests, realtime=True, on_tick=None, ahead=2):
         """Serve `requests` (sorted by arrival).  realtime: arrivals follow
         their timestamps on the wall clock; else all are queued at t = 0.
-        Returns the list of served requests with t_admit / t_done set."""
+        The host never waits for the whole device: a tick's steps are queued
+        on the batches' streams and followed by an event; a request whose
+        last step is queued holds its memory until that event has passed
+        (its dt state comes back through an asynchronous snapshot), and at
+        most `ahead` ticks are queued beyond the ones known complete.
+        Returns the list of served requests with t_admit and t_done (when
+        the device finished its last step) set, in seconds from the start."""
         import torch
         pending = deque(sorted(requests, key=lambda r: r.t_arrive))
         waiting = deque()
-        active = []
+        active = []      # admitted, steps still to queue
+        retiring = []    # every step queued; memory held until its event
         served = []
+        inflight = deque()  # events of the queued ticks, oldest first
         t0 = time.perf_counter()
+        base = None      # event at host time t_base (seconds from t0)
+        t_base = 0.0
         ticks = 0
-        while pending or waiting or active:
+
+        def retire(block):
+            nonlocal retiring
+            left = []
+            for r in retiring:
+                if not (block or r.ev.query()):
+                    left.append(r)
+                    continue
+                r.ev.synchronize()
+                r.t_done = t_base + base.elapsed_time(r.ev) / 1e3
+                b = self.batch[r.kind]
+                if self.keep_results:
+                    r.result = b.read(r.slot, r.n)
+                r.status = b.snapshot_status(r.slot)[0]
+                b.remove(r.slot, wait=False)
+                r.ev = None
+                served.append(r)
+            retiring = left
+
+        while pending or waiting or active or retiring:
+            if retiring:
+                retire(block=False)
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].t_arrive <= now):
                 r = pending.popleft()
@@ -412,9 +465,9 @@ class Server:
                     continue
                 waiting.append(r)
             if self.admission == "backfill":
-                self._admit_backfill(waiting, active, t0)
+                self._admit_backfill(waiting, active, retiring, t0)
             i = 0
-            free = self._free_slots(active)
+            free = self._free_slots(active + retiring)
             while self.admission != "backfill" and i < len(waiting):
                 r = waiting[i]
                 b = self.batch[r.kind]
@@ -432,29 +485,46 @@ class Server:
                 del waiting[i]
                 active.append(r)
             if not active:
-                if pending and realtime:
+                if retiring:
+                    retire(block=True)  # memory comes back only from them
+                elif pending and realtime:
                     time.sleep(max(0.0, pending[0].t_arrive - (time.perf_counter() - t0)))
                 continue
+            while len(inflight) >= ahead:
+                for ev in inflight.popleft():
+                    ev.synchronize()
+            # admissions ran on the current stream (pc_batch_add_v1 returns
+            # once they have); each tick's steps after them
+            cur = torch.cuda.current_stream()
+            if base is None:
+                base = torch.cuda.Event(enable_timing=True)
+                base.record(cur)
+                t_base = time.perf_counter() - t0
+            evs = {}
             for kind, b in self.batch.items():
-                b.step([r.slot for r in active if r.kind == kind], self.streams[kind])
+                mine = [r for r in active if r.kind == kind]
+                if not mine:
+                    continue
+                st = self.streams[kind] or cur
+                if st is not cur:
+                    st.wait_stream(cur)
+                b.step([r.slot for r in mine], st)
+                for r in mine:
+                    r.done += 1
+                    if r.done == r.nsteps:
+                        b.snapshot(r.slot, st)
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record(st)
+                evs[kind] = ev
+            inflight.append(list(evs.values()))
             ticks += 1
             for r in active:
-                r.done += 1
-            finished = [r for r in active if r.done == r.nsteps]
-            if finished:
-                torch.cuda.synchronize()
-                t = time.perf_counter() - t0
-                for r in finished:
-                    r.t_done = t
-                    b = self.batch[r.kind]
-                    if self.keep_results:
-                        r.result = b.read(r.slot, r.n)
-                    r.status = b.status(r.slot)[0]
-                    b.remove(r.slot)
-                    served.append(r)
-                active = [r for r in active if r.done < r.nsteps]
+                if r.done == r.nsteps:
+                    r.ev = evs[r.kind]
+                    retiring.append(r)
+            active = [r for r in active if r.done < r.nsteps]
             if on_tick:
-                on_tick(ticks, active)
+                on_tick(ticks, active + retiring)
         self.wall = time.perf_counter() - t0
         self.ticks = ticks
         return served
