@@ -418,7 +418,9 @@ PCD FP flatten_line(bool inside, PF&& pq, UF&& unq) {
 
 // --------------------------------------------------------------- PPM (A7) --
 // Source/PPM.H:28-115: CW84 edge values, van Leer slopes, flattening blend,
-// Colella-Sekora parabola limiter.
+// Colella-Sekora parabola limiter.  The reference's sign(dsc) * min(|.|)
+// is written copysign(min(|.|), dsc): the minimum is never negative, so the
+// product by +-1 is that value with dsc's sign, bit for bit (zeros included).
 template <class FP>
 PCD void ppm_reconstruct(const FP s[5], FP flatn, FP& sm,
                          FP& sp) {
@@ -427,14 +429,14 @@ PCD void ppm_reconstruct(const FP s[5], FP flatn, FP& sm,
   FP dsvl_l = FP(0.0);
   if (dsl * dsr > FP(0.0)) {
     const FP dsc = FP(0.5) * (s[2] - s[0]);
-    dsvl_l = copysign(FP(1.0), dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
+    dsvl_l = copysign(mn(fabs(dsc), mn(fabs(dsl), fabs(dsr))), dsc);
   }
   dsl = FP(2.0) * (s[2] - s[1]);
   dsr = FP(2.0) * (s[3] - s[2]);
   FP dsvl_r = FP(0.0);
   if (dsl * dsr > FP(0.0)) {
     const FP dsc = FP(0.5) * (s[3] - s[1]);
-    dsvl_r = copysign(FP(1.0), dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
+    dsvl_r = copysign(mn(fabs(dsc), mn(fabs(dsl), fabs(dsr))), dsc);
   }
   sm = FP(0.5) * (s[2] + s[1]) - (FP(1.0) / FP(6.0)) * (dsvl_r - dsvl_l);
   sm = mx(sm, mn(s[2], s[1]));
@@ -446,7 +448,7 @@ PCD void ppm_reconstruct(const FP s[5], FP flatn, FP& sm,
   dsvl_r = FP(0.0);
   if (dsl * dsr > FP(0.0)) {
     const FP dsc = FP(0.5) * (s[4] - s[2]);
-    dsvl_r = copysign(FP(1.0), dsc) * mn(fabs(dsc), mn(fabs(dsl), fabs(dsr)));
+    dsvl_r = copysign(mn(fabs(dsc), mn(fabs(dsl), fabs(dsr))), dsc);
   }
   sp = FP(0.5) * (s[3] + s[2]) - (FP(1.0) / FP(6.0)) * (dsvl_r - dsvl_l);
   sp = mx(sp, mn(s[3], s[2]));
