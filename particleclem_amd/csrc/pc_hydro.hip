@@ -403,9 +403,11 @@ static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz) {
   int lo[3], n[3];
   stage_region(st, J.lo, J.hi, lo, n);
   if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return 0;
-  if (lx < 0) return ((long long)n[0] * n[1] * n[2] + BLOCK - 1) / BLOCK;
+  // the transverse stage runs two threads per face (k_trans)
+  const int per = (st == ST_TRANS_XY || st == ST_TRANS_YX || st == ST_TRANS_ZX) ? 2 : 1;
+  if (lx < 0) return (per * (long long)n[0] * n[1] * n[2] + BLOCK - 1) / BLOCK;
   auto nt = [](int m, int l) { return (long long)((m + (1 << l) - 1) >> l); };
-  return nt(n[0], lx) * nt(n[1], ly) * nt(n[2], lz);
+  return per * nt(n[0], lx) * nt(n[1], ly) * nt(n[2], lz);
 }
 
 static int grid1(int st, const Job& J) {

@@ -760,17 +760,62 @@ PCD void dflux(const VW& f, int i, int j, int k, FP c, DFlux<NS, FP>& D) {
 }
 
 // ------------------------------------- K9-16 transverse (d|t) fluxes ----
-// pc_transdo on the two cells of d-face (i,j,k), then pc_cmpflx
-// (Source/Godunov.cpp:246-384).
-template <class EOS, int D, int T, class FP>
-PCD void trans_face(const JobT<FP>& J, int i, int j, int k,
-                    Flux<EOS::NS, FP>& F) {
+// One of two views of the job by a per-lane condition.  Each field is first
+// pinned in a register (an empty asm), so the selection is made on the
+// values: a select between the two fields' addresses would keep the job's
+// copy in private (scratch) memory.
+template <class T>
+PCD VT<T> lane_select(bool c, const VT<T>& a, const VT<T>& b) {
+  unsigned long long pa = (unsigned long long)a.p, pb = (unsigned long long)b.p;
+  int fa[6] = {a.js, a.ks, a.ns, a.b0, a.b1, a.b2};
+  int fb[6] = {b.js, b.ks, b.ns, b.b0, b.b1, b.b2};
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(pa), "+v"(pb));
+#pragma unroll
+  for (int m = 0; m < 6; ++m) asm("" : "+v"(fa[m]), "+v"(fb[m]));
+#endif
+  VT<T> r;
+  r.p = (T*)(c ? pa : pb);
+  r.js = c ? fa[0] : fb[0];
+  r.ks = c ? fa[1] : fb[1];
+  r.ns = c ? fa[2] : fb[2];
+  r.b0 = c ? fa[3] : fb[3];
+  r.b1 = c ? fa[4] : fb[4];
+  r.b2 = c ? fa[5] : fb[5];
+  return r;
+}
+
+// pc_transdo on the two cells of a d-face, then pc_cmpflx
+// (Source/Godunov.cpp:246-384), for both transverse fluxes of the face by a
+// pair of neighbouring lanes: the even lane solves (D|t0), the odd lane
+// (D|t1).  The two lanes load the face's traces by the same instruction, so
+// each trace is fetched from HBM once for both fluxes (one thread solving
+// both read them twice, the second time mostly from HBM again), and each
+// lane holds one pair of states.  The launch has two threads per face of the
+// region (stage_blocks doubles the stage's blocks); the face index of a
+// thread is its local index halved.
+template <class EOS, int D, class FP>
+PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
+  constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  const bool second = (local & 1) != 0;
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_TRANS_XY + 2 * D, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local >> 1, i, j, k)) return;
+  (void)ka;
+  const int c[3] = {i, j, k};
+  // (D|t) is read on the faces whose t index lies in [L, H]
+  const int ct = second ? c[T1] : c[T0];
+  const int tl = second ? J.lo[T1] : J.lo[T0], th = second ? J.hi[T1] : J.hi[T0];
+  if (ct < tl || ct > th) return;
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
+  const VS<FP> tr = split(J.tr[D]), qa = split(J.qa);
+  const VT<FP> dl = lane_select(second, J.dl[T1], J.dl[T0]);
   FaceState<NS, FP> s, L, R;
   TCorr<NS, FP> df;
-  const VS<FP> tr = split(J.tr[D]), dl = split(J.dl[T]), qa = split(J.qa);
   // minus side: cell f - e_D, state on its high face
   load_trace<NS, FP>(tr, i, j, k, 0, s);
   load_tcorr<NS, FP>(dl, im, jm, km, df);
@@ -781,32 +826,11 @@ PCD void trans_face(const JobT<FP>& J, int i, int j, int k,
   transdo_side<NS>(s, df, R);
   const FP cav = FP(0.5) * (qa(i, j, k, QC) + qa(im, jm, km, QC));
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+  Flux<NS, FP> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
-}
-// Both transverse fluxes of a d-face in one thread: the second reads the
-// face's traces again right after the first (from L2, not HBM).
-template <class EOS, int D, class FP>
-PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
-  constexpr int NS = EOS::NS;
-  constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
-  int local;
-  const JobT<FP> J = load_job<FP>(sc, lb_, local);
-  int lo[3], n[3], i, j, k;
-  stage_region(ST_TRANS_XY + 2 * D, J.lo, J.hi, lo, n);
-  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  (void)ka;
-  const int c[3] = {i, j, k};
-  // (D|t) is read on the faces whose t index lies in [L, H]
-  if (c[T0] >= J.lo[T0] && c[T0] <= J.hi[T0]) {
-    Flux<NS, FP> F;
-    trans_face<EOS, D, T0, FP>(J, i, j, k, F);
-    store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T0)]), i, j, k, F);
-  }
-  if (c[T1] >= J.lo[T1] && c[T1] <= J.hi[T1]) {
-    Flux<NS, FP> F;
-    trans_face<EOS, D, T1, FP>(J, i, j, k, F);
-    store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T1)]), i, j, k, F);
-  }
+  const VT<FP> ft =
+      lane_select(second, J.ft[2 * D + tslot(D, T1)], J.ft[2 * D + tslot(D, T0)]);
+  store_flux<NS, FP>(ft, i, j, k, F);
 }
 template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)

@@ -162,6 +162,21 @@ struct LiDryerTab {
        -9.227977e+02}};
 };
 
+// A compile-time FP64 constant that is the addend of an FMA, held in a scalar
+// register pair.  gfx950's three-operand (VOP3) encoding takes no 64-bit
+// literal, and left alone the compiler puts each such addend in a vector
+// register pair (two v_mov_b32, vector issue slots) to use the two-address
+// v_fmac form; from an SGPR pair (two scalar moves, issued beside other
+// waves' vector work) the addend feeds v_fma_f64 directly.  The empty asm
+// only pins the register class: the value and the arithmetic are unchanged.
+template <class FP>
+PCD FP kc(FP c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(FP) == 8) asm("" : "+s"(c));
+#endif
+  return c;
+}
+
 // Species K's polynomials in one NASA range with every coefficient
 // pre-multiplied by RU / W_K (compile-time literals):
 //   cv_K(T) = v0 + v1 T + v2 T^2 + v3 T^3 + v4 T^4                erg/(g K)
@@ -184,14 +199,16 @@ struct Nasa {
   template <class FP>
   PCD static FP cv(FP T) {
     if constexpr (flat) return FP(v0);
-    return fma(T, fma(T, fma(T, fma(T, FP(v4), FP(v3)), FP(v2)), FP(v1)), FP(v0));
+    return fma(T, fma(T, fma(T, fma(T, FP(v4), kc(FP(v3))), kc(FP(v2))), kc(FP(v1))),
+               kc(FP(v0)));
   }
   template <class FP>
   PCD static FP e(FP T) {
-    if constexpr (flat) return fma(T, FP(u1), FP(u0));
-    return fma(T, fma(T, fma(T, fma(T, fma(T, FP(u5), FP(u4)), FP(u3)), FP(u2)),
-                      FP(u1)),
-               FP(u0));
+    if constexpr (flat) return fma(T, FP(u1), kc(FP(u0)));
+    return fma(T, fma(T, fma(T, fma(T, fma(T, FP(u5), kc(FP(u4))), kc(FP(u3))),
+                             kc(FP(u2))),
+                      kc(FP(u1))),
+               kc(FP(u0)));
   }
 };
 
