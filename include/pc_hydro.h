@@ -211,11 +211,15 @@ int pc_hydro_source_box_noalloc_v1(pc_ctx_t* ctx, pc_box_t bx,
  * its own FAB of NVAR comps and 4 ghost cells in one device arena, uniform
  * Cartesian geometry (area = dx*dx, vol = dx^3 as amrex::Geometry on a
  * Cartesian grid, Source/PeleC.cpp:600-617).  The level owns the state and
- * runs the whole step on the device: ghost fill (intra-level copy, periodic,
- * FOExtrap/reflect by the BC map of Source/Setup.cpp:42-121), hydro source,
- * S_new = S_old + dt*hydro_source, reset_internal_energy + computeTemp, and
- * the next dt (estTimeStep + computeNewDt limits), with dt kept in device
- * memory so consecutive steps need no host synchronisation. */
+ * runs the whole step on the device: ghost values (intra-level copy,
+ * periodic, FOExtrap/reflect by the BC map of Source/Setup.cpp:42-121),
+ * hydro source, S_new = S_old + dt*hydro_source, reset_internal_energy +
+ * computeTemp, and the next dt (estTimeStep + computeNewDt limits), with dt
+ * kept in device memory so consecutive steps need no host synchronisation.
+ * The Godunov step reads each ghost value from its source cell inside the
+ * primitives kernel and stores only the first ghost layer back into the FAB
+ * (the artificial viscosity reads it); the method-of-lines step fills all
+ * four layers.  pc_level_fill_ghosts_v1 fills all four on request. */
 typedef struct pc_level pc_level_t;
 
 /* Ghost cells beyond Inflow and UserBC domain faces: the reference fills
@@ -300,6 +304,8 @@ int pc_write_plotfile_v1(const char* dir, int nvars, const char* const* names,
  * last event (so timed steps do not overlap the host).  Kernel ids, in
  * launch order: */
 enum {
+  /* the Godunov step has no ghost-fill launch (its primitives kernel reads
+   * ghost values from their sources): PC_K_GHOST reads 0 there */
   PC_K_GHOST = 0, PC_K_CTOPRIM, PC_K_DIVU, PC_K_TRACE_X, PC_K_TRACE_Y,
   PC_K_TRACE_Z, PC_K_FLUX1_X, PC_K_FLUX1_Y, PC_K_FLUX1_Z,
   /* the transverse fluxes of d: both (d|t) in one launch, timed under the

@@ -1202,7 +1202,9 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
   auto tr = [&](int d) { return sub < 0 || sub == d; };
   switch (s) {
     case ST_GHOST: LAUNCH((k_ghost<EOS::NS, P>), grid, st, sc, ka, L->la); break;
-    case ST_CTOPRIM: LAUNCH((k_ctoprim<EOS, FP, P>), grid, st, sc, ka); break;
+    // the Godunov step's primitives read ghost values from their source
+    // cells (no ghost fill before them)
+    case ST_CTOPRIM: LAUNCH((k_ctoprim_ghost<EOS, FP, P>), grid, st, sc, ka, L->la); break;
     case ST_DIVU: LAUNCH((k_divu<FP, P>), grid, st, sc, ka); break;
     case ST_TRACE:
       if (L->prm.ppm_type == 1 && L->fuse) {
@@ -1366,6 +1368,13 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
       !dalloc((void**)&L->ds, sizeof(DtState))) {
     level_free(L);
     return fail(c, PC_ERR_NOMEM, "level allocation failed");
+  }
+  // the Godunov step stores only the first ghost layer: the outer layers
+  // start at zero rather than as whatever the allocation held
+  if (hipMemset(L->state, 0, o * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    level_free(L);
+    return fail(c, PC_ERR_HIP, "level state initialisation failed");
   }
   // jobs
   std::vector<Job> jobs(d->nboxes);
@@ -1695,13 +1704,11 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st,
         run(ST_MOL_UPD, h == 0 ? PC_K_MOL_UPD1 : PC_K_MOL_UPD2, h);
       }
     } else if (brm) {
-      run(ST_GHOST, PC_K_GHOST);
       run(ST_CTOPRIM, PC_K_CTOPRIM);
       run(ST_DIVU, PC_K_DIVU);
       branch_ctu_fluxes<EOS>(L, S, P, st, L->brs);
       run(ST_UPDATE, PC_K_UPDATE);
     } else {
-      run(ST_GHOST, PC_K_GHOST);
       run(ST_CTOPRIM, PC_K_CTOPRIM);
       run(ST_DIVU, PC_K_DIVU);
       for (int d = 0; d < 3; ++d) run(ST_TRACE, PC_K_TRACE_X + d, d);
@@ -2741,7 +2748,6 @@ static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
                        B->ds, (const int*)B->dids, n, B->la, DT_NEW);
     return;
   }
-  launch_stage<EOS>(B, ST_GHOST, S[ST_GHOST], P[ST_GHOST], st);
   launch_stage<EOS>(B, ST_CTOPRIM, S[ST_CTOPRIM], P[ST_CTOPRIM], st);
   launch_stage<EOS>(B, ST_DIVU, S[ST_DIVU], P[ST_DIVU], st);
   if (B->brs.on) {
@@ -2983,6 +2989,10 @@ int pc_batch_add_v1(pc_batch_t* B, pc_box_t bx, const int* bclo,
   HIPCHK(c, hipMemcpyAsync(B->ds + id, B->hds0 + id, sizeof(z), hipMemcpyHostToDevice, st));
   const Sched s0 = one_job_sched(B->dslot + id);
   const int g = grid1(ST_UPDATE, J);
+  // the Godunov step stores only the first ghost layer: the outer layers
+  // start at zero rather than as the arena's previous contents
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((S.nstate + 255) / 256)), dim3(256), 0, st,
+                     base, (long long)S.nstate, 0.0);
   if (c->ns == 1) {
     LAUNCH(k_prim_to_cons<EosGamma>, g, st, s0, B->ka, pv, prim_mode);
     LAUNCH((k_estdt<EosGamma, false>), g, st, s0, B->ka, B->ds + id);

@@ -386,36 +386,31 @@ PCD void step(int& i, int& j, int& k, int s) {
 }
 
 // ------------------------------------------------------------ K1 ctoprim --
-// Source/PeleCUtilities.H:242-320
-// (REG: the region's stage, ST_CTOPRIM or the MOL step's ST_MOL_PRIM)
-template <class EOS, class FP, int REG>
-PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
+// Source/PeleCUtilities.H:242-320: the primitives and auxiliaries of cell
+// (i,j,k) from its conserved state Uc (every component but UEINT is read)
+template <class EOS, class FP>
+PCD void ctoprim_cell(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
+                      const double* Uc) {
   constexpr int NS = EOS::NS;
-  int local;
-  const JobT<FP> J = load_job<FP>(sc, lb_, local);
-  int lo[3], n[3], i, j, k;
-  stage_region(REG, J.lo, J.hi, lo, n);
-  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
-  const auto& u = J.U;
   const auto& q = J.q;
   const auto& qa = J.qa;
-  const double rho = u(i, j, k, URHO);
+  const double rho = Uc[URHO];
   const double rhoinv = 1.0 / rho;
-  const double vx = u(i, j, k, UMX) * rhoinv;
-  const double vy = u(i, j, k, UMY) * rhoinv;
-  const double vz = u(i, j, k, UMZ) * rhoinv;
+  const double vx = Uc[UMX] * rhoinv;
+  const double vy = Uc[UMY] * rhoinv;
+  const double vz = Uc[UMZ] * rhoinv;
   const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz) -
                         0.5 * rho * 0.0 * 0.0 * 0.0 * 0.0;
   double Y[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    double y = div_by((double)u(i, j, k, UFS + s), rho, rhoinv);
+    double y = div_by(Uc[UFS + s], rho, rhoinv);
     if ((-1e-4 * DEPS < y) && (y < 1e-4 * DEPS)) y = 0.0;
     Y[s] = y;
   }
-  const double e = (u(i, j, k, UEDEN) - kineng) * rhoinv;
+  const double e = (Uc[UEDEN] - kineng) * rhoinv;
   const typename EOS::Thermo th(Y);
-  const double T = th.EY2T(e, u(i, j, k, UTEMP));
+  const double T = th.EY2T(e, Uc[UTEMP]);
   const double p = th.RTY2P(rho, T);
   const double cs = th.RTY2Cs(rho, T);
   const double gam1 = th.RTY2G(rho, T);
@@ -440,6 +435,22 @@ PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
     qa(i, j, k, QCSML) = mx(small_num, small_num * cs);
     qa(i, j, k, QRSPEC) = RU / w;
   }
+}
+
+// (REG: the region's stage, ST_CTOPRIM or the MOL step's ST_MOL_PRIM)
+template <class EOS, class FP, int REG>
+PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
+  constexpr int NVAR = 7 + EOS::NS;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(REG, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  double Uc[NVAR];
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m)
+    if (m != UEINT) Uc[m] = J.U(i, j, k, m);
+  ctoprim_cell<EOS, FP>(J, ka, i, j, k, Uc);
 }
 template <class EOS, class FP = double, bool PERSIST = false,
           int REG = ST_CTOPRIM>

@@ -33,13 +33,13 @@ PCD int ghost_src(int x, int lo, int hi, int bcl, int bch, int& refl) {
   return (x < lo) ? lo : hi;
 }
 
-template <int NS>
-PCD void b_ghost(const Sched& sc, const KArgs& ka, int lb_, LevelArgs la) {
-  constexpr int NVAR = 7 + NS;
-  int local;
-  const Job J = load_job(sc, lb_, local);
-  int i, j, k;
-  if (!ghost_shell_cell(J.lo, J.hi, local, i, j, k)) return;
+// Where ghost cell (i,j,k) of job J takes its values from: the source cell
+// (s[3]), the state it lies in (the job's own, or in a level whose boxes tile
+// the domain the box that owns the source cell) and the reflection kind per
+// direction (r[3], ghost_src).
+template <class J_>
+PCD void ghost_source(const Sched& sc, const J_& J, const LevelArgs& la, int i,
+                      int j, int k, int s[3], int r[3], V& src) {
   // periodic_self: 0 = boxes tile the level domain, 1 = every box is its own
   // periodic domain, 2 = every box is its own domain with its own BCs
   int D0[3], D1[3], bl[3], bh[3];
@@ -51,27 +51,42 @@ PCD void b_ghost(const Sched& sc, const KArgs& ka, int lb_, LevelArgs la) {
     bh[d] = la.periodic_self == 1 ? BC_INTERIOR
                                   : (la.periodic_self == 2 ? J.bchi[d] : la.bchi[d]);
   }
-  int r0, r1, r2;
-  const int s0 = ghost_src(i, D0[0], D1[0], bl[0], bh[0], r0);
-  const int s1 = ghost_src(j, D0[1], D1[1], bl[1], bh[1], r1);
-  const int s2 = ghost_src(k, D0[2], D1[2], bl[2], bh[2], r2);
-  V src = J.U;
+  s[0] = ghost_src(i, D0[0], D1[0], bl[0], bh[0], r[0]);
+  s[1] = ghost_src(j, D0[1], D1[1], bl[1], bh[1], r[1]);
+  s[2] = ghost_src(k, D0[2], D1[2], bl[2], bh[2], r[2]);
+  src = J.U;
   if (!la.periodic_self) {
     const long nx = D1[0] - D0[0] + 1, ny = D1[1] - D0[1] + 1;
-    const int sb = la.cellmap[(s0 - D0[0]) + nx * ((s1 - D0[1]) + ny * (long)(s2 - D0[2]))];
+    const int sb =
+        la.cellmap[(s[0] - D0[0]) + nx * ((s[1] - D0[1]) + ny * (long)(s[2] - D0[2]))];
     src = sc.jobs[sb].U;
   }
-  const int r[3] = {r0, r1, r2};
+}
+
+// Sign of component m of a ghost value: momenta flip across a reflecting
+// face (the normal one for Symmetry/SlipWall, all three for NoSlipWall).
+PCD double ghost_sign(const int r[3], int m) {
+  double sg = 1.0;
 #pragma unroll
-  for (int m = 0; m < NVAR; ++m) {
-    double sg = 1.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const bool mom = (m == UMX || m == UMY || m == UMZ);
-      if ((r[d] == 2 && mom) || (r[d] == 1 && m == UMX + d)) sg = -sg;
-    }
-    J.U(i, j, k, m) = sg * src(s0, s1, s2, m);
+  for (int d = 0; d < 3; ++d) {
+    const bool mom = (m == UMX || m == UMY || m == UMZ);
+    if ((r[d] == 2 && mom) || (r[d] == 1 && m == UMX + d)) sg = -sg;
   }
+  return sg;
+}
+
+template <int NS>
+PCD void b_ghost(const Sched& sc, const KArgs& ka, int lb_, LevelArgs la) {
+  constexpr int NVAR = 7 + NS;
+  int local;
+  const Job J = load_job(sc, lb_, local);
+  int i, j, k;
+  if (!ghost_shell_cell(J.lo, J.hi, local, i, j, k)) return;
+  int s[3], r[3];
+  V src;
+  ghost_source(sc, J, la, i, j, k, s, r, src);
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m) J.U(i, j, k, m) = ghost_sign(r, m) * src(s[0], s[1], s[2], m);
 }
 template <int NS, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
@@ -81,6 +96,57 @@ __global__ void __launch_bounds__(BLOCK)
       b_ghost<NS>(sc, ka, lb_, la);
   } else {
     b_ghost<NS>(sc, ka, first_block(), la);
+  }
+}
+
+// The primitives of grow(bx,4) with the ghost fill folded in (the Godunov
+// level and batch steps): a valid cell reads its own state; a ghost cell
+// reads its source cell's state with the reflection signs, the values the
+// ghost fill would have written, so the primitives are the same bits.  Of
+// the ghost cells only the first layer (grow(bx,1), which the final fluxes'
+// artificial viscosity reads) is written back to the state; the outer
+// layers are not stored (pc_level_fill_ghosts_v1 fills all four).
+template <class EOS, class FP>
+PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
+                         const LevelArgs& la) {
+  constexpr int NVAR = 7 + EOS::NS;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  double Uc[NVAR];
+  // distance outside the valid box (0 inside)
+  auto out = [](int x, int l, int h) { return x < l ? l - x : (x > h ? x - h : 0); };
+  const int d0 = out(i, J.lo[0], J.hi[0]), d1 = out(j, J.lo[1], J.hi[1]),
+            d2 = out(k, J.lo[2], J.hi[2]);
+  const int dist = d0 > d1 ? (d0 > d2 ? d0 : d2) : (d1 > d2 ? d1 : d2);
+  if (dist <= 0) {
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m)
+      if (m != UEINT) Uc[m] = J.U(i, j, k, m);
+  } else {
+    int s[3], r[3];
+    V src;
+    ghost_source(sc, J, la, i, j, k, s, r, src);
+    const bool store = dist == 1;
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m) {
+      if (m == UEINT && !store) continue;
+      Uc[m] = ghost_sign(r, m) * src(s[0], s[1], s[2], m);
+      if (store) J.U(i, j, k, m) = Uc[m];
+    }
+  }
+  ctoprim_cell<EOS, FP>(J, ka, i, j, k, Uc);
+}
+template <class EOS, class FP = double, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_ctoprim_ghost(Sched sc, KArgs ka, LevelArgs la) {
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
+      b_ctoprim_ghost<EOS, FP>(sc, ka, lb_, la);
+  } else {
+    b_ctoprim_ghost<EOS, FP>(sc, ka, first_block(), la);
   }
 }
 
