@@ -25,7 +25,12 @@ PCD int ghost_src(int x, int lo, int hi, int bcl, int bch, int& refl) {
   if (x >= lo && x <= hi) return x;
   const int L = hi - lo + 1;
   const int bc = (x < lo) ? bcl : bch;
-  if (bc == BC_INTERIOR) return lo + (((x - lo) % L) + L) % L;
+  if (bc == BC_INTERIOR) {
+    // a ghost cell is at most 4 cells out: one period, unless the domain is
+    // shorter than that
+    if (x >= lo - L && x <= hi + L) return x < lo ? x + L : x - L;
+    return lo + (((x - lo) % L) + L) % L;
+  }
   if (bc == BC_SYMMETRY || bc == BC_SLIPWALL || bc == BC_NOSLIPWALL) {
     refl = (bc == BC_NOSLIPWALL) ? 2 : 1;
     return (x < lo) ? 2 * lo - 1 - x : 2 * hi + 1 - x;
