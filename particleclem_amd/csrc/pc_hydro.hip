@@ -391,7 +391,11 @@ struct StagePlan {
 // logical blocks of one job for a stage and block shape
 // lx = -2: the LDS tiles of the PPM trace, -3: the pencil tiles of the fused
 // PPM trace + first fluxes (pc_trace_lds.h)
-static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz) {
+// fp32: the launch is the mixed-precision step's (the transverse stage's
+// threads per face, trans_lanes, differ by precision)
+static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz,
+                              bool fp32 = false) {
+  const int tl = fp32 ? trans_lanes<float>() : trans_lanes<double>();
   if (lx == -2 || lx == -3) {
     int lo[3], n[3];
     stage_region(st, J.lo, J.hi, lo, n);
@@ -403,8 +407,7 @@ static long long stage_blocks(int st, const Job& J, int lx, int ly, int lz) {
   int lo[3], n[3];
   stage_region(st, J.lo, J.hi, lo, n);
   if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return 0;
-  // the transverse stage runs two threads per face (k_trans)
-  const int per = (st == ST_TRANS_XY || st == ST_TRANS_YX || st == ST_TRANS_ZX) ? 2 : 1;
+  const int per = (st == ST_TRANS_XY || st == ST_TRANS_YX || st == ST_TRANS_ZX) ? tl : 1;
   if (lx < 0) return (per * (long long)n[0] * n[1] * n[2] + BLOCK - 1) / BLOCK;
   auto nt = [](int m, int l) { return (long long)((m + (1 << l) - 1) >> l); };
   return per * nt(n[0], lx) * nt(n[1], ly) * nt(n[2], lz);
@@ -1138,7 +1141,7 @@ static bool plan_make(pc_level* L, int s, int lx, int ly, int lz, int cap,
   std::vector<int> start(nb), map;
   for (int b = 0; b < nb; ++b) {
     start[b] = (int)map.size();
-    const long long n = stage_blocks(s, L->hjobs[b], lx, ly, lz);
+    const long long n = stage_blocks(s, L->hjobs[b], lx, ly, lz, L->mixed != 0);
     for (long long k = 0; k < n; ++k) map.push_back(b);
   }
   P.nblocks = (int)map.size();
@@ -2018,7 +2021,26 @@ int pc_level_set_precision_v1(pc_level_t* L, int mixed) {
   if (!L || (mixed != 0 && mixed != 1)) return PC_ERR_ARG;
   HIPCHK(L->ctx, hipDeviceSynchronize());
   graph_drop(L);
+  if (L->mixed == mixed) return PC_OK;
   L->mixed = mixed;
+  // the transverse stage's threads per face follow the precision: re-plan
+  // its three launches, keeping their shapes
+  const int st[3] = {ST_TRANS_XY, ST_TRANS_YX, ST_TRANS_ZX};
+  StagePlan P[3];
+  for (int x = 0; x < 3; ++x) {
+    const StagePlan& O = L->plans[st[x]];
+    const int cap = O.grid < O.nblocks ? O.grid : 0;
+    if (!plan_make(L, st[x], O.lx, O.ly, O.lz, cap, P[x])) {
+      for (int y = 0; y < x; ++y) plan_free(P[y]);
+      L->mixed = !mixed;
+      return fail(L->ctx, PC_ERR_NOMEM, "set_precision: plan allocation failed");
+    }
+  }
+  for (int x = 0; x < 3; ++x) {
+    plan_free(L->plans[st[x]]);
+    L->plans[st[x]] = P[x];
+    L->scs[st[x]] = plan_sched(L, P[x]);
+  }
   return PC_OK;
 }
 
@@ -2712,7 +2734,7 @@ static int batch_tables(pc_batch* B, const std::vector<int>& ids,
                        ? (B->fuse ? -3 : -2) : -1;
     for (int k = 0; k < n; ++k) {
       start[(size_t)s * B->max_slots + k] = tot;
-      tot += (int)stage_blocks(s, jobs[k], lx, 0, 0);
+      tot += (int)stage_blocks(s, jobs[k], lx, 0, 0, B->mixed != 0);
     }
     B->plans[s].nblocks = B->plans[s].grid = tot;
     B->plans[s].lx = -1;

@@ -796,37 +796,17 @@ PCD VT<T> lane_select(bool c, const VT<T>& a, const VT<T>& b) {
   return r;
 }
 
-// pc_transdo on the two cells of a d-face, then pc_cmpflx
-// (Source/Godunov.cpp:246-384), for both transverse fluxes of the face by a
-// pair of neighbouring lanes: the even lane solves (D|t0), the odd lane
-// (D|t1).  The two lanes load the face's traces by the same instruction, so
-// each trace is fetched from HBM once for both fluxes (one thread solving
-// both read them twice, the second time mostly from HBM again), and each
-// lane holds one pair of states.  The launch has two threads per face of the
-// region (stage_blocks doubles the stage's blocks); the face index of a
-// thread is its local index halved.
-template <class EOS, int D, class FP>
-PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
+// pc_transdo on the two cells of d-face (i,j,k) with the corrections of
+// direction T (view dl), then pc_cmpflx (Source/Godunov.cpp:246-384).
+template <class EOS, int D, class FP, class TV, class DV>
+PCD void trans_face(const JobT<FP>& J, const TV& tr, const DV& dl, int i, int j,
+                    int k, Flux<EOS::NS, FP>& F) {
   constexpr int NS = EOS::NS;
-  constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
-  int local;
-  const JobT<FP> J = load_job<FP>(sc, lb_, local);
-  const bool second = (local & 1) != 0;
-  int lo[3], n[3], i, j, k;
-  stage_region(ST_TRANS_XY + 2 * D, J.lo, J.hi, lo, n);
-  if (!region_ijk(sc, lo, n, local >> 1, i, j, k)) return;
-  (void)ka;
-  const int c[3] = {i, j, k};
-  // (D|t) is read on the faces whose t index lies in [L, H]
-  const int ct = second ? c[T1] : c[T0];
-  const int tl = second ? J.lo[T1] : J.lo[T0], th = second ? J.hi[T1] : J.hi[T0];
-  if (ct < tl || ct > th) return;
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
-  const VS<FP> tr = split(J.tr[D]), qa = split(J.qa);
-  const VT<FP> dl = lane_select(second, J.dl[T1], J.dl[T0]);
   FaceState<NS, FP> s, L, R;
   TCorr<NS, FP> df;
+  const VS<FP> qa = split(J.qa);
   // minus side: cell f - e_D, state on its high face
   load_trace<NS, FP>(tr, i, j, k, 0, s);
   load_tcorr<NS, FP>(dl, im, jm, km, df);
@@ -837,11 +817,61 @@ PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
   transdo_side<NS>(s, df, R);
   const FP cav = FP(0.5) * (qa(i, j, k, QC) + qa(im, jm, km, QC));
   const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-  Flux<NS, FP> F;
   cmpflx<EOS, D>(L, R, cav, idx, J.dlo[D], J.dhi[D], J.bclo[D], J.bchi[D], F);
-  const VT<FP> ft =
-      lane_select(second, J.ft[2 * D + tslot(D, T1)], J.ft[2 * D + tslot(D, T0)]);
-  store_flux<NS, FP>(ft, i, j, k, F);
+}
+
+// Threads per face of the transverse stage: two in FP64 (below), one in FP32.
+template <class FP>
+__host__ __device__ constexpr int trans_lanes() {
+  return sizeof(FP) == 8 ? 2 : 1;
+}
+
+// Both transverse fluxes of a d-face, (D|t0) and (D|t1), each where its t
+// lies in [L, H].  FP64: a pair of neighbouring lanes per face, the even
+// lane solving (D|t0), the odd lane (D|t1); the two lanes load the face's
+// traces by the same instruction, so each trace is fetched from HBM once for
+// both fluxes (one thread solving both read them twice, the second time
+// mostly from HBM again), and each lane holds one pair of states; the face
+// index of a thread is its local index halved.  FP32: one thread solves
+// both, reading the face's traces again for the second flux (half the bytes:
+// the second read finds them in L2).
+template <class EOS, int D, class FP>
+PCD void b_trans(const Sched& sc, const KArgs& ka, int lb_) {
+  constexpr int NS = EOS::NS;
+  constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_TRANS_XY + 2 * D, J.lo, J.hi, lo, n);
+  (void)ka;
+  const VS<FP> tr = split(J.tr[D]);
+  if constexpr (trans_lanes<FP>() == 2) {
+    const bool second = (local & 1) != 0;
+    if (!region_ijk(sc, lo, n, local >> 1, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    const int ct = second ? c[T1] : c[T0];
+    const int tl = second ? J.lo[T1] : J.lo[T0], th = second ? J.hi[T1] : J.hi[T0];
+    if (ct < tl || ct > th) return;
+    const VT<FP> dl = lane_select(second, J.dl[T1], J.dl[T0]);
+    Flux<NS, FP> F;
+    trans_face<EOS, D, FP>(J, tr, dl, i, j, k, F);
+    const VT<FP> ft =
+        lane_select(second, J.ft[2 * D + tslot(D, T1)], J.ft[2 * D + tslot(D, T0)]);
+    store_flux<NS, FP>(ft, i, j, k, F);
+  } else {
+    if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    if (c[T0] >= J.lo[T0] && c[T0] <= J.hi[T0]) {
+      Flux<NS, FP> F;
+      trans_face<EOS, D, FP>(J, tr, split(J.dl[T0]), i, j, k, F);
+      store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T0)]), i, j, k, F);
+    }
+    if (c[T1] >= J.lo[T1] && c[T1] <= J.hi[T1]) {
+      Flux<NS, FP> F;
+      trans_face<EOS, D, FP>(J, tr, split(J.dl[T1]), i, j, k, F);
+      store_flux<NS, FP>(split(J.ft[2 * D + tslot(D, T1)]), i, j, k, F);
+    }
+  }
 }
 template <class EOS, int D, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)

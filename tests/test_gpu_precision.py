@@ -84,3 +84,44 @@ def test_mixed_ragged_with_graph(cuda):
         x.close()
     ca.close()
     cb.close()
+
+
+def test_precision_switch_replans_transverse_stage(cuda):
+    """The transverse stage runs two threads per face in FP64 and one in
+    FP32, so switching a level's precision re-plans it, keeping a shape set
+    before the switch: a level with tiles for that stage, switched FP64 ->
+    mixed -> FP64, equals a level with the default plan switched the same
+    way, bitwise after each phase, and its plan has twice the blocks in
+    FP64."""
+    w = wl.premixed(n=32, box=16, nsteps=0)
+    over = dict(fixed_dt=2.0e-8)
+    c0, ref64 = level(w, False, **over)
+    c1, ref32 = level(w, False, **over)
+    c2, sw = level(w, False, **over)
+    sw.set_plan("trans_xy", (5, 3, 0), 0)
+    for x in (ref64, ref32, sw):
+        x.init_dt()
+    ref64.step(2)
+    ref32.step(2)
+    sw.step(2)
+    assert all(np.array_equal(sw.read_box(b), ref64.read_box(b)) for b in range(len(w.boxes)))
+    # both switch to mixed for two steps, then back (ref32 keeps runs of
+    # 256 faces for its transverse stage)
+    ref32.set_precision(True)
+    sw.set_precision(True)
+    ref32.step(2)
+    sw.step(2)
+    assert all(np.array_equal(sw.read_box(b), ref32.read_box(b)) for b in range(len(w.boxes)))
+    n32 = sw.plans()["trans_xy"][2]
+    sw.set_precision(False)
+    ref32.set_precision(False)
+    sw.step(2)
+    ref32.step(2)
+    assert all(np.array_equal(sw.read_box(b), ref32.read_box(b)) for b in range(len(w.boxes)))
+    shape, _, n64, _ = sw.plans()["trans_xy"]
+    assert shape == (32, 8, 1) and n64 == 2 * n32, (shape, n64, n32)
+    assert sw.status()[0] == 0
+    for x in (ref64, ref32, sw):
+        x.close()
+    for c in (c0, c1, c2):
+        c.close()
