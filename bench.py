@@ -142,6 +142,28 @@ def profiled_traffic(summary, stage):
     return None
 
 
+def profiled_columns(summary, stage, names):
+    """Columns `names` of the row of `stage` in a committed rocprofv3
+    summary, as floats (None where absent)."""
+    path = os.path.join(ROOT, "profiles", summary)
+    if not os.path.exists(path):
+        return {k: None for k in names}
+    head = None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if "HBM MB" in cols and "stage" in cols:
+            head = cols
+        elif head is not None and len(cols) == len(head) and cols[1] == stage:
+            out = {}
+            for k in names:
+                try:
+                    out[k] = float(cols[head.index(k)])
+                except (ValueError, IndexError):
+                    out[k] = None
+            return out
+    return {k: None for k in names}
+
+
 def profiled_step_traffic(summary):
     """Counted HBM bytes of one level step from the same summary, or None."""
     path = os.path.join(ROOT, "profiles", summary)
@@ -744,6 +766,13 @@ def main():
                    "traffic_source": (f"profiles/{summary}" if traffic else None),
                    "bytes_per_launch": kbytes[dom], "ms_per_launch": dom_ms,
                    "launches_per_step": launches.get(dom, 1)}
+    if traffic is not None:
+        # the clock the kernel ran at and its FP64 vector pipe occupancy
+        # (FP64 instructions x 4 cycles over the 1024 SIMDs' cycles)
+        pc = profiled_columns(summary, dom, ["clock GHz", "FP64 pipe"])
+        if pc["clock GHz"] is not None:
+            roof_kernel["clock_ghz"] = pc["clock GHz"]
+            roof_kernel["fp64_pipe"] = pc["FP64 pipe"]
     nvar = w.nvar
     # the step against HBM on the algorithmic bytes of the workload, B_alg =
     # 2 NVAR 8 bytes per cell-update (read U^n, write U^n+1 once)
