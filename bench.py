@@ -46,7 +46,7 @@ PROFILE_SUMMARIES = {
 }
 FP32_PEAK = 157.3e12
 # launches per step of the kernels the MOL step runs in both Heun stages
-MOL_LAUNCHES = {"ghost": 2, "ctoprim": 2, "mol_x": 2, "mol_y": 2, "mol_z": 2}
+MOL_LAUNCHES = {"ctoprim": 2, "mol_x": 2, "mol_y": 2, "mol_z": 2}
 
 
 def stage_bytes(nspec, boxes, mol=False, fused=True):
@@ -67,11 +67,10 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
         g4 = (n[0] + 8) * (n[1] + 8) * (n[2] + 8)
         v = n[0] * n[1] * n[2]
         if mol:
-            # ghost fill; primitives on grow(bx,2); per d-face the primitives
-            # the slopes read (rho, u, v, w, p, Y, c) of one cell and the
-            # NVAR fluxes written; the updates read U, the fluxes (and U^n)
-            # and write U (and U^n)
-            add("ghost", g4 - v, 2 * N)
+            # primitives on grow(bx,2) (ghost values read at their source
+            # cells); per d-face the primitives the slopes read (rho, u, v,
+            # w, p, Y, c) of one cell and the NVAR fluxes written; the
+            # updates read U, the fluxes (and U^n) and write U (and U^n)
             add("ctoprim", (n[0] + 4) * (n[1] + 4) * (n[2] + 4), N + Q + 6)
             for d, a in enumerate("xyz"):
                 f = list(n)

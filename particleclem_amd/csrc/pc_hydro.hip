@@ -1265,7 +1265,7 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
       break;
     case ST_UPDATE: LAUNCH((k_update<EOS, P>), grid, st, sc, ka, L->ds); break;
     case ST_MOL_PRIM:
-      LAUNCH((k_ctoprim<EOS, FP, P, ST_MOL_PRIM>), grid, st, sc, ka);
+      LAUNCH((k_ctoprim_ghost<EOS, FP, P, ST_MOL_PRIM>), grid, st, sc, ka, L->la);
       break;
     case ST_MOL_X:
       if (L->laxf) LAUNCH((k_mol_flux<EOS, 0, true, FP, P>), grid, st, sc, ka);
@@ -1697,7 +1697,6 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st,
     const bool brm = L->brs.on && !tm && !capturing;
     if (L->mol) {
       for (int h = 0; h < 2; ++h) {
-        run(ST_GHOST, PC_K_GHOST);
         run(ST_MOL_PRIM, PC_K_CTOPRIM);
         if (brm) {
           branch_mol_fluxes<EOS>(L, S, P, st, L->brs);
@@ -2756,8 +2755,7 @@ static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
   const StagePlan* P = B->plans;
   if (B->mol) {
     for (int h = 0; h < 2; ++h) {
-      for (int x : {ST_GHOST, ST_MOL_PRIM})
-        launch_stage<EOS>(B, x, S[x], P[x], st);
+      launch_stage<EOS>(B, ST_MOL_PRIM, S[ST_MOL_PRIM], P[ST_MOL_PRIM], st);
       if (B->brs.on) {
         branch_mol_fluxes<EOS>(B, S, P, st, B->brs);
       } else {

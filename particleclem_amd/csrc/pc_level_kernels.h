@@ -104,21 +104,24 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
-// The primitives of grow(bx,4) with the ghost fill folded in (the Godunov
-// level and batch steps): a valid cell reads its own state; a ghost cell
-// reads its source cell's state with the reflection signs, the values the
-// ghost fill would have written, so the primitives are the same bits.  Of
-// the ghost cells only the first layer (grow(bx,1), which the final fluxes'
-// artificial viscosity reads) is written back to the state; the outer
-// layers are not stored (pc_level_fill_ghosts_v1 fills all four).
-template <class EOS, class FP>
+// The primitives with the ghost fill folded in (the level and batch steps):
+// a valid cell reads its own state; a ghost cell reads its source cell's
+// state with the reflection signs, the values the ghost fill would have
+// written, so the primitives are the same bits.  REG: the Godunov step's
+// grow(bx,4) (ST_CTOPRIM), of whose ghost cells the first layer (grow(bx,1),
+// which the final fluxes' artificial viscosity reads) is written back to the
+// state, or the method-of-lines step's grow(bx,2) (ST_MOL_PRIM), which writes
+// none back (its kernels read the state on the valid cells only).  The
+// other ghost cells are not stored (pc_level_fill_ghosts_v1 fills all four
+// layers).
+template <class EOS, class FP, int REG>
 PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
                          const LevelArgs& la) {
   constexpr int NVAR = 7 + EOS::NS;
   int local;
   const JobT<FP> J = load_job<FP>(sc, lb_, local);
   int lo[3], n[3], i, j, k;
-  stage_region(ST_CTOPRIM, J.lo, J.hi, lo, n);
+  stage_region(REG, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   double Uc[NVAR];
   // distance outside the valid box (0 inside)
@@ -134,7 +137,7 @@ PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
     int s[3], r[3];
     V src;
     ghost_source(sc, J, la, i, j, k, s, r, src);
-    const bool store = dist == 1;
+    const bool store = REG == ST_CTOPRIM && dist == 1;
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) {
       if (m == UEINT && !store) continue;
@@ -144,14 +147,14 @@ PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
   }
   ctoprim_cell<EOS, FP>(J, ka, i, j, k, Uc);
 }
-template <class EOS, class FP = double, bool PERSIST = false>
+template <class EOS, class FP = double, bool PERSIST = false, int REG = ST_CTOPRIM>
 __global__ void __launch_bounds__(BLOCK)
     k_ctoprim_ghost(Sched sc, KArgs ka, LevelArgs la) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_ctoprim_ghost<EOS, FP>(sc, ka, lb_, la);
+      b_ctoprim_ghost<EOS, FP, REG>(sc, ka, lb_, la);
   } else {
-    b_ctoprim_ghost<EOS, FP>(sc, ka, first_block(), la);
+    b_ctoprim_ghost<EOS, FP, REG>(sc, ka, first_block(), la);
   }
 }
 
