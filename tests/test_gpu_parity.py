@@ -98,3 +98,15 @@ def test_long_box_unfused_ppm_trace(cuda, idir):
     w = wl.sod(n=300, ny=8, nz=8, idir=idir, nsteps=2)
     assert len(w.boxes) == 1
     assert_parity(w, 2, label=f"sod 300x8x8 along {'xyz'[idir]}")
+
+
+@pytest.mark.parametrize("over", [{}, dict(do_mol=1)], ids=["ctu", "mol"])
+def test_sod_thin_periodic_domain(cuda, over):
+    """A tube 2 cells thick in y and 3 in z, periodic there, split along x
+    into boxes: ghost cells 3 and 4 cells out lie more than one period away,
+    so their values come from the cell that several wraps reach (the
+    primitives' ghost-source mapping on the device, the ghost fill in the
+    oracle)."""
+    w = wl.sod(n=24, ny=2, nz=3, max_grid=8, nsteps=3)
+    assert len(w.boxes) == 3
+    assert_parity(w, 3, over, label=f"sod 24x2x3 periodic y/z {over}")
