@@ -1263,7 +1263,9 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
       LAUNCH((k_final<EOS, 2, false, FP, P>), grid, st, sc, ka);
       if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2, FP, P>), grid, st, sc, ka);
       break;
-    case ST_UPDATE: LAUNCH((k_update<EOS, P>), grid, st, sc, ka, L->ds); break;
+    case ST_UPDATE:
+      LAUNCH((k_update<EOS, P, sizeof(FP) == 4>), grid, st, sc, ka, L->ds);
+      break;
     case ST_MOL_PRIM:
       LAUNCH((k_ctoprim_ghost<EOS, FP, P, ST_MOL_PRIM>), grid, st, sc, ka, L->la);
       break;

@@ -101,6 +101,21 @@ struct VB {
 #endif
   }
 };
+// The same array seen as FP32 elements with the same element strides (the
+// data occupies the first half of each component's bytes): the mixed step
+// stores its final fluxes and qec so.
+PCD VT<float> as_fp32(const VT<double>& v) {
+  VT<float> f;
+  f.p = (float*)v.p;
+  f.js = v.js;
+  f.ks = v.ks;
+  f.ns = v.ns;
+  f.b0 = v.b0;
+  f.b1 = v.b1;
+  f.b2 = v.b2;
+  return f;
+}
+
 template <class T>
 PCD VB<T> bufview(const VT<T>& v) {
   VB<T> b;
@@ -1043,11 +1058,25 @@ PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
   }
   // pc_ext_flx (Source/Hydro.H:854-865)
   const double area = J.area[D].p ? J.area[D](i, j, k, 0) : J.area_u[D];
-  const auto& fo = J.flx[D];
+  if constexpr (sizeof(FP) == 8) {
+    const auto& fo = J.flx[D];
 #pragma unroll
-  for (int m = 0; m < NVAR; ++m) fo(i, j, k, m) = fl[m] * area;
-  J.qec[D](i, j, k, 0) = F.pgd;
-  J.qec[D](i, j, k, 1) = F.ugd;
+    for (int m = 0; m < NVAR; ++m) fo(i, j, k, m) = fl[m] * area;
+    J.qec[D](i, j, k, 0) = F.pgd;
+    J.qec[D](i, j, k, 1) = F.ugd;
+  } else {
+    // mixed step: the face fluxes and qec in FP32, half the bytes written
+    // here and read by the update; the density flux is not stored (the
+    // update takes the sum of the species fluxes, so sum_k rho Y_k = rho
+    // holds to FP64 rounding) and neither is the zero UTEMP flux
+    const VT<float> fo = as_fp32(J.flx[D]);
+#pragma unroll
+    for (int m = 0; m < NVAR; ++m)
+      if (m != URHO && m != UTEMP) fo(i, j, k, m) = (float)(fl[m] * area);
+    const VT<float> qe = as_fp32(J.qec[D]);
+    qe(i, j, k, 0) = F.pgd;
+    qe(i, j, k, 1) = F.ugd;
+  }
 }
 
 template <class EOS, int D, bool SRC, class FP>
@@ -1148,6 +1177,49 @@ PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
     h[m] += (f0(i, j, k, m) - f0(i + 1, j, k, m) + f1(i, j, k, m) -
              f1(i, j + 1, k, m) + f2(i, j, k, m) - f2(i, j, k + 1, m)) /
             v;
+  h[UEINT] -= pdivu;
+}
+
+// The same for the mixed step's FP32 face fluxes and qec (final_tail), in
+// FP64 arithmetic: the density flux of a face is the sum of its species
+// fluxes (species first to last).
+template <int NS>
+PCD void cell_divergence_fp32(const Job& J, const KArgs& ka, int i, int j, int k,
+                              double* h) {
+  constexpr int NVAR = 7 + NS;
+  auto A = [&](int d, int a, int b, int c) {
+    return J.area[d].p ? J.area[d](a, b, c, 0) : J.area_u[d];
+  };
+  const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
+  const VT<float> q1 = as_fp32(J.qec[0]);
+  const VT<float> q2 = as_fp32(J.qec[1]);
+  const VT<float> q3 = as_fp32(J.qec[2]);
+  auto Q = [](const VT<float>& q, int a, int b, int c, int n) { return (double)q(a, b, c, n); };
+  const double pdivu =
+      0.5 *
+      ((Q(q1, i + 1, j, k, 0) + Q(q1, i, j, k, 0)) *
+           (Q(q1, i + 1, j, k, 1) * A(0, i + 1, j, k) - Q(q1, i, j, k, 1) * A(0, i, j, k)) +
+       (Q(q2, i, j + 1, k, 0) + Q(q2, i, j, k, 0)) *
+           (Q(q2, i, j + 1, k, 1) * A(1, i, j + 1, k) - Q(q2, i, j, k, 1) * A(1, i, j, k)) +
+       (Q(q3, i, j, k + 1, 0) + Q(q3, i, j, k, 0)) *
+           (Q(q3, i, j, k + 1, 1) * A(2, i, j, k + 1) - Q(q3, i, j, k, 1) * A(2, i, j, k))) /
+      v;
+  const VB<float> f[3] = {bufview(as_fp32(J.flx[0])), bufview(as_fp32(J.flx[1])),
+                          bufview(as_fp32(J.flx[2]))};
+  double drho = 0.0;
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m) {
+    if (m == URHO || m == UTEMP) continue;
+    double dm = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int a = i + (d == 0), b = j + (d == 1), c = k + (d == 2);
+      dm += (double)f[d](i, j, k, m) - (double)f[d](a, b, c, m);
+    }
+    h[m] += dm / v;
+    if (m >= UFS) drho += dm;
+  }
+  h[URHO] += drho / v;
   h[UEINT] -= pdivu;
 }
 

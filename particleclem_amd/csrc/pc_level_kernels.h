@@ -229,7 +229,8 @@ PCD void reset_and_temp(double* S, const typename EOS::Thermo& th) {
 // (construct_Snew, Source/Advance.cpp:397-416), reset + e->T twice
 // (computeTemp in do_sdc_iteration and in post_timestep), NaN check and the
 // CFL partial minimum on the new state.
-template <class EOS>
+// F32: the final fluxes are the mixed step's FP32 ones (cell_divergence_fp32)
+template <class EOS, bool F32>
 PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState*& ds) {
   constexpr int NS = EOS::NS;
   constexpr int NVAR = 7 + NS;
@@ -244,7 +245,8 @@ PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState*& ds) {
     double h[NVAR];
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) h[m] = 0.0;
-    cell_divergence<NS>(J, ka, i, j, k, h);
+    if constexpr (F32) cell_divergence_fp32<NS>(J, ka, i, j, k, h);
+    else cell_divergence<NS>(J, ka, i, j, k, h);
     double S[NVAR];
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) S[m] = J.U(i, j, k, m) + dt * h[m];
@@ -265,15 +267,15 @@ PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState*& ds) {
   return mydt;
 }
 
-template <class EOS, bool PERSIST = false>
+template <class EOS, bool PERSIST = false, bool F32 = false>
 __global__ void __launch_bounds__(BLOCK)
     k_update(Sched sc, KArgs ka, DtState* ds) {
   double m = 1.7976931348623157e308;
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      m = mn(m, b_update<EOS>(sc, ka, lb_, ds));
+      m = mn(m, b_update<EOS, F32>(sc, ka, lb_, ds));
   } else {
-    m = b_update<EOS>(sc, ka, first_block(), ds);
+    m = b_update<EOS, F32>(sc, ka, first_block(), ds);
   }
   block_min_to(m, &ds->minbits);
 }
