@@ -1282,8 +1282,8 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
       else LAUNCH((k_mol_flux<EOS, 2, false, FP, P>), grid, st, sc, ka);
       break;
     case ST_MOL_UPD:  // sub: the Heun stage (0 or 1)
-      if (sub == 0) LAUNCH((k_mol_update<EOS, 0, P>), grid, st, sc, ka, L->ds);
-      else LAUNCH((k_mol_update<EOS, 1, P>), grid, st, sc, ka, L->ds);
+      if (sub == 0) LAUNCH((k_mol_update<EOS, 0, P, sizeof(FP) == 4>), grid, st, sc, ka, L->ds);
+      else LAUNCH((k_mol_update<EOS, 1, P, sizeof(FP) == 4>), grid, st, sc, ka, L->ds);
       break;
     default: break;
   }

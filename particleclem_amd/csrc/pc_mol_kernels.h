@@ -136,16 +136,31 @@ PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int MT = (D == 0) ? UMY : UMX;
   constexpr int MTT = (D == 2) ? UMY : UMZ;
   const double area = J.area[D].p ? J.area[D](i, j, k, 0) : J.area_u[D];
-  const auto& fo = J.flx[D];
-  fo(i, j, k, URHO) = 0.0 + (double)o.f_rho * area;
-  fo(i, j, k, MU) = 0.0 + (double)o.f_un * area;
-  fo(i, j, k, MT) = 0.0 + (double)o.f_ut * area;
-  fo(i, j, k, MTT) = 0.0 + (double)o.f_utt * area;
-  fo(i, j, k, UEDEN) = 0.0 + (double)o.f_eden * area;
-  fo(i, j, k, UEINT) = 0.0 + (double)o.f_eint * area;
-  fo(i, j, k, UTEMP) = 0.0;
+  if constexpr (sizeof(FP) == 8) {
+    const auto& fo = J.flx[D];
+    fo(i, j, k, URHO) = 0.0 + (double)o.f_rho * area;
+    fo(i, j, k, MU) = 0.0 + (double)o.f_un * area;
+    fo(i, j, k, MT) = 0.0 + (double)o.f_ut * area;
+    fo(i, j, k, MTT) = 0.0 + (double)o.f_utt * area;
+    fo(i, j, k, UEDEN) = 0.0 + (double)o.f_eden * area;
+    fo(i, j, k, UEINT) = 0.0 + (double)o.f_eint * area;
+    fo(i, j, k, UTEMP) = 0.0;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = 0.0 + (double)o.f_Y[s] * area;
+    for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = 0.0 + (double)o.f_Y[s] * area;
+  } else {
+    // mixed step: the FP32 Riemann solve's face fluxes stored in FP32 (the
+    // updates read them into FP64 arithmetic); the zero UTEMP flux is not
+    // stored
+    const VT<float> fo = as_fp32(J.flx[D]);
+    fo(i, j, k, URHO) = (float)(0.0 + (double)o.f_rho * area);
+    fo(i, j, k, MU) = (float)(0.0 + (double)o.f_un * area);
+    fo(i, j, k, MT) = (float)(0.0 + (double)o.f_ut * area);
+    fo(i, j, k, MTT) = (float)(0.0 + (double)o.f_utt * area);
+    fo(i, j, k, UEDEN) = (float)(0.0 + (double)o.f_eden * area);
+    fo(i, j, k, UEINT) = (float)(0.0 + (double)o.f_eint * area);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = (float)(0.0 + (double)o.f_Y[s] * area);
+  }
 }
 template <class EOS, int D, bool LAXF, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
@@ -171,6 +186,26 @@ PCD void mol_divergence(const Job& J, int i, int j, int k, double* D) {
     D[m] = -(f0(i + 1, j, k, m) - f0(i, j, k, m) + f1(i, j + 1, k, m) -
              f1(i, j, k, m) + f2(i, j, k + 1, m) - f2(i, j, k, m)) /
            v;
+}
+// the same for the mixed step's FP32 face fluxes (FP64 arithmetic)
+template <int NS>
+PCD void mol_divergence_fp32(const Job& J, int i, int j, int k, double* D) {
+  constexpr int NVAR = 7 + NS;
+  const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
+  const VB<float> f0 = bufview(as_fp32(J.flx[0]));
+  const VB<float> f1 = bufview(as_fp32(J.flx[1]));
+  const VB<float> f2 = bufview(as_fp32(J.flx[2]));
+  auto F = [](const VB<float>& f, int a, int b, int c, int m) { return (double)f(a, b, c, m); };
+#pragma unroll
+  for (int m = 0; m < NVAR; ++m) {
+    if (m == UTEMP) {
+      D[m] = -0.0 / v;
+      continue;
+    }
+    D[m] = -(F(f0, i + 1, j, k, m) - F(f0, i, j, k, m) + F(f1, i, j + 1, k, m) -
+             F(f1, i, j, k, m) + F(f2, i, j, k + 1, m) - F(f2, i, j, k, m)) /
+           v;
+  }
 }
 
 // The MOL source of one box into J.upd (getMOLSrcTerm's copy of Dterm to
@@ -202,7 +237,8 @@ __global__ void __launch_bounds__(BLOCK)
 // STAGE 0: U* = 1.0 U^n + dt S(U^n) (U^n kept in J.sold), computeTemp;
 // STAGE 1: U^{n+1} = 0.5 U* + 0.5 U^n, += (0.5 dt) S(U*), computeTemp and
 // post_timestep's computeTemp, NaN check and the CFL partial minimum.
-template <class EOS, int STAGE>
+// F32: the face fluxes are the mixed step's FP32 ones (mol_divergence_fp32)
+template <class EOS, int STAGE, bool F32>
 PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
                         DtState*& ds) {
   constexpr int NS = EOS::NS;
@@ -216,7 +252,8 @@ PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
   if (region_ijk(sc, lo, n, local, i, j, k)) {
     const double dt = get_dt(J, ka);
     double D[NVAR];
-    mol_divergence<NS>(J, i, j, k, D);
+    if constexpr (F32) mol_divergence_fp32<NS>(J, i, j, k, D);
+    else mol_divergence<NS>(J, i, j, k, D);
     double S[NVAR];
     if constexpr (STAGE == 0) {
 #pragma unroll
@@ -252,15 +289,15 @@ PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
   return mydt;
 }
 
-template <class EOS, int STAGE, bool PERSIST = false>
+template <class EOS, int STAGE, bool PERSIST = false, bool F32 = false>
 __global__ void __launch_bounds__(BLOCK)
     k_mol_update(Sched sc, KArgs ka, DtState* ds) {
   double m = 1.7976931348623157e308;
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      m = mn(m, b_mol_update<EOS, STAGE>(sc, ka, lb_, ds));
+      m = mn(m, b_mol_update<EOS, STAGE, F32>(sc, ka, lb_, ds));
   } else {
-    m = b_mol_update<EOS, STAGE>(sc, ka, first_block(), ds);
+    m = b_mol_update<EOS, STAGE, F32>(sc, ka, first_block(), ds);
   }
   if constexpr (STAGE == 1) block_min_to(m, &ds->minbits);
 }
