@@ -404,7 +404,8 @@ def forms(args, ctx, w, fp64_ms):
     """The forms beside the plain FP64 level step, measured in the same run:
     mixed precision on the same level (timed as the FP64 step is: branch
     streams, warm-up steps, then the same number of timed steps; error norms
-    from FP64 and mixed levels stepped in lockstep), the ragged 300-box level
+    from FP64 and mixed levels stepped in lockstep), the method-of-lines step
+    on the same level (FP64 and mixed, timed alike), the ragged 300-box level
     (FP64, branch streams) and a serving run of the ensemble generator (sides
     32-128, 10-200 steps, mixed precision)."""
     import torch
@@ -443,6 +444,32 @@ def forms(args, ctx, w, fp64_ms):
                                  "note": "one stream, host-synchronised at "
                                          "each checkpoint, first steps included"},
         "errors_vs_fp64": pc["errors"], "wall_s": time.perf_counter() - t0}
+    # the method-of-lines step on the same level, FP64 and mixed
+    t0 = time.perf_counter()
+    mol = {}
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        lv = DeviceLevel.from_workload(ctx, w, do_mol=1)
+        lv.set_precision(mixed)
+        if args.branches:
+            lv.set_branches(True)
+        lv.init_dt()
+        lv.step(args.warmup)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        lv.step(args.steps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - ts) / args.steps * 1e3
+        mol[name] = {"ms_per_step": ms, "value": cells / (ms * 1e-3) / 1e6,
+                     "status": lv.status()[0]}
+        lv.close()
+    out["mol"] = {
+        "workload": "the same premixed 128^3 level stepped by the method-of-lines "
+                    "step (do_mol = 1: two Heun stages of piecewise-linear face "
+                    "states and Riemann fluxes), FP64 and mixed precision, timed "
+                    "as the FP64 line",
+        "fp64": mol["fp64"], "mixed": mol["mixed"],
+        "speedup": mol["fp64"]["ms_per_step"] / mol["mixed"]["ms_per_step"],
+        "wall_s": time.perf_counter() - t0}
     # ragged level (c2)
     t0 = time.perf_counter()
     wr = workloads.ragged(nboxes=300, nsteps=args.forms_steps)
@@ -620,9 +647,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
-                    help="premixed default workload: skip the three forms "
-                         "measured after the main line (mixed precision, the "
-                         "ragged 300-box level, a serving run)")
+                    help="premixed default workload: skip the forms measured "
+                         "after the main line (mixed precision, the "
+                         "method-of-lines step, the ragged 300-box level, a "
+                         "serving run)")
     ap.add_argument("--forms-steps", type=int, default=10,
                     help="timed steps of the ragged-level form")
     ap.add_argument("--serving-requests", type=int, default=200,

@@ -97,6 +97,39 @@ def test_mol_batch_equals_mol_level_and_mix_independence(cuda):
         assert np.array_equal(got.result, out[r.rid].result)
 
 
+@pytest.mark.parametrize("mol", [False, True], ids=["ctu", "mol"])
+def test_mixed_batch_equals_mixed_level(cuda, mol):
+    """The mixed-precision batch (FP32 intermediates and face fluxes) against
+    the mixed-precision level stepper on the same box, BCs and fields: the
+    same kernels and arithmetic, so bitwise equal, for the Godunov and the
+    method-of-lines step."""
+    from particleclem_amd.hydro import DeviceLevel, HydroContext, a4_from_tensor
+    reqs = [sv.Request(0, sv.TUBE, 12, 5, 10.0, 0.8, 0.0, 5),
+            sv.Request(1, sv.SLAB, 10, 4, 2.0, 1.1, 0.0, 6)]
+    srv = sv.Server(mixed=True, budget_bytes=1 << 30, keep_results=True, mol=mol)
+    out = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    srv.close()
+    V = (slice(None), slice(4, -4), slice(4, -4), slice(4, -4))
+    for r in reqs:
+        P, bclo, bchi, mode = sv.request_fields(r)
+        ctx = HydroContext(sv.NSPEC[r.kind])
+        lev = DeviceLevel(ctx, [([0, 0, 0], [r.n - 1] * 3)], [0, 0, 0], [r.n - 1] * 3,
+                          bclo, bchi, [1.0 / r.n] * 3,
+                          params=dict(ppm_type=1, cfl=0.3, do_mol=int(mol)))
+        lev.set_precision(True)
+        lev._chk(lev.L.pc_level_prim_to_cons_v1(lev.h, 0, a4_from_tensor(P, [0, 0, 0]),
+                                                 mode, None))
+        lev.init_dt()
+        lev.step(r.nsteps)
+        ref = lev.read_box(0)
+        nd = int(np.count_nonzero(ref[V] != out[r.rid].result[V]))
+        print(f"mixed {'MOL' if mol else 'CTU'} request {r.rid} kind {r.kind}: {nd} values "
+              f"differ from the mixed level stepper")
+        assert out[r.rid].status == 0 and nd == 0
+        lev.close()
+        ctx.close()
+
+
 def test_admission_under_a_tight_budget(cuda):
     reqs = small_stream(10, seed=3)
     probe = sv.Server(mixed=True, budget_bytes=64 << 20)
