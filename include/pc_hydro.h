@@ -218,8 +218,10 @@ int pc_hydro_source_box_noalloc_v1(pc_ctx_t* ctx, pc_box_t bx,
  * kept in device memory so consecutive steps need no host synchronisation.
  * The Godunov step reads each ghost value from its source cell inside the
  * primitives kernel and stores only the first ghost layer back into the FAB
- * (the artificial viscosity reads it); the method-of-lines step fills all
- * four layers.  pc_level_fill_ghosts_v1 fills all four on request. */
+ * (the artificial viscosity reads it); the method-of-lines step reads them
+ * the same way and stores none.  The FAB's other ghost cells therefore hold
+ * stale values after a step; pc_level_fill_ghosts_v1 fills all four layers
+ * on request (pc_level_write_plotfile_v1 fills them itself for magvort). */
 typedef struct pc_level pc_level_t;
 
 /* Ghost cells beyond Inflow and UserBC domain faces: the reference fills
