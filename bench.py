@@ -69,14 +69,15 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
         if mol:
             # primitives on grow(bx,2) (ghost values read at their source
             # cells); per d-face the primitives the slopes read (rho, u, v,
-            # w, p, Y, c) of one cell and the NVAR fluxes written; the
-            # updates read U, the fluxes (and U^n) and write U (and U^n)
+            # w, p, Y, c) of one cell and the NVAR fluxes written; the first
+            # update reads U^n and the three fluxes and writes U* to the stage
+            # array, the second reads U*, U^n and the fluxes and writes U
             add("ctoprim", (n[0] + 4) * (n[1] + 4) * (n[2] + 4), N + Q + 6)
             for d, a in enumerate("xyz"):
                 f = list(n)
                 f[d] += 1
                 add("mol_" + a, f[0] * f[1] * f[2], 6 + nspec + N)
-            add("mol_update1", v, 6 * N)
+            add("mol_update1", v, 5 * N)
             add("mol_update2", v, 6 * N)
             add("dt", 0, 0)
             continue

@@ -1266,8 +1266,11 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
     case ST_UPDATE:
       LAUNCH((k_update<EOS, P, sizeof(FP) == 4>), grid, st, sc, ka, L->ds);
       break;
-    case ST_MOL_PRIM:
-      LAUNCH((k_ctoprim_ghost<EOS, FP, P, ST_MOL_PRIM>), grid, st, sc, ka, L->la);
+    case ST_MOL_PRIM:  // sub: the Heun stage (1: of U* in the stage arrays)
+      if (sub == 1)
+        LAUNCH((k_ctoprim_ghost<EOS, FP, P, ST_MOL_PRIM, true>), grid, st, sc, ka, L->la);
+      else
+        LAUNCH((k_ctoprim_ghost<EOS, FP, P, ST_MOL_PRIM>), grid, st, sc, ka, L->la);
       break;
     case ST_MOL_X:
       if (L->laxf) LAUNCH((k_mol_flux<EOS, 0, true, FP, P>), grid, st, sc, ka);
@@ -1699,7 +1702,7 @@ static int level_step_eos(pc_level* L, int nsteps, hipStream_t st,
     const bool brm = L->brs.on && !tm && !capturing;
     if (L->mol) {
       for (int h = 0; h < 2; ++h) {
-        run(ST_MOL_PRIM, PC_K_CTOPRIM);
+        run(ST_MOL_PRIM, PC_K_CTOPRIM, h);
         if (brm) {
           branch_mol_fluxes<EOS>(L, S, P, st, L->brs);
         } else {
@@ -1770,14 +1773,14 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
   DtState ds0;
   HIPCHK(c, hipMemcpy(&ds0, L->ds, sizeof(DtState), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpyAsync(saved, L->state, sbytes, hipMemcpyDeviceToDevice, st));
-  // intermediates of one step (the MOL first-stage update also leaves U^n in
-  // the workspace for the second-stage kernel; the state is restored after)
+  // intermediates of one step (the MOL first-stage update also leaves U* in
+  // the stage arrays, which the second-stage update timed below reads; the
+  // state is restored before each of its runs)
   std::vector<int> tune;
   if (L->mol) {
     for (int s : {ST_GHOST, ST_MOL_PRIM, ST_MOL_X, ST_MOL_Y, ST_MOL_Z})
       launch_stage<EOS>(L, s, L->scs[s], L->plans[s], st);
     launch_stage<EOS>(L, ST_MOL_UPD, L->scs[ST_MOL_UPD], L->plans[ST_MOL_UPD], st, 0);
-    HIPCHK(c, hipMemcpyAsync(L->state, saved, sbytes, hipMemcpyDeviceToDevice, st));
     tune = {ST_GHOST, ST_MOL_PRIM, ST_MOL_X, ST_MOL_Y, ST_MOL_Z, ST_MOL_UPD};
   } else {
     for (int s : {ST_GHOST, ST_CTOPRIM, ST_DIVU, ST_TRACE, ST_FLUX1_X, ST_FLUX1_Y,
@@ -2757,7 +2760,7 @@ static void batch_step_eos(pc_batch* B, int n, hipStream_t st) {
   const StagePlan* P = B->plans;
   if (B->mol) {
     for (int h = 0; h < 2; ++h) {
-      launch_stage<EOS>(B, ST_MOL_PRIM, S[ST_MOL_PRIM], P[ST_MOL_PRIM], st);
+      launch_stage<EOS>(B, ST_MOL_PRIM, S[ST_MOL_PRIM], P[ST_MOL_PRIM], st, h);
       if (B->brs.on) {
         branch_mol_fluxes<EOS>(B, S, P, st, B->brs);
       } else {

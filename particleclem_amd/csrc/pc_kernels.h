@@ -172,7 +172,7 @@ struct JobT {
   VT<double> area[3], vol;          // p == 0: uniform geometry
   VT<double> upd;                   // uout (dU/dt accumulator) or p == 0
   VT<double> divu;                  // nodes [L, H+1]^3
-  VT<double> sold;                  // MOL: U^n on the valid box (or p == 0)
+  VT<double> sold;                  // MOL: the first stage's U* on the valid box (or p == 0)
   VT<FP> flat;                      // flattening coefficient on grow(bx,1)
 };
 using Job = JobT<double>;

@@ -44,7 +44,7 @@ PCD int ghost_src(int x, int lo, int hi, int bcl, int bch, int& refl) {
 // direction (r[3], ghost_src).
 template <class J_>
 PCD void ghost_source(const Sched& sc, const J_& J, const LevelArgs& la, int i,
-                      int j, int k, int s[3], int r[3], V& src) {
+                      int j, int k, int s[3], int r[3], V& src, bool sold = false) {
   // periodic_self: 0 = boxes tile the level domain, 1 = every box is its own
   // periodic domain, 2 = every box is its own domain with its own BCs
   int D0[3], D1[3], bl[3], bh[3];
@@ -59,12 +59,12 @@ PCD void ghost_source(const Sched& sc, const J_& J, const LevelArgs& la, int i,
   s[0] = ghost_src(i, D0[0], D1[0], bl[0], bh[0], r[0]);
   s[1] = ghost_src(j, D0[1], D1[1], bl[1], bh[1], r[1]);
   s[2] = ghost_src(k, D0[2], D1[2], bl[2], bh[2], r[2]);
-  src = J.U;
+  src = sold ? J.sold : J.U;
   if (!la.periodic_self) {
     const long nx = D1[0] - D0[0] + 1, ny = D1[1] - D0[1] + 1;
     const int sb =
         la.cellmap[(s[0] - D0[0]) + nx * ((s[1] - D0[1]) + ny * (long)(s[2] - D0[2]))];
-    src = sc.jobs[sb].U;
+    src = sold ? sc.jobs[sb].sold : sc.jobs[sb].U;
   }
 }
 
@@ -113,8 +113,10 @@ __global__ void __launch_bounds__(BLOCK)
 // state, or the method-of-lines step's grow(bx,2) (ST_MOL_PRIM), which writes
 // none back (its kernels read the state on the valid cells only).  The
 // other ghost cells are not stored (pc_level_fill_ghosts_v1 fills all four
-// layers).
-template <class EOS, class FP, int REG>
+// layers).  SOLD: the second MOL stage's primitives, of the first stage's
+// result U*, which the first update leaves in each box's stage array sold
+// (the valid cells and every ghost source read there).
+template <class EOS, class FP, int REG, bool SOLD = false>
 PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
                          const LevelArgs& la) {
   constexpr int NVAR = 7 + EOS::NS;
@@ -130,13 +132,14 @@ PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
             d2 = out(k, J.lo[2], J.hi[2]);
   const int dist = d0 > d1 ? (d0 > d2 ? d0 : d2) : (d1 > d2 ? d1 : d2);
   if (dist <= 0) {
+    const V& u = SOLD ? J.sold : J.U;
 #pragma unroll
     for (int m = 0; m < NVAR; ++m)
-      if (m != UEINT) Uc[m] = J.U(i, j, k, m);
+      if (m != UEINT) Uc[m] = u(i, j, k, m);
   } else {
     int s[3], r[3];
     V src;
-    ghost_source(sc, J, la, i, j, k, s, r, src);
+    ghost_source(sc, J, la, i, j, k, s, r, src, SOLD);
     const bool store = REG == ST_CTOPRIM && dist == 1;
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) {
@@ -147,14 +150,15 @@ PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
   }
   ctoprim_cell<EOS, FP>(J, ka, i, j, k, Uc);
 }
-template <class EOS, class FP = double, bool PERSIST = false, int REG = ST_CTOPRIM>
+template <class EOS, class FP = double, bool PERSIST = false, int REG = ST_CTOPRIM,
+          bool SOLD = false>
 __global__ void __launch_bounds__(BLOCK)
     k_ctoprim_ghost(Sched sc, KArgs ka, LevelArgs la) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_ctoprim_ghost<EOS, FP, REG>(sc, ka, lb_, la);
+      b_ctoprim_ghost<EOS, FP, REG, SOLD>(sc, ka, lb_, la);
   } else {
-    b_ctoprim_ghost<EOS, FP, REG>(sc, ka, first_block(), la);
+    b_ctoprim_ghost<EOS, FP, REG, SOLD>(sc, ka, first_block(), la);
   }
 }
 

@@ -234,9 +234,11 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // One Heun stage of do_mol_advance on a cell (Source/Advance.cpp:110-176):
-// STAGE 0: U* = 1.0 U^n + dt S(U^n) (U^n kept in J.sold), computeTemp;
-// STAGE 1: U^{n+1} = 0.5 U* + 0.5 U^n, += (0.5 dt) S(U*), computeTemp and
-// post_timestep's computeTemp, NaN check and the CFL partial minimum.
+// STAGE 0: U* = 1.0 U^n + dt S(U^n), computeTemp, into the stage array
+// J.sold (the state keeps U^n, so it is never copied; the second stage's
+// primitives read U* there); STAGE 1: U^{n+1} = 0.5 U* + 0.5 U^n, += (0.5
+// dt) S(U*), computeTemp and post_timestep's computeTemp, into the state;
+// NaN check and the CFL partial minimum.
 // F32: the face fluxes are the mixed step's FP32 ones (mol_divergence_fp32)
 template <class EOS, int STAGE, bool F32>
 PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
@@ -257,16 +259,12 @@ PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
     double S[NVAR];
     if constexpr (STAGE == 0) {
 #pragma unroll
-      for (int m = 0; m < NVAR; ++m) {
-        const double u = J.U(i, j, k, m);
-        J.sold(i, j, k, m) = u;
-        S[m] = 1.0 * u + dt * D[m];
-      }
+      for (int m = 0; m < NVAR; ++m) S[m] = 1.0 * J.U(i, j, k, m) + dt * D[m];
     } else {
       const double hdt = 0.5 * dt;
 #pragma unroll
       for (int m = 0; m < NVAR; ++m) {
-        S[m] = 0.5 * J.U(i, j, k, m) + 0.5 * J.sold(i, j, k, m);
+        S[m] = 0.5 * J.sold(i, j, k, m) + 0.5 * J.U(i, j, k, m);
         S[m] += hdt * D[m];
       }
     }
@@ -276,9 +274,10 @@ PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
     reset_and_temp<EOS>(S, th);
     if constexpr (STAGE == 1) reset_and_temp<EOS>(S, th);
     bool bad = false;
+    const V& out = STAGE == 0 ? J.sold : J.U;
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) {
-      J.U(i, j, k, m) = S[m];
+      out(i, j, k, m) = S[m];
       bad |= !isfinite(S[m]);
     }
     if constexpr (STAGE == 1) {
