@@ -4,8 +4,9 @@
 // dU/dt = -div(F)/V with fluxes from limited piecewise-linear face states
 // and the same approximate Riemann solver as the Godunov path (or the
 // Lax-Friedrichs flux).  Per stage the launches are
-//   ghost fill -> primitives on grow(bx,2) -> one flux launch per direction
-//   (d-faces [L, H+1] x transverse [L, H]) -> one update launch on [L, H].
+//   primitives on grow(bx,2) (ghost values read at their source cells) ->
+//   one flux launch per direction (d-faces [L, H+1] x transverse [L, H]) ->
+//   one update launch on [L, H].
 // A flux thread recomputes the slopes of the two cells of its face from
 // the primitives instead of reading a stored slope array: the slope of a
 // cell costs ~60 flops on values that sit in L2, storing it would cost
