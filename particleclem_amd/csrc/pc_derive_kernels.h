@@ -103,7 +103,7 @@ PCD void b_derive(const Sched& sc, const KArgs& ka, int lb_, const DeriveArgs& d
 template <class EOS>
 __global__ void __launch_bounds__(BLOCK)
     k_derive(Sched sc, KArgs ka, DeriveArgs da) {
-  b_derive<EOS>(sc, ka, first_block(), da);
+  b_derive<EOS>(sc, ka, first_block(sc), da);
 }
 
 }  // namespace pcd

@@ -1085,7 +1085,20 @@ struct Branches {
   }
 };
 
+// Stages whose workgroups walk their logical blocks backwards (Sched::rev):
+// such a launch starts on the boxes its producer wrote last, whose arrays
+// are still in the last-level cache.  The divergence after the primitives,
+// the transverse fluxes after the traces, the update after the final fluxes
+// (the next primitives then run forwards after it), and the MOL primitives
+// after the MOL update.  On a level of 64 boxes of 32^3 this takes ~1% off
+// the Godunov step and 2-4% off the MOL step; reversing every stage is
+// slower than reversing none.
+constexpr unsigned kReversedStages =
+    (1u << ST_DIVU) | (1u << ST_TRANS_XY) | (1u << ST_TRANS_YX) |
+    (1u << ST_TRANS_ZX) | (1u << ST_UPDATE) | (1u << ST_MOL_PRIM);
+
 struct pc_level {
+  unsigned rev_mask = kReversedStages;
   pc_ctx* ctx = nullptr;
   pc_params_t prm;
   std::vector<IBox> boxes;
@@ -1167,9 +1180,11 @@ static bool plan_make(pc_level* L, int s, int lx, int ly, int lz, int cap,
   return true;
 }
 
-static Sched plan_sched(const pc_level* L, const StagePlan& P) {
-  return Sched{L->jobs, P.blkmap, P.blkstart, P.nblocks, P.lx, P.ly, P.lz,
-               (int)L->hjobs.size()};
+static Sched plan_sched(const pc_level* L, const StagePlan& P, int stage) {
+  Sched s{L->jobs, P.blkmap, P.blkstart, P.nblocks, P.lx, P.ly, P.lz,
+          (int)L->hjobs.size()};
+  s.rev = P.grid >= P.nblocks && ((L->rev_mask >> stage) & 1u);
+  return s;
 }
 
 static void graph_drop(pc_level* L) {
@@ -1427,7 +1442,7 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
       level_free(L);
       return fail(c, PC_ERR_NOMEM, "plan allocation failed");
     }
-    L->scs[s] = plan_sched(L, L->plans[s]);
+    L->scs[s] = plan_sched(L, L->plans[s], s);
   }
   // ghost-fill cell map over the domain
   memset(&L->la, 0, sizeof(L->la));
@@ -1809,7 +1824,7 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
         rc = fail(c, PC_ERR_NOMEM, "autotune: plan allocation failed");
         break;
       }
-      const Sched sc = plan_sched(L, P);
+      const Sched sc = plan_sched(L, P, s);
       double tot = 0.0;
       for (int r = 0; r <= reps; ++r) {  // r = 0 warms up
         if (s == ST_UPDATE || s == ST_MOL_UPD)
@@ -1838,7 +1853,7 @@ static int level_autotune(pc_level* L, int reps, hipStream_t st) {
     if (bestc >= 0) {
       plan_free(L->plans[s]);
       L->plans[s] = bestP;
-      L->scs[s] = plan_sched(L, bestP);
+      L->scs[s] = plan_sched(L, bestP, s);
       L->tuned_ms[s] = best;
     }
   }
@@ -2016,7 +2031,7 @@ int pc_level_set_plan_v1(pc_level_t* L, int stage, int lx, int ly, int lz,
   graph_drop(L);
   plan_free(L->plans[stage]);
   L->plans[stage] = P;
-  L->scs[stage] = plan_sched(L, P);
+  L->scs[stage] = plan_sched(L, P, stage);
   L->tuned_ms[stage] = 0.0;
   return PC_OK;
 }
@@ -2043,7 +2058,7 @@ int pc_level_set_precision_v1(pc_level_t* L, int mixed) {
   for (int x = 0; x < 3; ++x) {
     plan_free(L->plans[st[x]]);
     L->plans[st[x]] = P[x];
-    L->scs[st[x]] = plan_sched(L, P[x]);
+    L->scs[st[x]] = plan_sched(L, P[x], st[x]);
   }
   return PC_OK;
 }

@@ -206,6 +206,7 @@ struct Sched {
   int nlog;             // logical blocks (0: one per workgroup)
   int lx, ly, lz;       // tile shape (log2), lx = -1 for runs
   int njobs;            // jobs in blkstart (used without blkmap)
+  int rev = 0;          // 1: each XCD walks its range of logical blocks backwards
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
@@ -222,6 +223,15 @@ PCD int xcd_swizzle(int i, int n) {
 typedef const __attribute__((address_space(4))) int CInt;
 
 PCD int first_block() { return xcd_swizzle(blockIdx.x, gridDim.x); }
+// One workgroup per logical block: the XCD's range in order, or backwards
+// (s.rev), so that a launch can start where the previous one ended.
+PCD int first_block(const Sched& s) {
+  const int i = blockIdx.x, n = gridDim.x;
+  if (!s.rev) return xcd_swizzle(i, n);
+  const int x = i & 7, q = n >> 3, r = n & 7;
+  const int cnt = q + (x < r ? 1 : 0);
+  return x * q + (x < r ? x : r) + (cnt - 1 - (i >> 3));
+}
 PCD int nlogical(const Sched& s) { return s.nlog ? s.nlog : (int)gridDim.x; }
 
 template <class FP = double>
@@ -475,7 +485,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_ctoprim<EOS, FP, REG>(sc, ka, lb_);
   } else {
-    b_ctoprim<EOS, FP, REG>(sc, ka, first_block());
+    b_ctoprim<EOS, FP, REG>(sc, ka, first_block(sc));
   }
 }
 
@@ -522,7 +532,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_srctoprim<NS>(sc, ka, lb_, src);
   } else {
-    b_srctoprim<NS>(sc, ka, first_block(), src);
+    b_srctoprim<NS>(sc, ka, first_block(sc), src);
   }
 }
 
@@ -570,7 +580,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_divu<FP>(sc, ka, lb_);
   } else {
-    b_divu<FP>(sc, ka, first_block());
+    b_divu<FP>(sc, ka, first_block(sc));
   }
 }
 
@@ -655,7 +665,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_trace_plm<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_trace_plm<EOS, D, FP>(sc, ka, first_block());
+    b_trace_plm<EOS, D, FP>(sc, ka, first_block(sc));
   }
 }
 
@@ -686,7 +696,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_flux1<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_flux1<EOS, D, FP>(sc, ka, first_block());
+    b_flux1<EOS, D, FP>(sc, ka, first_block(sc));
   }
 }
 
@@ -765,7 +775,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_tcorr<D, FP, NS>(sc, ka, lb_);
   } else {
-    b_tcorr<D, FP, NS>(sc, ka, first_block());
+    b_tcorr<D, FP, NS>(sc, ka, first_block(sc));
   }
 }
 
@@ -895,7 +905,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_trans<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_trans<EOS, D, FP>(sc, ka, first_block());
+    b_trans<EOS, D, FP>(sc, ka, first_block(sc));
   }
 }
 
@@ -1111,7 +1121,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_final<EOS, D, SRC, FP>(sc, ka, lb_);
   } else {
-    b_final<EOS, D, SRC, FP>(sc, ka, first_block());
+    b_final<EOS, D, SRC, FP>(sc, ka, first_block(sc));
   }
 }
 
@@ -1143,7 +1153,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_final_loworder<EOS, D, FP>(sc, ka, lb_);
   } else {
-    b_final_loworder<EOS, D, FP>(sc, ka, first_block());
+    b_final_loworder<EOS, D, FP>(sc, ka, first_block(sc));
   }
 }
 
@@ -1246,7 +1256,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_consup<NS>(sc, ka, lb_);
   } else {
-    b_consup<NS>(sc, ka, first_block());
+    b_consup<NS>(sc, ka, first_block(sc));
   }
 }
 

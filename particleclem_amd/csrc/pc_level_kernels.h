@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_ghost<NS>(sc, ka, lb_, la);
   } else {
-    b_ghost<NS>(sc, ka, first_block(), la);
+    b_ghost<NS>(sc, ka, first_block(sc), la);
   }
 }
 
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_ctoprim_ghost<EOS, FP, REG, SOLD>(sc, ka, lb_, la);
   } else {
-    b_ctoprim_ghost<EOS, FP, REG, SOLD>(sc, ka, first_block(), la);
+    b_ctoprim_ghost<EOS, FP, REG, SOLD>(sc, ka, first_block(sc), la);
   }
 }
 
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       m = mn(m, b_update<EOS, F32>(sc, ka, lb_, ds));
   } else {
-    m = b_update<EOS, F32>(sc, ka, first_block(), ds);
+    m = b_update<EOS, F32>(sc, ka, first_block(sc), ds);
   }
   block_min_to(m, &ds->minbits);
 }
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       m = mn(m, b_estdt<EOS>(sc, ka, lb_, ds));
   } else {
-    m = b_estdt<EOS>(sc, ka, first_block(), ds);
+    m = b_estdt<EOS>(sc, ka, first_block(sc), ds);
   }
   block_min_to(m, &ds->minbits);
 }
@@ -421,7 +421,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_prim_to_cons<EOS>(sc, ka, lb_, prim, mode);
   } else {
-    b_prim_to_cons<EOS>(sc, ka, first_block(), prim, mode);
+    b_prim_to_cons<EOS>(sc, ka, first_block(sc), prim, mode);
   }
 }
 

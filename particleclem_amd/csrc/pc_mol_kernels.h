@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_mol_flux<EOS, D, LAXF, FP>(sc, ka, lb_);
   } else {
-    b_mol_flux<EOS, D, LAXF, FP>(sc, ka, first_block());
+    b_mol_flux<EOS, D, LAXF, FP>(sc, ka, first_block(sc));
   }
 }
 
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       b_mol_div<NS>(sc, ka, lb_);
   } else {
-    b_mol_div<NS>(sc, ka, first_block());
+    b_mol_div<NS>(sc, ka, first_block(sc));
   }
 }
 
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
       m = mn(m, b_mol_update<EOS, STAGE, F32>(sc, ka, lb_, ds));
   } else {
-    m = b_mol_update<EOS, STAGE, F32>(sc, ka, first_block(), ds);
+    m = b_mol_update<EOS, STAGE, F32>(sc, ka, first_block(sc), ds);
   }
   if constexpr (STAGE == 1) block_min_to(m, &ds->minbits);
 }

@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(BLOCK)
       __syncthreads();  // the next tile restages the buffer
     }
   } else {
-    b_trace_ppm_lds<EOS, D, FP>(sc, ka, first_block(), lds);
+    b_trace_ppm_lds<EOS, D, FP>(sc, ka, first_block(sc), lds);
   }
 }
 
@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(BLOCK)
       __syncthreads();  // the next tile restages the buffer
     }
   } else {
-    b_trace_flux1_lds<EOS, D, FP>(sc, ka, first_block(), lds);
+    b_trace_flux1_lds<EOS, D, FP>(sc, ka, first_block(sc), lds);
   }
 }
 
