@@ -13,6 +13,7 @@
 // (4 + NS) words of HBM traffic per cell and direction.
 #pragma once
 #include "pc_level_kernels.h"
+#include "pc_trace_lds.h"
 
 namespace pcd {
 
@@ -30,8 +31,8 @@ PCD FP mol_lim(FP l, FP r) {
 // state of Source/MOL.cpp:61-104.  LEFT: qtempl of face c+1/2 (+ half a
 // slope); otherwise qtempr of face c-1/2 (- half a slope).  Outputs in the
 // normal frame: rho, un, ut (first transverse), utt (second), p, Y.
-template <int NS, int D, bool LEFT, class FP, class VW>
-PCD void mol_face_state(const VW& q, const VW& qa, bool slopes, int i,
+template <int NS, int D, bool LEFT, class FP, class VW, class VA>
+PCD void mol_face_state(const VW& q, const VA& qa, bool slopes, int i,
                         int j, int k, FP& rho, FP& un, FP& ut, FP& utt, FP& p,
                         FP* Y) {
   constexpr int IU = QU + D;
@@ -104,24 +105,20 @@ PCD void mol_face_state(const VW& q, const VW& qa, bool slopes, int i,
   rho = rs;
 }
 
-// pc_compute_hyp_mol_flux for the d-faces of one box (Source/MOL.cpp:6-190,
-// regular cells, no rotating frame, no advected or linear scalars):
-// face (i,j,k) between cells (i,j,k)-e_D and (i,j,k); the extensive flux
-// goes to flx[D] (the flux register starts from zero: 0 + F * area).
-template <class EOS, int D, bool LAXF, class FP>
-PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
+// pc_compute_hyp_mol_flux on d-face (i,j,k) between cells (i,j,k)-e_D and
+// (i,j,k) (Source/MOL.cpp:6-190, regular cells, no rotating frame, no
+// advected or linear scalars), the primitives read through the views q/qa;
+// the extensive flux goes to flx[D] (the flux register starts from zero:
+// 0 + F * area).
+template <class EOS, int D, bool LAXF, class FP, class VQ, class VA>
+PCD void mol_face_flux(const JobT<FP>& J, const KArgs& ka, const VQ& q, const VA& qa,
+                       int i, int j, int k) {
   constexpr int NS = EOS::NS;
-  int local;
-  const JobT<FP> J = load_job<FP>(sc, lb_, local);
-  int lo[3], n[3], i, j, k;
-  stage_region(ST_MOL_X + D, J.lo, J.hi, lo, n);
-  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const bool slopes = ka.plm_iorder != 1;  // Source/Diffusion.cpp:371-373
   int im = i, jm = j, km = k;
   step<D>(im, jm, km, -1);
   FP rl, ul, vl, v2l, pl, rr, ur, vr, v2r, pr;
   FP Yl[NS], Yr[NS];
-  const VB<FP> q = bufview(J.q), qa = bufview(J.qa);
   mol_face_state<NS, D, true, FP>(q, qa, slopes, im, jm, km, rl, ul, vl, v2l,
                                   pl, Yl);
   mol_face_state<NS, D, false, FP>(q, qa, slopes, i, j, k, rr, ur, vr, v2r,
@@ -130,7 +127,7 @@ PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
   if constexpr (LAXF) {
     o = laxf<EOS, FP>(rl, ul, vl, v2l, pl, Yl, rr, ur, vr, v2r, pr, Yr);
   } else {
-    const FP cavg = FP(0.5) * (J.qa(i, j, k, QC) + J.qa(im, jm, km, QC));
+    const FP cavg = FP(0.5) * (qa(i, j, k, QC) + qa(im, jm, km, QC));
     o = riemann<EOS, FP>(rl, ul, vl, v2l, pl, Yl, rr, ur, vr, v2r, pr, Yr, cavg);
   }
   constexpr int MU = UMX + D;
@@ -163,6 +160,19 @@ PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
     for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = (float)(0.0 + (double)o.f_Y[s] * area);
   }
 }
+
+// The MOL fluxes of the d-faces of one box, one thread per face, the
+// primitives read from the level arrays.
+template <class EOS, int D, bool LAXF, class FP>
+PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3], i, j, k;
+  stage_region(ST_MOL_X + D, J.lo, J.hi, lo, n);
+  if (!region_ijk(sc, lo, n, local, i, j, k)) return;
+  const VB<FP> q = bufview(J.q), qa = bufview(J.qa);
+  mol_face_flux<EOS, D, LAXF, FP>(J, ka, q, qa, i, j, k);
+}
 template <class EOS, int D, bool LAXF, class FP = double, bool PERSIST = false>
 __global__ void __launch_bounds__(BLOCK)
     k_mol_flux(Sched sc, KArgs ka) {
@@ -171,6 +181,147 @@ __global__ void __launch_bounds__(BLOCK)
       b_mol_flux<EOS, D, LAXF, FP>(sc, ka, lb_);
   } else {
     b_mol_flux<EOS, D, LAXF, FP>(sc, ka, first_block(sc));
+  }
+}
+
+// ------------------------------------ MOL fluxes from tiles in LDS --
+// A workgroup owns a tile of d-faces: for x, whole rows of x-faces (ta =
+// n[0] faces) by tb rows along y; for y and z, ta <= 32 faces along x by tb
+// along D (axis b = trace_axis_b(D) = D).  The primitives of the tile's
+// cells, [f0 - 2, f0 + extent] along D (each face reads the three-cell slope
+// stencils of its two cells), are loaded from HBM once, coalesced along x,
+// into LDS (rho, the three velocities, p, the species and c: 6 + NS
+// components); every face thread then reads its stencils there.  Arithmetic
+// and its order are those of mol_face_flux with the level-array views, so
+// the fluxes are the same bits.
+constexpr int MOL_LDS_CELLS = BLOCK + 4 * 64;
+
+__host__ __device__ inline TraceTile mol_tile(int D, const int n[3]) {
+  TraceTile t;
+  const int b = trace_axis_b(D), c = 3 - b;
+  if (D == 0) {
+    t.ta = n[0];
+    t.tb = BLOCK / t.ta;
+    const int fit = MOL_LDS_CELLS / (t.ta + 3);
+    if (t.tb > fit) t.tb = fit;
+    if (t.tb > n[b]) t.tb = n[b];
+  } else {
+    // rows of up to 32 cells along x (coalesced staging), as many faces
+    // along D as the workgroup holds; a tile stages its faces' cells plus
+    // three (two below, one above) along D
+    t.ta = n[0] < 32 ? n[0] : 32;
+    t.tb = BLOCK / t.ta;
+    const int fit = MOL_LDS_CELLS / t.ta - 3;
+    if (t.tb > fit) t.tb = fit;
+    if (t.tb > n[b]) t.tb = n[b];
+  }
+  t.nta = (n[0] + t.ta - 1) / t.ta;
+  t.ntb = (n[b] + t.tb - 1) / t.tb;
+  t.nc = n[c];
+  return t;
+}
+
+// Whether the x-face rows fit one workgroup (n: the valid box's extents;
+// the x-face region is n[0] + 1 long).
+__host__ __device__ inline bool mol_pencils_fit(const int n[3]) {
+  return n[0] + 1 <= BLOCK;
+}
+
+// Window cells of the tile of direction D over the d-face region n.
+__host__ __device__ inline int mol_window_cells(int D, const int n[3]) {
+  const TraceTile t = mol_tile(D, n);
+  return D == 0 ? (t.ta + 3) * t.tb : t.ta * (t.tb + 3);
+}
+
+// Component slot of a primitive in the window (QA: the qaux view, whose
+// only component read is QC).
+template <int NS, bool QA>
+PCD int mol_slot(int comp) {
+  if (QA) return 5 + NS;
+  return comp == QRHO ? 0 : comp == QU ? 1 : comp == QV ? 2 : comp == QW ? 3
+       : comp == QPRES ? 4 : 5 + (comp - QFS);
+}
+
+// A view of the window by absolute cell index: window cell (i - x0) +
+// wa (c[AB] - b0), component slot m at lds[m * nwin + cell].
+template <class FP, int NS, int AB, bool QA>
+struct MolWin {
+  const FP* lds;
+  int nwin, wa, x0, b0;
+  PCD FP operator()(int i, int j, int k, int comp) const {
+    const int cb = AB == 1 ? j : k;
+    return lds[mol_slot<NS, QA>(comp) * nwin + (i - x0) + wa * (cb - b0)];
+  }
+};
+
+template <class EOS, int D, bool LAXF, class FP>
+PCD void b_mol_flux_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
+  constexpr int NS = EOS::NS;
+  constexpr int AB = (D == 2) ? 2 : 1;  // axis b
+  constexpr int AC = 3 - AB;            // axis c
+  int local;
+  const JobT<FP> J = load_job<FP>(sc, lb_, local);
+  int lo[3], n[3];
+  stage_region(ST_MOL_X + D, J.lo, J.hi, lo, n);
+  const TraceTile T = mol_tile(D, n);
+  const int tile = local / BLOCK, t = local % BLOCK;
+  if (tile >= T.nta * T.ntb * T.nc) return;  // whole workgroup: no barrier crossed
+  const int ia = tile % T.nta, r = tile / T.nta;
+  const int ib = r % T.ntb, ic = r / T.ntb;
+  const int ta_ = t % T.ta, tb_ = t / T.ta;
+  int ijk[3];
+  ijk[0] = lo[0] + ia * T.ta + ta_;
+  ijk[AB] = lo[AB] + ib * T.tb + tb_;
+  ijk[AC] = lo[AC] + ic;
+  const bool active = tb_ < T.tb && ijk[0] < lo[0] + n[0] && ijk[AB] < lo[AB] + n[AB];
+  // window: the tile's cells, two more below and one more above along D
+  constexpr int HA = (D == 0) ? 2 : 0, HB = (D == 0) ? 0 : 2;
+  const int wa = T.ta + (D == 0 ? 3 : 0), wb = T.tb + (D == 0 ? 0 : 3);
+  const int nwin = wa * wb;
+  const int x0 = lo[0] + ia * T.ta - HA, b0 = lo[AB] + ib * T.tb - HB;
+  // cells of the window inside the tile's part of the region (the last
+  // tile of a row may be partial): along D [lo - 2, lo + n], across [lo, lo + n)
+  const int xe = (D == 0) ? lo[0] + n[0] : lo[0] + n[0] - 1;
+  const int be = (D == 0) ? lo[AB] + n[AB] - 1 : lo[AB] + n[AB];
+  {
+    const VB<FP> q = bufview(J.q), qa = bufview(J.qa);
+    for (int w = threadIdx.x; w < nwin; w += BLOCK) {
+      int g[3];
+      g[0] = x0 + w % wa;
+      g[AB] = b0 + w / wa;
+      g[AC] = lo[AC] + ic;
+      if (g[0] > xe || g[AB] > be) continue;
+      lds[0 * nwin + w] = q(g[0], g[1], g[2], QRHO);
+      lds[1 * nwin + w] = q(g[0], g[1], g[2], QU);
+      lds[2 * nwin + w] = q(g[0], g[1], g[2], QV);
+      lds[3 * nwin + w] = q(g[0], g[1], g[2], QW);
+      lds[4 * nwin + w] = q(g[0], g[1], g[2], QPRES);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) lds[(5 + s) * nwin + w] = q(g[0], g[1], g[2], QFS + s);
+      lds[(5 + NS) * nwin + w] = qa(g[0], g[1], g[2], QC);
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const MolWin<FP, NS, AB, false> qv{lds, nwin, wa, x0, b0};
+  const MolWin<FP, NS, AB, true> qav{lds, nwin, wa, x0, b0};
+  mol_face_flux<EOS, D, LAXF, FP>(J, ka, qv, qav, ijk[0], ijk[1], ijk[2]);
+}
+
+// Dynamic LDS: (6 + NS) components of the largest window of the launch's
+// boxes (mol_window_cells), sizeof(FP) each.
+template <class EOS, int D, bool LAXF, class FP = double, bool PERSIST = false>
+__global__ void __launch_bounds__(BLOCK)
+    k_mol_flux_lds(Sched sc, KArgs ka) {
+  extern __shared__ __align__(16) unsigned char pc_mol_dyn[];
+  FP* lds = (FP*)pc_mol_dyn;
+  if constexpr (PERSIST) {
+    for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x) {
+      b_mol_flux_lds<EOS, D, LAXF, FP>(sc, ka, lb_, lds);
+      __syncthreads();  // the next tile restages the buffer
+    }
+  } else {
+    b_mol_flux_lds<EOS, D, LAXF, FP>(sc, ka, first_block(sc), lds);
   }
 }
 
