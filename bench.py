@@ -783,7 +783,8 @@ def main():
             and not args.autotune and not args.laxf):
         traffic = profiled_traffic(summary, dom)
     kname = ("k_mol_update" if dom.startswith("mol_update") else
-             "k_mol_flux" if dom.startswith("mol_") else
+             ("k_mol_flux_lds" if args.precision == "fp64" else "k_mol_flux")
+             if dom.startswith("mol_") else
              "k_trace_flux1_lds" if dom.startswith("trace_") and kt.get("flux1_x", 1) < 0.02
              else "k_" + dom.split("_")[0])
     # the dominant kernel against HBM on its own stage bytes (every input
