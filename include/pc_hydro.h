@@ -335,10 +335,13 @@ int pc_level_kernel_times_v1(pc_level_t* lev, double* ms, int n);
  * MFIter tiling in the reference (Source/Hydro.cpp:60-73); no counterpart
  * to the timing itself. */
 int pc_level_autotune_v1(pc_level_t* lev, int reps, void* stream);
-/* shape[5] = {lx (-1: runs), ly, lz, workgroups, logical blocks} of stage
- * `stage` (0..16: ctoprim, divu, trace, flux1 x/y/z, trans xy/xz/yx/yz/zx/zy,
- * final x/y/z, update, ghost); *ms = its tuned time per launch (0 if never
- * tuned; may be NULL). */
+/* shape[5] = {lx, ly, lz, workgroups, logical blocks} of stage `stage`
+ * (0..21: ctoprim, divu, trace, flux1 x/y/z, trans xy/xz/yx/yz/zx/zy, final
+ * x/y/z, update, ghost, MOL primitives, MOL fluxes x/y/z, MOL update); lx =
+ * -1: runs, -2 / -3: the PPM trace's LDS tiles / pencils, -4: the FP64 MOL
+ * fluxes' LDS face tiles (those stages always run on their tiles; a plan set
+ * for them changes only the persistence); *ms = its tuned time per launch
+ * (0 if never tuned; may be NULL). */
 int pc_level_plan_v1(const pc_level_t* lev, int stage, int* shape, double* ms);
 /* Set stage `stage`'s plan by hand (e.g. a shape saved from an earlier
  * autotune): lx = -1 for runs, else a tile with lx + ly + lz = 8; wg_per_cu

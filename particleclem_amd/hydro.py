@@ -241,14 +241,16 @@ class DeviceLevel:
         self._chk(self.L.pc_level_autotune_v1(self.h, int(reps), _stream_ptr(stream)))
 
     def plans(self):
-        """{stage: (shape (lx, ly, lz) or 'runs', workgroups, logical
-        blocks, tuned ms)}"""
+        """{stage: (shape (lx, ly, lz), 'runs' or 'tiles' (the LDS tiles of
+        the PPM trace and of the FP64 MOL fluxes), workgroups, logical blocks,
+        tuned ms)}"""
         out = {}
         for sid, name in enumerate(_lib.STAGES):
             sh = (C.c_int * 5)()
             ms = C.c_double()
             self._chk(self.L.pc_level_plan_v1(self.h, sid, sh, C.byref(ms)))
-            shape = "runs" if sh[0] < 0 else (1 << sh[0], 1 << sh[1], 1 << sh[2])
+            shape = ("runs" if sh[0] == -1 else "tiles" if sh[0] < 0
+                     else (1 << sh[0], 1 << sh[1], 1 << sh[2]))
             out[name] = (shape, sh[3], sh[4], ms.value)
         return out
 
