@@ -61,6 +61,19 @@ def test_mol_odd_boxes_every_boundary_kind(cuda, over):
     assert_parity(w, 3, over, label=f"sod 21x13x7 MOL, all BC kinds {over}")
 
 
+@pytest.mark.parametrize("over", [MOL, LAXF], ids=["riemann", "laxf"])
+@pytest.mark.parametrize("idir", [0, 1])
+def test_mol_large_odd_box_partial_tiles(cuda, over, idir):
+    """One box of 45 x 37 x 19 cells (the tube along x, or 19 x 45 x 37 with
+    it along y): x-face rows longer than a wave and fewer than a workgroup's
+    worth per tile, y- and z-face tiles split along x (32 + the rest) and
+    along the direction with a shorter last tile, which the FP64 face tiles
+    staged in LDS must cover exactly."""
+    w = wl.sod(n=45, ny=37, nz=19, nsteps=2, idir=idir)
+    assert len(w.boxes) == 1
+    assert_parity(w, 2, over, label=f"sod one odd box idir={idir} MOL {over}")
+
+
 def _box_inputs(kind):
     w = wl.sod(n=16, nsteps=0) if kind == 1 else wl.premixed(n=16, box=16, nsteps=0)
     H = ob.HostLevel(w.nspec, w.boxes, w.domlo, w.domhi, w.bclo, w.bchi, w.dx)
