@@ -1,6 +1,6 @@
 """The method-of-lines path on the GPU (through the C ABI) against the CPU
 oracle on the same seeded inputs: whole MOL level steps (do_mol_advance,
-both Heun stages, ghost fills, computeTemp, dt) for the gamma-law gas and
+both Heun stages, ghost values, computeTemp, dt) for the gamma-law gas and
 the 9-species mixture, with the approximate Riemann solver and with the
 Lax-Friedrichs flux, with and without slopes; the one-box MOL source term
 pc_mol_src_box_v1; launch shapes, autotuning and hipGraph replay of MOL
