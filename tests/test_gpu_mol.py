@@ -208,6 +208,46 @@ def test_mixed_precision_mol_close_to_fp64(cuda):
     c32.close()
 
 
+def test_mol_precision_switch_replans_face_fluxes(cuda):
+    """The FP64 MOL fluxes run on face tiles staged in LDS, the mixed ones one
+    thread per face, so switching a MOL level's precision re-plans those
+    stages: a level switched FP64 -> mixed -> FP64 between steps equals a
+    level stepped the same way with the plans made fresh at each switch
+    (a new level from the state), bitwise after each phase."""
+    w = wl.premixed(n=32, box=16, nsteps=0)
+    over = dict(do_mol=1, fixed_dt=2.0e-8)
+    c, sw = _level(w, **over)
+    assert sw.plans()["mol_x"][0] == "tiles"
+    sw.step(2)
+    sw.set_precision(True)
+    assert sw.plans()["mol_x"][0] == "runs"
+    sw.step(2)
+    mid = _states(sw, len(w.boxes))
+    sw.set_precision(False)
+    assert sw.plans()["mol_x"][0] == "tiles"
+    sw.step(2)
+    got = _states(sw, len(w.boxes))
+    # the same phases on fresh levels loaded with the state at each switch
+    c1, a = _level(w, **over)
+    a.step(2)
+    s2 = _states(a, len(w.boxes))
+    c2, b = _level(w, **over)
+    b.set_precision(True)
+    for k, f in enumerate(s2):
+        b.write_box(k, f)
+    b.step(2)
+    assert all(np.array_equal(x, y) for x, y in zip(_states(b, len(w.boxes)), mid))
+    c3, d = _level(w, **over)
+    for k, f in enumerate(mid):
+        d.write_box(k, f)
+    d.step(2)
+    assert all(np.array_equal(x, y) for x, y in zip(_states(d, len(w.boxes)), got))
+    assert sw.status()[0] == 0
+    for x, cc in ((sw, c), (a, c1), (b, c2), (d, c3)):
+        x.close()
+        cc.close()
+
+
 def test_mol_option_errors(cuda):
     from particleclem_amd._lib import HydroError
     from particleclem_amd.hydro import DeviceLevel, HydroContext
