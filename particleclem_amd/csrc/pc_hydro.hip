@@ -1180,9 +1180,9 @@ static bool plan_make(pc_level* L, int s, int lx, int ly, int lz, int cap,
                       StagePlan& P) {
   // the PPM trace always runs on its LDS tiles (only persistence varies)
   if (s == ST_TRACE && L->prm.ppm_type == 1 && !L->mol) lx = L->fuse ? -3 : -2;
-  // so do the FP64 MOL fluxes on their face tiles
+  // so do the MOL fluxes on their face tiles (mixed: the x fluxes only)
   if (s >= ST_MOL_X && s <= ST_MOL_Z)
-    lx = (L->mol_lds && !L->mixed) ? -4 : (lx == -4 ? -1 : lx);
+    lx = (L->mol_lds && (!L->mixed || s == ST_MOL_X)) ? -4 : (lx == -4 ? -1 : lx);
   const int nb = (int)L->hjobs.size();
   std::vector<int> start(nb), map;
   for (int b = 0; b < nb; ++b) {
@@ -1243,15 +1243,15 @@ static void level_free(pc_level* L) {
   delete L;
 }
 
-// The MOL fluxes of direction D: in FP64 from face tiles staged in LDS when
-// every box's x-face rows fit one workgroup, else (and in the mixed step,
-// whose face threads overlap their loads well enough: staged, the FP32
-// kernels ran 2-6% faster alone but the step with its three directions on
-// branch streams 1.7% slower) one thread per face from the level arrays.
+// The MOL fluxes of direction D: from face tiles staged in LDS when every
+// box's x-face rows fit one workgroup (in the mixed step only the x fluxes:
+// staged, the FP32 y and z kernels ran faster alone but the step with its
+// three directions on branch streams slower), else one thread per face from
+// the level arrays.
 template <class EOS, int D, class FP, bool P, class H>
 static void launch_mol_flux(H* L, int grid, hipStream_t st, const Sched& sc) {
   const KArgs& ka = L->ka;
-  if (L->mol_lds && sizeof(FP) == 8) {
+  if (L->mol_lds && (sizeof(FP) == 8 || D == 0)) {
     const size_t sh = (size_t)L->mol_cells * (6 + EOS::NS) * sizeof(FP);
     if (L->laxf) LAUNCH_SH((k_mol_flux_lds<EOS, D, true, FP, P>), grid, sh, st, sc, ka);
     else LAUNCH_SH((k_mol_flux_lds<EOS, D, false, FP, P>), grid, sh, st, sc, ka);
@@ -2799,7 +2799,7 @@ static int batch_tables(pc_batch* B, const std::vector<int>& ids,
     int tot = 0;
     const int lx = (s == ST_TRACE && B->prm.ppm_type == 1 && !B->mol)
                        ? (B->fuse ? -3 : -2)
-                       : (s >= ST_MOL_X && s <= ST_MOL_Z && B->mol_lds && !B->mixed) ? -4
+                       : (s >= ST_MOL_X && s <= ST_MOL_Z && B->mol_lds && (!B->mixed || s == ST_MOL_X)) ? -4
                                                                                    : -1;
     for (int k = 0; k < n; ++k) {
       start[(size_t)s * B->max_slots + k] = tot;

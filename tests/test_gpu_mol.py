@@ -209,22 +209,22 @@ def test_mixed_precision_mol_close_to_fp64(cuda):
 
 
 def test_mol_precision_switch_replans_face_fluxes(cuda):
-    """The FP64 MOL fluxes run on face tiles staged in LDS, the mixed ones one
-    thread per face, so switching a MOL level's precision re-plans those
-    stages: a level switched FP64 -> mixed -> FP64 between steps equals a
-    level stepped the same way with the plans made fresh at each switch
-    (a new level from the state), bitwise after each phase."""
+    """The FP64 MOL fluxes run on face tiles staged in LDS, the mixed y and z
+    fluxes one thread per face, so switching a MOL level's precision
+    re-plans those stages: a level switched FP64 -> mixed -> FP64 between
+    steps equals a level stepped the same way with the plans made fresh at
+    each switch (a new level from the state), bitwise after each phase."""
     w = wl.premixed(n=32, box=16, nsteps=0)
     over = dict(do_mol=1, fixed_dt=2.0e-8)
     c, sw = _level(w, **over)
-    assert sw.plans()["mol_x"][0] == "tiles"
+    assert sw.plans()["mol_y"][0] == "tiles" and sw.plans()["mol_x"][0] == "tiles"
     sw.step(2)
     sw.set_precision(True)
-    assert sw.plans()["mol_x"][0] == "runs"
+    assert sw.plans()["mol_y"][0] == "runs" and sw.plans()["mol_x"][0] == "tiles"
     sw.step(2)
     mid = _states(sw, len(w.boxes))
     sw.set_precision(False)
-    assert sw.plans()["mol_x"][0] == "tiles"
+    assert sw.plans()["mol_y"][0] == "tiles"
     sw.step(2)
     got = _states(sw, len(w.boxes))
     # the same phases on fresh levels loaded with the state at each switch
