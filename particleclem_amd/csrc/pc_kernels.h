@@ -412,8 +412,10 @@ PCD void step(int& i, int& j, int& k, int s) {
 
 // ------------------------------------------------------------ K1 ctoprim --
 // Source/PeleCUtilities.H:242-320: the primitives and auxiliaries of cell
-// (i,j,k) from its conserved state Uc (every component but UEINT is read)
-template <class EOS, class FP>
+// (i,j,k) from its conserved state Uc (every component but UEINT is read).
+// MOLQ: for the MOL fluxes, which read rho, the velocities, p, the species
+// and c only (QREINT and QGAMC are not stored).
+template <class EOS, class FP, bool MOLQ = false>
 PCD void ctoprim_cell(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
                       const double* Uc) {
   constexpr int NS = EOS::NS;
@@ -445,9 +447,9 @@ PCD void ctoprim_cell(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
   q(i, j, k, QW) = vz;
 #pragma unroll
   for (int s = 0; s < NS; ++s) q(i, j, k, QFS + s) = Y[s];
-  q(i, j, k, QREINT) = e * rho;
+  if (!MOLQ) q(i, j, k, QREINT) = e * rho;
   q(i, j, k, QPRES) = p;
-  qa(i, j, k, QGAMC) = gam1;
+  if (!MOLQ) qa(i, j, k, QGAMC) = gam1;
   qa(i, j, k, QC) = cs;
   if (ka.full_prims) {
     const double w = th.wbar();
@@ -475,7 +477,7 @@ PCD void b_ctoprim(const Sched& sc, const KArgs& ka, int lb_) {
 #pragma unroll
   for (int m = 0; m < NVAR; ++m)
     if (m != UEINT) Uc[m] = J.U(i, j, k, m);
-  ctoprim_cell<EOS, FP>(J, ka, i, j, k, Uc);
+  ctoprim_cell<EOS, FP, REG == ST_MOL_PRIM>(J, ka, i, j, k, Uc);
 }
 template <class EOS, class FP = double, bool PERSIST = false,
           int REG = ST_CTOPRIM>

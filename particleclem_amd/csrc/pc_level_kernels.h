@@ -148,7 +148,7 @@ PCD void b_ctoprim_ghost(const Sched& sc, const KArgs& ka, int lb_,
       if (store) J.U(i, j, k, m) = Uc[m];
     }
   }
-  ctoprim_cell<EOS, FP>(J, ka, i, j, k, Uc);
+  ctoprim_cell<EOS, FP, REG == ST_MOL_PRIM>(J, ka, i, j, k, Uc);
 }
 template <class EOS, class FP = double, bool PERSIST = false, int REG = ST_CTOPRIM,
           bool SOLD = false>
