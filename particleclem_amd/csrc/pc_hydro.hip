@@ -1538,6 +1538,8 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   L->ka.difmag = prm->difmag;
   L->ka.plm_iorder = prm->plm_iorder;
   L->ka.use_flattening = prm->use_flattening;
+  // only the PLM traces and the low-order boundary faces read q's rho e
+  L->ka.skip_reint = prm->ppm_type == 1 && !L->lowo[0] && !L->lowo[1] && !L->lowo[2];
   DtState z;
   z.dt = 0.0;
   z.minbits = 0x7FEFFFFFFFFFFFFFull;

@@ -185,6 +185,7 @@ struct KArgs {
   int use_flattening;
   int nblocks;
   int full_prims;  // 1: write every q/qaux comp; 0: only what the step reads
+  int skip_reint;  // 1: q's rho e is not stored (PPM steps without low-order faces)
 };
 
 template <class J_>
@@ -447,7 +448,7 @@ PCD void ctoprim_cell(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
   q(i, j, k, QW) = vz;
 #pragma unroll
   for (int s = 0; s < NS; ++s) q(i, j, k, QFS + s) = Y[s];
-  if (!MOLQ) q(i, j, k, QREINT) = e * rho;
+  if (!MOLQ && !ka.skip_reint) q(i, j, k, QREINT) = e * rho;
   q(i, j, k, QPRES) = p;
   if (!MOLQ) qa(i, j, k, QGAMC) = gam1;
   qa(i, j, k, QC) = cs;
