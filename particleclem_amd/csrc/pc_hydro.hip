@@ -1256,8 +1256,8 @@ static void launch_mol_flux(H* L, int grid, hipStream_t st, const Sched& sc) {
     if (L->laxf) LAUNCH_SH((k_mol_flux_lds<EOS, D, true, FP, P>), grid, sh, st, sc, ka);
     else LAUNCH_SH((k_mol_flux_lds<EOS, D, false, FP, P>), grid, sh, st, sc, ka);
   } else {
-    if (L->laxf) LAUNCH((k_mol_flux<EOS, D, true, FP, P>), grid, st, sc, ka);
-    else LAUNCH((k_mol_flux<EOS, D, false, FP, P>), grid, st, sc, ka);
+    if (L->laxf) LAUNCH((k_mol_flux<EOS, D, true, FP, P, false>), grid, st, sc, ka);
+    else LAUNCH((k_mol_flux<EOS, D, false, FP, P, false>), grid, st, sc, ka);
   }
 }
 

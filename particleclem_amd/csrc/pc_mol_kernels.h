@@ -110,7 +110,9 @@ PCD void mol_face_state(const VW& q, const VA& qa, bool slopes, int i,
 // advected or linear scalars), the primitives read through the views q/qa;
 // the extensive flux goes to flx[D] (the flux register starts from zero:
 // 0 + F * area).
-template <class EOS, int D, bool LAXF, class FP, class VQ, class VA>
+// STORE_T: store the zero UTEMP flux (the box entry point's caller reads
+// it; the level and batch updates take its divergence as zero).
+template <class EOS, int D, bool LAXF, class FP, bool STORE_T, class VQ, class VA>
 PCD void mol_face_flux(const JobT<FP>& J, const KArgs& ka, const VQ& q, const VA& qa,
                        int i, int j, int k) {
   constexpr int NS = EOS::NS;
@@ -142,7 +144,7 @@ PCD void mol_face_flux(const JobT<FP>& J, const KArgs& ka, const VQ& q, const VA
     fo(i, j, k, MTT) = 0.0 + (double)o.f_utt * area;
     fo(i, j, k, UEDEN) = 0.0 + (double)o.f_eden * area;
     fo(i, j, k, UEINT) = 0.0 + (double)o.f_eint * area;
-    fo(i, j, k, UTEMP) = 0.0;
+    if (STORE_T) fo(i, j, k, UTEMP) = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = 0.0 + (double)o.f_Y[s] * area;
   } else {
@@ -163,7 +165,7 @@ PCD void mol_face_flux(const JobT<FP>& J, const KArgs& ka, const VQ& q, const VA
 
 // The MOL fluxes of the d-faces of one box, one thread per face, the
 // primitives read from the level arrays.
-template <class EOS, int D, bool LAXF, class FP>
+template <class EOS, int D, bool LAXF, class FP, bool STORE_T>
 PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
   int local;
   const JobT<FP> J = load_job<FP>(sc, lb_, local);
@@ -171,16 +173,17 @@ PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
   stage_region(ST_MOL_X + D, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const VB<FP> q = bufview(J.q), qa = bufview(J.qa);
-  mol_face_flux<EOS, D, LAXF, FP>(J, ka, q, qa, i, j, k);
+  mol_face_flux<EOS, D, LAXF, FP, STORE_T>(J, ka, q, qa, i, j, k);
 }
-template <class EOS, int D, bool LAXF, class FP = double, bool PERSIST = false>
+template <class EOS, int D, bool LAXF, class FP = double, bool PERSIST = false,
+          bool STORE_T = true>
 __global__ void __launch_bounds__(BLOCK)
     k_mol_flux(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_mol_flux<EOS, D, LAXF, FP>(sc, ka, lb_);
+      b_mol_flux<EOS, D, LAXF, FP, STORE_T>(sc, ka, lb_);
   } else {
-    b_mol_flux<EOS, D, LAXF, FP>(sc, ka, first_block(sc));
+    b_mol_flux<EOS, D, LAXF, FP, STORE_T>(sc, ka, first_block(sc));
   }
 }
 
@@ -305,7 +308,7 @@ PCD void b_mol_flux_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
   if (!active) return;
   const MolWin<FP, NS, AB, false> qv{lds, nwin, wa, x0, b0};
   const MolWin<FP, NS, AB, true> qav{lds, nwin, wa, x0, b0};
-  mol_face_flux<EOS, D, LAXF, FP>(J, ka, qv, qav, ijk[0], ijk[1], ijk[2]);
+  mol_face_flux<EOS, D, LAXF, FP, false>(J, ka, qv, qav, ijk[0], ijk[1], ijk[2]);
 }
 
 // Dynamic LDS: (6 + NS) components of the largest window of the launch's
@@ -326,7 +329,9 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // pc_flux_div (Source/Diffterm.H:470-487): -div(F)/V of one cell.
-template <int NS>
+// T0: the UTEMP fluxes are not stored (all zero): its divergence is
+// -(0 - 0 + 0 - 0 + 0 - 0) / v = -0.0, the value the sum gives.
+template <int NS, bool T0 = false>
 PCD void mol_divergence(const Job& J, int i, int j, int k, double* D) {
   constexpr int NVAR = 7 + NS;
   const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
@@ -334,10 +339,15 @@ PCD void mol_divergence(const Job& J, int i, int j, int k, double* D) {
   const V& f1 = J.flx[1];
   const V& f2 = J.flx[2];
 #pragma unroll
-  for (int m = 0; m < NVAR; ++m)
+  for (int m = 0; m < NVAR; ++m) {
+    if (T0 && m == UTEMP) {
+      D[m] = -0.0 / v;
+      continue;
+    }
     D[m] = -(f0(i + 1, j, k, m) - f0(i, j, k, m) + f1(i, j + 1, k, m) -
              f1(i, j, k, m) + f2(i, j, k + 1, m) - f2(i, j, k, m)) /
            v;
+  }
 }
 // the same for the mixed step's FP32 face fluxes (FP64 arithmetic)
 template <int NS>
@@ -407,7 +417,7 @@ PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
     const double dt = get_dt(J, ka);
     double D[NVAR];
     if constexpr (F32) mol_divergence_fp32<NS>(J, i, j, k, D);
-    else mol_divergence<NS>(J, i, j, k, D);
+    else mol_divergence<NS, true>(J, i, j, k, D);
     double S[NVAR];
     if constexpr (STAGE == 0) {
 #pragma unroll
