@@ -1318,16 +1318,16 @@ static void launch_stage_t(H* L, int s, const Sched& sc, int grid,
     case ST_TRANS_ZX: LAUNCH((k_trans<EOS, 2, FP, P>), grid, st, sc, ka); break;
     case ST_TRANS_XZ: case ST_TRANS_YZ: case ST_TRANS_ZY: break;
     case ST_FINAL_X:
-      LAUNCH((k_final<EOS, 0, false, FP, P>), grid, st, sc, ka);
-      if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0, FP, P>), grid, st, sc, ka);
+      LAUNCH((k_final<EOS, 0, false, FP, P, false>), grid, st, sc, ka);
+      if (L->lowo[0]) LAUNCH((k_final_loworder<EOS, 0, FP, P, false>), grid, st, sc, ka);
       break;
     case ST_FINAL_Y:
-      LAUNCH((k_final<EOS, 1, false, FP, P>), grid, st, sc, ka);
-      if (L->lowo[1]) LAUNCH((k_final_loworder<EOS, 1, FP, P>), grid, st, sc, ka);
+      LAUNCH((k_final<EOS, 1, false, FP, P, false>), grid, st, sc, ka);
+      if (L->lowo[1]) LAUNCH((k_final_loworder<EOS, 1, FP, P, false>), grid, st, sc, ka);
       break;
     case ST_FINAL_Z:
-      LAUNCH((k_final<EOS, 2, false, FP, P>), grid, st, sc, ka);
-      if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2, FP, P>), grid, st, sc, ka);
+      LAUNCH((k_final<EOS, 2, false, FP, P, false>), grid, st, sc, ka);
+      if (L->lowo[2]) LAUNCH((k_final_loworder<EOS, 2, FP, P, false>), grid, st, sc, ka);
       break;
     case ST_UPDATE:
       LAUNCH((k_update<EOS, P, sizeof(FP) == 4>), grid, st, sc, ka, L->ds);

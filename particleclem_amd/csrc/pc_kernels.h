@@ -1012,7 +1012,9 @@ PCD bool loworder_face(const JobT<FP>& J, int D, int idx) {
           (J.bchi[D] == BC_INFLOW || J.bchi[D] == BC_USER));
 }
 
-template <class EOS, int D, class FP>
+// STORE_T: store the zero UTEMP flux in FP64 (the box entry points' callers
+// read it; the level and batch updates take its divergence as zero)
+template <class EOS, int D, class FP, bool STORE_T = true>
 PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
                     const FaceState<EOS::NS, FP>& L, const FaceState<EOS::NS, FP>& R) {
   constexpr int NS = EOS::NS;
@@ -1074,7 +1076,8 @@ PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
   if constexpr (sizeof(FP) == 8) {
     const auto& fo = J.flx[D];
 #pragma unroll
-    for (int m = 0; m < NVAR; ++m) fo(i, j, k, m) = fl[m] * area;
+    for (int m = 0; m < NVAR; ++m)
+      if (STORE_T || m != UTEMP) fo(i, j, k, m) = fl[m] * area;
     J.qec[D](i, j, k, 0) = F.pgd;
     J.qec[D](i, j, k, 1) = F.ugd;
   } else {
@@ -1092,7 +1095,7 @@ PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
   }
 }
 
-template <class EOS, int D, bool SRC, class FP>
+template <class EOS, int D, bool SRC, class FP, bool STORE_T>
 PCD void b_final(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   constexpr int T0 = (D == 0) ? 1 : 0;
@@ -1115,23 +1118,24 @@ PCD void b_final(const Sched& sc, const KArgs& ka, int lb_) {
   final_corr<NS, D, SRC, FP>(J, im, jm, km, hdt, c0, c1, s, L);
   load_trace<NS, FP>(J.tr[D], i, j, k, 1, s);
   final_corr<NS, D, SRC, FP>(J, i, j, k, hdt, c0, c1, s, R);
-  final_tail<EOS, D, FP>(J, ka, i, j, k, L, R);
+  final_tail<EOS, D, FP, STORE_T>(J, ka, i, j, k, L, R);
 }
-template <class EOS, int D, bool SRC, class FP = double, bool PERSIST = false>
+template <class EOS, int D, bool SRC, class FP = double, bool PERSIST = false,
+          bool STORE_T = true>
 __global__ void __launch_bounds__(BLOCK)
     k_final(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_final<EOS, D, SRC, FP>(sc, ka, lb_);
+      b_final<EOS, D, SRC, FP, STORE_T>(sc, ka, lb_);
   } else {
-    b_final<EOS, D, SRC, FP>(sc, ka, first_block(sc));
+    b_final<EOS, D, SRC, FP, STORE_T>(sc, ka, first_block(sc));
   }
 }
 
 // pc_low_order_boundary (Source/Godunov.cpp:7-55,440-468): at Hard/UserBC
 // domain faces the flux comes from first-order-in-time PLM states without
 // transverse terms.  Launched only when such a boundary exists.
-template <class EOS, int D, class FP>
+template <class EOS, int D, class FP, bool STORE_T>
 PCD void b_final_loworder(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
@@ -1147,22 +1151,24 @@ PCD void b_final_loworder(const Sched& sc, const KArgs& ka, int lb_) {
   FaceState<NS, FP> L, R, unused;
   plm_cell<EOS, D, FP>(J, ka, dtdx, im, jm, km, unused, L);
   plm_cell<EOS, D, FP>(J, ka, dtdx, i, j, k, R, unused);
-  final_tail<EOS, D, FP>(J, ka, i, j, k, L, R);
+  final_tail<EOS, D, FP, STORE_T>(J, ka, i, j, k, L, R);
 }
-template <class EOS, int D, class FP = double, bool PERSIST = false>
+template <class EOS, int D, class FP = double, bool PERSIST = false, bool STORE_T = true>
 __global__ void __launch_bounds__(BLOCK)
     k_final_loworder(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_final_loworder<EOS, D, FP>(sc, ka, lb_);
+      b_final_loworder<EOS, D, FP, STORE_T>(sc, ka, lb_);
   } else {
-    b_final_loworder<EOS, D, FP>(sc, ka, first_block(sc));
+    b_final_loworder<EOS, D, FP, STORE_T>(sc, ka, first_block(sc));
   }
 }
 
 // pdivu (Source/Godunov.H:1033-1060) and the flux divergence of one cell
 // (Source/Hydro.H:888-907); h receives dU/dt.
-template <int NS>
+// T0: the UTEMP fluxes are not stored (all zero), so h[UTEMP] keeps its
+// value: adding (0 - 0 + 0 - 0 + 0 - 0) / v = +0.0 changes no bit.
+template <int NS, bool T0 = false>
 PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
                          double* h) {
   constexpr int NVAR = 7 + NS;
@@ -1186,10 +1192,12 @@ PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
   const VB<double> f1 = bufview(J.flx[1]);
   const VB<double> f2 = bufview(J.flx[2]);
 #pragma unroll
-  for (int m = 0; m < NVAR; ++m)
+  for (int m = 0; m < NVAR; ++m) {
+    if (T0 && m == UTEMP) continue;
     h[m] += (f0(i, j, k, m) - f0(i + 1, j, k, m) + f1(i, j, k, m) -
              f1(i, j + 1, k, m) + f2(i, j, k, m) - f2(i, j, k + 1, m)) /
             v;
+  }
   h[UEINT] -= pdivu;
 }
 

@@ -250,7 +250,7 @@ PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState*& ds) {
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) h[m] = 0.0;
     if constexpr (F32) cell_divergence_fp32<NS>(J, ka, i, j, k, h);
-    else cell_divergence<NS>(J, ka, i, j, k, h);
+    else cell_divergence<NS, true>(J, ka, i, j, k, h);
     double S[NVAR];
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) S[m] = J.U(i, j, k, m) + dt * h[m];
