@@ -1012,9 +1012,12 @@ PCD bool loworder_face(const JobT<FP>& J, int D, int idx) {
           (J.bchi[D] == BC_INFLOW || J.bchi[D] == BC_USER));
 }
 
-// STORE_T: store the zero UTEMP flux in FP64 (the box entry points' callers
-// read it; the level and batch updates take its divergence as zero)
-template <class EOS, int D, class FP, bool STORE_T = true>
+// FULL: store every face quantity the box entry points' callers read.  The
+// level and batch steps store neither the zero UTEMP flux nor the UEINT flux
+// and qec (p_gd, u_gd): their update resets UEINT to UEDEN - rho |u|^2 / 2
+// (reset_internal_energy with allow_small_energy = allow_negative_energy =
+// 1), so the UEINT flux divergence and pdivu never reach the new state.
+template <class EOS, int D, class FP, bool FULL = true>
 PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
                     const FaceState<EOS::NS, FP>& L, const FaceState<EOS::NS, FP>& R) {
   constexpr int NS = EOS::NS;
@@ -1077,25 +1080,25 @@ PCD void final_tail(const JobT<FP>& J, const KArgs& ka, int i, int j, int k,
     const auto& fo = J.flx[D];
 #pragma unroll
     for (int m = 0; m < NVAR; ++m)
-      if (STORE_T || m != UTEMP) fo(i, j, k, m) = fl[m] * area;
-    J.qec[D](i, j, k, 0) = F.pgd;
-    J.qec[D](i, j, k, 1) = F.ugd;
+      if (FULL || (m != UTEMP && m != UEINT)) fo(i, j, k, m) = fl[m] * area;
+    if (FULL) {
+      J.qec[D](i, j, k, 0) = F.pgd;
+      J.qec[D](i, j, k, 1) = F.ugd;
+    }
   } else {
-    // mixed step: the face fluxes and qec in FP32, half the bytes written
-    // here and read by the update; the density flux is not stored (the
-    // update takes the sum of the species fluxes, so sum_k rho Y_k = rho
-    // holds to FP64 rounding) and neither is the zero UTEMP flux
+    // mixed step (level and batch): the face fluxes in FP32, half the bytes
+    // written here and read by the update; the density flux is not stored
+    // (the update takes the sum of the species fluxes, so sum_k rho Y_k =
+    // rho holds to FP64 rounding), nor are the zero UTEMP flux, the UEINT
+    // flux and qec (above)
     const VT<float> fo = as_fp32(J.flx[D]);
 #pragma unroll
     for (int m = 0; m < NVAR; ++m)
-      if (m != URHO && m != UTEMP) fo(i, j, k, m) = (float)(fl[m] * area);
-    const VT<float> qe = as_fp32(J.qec[D]);
-    qe(i, j, k, 0) = F.pgd;
-    qe(i, j, k, 1) = F.ugd;
+      if (m != URHO && m != UTEMP && m != UEINT) fo(i, j, k, m) = (float)(fl[m] * area);
   }
 }
 
-template <class EOS, int D, bool SRC, class FP, bool STORE_T>
+template <class EOS, int D, bool SRC, class FP, bool FULL>
 PCD void b_final(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   constexpr int T0 = (D == 0) ? 1 : 0;
@@ -1118,24 +1121,24 @@ PCD void b_final(const Sched& sc, const KArgs& ka, int lb_) {
   final_corr<NS, D, SRC, FP>(J, im, jm, km, hdt, c0, c1, s, L);
   load_trace<NS, FP>(J.tr[D], i, j, k, 1, s);
   final_corr<NS, D, SRC, FP>(J, i, j, k, hdt, c0, c1, s, R);
-  final_tail<EOS, D, FP, STORE_T>(J, ka, i, j, k, L, R);
+  final_tail<EOS, D, FP, FULL>(J, ka, i, j, k, L, R);
 }
 template <class EOS, int D, bool SRC, class FP = double, bool PERSIST = false,
-          bool STORE_T = true>
+          bool FULL = true>
 __global__ void __launch_bounds__(BLOCK)
     k_final(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_final<EOS, D, SRC, FP, STORE_T>(sc, ka, lb_);
+      b_final<EOS, D, SRC, FP, FULL>(sc, ka, lb_);
   } else {
-    b_final<EOS, D, SRC, FP, STORE_T>(sc, ka, first_block(sc));
+    b_final<EOS, D, SRC, FP, FULL>(sc, ka, first_block(sc));
   }
 }
 
 // pc_low_order_boundary (Source/Godunov.cpp:7-55,440-468): at Hard/UserBC
 // domain faces the flux comes from first-order-in-time PLM states without
 // transverse terms.  Launched only when such a boundary exists.
-template <class EOS, int D, class FP, bool STORE_T>
+template <class EOS, int D, class FP, bool FULL>
 PCD void b_final_loworder(const Sched& sc, const KArgs& ka, int lb_) {
   constexpr int NS = EOS::NS;
   int local;
@@ -1151,24 +1154,26 @@ PCD void b_final_loworder(const Sched& sc, const KArgs& ka, int lb_) {
   FaceState<NS, FP> L, R, unused;
   plm_cell<EOS, D, FP>(J, ka, dtdx, im, jm, km, unused, L);
   plm_cell<EOS, D, FP>(J, ka, dtdx, i, j, k, R, unused);
-  final_tail<EOS, D, FP, STORE_T>(J, ka, i, j, k, L, R);
+  final_tail<EOS, D, FP, FULL>(J, ka, i, j, k, L, R);
 }
-template <class EOS, int D, class FP = double, bool PERSIST = false, bool STORE_T = true>
+template <class EOS, int D, class FP = double, bool PERSIST = false, bool FULL = true>
 __global__ void __launch_bounds__(BLOCK)
     k_final_loworder(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_final_loworder<EOS, D, FP, STORE_T>(sc, ka, lb_);
+      b_final_loworder<EOS, D, FP, FULL>(sc, ka, lb_);
   } else {
-    b_final_loworder<EOS, D, FP, STORE_T>(sc, ka, first_block(sc));
+    b_final_loworder<EOS, D, FP, FULL>(sc, ka, first_block(sc));
   }
 }
 
 // pdivu (Source/Godunov.H:1033-1060) and the flux divergence of one cell
 // (Source/Hydro.H:888-907); h receives dU/dt.
-// T0: the UTEMP fluxes are not stored (all zero), so h[UTEMP] keeps its
-// value: adding (0 - 0 + 0 - 0 + 0 - 0) / v = +0.0 changes no bit.
-template <int NS, bool T0 = false>
+// LEVEL: the level and batch updates, whose final fluxes store neither the
+// zero UTEMP flux (adding its (0 - 0 + 0 - 0 + 0 - 0) / v = +0.0 changes no
+// bit) nor the UEINT flux and qec (the update resets UEINT; final_tail):
+// h[UTEMP] and h[UEINT] are left as they are.
+template <int NS, bool LEVEL = false>
 PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
                          double* h) {
   constexpr int NVAR = 7 + NS;
@@ -1176,61 +1181,48 @@ PCD void cell_divergence(const Job& J, const KArgs& ka, int i, int j, int k,
     return J.area[d].p ? J.area[d](a, b, c, 0) : J.area_u[d];
   };
   const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
-  const V& q1 = J.qec[0];
-  const V& q2 = J.qec[1];
-  const V& q3 = J.qec[2];
-  const double pdivu =
-      0.5 *
-      ((q1(i + 1, j, k, 0) + q1(i, j, k, 0)) *
-           (q1(i + 1, j, k, 1) * A(0, i + 1, j, k) - q1(i, j, k, 1) * A(0, i, j, k)) +
-       (q2(i, j + 1, k, 0) + q2(i, j, k, 0)) *
-           (q2(i, j + 1, k, 1) * A(1, i, j + 1, k) - q2(i, j, k, 1) * A(1, i, j, k)) +
-       (q3(i, j, k + 1, 0) + q3(i, j, k, 0)) *
-           (q3(i, j, k + 1, 1) * A(2, i, j, k + 1) - q3(i, j, k, 1) * A(2, i, j, k))) /
-      v;
   const VB<double> f0 = bufview(J.flx[0]);
   const VB<double> f1 = bufview(J.flx[1]);
   const VB<double> f2 = bufview(J.flx[2]);
 #pragma unroll
   for (int m = 0; m < NVAR; ++m) {
-    if (T0 && m == UTEMP) continue;
+    if (LEVEL && (m == UTEMP || m == UEINT)) continue;
     h[m] += (f0(i, j, k, m) - f0(i + 1, j, k, m) + f1(i, j, k, m) -
              f1(i, j + 1, k, m) + f2(i, j, k, m) - f2(i, j, k + 1, m)) /
             v;
   }
-  h[UEINT] -= pdivu;
+  if constexpr (!LEVEL) {
+    const V& q1 = J.qec[0];
+    const V& q2 = J.qec[1];
+    const V& q3 = J.qec[2];
+    const double pdivu =
+        0.5 *
+        ((q1(i + 1, j, k, 0) + q1(i, j, k, 0)) *
+             (q1(i + 1, j, k, 1) * A(0, i + 1, j, k) - q1(i, j, k, 1) * A(0, i, j, k)) +
+         (q2(i, j + 1, k, 0) + q2(i, j, k, 0)) *
+             (q2(i, j + 1, k, 1) * A(1, i, j + 1, k) - q2(i, j, k, 1) * A(1, i, j, k)) +
+         (q3(i, j, k + 1, 0) + q3(i, j, k, 0)) *
+             (q3(i, j, k + 1, 1) * A(2, i, j, k + 1) - q3(i, j, k, 1) * A(2, i, j, k))) /
+        v;
+    h[UEINT] -= pdivu;
+  }
 }
 
-// The same for the mixed step's FP32 face fluxes and qec (final_tail), in
-// FP64 arithmetic: the density flux of a face is the sum of its species
-// fluxes (species first to last).
+// The same, as the level update with LEVEL, for the mixed step's FP32 face
+// fluxes (final_tail), in FP64 arithmetic: the density flux of a face is the
+// sum of its species fluxes (species first to last).
 template <int NS>
 PCD void cell_divergence_fp32(const Job& J, const KArgs& ka, int i, int j, int k,
                               double* h) {
   constexpr int NVAR = 7 + NS;
-  auto A = [&](int d, int a, int b, int c) {
-    return J.area[d].p ? J.area[d](a, b, c, 0) : J.area_u[d];
-  };
+  (void)ka;
   const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
-  const VT<float> q1 = as_fp32(J.qec[0]);
-  const VT<float> q2 = as_fp32(J.qec[1]);
-  const VT<float> q3 = as_fp32(J.qec[2]);
-  auto Q = [](const VT<float>& q, int a, int b, int c, int n) { return (double)q(a, b, c, n); };
-  const double pdivu =
-      0.5 *
-      ((Q(q1, i + 1, j, k, 0) + Q(q1, i, j, k, 0)) *
-           (Q(q1, i + 1, j, k, 1) * A(0, i + 1, j, k) - Q(q1, i, j, k, 1) * A(0, i, j, k)) +
-       (Q(q2, i, j + 1, k, 0) + Q(q2, i, j, k, 0)) *
-           (Q(q2, i, j + 1, k, 1) * A(1, i, j + 1, k) - Q(q2, i, j, k, 1) * A(1, i, j, k)) +
-       (Q(q3, i, j, k + 1, 0) + Q(q3, i, j, k, 0)) *
-           (Q(q3, i, j, k + 1, 1) * A(2, i, j, k + 1) - Q(q3, i, j, k, 1) * A(2, i, j, k))) /
-      v;
   const VB<float> f[3] = {bufview(as_fp32(J.flx[0])), bufview(as_fp32(J.flx[1])),
                           bufview(as_fp32(J.flx[2]))};
   double drho = 0.0;
 #pragma unroll
   for (int m = 0; m < NVAR; ++m) {
-    if (m == URHO || m == UTEMP) continue;
+    if (m == URHO || m == UTEMP || m == UEINT) continue;
     double dm = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -1241,7 +1233,6 @@ PCD void cell_divergence_fp32(const Job& J, const KArgs& ka, int i, int j, int k
     if (m >= UFS) drho += dm;
   }
   h[URHO] += drho / v;
-  h[UEINT] -= pdivu;
 }
 
 // ----------------------------------------------------------- K28 consup --

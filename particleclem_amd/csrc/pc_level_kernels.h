@@ -251,9 +251,10 @@ PCD double b_update(const Sched& sc, const KArgs& ka, int lb_, DtState*& ds) {
     for (int m = 0; m < NVAR; ++m) h[m] = 0.0;
     if constexpr (F32) cell_divergence_fp32<NS>(J, ka, i, j, k, h);
     else cell_divergence<NS, true>(J, ka, i, j, k, h);
+    // UEINT is reset from UEDEN and the momenta below before it is read
     double S[NVAR];
 #pragma unroll
-    for (int m = 0; m < NVAR; ++m) S[m] = J.U(i, j, k, m) + dt * h[m];
+    for (int m = 0; m < NVAR; ++m) S[m] = m == UEINT ? 0.0 : J.U(i, j, k, m) + dt * h[m];
     double Y[NS];
     state_Y<NS>(S, Y);
     const typename EOS::Thermo th(Y);

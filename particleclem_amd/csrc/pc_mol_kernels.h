@@ -110,9 +110,10 @@ PCD void mol_face_state(const VW& q, const VA& qa, bool slopes, int i,
 // advected or linear scalars), the primitives read through the views q/qa;
 // the extensive flux goes to flx[D] (the flux register starts from zero:
 // 0 + F * area).
-// STORE_T: store the zero UTEMP flux (the box entry point's caller reads
-// it; the level and batch updates take its divergence as zero).
-template <class EOS, int D, bool LAXF, class FP, bool STORE_T, class VQ, class VA>
+// FULL: store the zero UTEMP flux and the UEINT flux (the box entry point's
+// caller reads them); the level and batch updates take the UTEMP divergence
+// as zero and reset UEINT from UEDEN and the momenta, so they need neither.
+template <class EOS, int D, bool LAXF, class FP, bool FULL, class VQ, class VA>
 PCD void mol_face_flux(const JobT<FP>& J, const KArgs& ka, const VQ& q, const VA& qa,
                        int i, int j, int k) {
   constexpr int NS = EOS::NS;
@@ -143,21 +144,22 @@ PCD void mol_face_flux(const JobT<FP>& J, const KArgs& ka, const VQ& q, const VA
     fo(i, j, k, MT) = 0.0 + (double)o.f_ut * area;
     fo(i, j, k, MTT) = 0.0 + (double)o.f_utt * area;
     fo(i, j, k, UEDEN) = 0.0 + (double)o.f_eden * area;
-    fo(i, j, k, UEINT) = 0.0 + (double)o.f_eint * area;
-    if (STORE_T) fo(i, j, k, UTEMP) = 0.0;
+    if (FULL) {
+      fo(i, j, k, UEINT) = 0.0 + (double)o.f_eint * area;
+      fo(i, j, k, UTEMP) = 0.0;
+    }
 #pragma unroll
     for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = 0.0 + (double)o.f_Y[s] * area;
   } else {
-    // mixed step: the FP32 Riemann solve's face fluxes stored in FP32 (the
-    // updates read them into FP64 arithmetic); the zero UTEMP flux is not
-    // stored
+    // mixed step (level and batch): the FP32 Riemann solve's face fluxes
+    // stored in FP32 (the updates read them into FP64 arithmetic); neither
+    // the zero UTEMP flux nor the UEINT flux is stored
     const VT<float> fo = as_fp32(J.flx[D]);
     fo(i, j, k, URHO) = (float)(0.0 + (double)o.f_rho * area);
     fo(i, j, k, MU) = (float)(0.0 + (double)o.f_un * area);
     fo(i, j, k, MT) = (float)(0.0 + (double)o.f_ut * area);
     fo(i, j, k, MTT) = (float)(0.0 + (double)o.f_utt * area);
     fo(i, j, k, UEDEN) = (float)(0.0 + (double)o.f_eden * area);
-    fo(i, j, k, UEINT) = (float)(0.0 + (double)o.f_eint * area);
 #pragma unroll
     for (int s = 0; s < NS; ++s) fo(i, j, k, UFS + s) = (float)(0.0 + (double)o.f_Y[s] * area);
   }
@@ -165,7 +167,7 @@ PCD void mol_face_flux(const JobT<FP>& J, const KArgs& ka, const VQ& q, const VA
 
 // The MOL fluxes of the d-faces of one box, one thread per face, the
 // primitives read from the level arrays.
-template <class EOS, int D, bool LAXF, class FP, bool STORE_T>
+template <class EOS, int D, bool LAXF, class FP, bool FULL>
 PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
   int local;
   const JobT<FP> J = load_job<FP>(sc, lb_, local);
@@ -173,17 +175,17 @@ PCD void b_mol_flux(const Sched& sc, const KArgs& ka, int lb_) {
   stage_region(ST_MOL_X + D, J.lo, J.hi, lo, n);
   if (!region_ijk(sc, lo, n, local, i, j, k)) return;
   const VB<FP> q = bufview(J.q), qa = bufview(J.qa);
-  mol_face_flux<EOS, D, LAXF, FP, STORE_T>(J, ka, q, qa, i, j, k);
+  mol_face_flux<EOS, D, LAXF, FP, FULL>(J, ka, q, qa, i, j, k);
 }
 template <class EOS, int D, bool LAXF, class FP = double, bool PERSIST = false,
-          bool STORE_T = true>
+          bool FULL = true>
 __global__ void __launch_bounds__(BLOCK)
     k_mol_flux(Sched sc, KArgs ka) {
   if constexpr (PERSIST) {
     for (int lb_ = first_block(); lb_ < nlogical(sc); lb_ += gridDim.x)
-      b_mol_flux<EOS, D, LAXF, FP, STORE_T>(sc, ka, lb_);
+      b_mol_flux<EOS, D, LAXF, FP, FULL>(sc, ka, lb_);
   } else {
-    b_mol_flux<EOS, D, LAXF, FP, STORE_T>(sc, ka, first_block(sc));
+    b_mol_flux<EOS, D, LAXF, FP, FULL>(sc, ka, first_block(sc));
   }
 }
 
@@ -329,9 +331,11 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // pc_flux_div (Source/Diffterm.H:470-487): -div(F)/V of one cell.
-// T0: the UTEMP fluxes are not stored (all zero): its divergence is
-// -(0 - 0 + 0 - 0 + 0 - 0) / v = -0.0, the value the sum gives.
-template <int NS, bool T0 = false>
+// LEVEL: the level and batch updates, whose face fluxes store neither the
+// zero UTEMP flux (its divergence -(0 - 0 + 0 - 0 + 0 - 0) / v = -0.0 is the
+// value the sum gives) nor the UEINT flux (UEINT is reset after the update,
+// D[UEINT] = 0 stands in).
+template <int NS, bool LEVEL = false>
 PCD void mol_divergence(const Job& J, int i, int j, int k, double* D) {
   constexpr int NVAR = 7 + NS;
   const double v = J.vol.p ? J.vol(i, j, k, 0) : J.vol_u;
@@ -340,8 +344,12 @@ PCD void mol_divergence(const Job& J, int i, int j, int k, double* D) {
   const V& f2 = J.flx[2];
 #pragma unroll
   for (int m = 0; m < NVAR; ++m) {
-    if (T0 && m == UTEMP) {
+    if (LEVEL && m == UTEMP) {
       D[m] = -0.0 / v;
+      continue;
+    }
+    if (LEVEL && m == UEINT) {
+      D[m] = 0.0;
       continue;
     }
     D[m] = -(f0(i + 1, j, k, m) - f0(i, j, k, m) + f1(i, j + 1, k, m) -
@@ -362,6 +370,10 @@ PCD void mol_divergence_fp32(const Job& J, int i, int j, int k, double* D) {
   for (int m = 0; m < NVAR; ++m) {
     if (m == UTEMP) {
       D[m] = -0.0 / v;
+      continue;
+    }
+    if (m == UEINT) {  // not stored (mol_face_flux); UEINT is reset
+      D[m] = 0.0;
       continue;
     }
     D[m] = -(F(f0, i + 1, j, k, m) - F(f0, i, j, k, m) + F(f1, i, j + 1, k, m) -
@@ -418,14 +430,20 @@ PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
     double D[NVAR];
     if constexpr (F32) mol_divergence_fp32<NS>(J, i, j, k, D);
     else mol_divergence<NS, true>(J, i, j, k, D);
+    // UEINT is reset from UEDEN and the momenta below before it is read, so
+    // neither stage reads it (nor does the first store it)
     double S[NVAR];
     if constexpr (STAGE == 0) {
 #pragma unroll
-      for (int m = 0; m < NVAR; ++m) S[m] = 1.0 * J.U(i, j, k, m) + dt * D[m];
+      for (int m = 0; m < NVAR; ++m) S[m] = m == UEINT ? 0.0 : 1.0 * J.U(i, j, k, m) + dt * D[m];
     } else {
       const double hdt = 0.5 * dt;
 #pragma unroll
       for (int m = 0; m < NVAR; ++m) {
+        if (m == UEINT) {
+          S[m] = 0.0;
+          continue;
+        }
         S[m] = 0.5 * J.sold(i, j, k, m) + 0.5 * J.U(i, j, k, m);
         S[m] += hdt * D[m];
       }
@@ -439,7 +457,7 @@ PCD double b_mol_update(const Sched& sc, const KArgs& ka, int lb_,
     const V& out = STAGE == 0 ? J.sold : J.U;
 #pragma unroll
     for (int m = 0; m < NVAR; ++m) {
-      out(i, j, k, m) = S[m];
+      if (STAGE == 1 || m != UEINT) out(i, j, k, m) = S[m];
       bad |= !isfinite(S[m]);
     }
     if constexpr (STAGE == 1) {
