@@ -1538,8 +1538,9 @@ int pc_level_create_v1(pc_ctx_t* c, const pc_level_desc_t* d,
   L->ka.difmag = prm->difmag;
   L->ka.plm_iorder = prm->plm_iorder;
   L->ka.use_flattening = prm->use_flattening;
-  // only the PLM traces and the low-order boundary faces read q's rho e
-  L->ka.skip_reint = prm->ppm_type == 1 && !L->lowo[0] && !L->lowo[1] && !L->lowo[2];
+  // q's rho e feeds only the traced states' rho e, which no flux reads
+  // (store_trace)
+  L->ka.skip_reint = 1;
   DtState z;
   z.dt = 0.0;
   z.minbits = 0x7FEFFFFFFFFFFFFFull;
@@ -2884,6 +2885,7 @@ int pc_batch_create_v1(pc_ctx_t* c, const pc_params_t* prm, int mixed,
   B->ka.difmag = prm->difmag;
   B->ka.plm_iorder = prm->plm_iorder;
   B->ka.use_flattening = prm->use_flattening;
+  B->ka.skip_reint = 1;  // as the level's
   memset(&B->la, 0, sizeof(B->la));
   B->la.periodic_self = 2;
   B->la.cfl = prm->cfl;

@@ -134,7 +134,8 @@ PCD VB<T> bufview(const VT<T>& v) {
 
 // Compact layouts of the stage intermediates.
 //   traces: comps [0, QT) = minus state (left of face), [QT, 2QT) = plus
-//           state; per state rho, u, v, w, p, rhoe, Y[NS]  (QT = 6 + NS)
+//           state; per state rho, u, v, w, p, rhoe (slot not stored), Y[NS]
+//           (QT = 6 + NS)
 //   fluxes: rho, mx, my, mz, rhoE, p_gd, un_gd, rhoY[NS]   (FT = 7 + NS)
 // FP is the element type of the primitives and of the stage intermediates
 // (double, or float in the mixed-precision step); the state, the final
@@ -185,7 +186,7 @@ struct KArgs {
   int use_flattening;
   int nblocks;
   int full_prims;  // 1: write every q/qaux comp; 0: only what the step reads
-  int skip_reint;  // 1: q's rho e is not stored (PPM steps without low-order faces)
+  int skip_reint;  // 1: q's rho e is not stored (the level and batch steps)
 };
 
 template <class J_>
@@ -373,10 +374,15 @@ PCD void load_trace(const VW& tr, int i, int j, int k, int side,
   s.v = tr(i, j, k, o + 2);
   s.w = tr(i, j, k, o + 3);
   s.p = tr(i, j, k, o + 4);
-  s.rhoe = tr(i, j, k, o + 5);
+  s.rhoe = FP(0.0);  // not stored (store_trace)
 #pragma unroll
   for (int n = 0; n < NS; ++n) s.Y[n] = tr(i, j, k, o + 6 + n);
 }
+// A traced state's rho e (slot 5) is not stored: the transverse and final
+// corrections carry it into their corrected states, but the Riemann solver
+// and cmpflx read rho, the velocities, p and the species only
+// (Source/Godunov.H:303-449, Source/Riemann.H:9-200), so no flux depends on
+// it.
 template <int NS, class FP>
 PCD void store_trace(const VT<FP>& tr, int i, int j, int k, int side,
                      const FaceState<NS, FP>& s) {
@@ -387,7 +393,6 @@ PCD void store_trace(const VT<FP>& tr, int i, int j, int k, int side,
   tr(i, j, k, o + 2) = s.v;
   tr(i, j, k, o + 3) = s.w;
   tr(i, j, k, o + 4) = s.p;
-  tr(i, j, k, o + 5) = s.rhoe;
 #pragma unroll
   for (int n = 0; n < NS; ++n) tr(i, j, k, o + 6 + n) = s.Y[n];
 }

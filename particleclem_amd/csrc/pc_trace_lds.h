@@ -159,8 +159,9 @@ PCD void ppm_cell_states(const FP* lds, int nwin, int wc, int wstep, FP flatv,
     P.u = vel[0];
     P.v = vel[1];
     P.w = vel[2];
-    const FP eint = EOS::RYP2E(P.rho, P.Y, P.p);
-    P.rhoe = P.rho * eint;
+    // the reference traces rho e = rho RYP2E(rho, Y, p) here
+    // (Source/PPM.cpp:324-330); no flux reads it (store_trace)
+    P.rhoe = FP(0.0);
   }
   // ---- minus state on face i+1 (fastest wave to the right)
   {
@@ -195,8 +196,7 @@ PCD void ppm_cell_states(const FP* lds, int nwin, int wc, int wstep, FP flatv,
     M.u = vel[0];
     M.v = vel[1];
     M.w = vel[2];
-    const FP eint = EOS::RYP2E(M.rho, M.Y, M.p);
-    M.rhoe = M.rho * eint;
+    M.rhoe = FP(0.0);  // Source/PPM.cpp:415-421; see the plus state
   }
 }
 
@@ -403,7 +403,6 @@ PCD void b_trace_flux1_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
     lds[2 * ncell + me] = M.v;
     lds[3 * ncell + me] = M.w;
     lds[4 * ncell + me] = M.p;
-    lds[5 * ncell + me] = M.rhoe;
 #pragma unroll
     for (int s = 0; s < NS; ++s) lds[(6 + s) * ncell + me] = M.Y[s];
   }
@@ -421,7 +420,7 @@ PCD void b_trace_flux1_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
     Lf.v = lds[2 * ncell + prev];
     Lf.w = lds[3 * ncell + prev];
     Lf.p = lds[4 * ncell + prev];
-    Lf.rhoe = lds[5 * ncell + prev];
+    Lf.rhoe = FP(0.0);
 #pragma unroll
     for (int s = 0; s < NS; ++s) Lf.Y[s] = lds[(6 + s) * ncell + prev];
     int im = i, jm = j, km = k;
