@@ -402,7 +402,10 @@ PCD void store_flux(const VW& f, int i, int j, int k, const Flux<NS, FP>& F) {
   f(i, j, k, 1) = F.f[1];
   f(i, j, k, 2) = F.f[2];
   f(i, j, k, 3) = F.f[3];
-  f(i, j, k, 4) = F.f[4];
+  // slot 4, the energy flux, is not stored: in the transverse corrections it
+  // only updates the corrected states' rho e, which no flux reads
+  // (store_trace); without it the Riemann solves of the first and transverse
+  // fluxes need no EOS call at the Godunov state
   f(i, j, k, 5) = F.pgd;
   f(i, j, k, 6) = F.ugd;
 #pragma unroll
@@ -718,7 +721,8 @@ template <int NS, class FP>
 PCD void make_tcorr(const Flux<NS, FP>& lo, const Flux<NS, FP>& hi, FP gamc,
                     FP c, TCorr<NS, FP>& D) {
 #pragma unroll
-  for (int m = 0; m < 5; ++m) D.d[m] = c * (hi.f[m] - lo.f[m]);
+  for (int m = 0; m < 4; ++m) D.d[m] = c * (hi.f[m] - lo.f[m]);
+  D.d[4] = FP(0.0);  // the energy flux difference feeds only rho e (store_flux)
 #pragma unroll
   for (int s = 0; s < NS; ++s) D.ds[s] = c * (hi.f[6 + s] - lo.f[6 + s]);
   const FP dAup = hi.pgd * hi.ugd - lo.pgd * lo.ugd;
@@ -729,7 +733,7 @@ PCD void make_tcorr(const Flux<NS, FP>& lo, const Flux<NS, FP>& hi, FP gamc,
 template <int NS, class FP, class VW>
 PCD void store_tcorr(const VW& v, int i, int j, int k, const TCorr<NS, FP>& D) {
 #pragma unroll
-  for (int m = 0; m < 5; ++m) v(i, j, k, m) = D.d[m];
+  for (int m = 0; m < 4; ++m) v(i, j, k, m) = D.d[m];  // slot 4 not stored (make_tcorr)
 #pragma unroll
   for (int s = 0; s < NS; ++s) v(i, j, k, 5 + s) = D.ds[s];
   v(i, j, k, 5 + NS) = D.pt;
@@ -737,7 +741,8 @@ PCD void store_tcorr(const VW& v, int i, int j, int k, const TCorr<NS, FP>& D) {
 template <int NS, class FP, class VW>
 PCD void load_tcorr(const VW& v, int i, int j, int k, TCorr<NS, FP>& D) {
 #pragma unroll
-  for (int m = 0; m < 5; ++m) D.d[m] = v(i, j, k, m);
+  for (int m = 0; m < 4; ++m) D.d[m] = v(i, j, k, m);
+  D.d[4] = FP(0.0);
 #pragma unroll
   for (int s = 0; s < NS; ++s) D.ds[s] = v(i, j, k, 5 + s);
   D.pt = v(i, j, k, 5 + NS);
@@ -746,7 +751,8 @@ PCD void load_tcorr(const VW& v, int i, int j, int k, TCorr<NS, FP>& D) {
 template <int NS, class FP, class VW>
 PCD void load_flux(const VW& f, int i, int j, int k, Flux<NS, FP>& F) {
 #pragma unroll
-  for (int m = 0; m < 5; ++m) F.f[m] = f(i, j, k, m);
+  for (int m = 0; m < 4; ++m) F.f[m] = f(i, j, k, m);
+  F.f[4] = FP(0.0);  // not stored (store_flux)
   F.pgd = f(i, j, k, 5);
   F.ugd = f(i, j, k, 6);
 #pragma unroll
@@ -932,9 +938,10 @@ PCD void final_corr(const JobT<FP>& J, int i, int j, int k, FP hdt,
   step<T1>(i1, j1, k1, 1);
   FP dd[5], dds[NS];
 #pragma unroll
-  for (int m = 0; m < 5; ++m)
+  for (int m = 0; m < 4; ++m)
     dd[m] = c0 * (fx(i0, j0, k0, m) - fx(i, j, k, m)) +
             c1 * (fy(i1, j1, k1, m) - fy(i, j, k, m));
+  dd[4] = FP(0.0);  // the energy fluxes are not stored (store_flux)
 #pragma unroll
   for (int s2 = 0; s2 < NS; ++s2)
     dds[s2] = c0 * (fx(i0, j0, k0, 7 + s2) - fx(i, j, k, 7 + s2)) +

@@ -435,7 +435,7 @@ PCD void b_trace_flux1_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
   __syncthreads();  // every M read: the buffer now takes the face fluxes
   if (face) {
 #pragma unroll
-    for (int m = 0; m < 5; ++m) lds[m * ncell + me] = F.f[m];
+    for (int m = 0; m < 4; ++m) lds[m * ncell + me] = F.f[m];  // no energy flux (store_flux)
 #pragma unroll
     for (int s = 0; s < NS; ++s) lds[(5 + s) * ncell + me] = F.f[6 + s];
     lds[(5 + NS) * ncell + me] = F.pgd;
@@ -446,7 +446,8 @@ PCD void b_trace_flux1_lds(const Sched& sc, const KArgs& ka, int lb_, FP* lds) {
   const int next = (D == 0) ? me + 1 : me + T.ta;
   Flux<NS, FP> Fh;
 #pragma unroll
-  for (int m = 0; m < 5; ++m) Fh.f[m] = lds[m * ncell + next];
+  for (int m = 0; m < 4; ++m) Fh.f[m] = lds[m * ncell + next];
+  Fh.f[4] = FP(0.0);
 #pragma unroll
   for (int s = 0; s < NS; ++s) Fh.f[6 + s] = lds[(5 + s) * ncell + next];
   Fh.pgd = lds[(5 + NS) * ncell + next];
