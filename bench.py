@@ -56,7 +56,14 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
     trace launch also forms the first fluxes and each cell's transverse
     correction (k_trace_flux1_lds); otherwise k_flux1 writes the face fluxes
     and k_tcorr turns them into the corrections."""
-    Q, N, QT, FT = 8 + nspec, 7 + nspec, 6 + nspec, 7 + nspec
+    Q, N = 8 + nspec, 7 + nspec
+    # the Godunov level step stores a traced state and a transverse
+    # correction without rho e and the energy flux difference (TS values:
+    # rho, u, v, w, p, Y; the correction's pressure term in place of the
+    # energy), a transverse flux without the energy flux (FS: rho, momenta,
+    # p_gd, u_gd, Y), and final fluxes without the UTEMP and UEINT fluxes and
+    # without qec (the update resets UEINT)
+    TS, FS = 5 + nspec, 6 + nspec
     tot = {}
 
     def add(k, units, per):
@@ -91,23 +98,26 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
         add("divu", (n[0] + 2) * (n[1] + 2) * (n[2] + 2), 6)
         for d, a in enumerate("xyz"):
             t = [x + 2 for x in n]
-            add("trace_" + a, g1, QT + 2 * QT)
+            add("trace_" + a, g1, TS + 2 * TS)
             f1 = list(t)
             f1[d] = n[d] + 1
             # cells [L, H] along d (transverse [L-1, H+1]) get a correction
-            # of QT values (rho, m, E, rho Y and the pressure term) from
-            # Gamma1 and their two faces' fluxes
+            # of TS values (rho, m, rho Y and the pressure term) from Gamma1
+            # and their two faces' fluxes
             c1 = list(t)
             c1[d] = n[d]
             if fused:  # the faces' states come from the trace, not from HBM
                 add("trace_" + a, f1[0] * f1[1] * f1[2], 1)
-                add("trace_" + a, c1[0] * c1[1] * c1[2], 1 + QT)
+                add("trace_" + a, c1[0] * c1[1] * c1[2], 1 + TS)
             else:
-                add("flux1_" + a, f1[0] * f1[1] * f1[2], 2 * QT + 1 + FT)
-                add("flux1_" + a, c1[0] * c1[1] * c1[2], FT + 1 + QT)
+                add("flux1_" + a, f1[0] * f1[1] * f1[2], 2 * TS + 1 + FS)
+                add("flux1_" + a, c1[0] * c1[1] * c1[2], FS + 1 + TS)
             fin = list(n)
             fin[d] = n[d] + 1
-            add("final_" + a, fin[0] * fin[1] * fin[2], 2 * QT + 2 * FT + 2 * N + 5)
+            # two traces, the two transverse fluxes (each once), U at one
+            # cell (artificial viscosity; neither UTEMP nor UEINT), the flux
+            # written, c and Gamma1 at two cells and div u
+            add("final_" + a, fin[0] * fin[1] * fin[2], 2 * TS + 2 * FS + 2 * (N - 2) + 5)
         for D, name in [(0, "xy"), (1, "yx"), (2, "zx")]:
             # one launch per direction for both (d|t): on the d-faces with
             # both transverse indices in [L-1, H+1] the face's traces and c
@@ -116,12 +126,13 @@ def stage_bytes(nspec, boxes, mol=False, fused=True):
             T0, T1 = [t for t in range(3) if t != D]
             r = [x + 2 for x in n]
             r[D] = n[D] + 1
-            add("trans_" + name, r[0] * r[1] * r[2], 2 * QT + 1)
+            add("trans_" + name, r[0] * r[1] * r[2], 2 * TS + 1)
             for T in (T0, T1):
                 e = list(r)
                 e[T] = n[T]
-                add("trans_" + name, e[0] * e[1] * e[2], QT + FT)
-        add("update", v, 6 * N + 6)
+                add("trans_" + name, e[0] * e[1] * e[2], TS + FS)
+        # U read (not UEINT) and written, three face fluxes per cell
+        add("update", v, (N - 1) + N + 3 * (N - 2))
         add("dt", 0, 0)
     return tot
 
